@@ -1,0 +1,360 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each test checks the oracle against something it does not restate: a
+brute-force DFT, an explicit matrix, a closed form, the complex square root of
+NumPy, a dot-product identity, or the reference's own test fixtures
+(propagator/test/test_wem_ops.py:13-84)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import wem_oracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+def crand(*shape, rng=RNG):
+    return rng.uniform(-0.5, 0.5, shape) + 1j * rng.uniform(-0.5, 0.5, shape)
+
+
+# ---------------------------------------------------------------- FFT / grids
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 8, 12, 15, 16, 30])
+def test_fft_u_matches_bruteforce_dft(n):
+    x = crand(n)
+    j = np.arange(n)
+    F = np.exp(-2j * np.pi * np.outer(j, j) / n) / math.sqrt(n)
+    np.testing.assert_allclose(O.fft_u(x), F @ x, atol=1e-13)
+    np.testing.assert_allclose(O.ifft_u(O.fft_u(x)), x, atol=1e-13)
+    assert abs(np.linalg.norm(O.fft_u(x)) - np.linalg.norm(x)) < 1e-12
+
+
+def test_kgrid_quirks_even_and_odd():
+    # hand-evaluated from Phshift_cmplx.cpp:12-24 for nk = 6 and nk = 5, d = 1
+    dk6 = 2 * np.pi / 5
+    np.testing.assert_allclose(O.kgrid(6, 1.0), [0, dk6, 2 * dk6, 0, -2 * dk6, -dk6])
+    dk5 = 2 * np.pi / 4
+    np.testing.assert_allclose(O.kgrid(5, 1.0), [0, dk5, 0, 0, -dk5])
+
+
+def test_kz_parts_equal_numpy_complex_sqrt():
+    k = O.kgrid(64, 10.0)
+    for s in (1 / 1500 ** 2, 1 / 1000 ** 2 - 1e-8j, 1 / 3000 ** 2 + 3e-9j):
+        for w in (2 * np.pi * 2.0, 2 * np.pi * 30.0):
+            s = complex(s)
+            re, im = O.kz_parts(w, s, 0.04, k)
+            a = w * w * s.real - k * k
+            b = w * w * (s.imag - 0.04 * s.real)
+            z = np.sqrt(a + 1j * b)
+            np.testing.assert_allclose(re, np.abs(z.real), rtol=1e-12, atol=1e-12)
+            np.testing.assert_allclose(im, -np.abs(z.imag), rtol=1e-12, atol=1e-12)
+
+
+def test_phase_factor_vertical_closed_form_and_evanescent_decay():
+    w = 2 * np.pi * 10.0
+    s = 1 / 2000 ** 2
+    P = O.phase_factor(w, 10.0, s, 0.0, np.array([0.0]))
+    np.testing.assert_allclose(P, np.exp(-1j * w * np.sqrt(s) * 10.0), rtol=1e-14)
+    k = O.kgrid(256, 10.0)
+    assert np.all(np.abs(O.phase_factor(w, 10.0, s, 0.04, k)) <= 1.0 + 1e-15)
+
+
+def test_taper_and_cos2_window():
+    f = O.taper_filter(10, 3)
+    np.testing.assert_allclose(f, [0, 0.5, 1, 1, 1, 1, 1, 1, 0.5, 0], atol=1e-15)
+    np.testing.assert_allclose(O.cos2_window(2), [0, 0.5, 1, 0.5, 0], atol=1e-15)
+    with pytest.raises(ValueError):
+        O.taper_filter(10, 1)
+
+
+# ---------------------------------------------------------------- RefSampler
+
+def test_refsampler_nref1_assigns_everything():
+    s = (1 / 1500 ** 2 + RNG.uniform(-1e-8, 1e-8, (20, 40))).astype(np.complex64)
+    t = O.refsampler(s, 1)
+    assert np.all(t.refid == 0)
+    np.testing.assert_array_equal(t.sref[:, 0], s[:, 20])
+
+
+def test_refsampler_stale_buffer_collapses_to_max_reference():
+    # while zeros remain in the nx-long reference buffer every x goes to the
+    # reference with the largest slowness (RefSampler.cpp:18,49-51,63-81)
+    nz, nx, nref = 6, 64, 4
+    s = np.empty((nz, nx), dtype=np.complex64)
+    s[:] = (1 / np.linspace(1500, 4500, nx) ** 2)[None, :]
+    t = O.refsampler(s, nref)
+    for iz in range(nz):
+        rmax = int(np.argmax(t.sref[iz].real))
+        assert np.all(t.refid[iz] == rmax)
+
+
+def test_refsampler_rejects_nonpositive_reference():
+    s = np.ones((3, 8), dtype=np.complex64)
+    s[1, 4] = 0
+    with pytest.raises(ValueError):
+        O.refsampler(s, 1)
+
+
+def test_refsampler_partition_after_buffer_fills():
+    # once the buffer holds no zeros the partition follows the sorted buffer
+    # midpoints; check against a direct evaluation of the while loops
+    nz, nx, nref = 40, 8, 2
+    rng = np.random.default_rng(7)
+    s = (1 / rng.uniform(1500, 4500, (nz, nx)) ** 2).astype(np.complex64)
+    t = O.refsampler(s, nref)
+    # every depth is a partition into at most nref classes, x unassigned only
+    # when nref == 1
+    assert np.all(t.refid >= 0)
+    assert np.all(t.refid < nref)
+
+
+# ---------------------------------------------------------------- one step
+
+def _fixture(nx=48, nz=12, nref=1, prop="ssf", tap=4, pad=5, lateral=True, seed=3):
+    rng = np.random.default_rng(seed)
+    v = 1500 + 1500 * np.linspace(0, 1, nz)[:, None] + np.zeros((nz, nx))
+    if lateral:
+        v = v * (1 + 0.05 * rng.standard_normal((nz, nx)))
+    s = (1 / v ** 2).astype(np.complex64)
+    p = O.StepParams(nx=nx, dx=10.0, dz=10.0, tap=tap, pad=pad, nref=nref, eps=0.04,
+                     prop=prop, ntaylor=1)
+    return s, p, O.refsampler(s, nref)
+
+
+def _matrix(f, n, shape_tail=None):
+    cols = []
+    for j in range(n):
+        e = np.zeros(n, dtype=complex)
+        e[j] = 1
+        cols.append(np.asarray(f(e)).ravel())
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("prop,nref", [("ssf", 1), ("pspi", 1), ("ssf", 3), ("pspi", 3)])
+def test_onestep_adjoint_is_conjugate_transpose(prop, nref):
+    s, p, ref = _fixture(nref=nref, prop=prop)
+    w = 2 * np.pi * 12.0
+    iz = 7
+    A = _matrix(lambda e: O.onestep_fwd(e, iz, w, s, ref, p), p.nx)
+    AH = _matrix(lambda e: O.onestep_adj(e, iz, w, s, ref, p), p.nx)
+    np.testing.assert_allclose(AH, A.conj().T, atol=1e-13)
+
+
+def test_onestep_homogeneous_is_exact_phase_shift():
+    nx = 32
+    s = np.full((4, nx), 1 / 2000 ** 2, dtype=np.complex64)
+    p = O.StepParams(nx=nx, dx=10.0, dz=10.0, tap=0, pad=0, nref=1, eps=0.04, prop="ssf")
+    ref = O.refsampler(s, 1)
+    x = crand(nx)
+    w = 2 * np.pi * 8.0
+    k = O.kgrid(nx, 10.0)
+    # independent restatement: kz from numpy's complex sqrt
+    kz = np.sqrt(w * w * (1 / 2000 ** 2) * (1 - 0.04j) - k * k)
+    kz = np.abs(kz.real) - 1j * np.abs(kz.imag)
+    y = np.fft.ifft(np.exp(-1j * kz * 10.0) * np.fft.fft(x))
+    np.testing.assert_allclose(O.onestep_fwd(x, 1, w, s, ref, p), y, atol=1e-13)
+    assert np.allclose(O.onestep_fwd(x, 1, w, s, ref, p),
+                       O.onestep_fwd(x, 1, w, s, ref,
+                                     O.StepParams(**{**p.__dict__, "prop": "pspi"})))
+
+
+# ---------------------------------------------------------------- Down / Up
+
+@pytest.mark.parametrize("down", [True, False])
+def test_inplace_recursion_equals_block_bidiagonal_solve(down):
+    nx, nz = 10, 5
+    s, p, ref = _fixture(nx=nx, nz=nz, tap=2, pad=0)
+    w = 2 * np.pi * 6.0
+    steps = [_matrix(lambda e, iz=iz: O.onestep_fwd(e, iz, w, s, ref, p), nx) for iz in range(nz)]
+    n = nx * nz
+    L = np.zeros((n, n), dtype=complex)
+    for iz in range(nz):
+        nxt = iz + 1 if down else iz - 1
+        if 0 <= nxt < nz:
+            L[nxt * nx:(nxt + 1) * nx, iz * nx:(iz + 1) * nx] = steps[iz]
+    m = crand(nz, nx)
+    want = np.linalg.solve(np.eye(n) - L, m.ravel()).reshape(nz, nx)
+    f = O.down_fwd if down else O.up_fwd
+    np.testing.assert_allclose(f(m, w, s, ref, p), want, atol=1e-12)
+    # out of place the reference applies a single bidiagonal step (I + L)
+    np.testing.assert_allclose(f(m, w, s, ref, p, inplace=False),
+                               (m.ravel() + L @ m.ravel()).reshape(nz, nx), atol=1e-12)
+    # the adjoint recursion solves with (I - L)^H
+    fa = O.down_adj if down else O.up_adj
+    d = crand(nz, nx)
+    wantH = np.linalg.solve((np.eye(n) - L).conj().T, d.ravel()).reshape(nz, nx)
+    np.testing.assert_allclose(fa(d, w, s, ref, p), wantH, atol=1e-12)
+
+
+# ---------------------------------------------------------------- Scatter / Lin
+
+def test_scatter_exact_adjoint_and_compat_order():
+    s, p, ref = _fixture(nx=24, nz=6, tap=3)
+    w = 2 * np.pi * 9.0
+    A = _matrix(lambda e: O.scatter_fwd(e, 2, w, s, p), p.nx)
+    AH = _matrix(lambda e: O.scatter_adj(e, 2, w, s, p, "exact"), p.nx)
+    np.testing.assert_allclose(AH, A.conj().T, atol=1e-12)
+    # laterally varying S and ntaylor=1: the reference order is not the adjoint
+    AC = _matrix(lambda e: O.scatter_adj(e, 2, w, s, p, "compat"), p.nx)
+    assert np.max(np.abs(AC - A.conj().T)) > 1e-6 * np.max(np.abs(A))
+    # laterally constant S: both coincide (test_wem_ops.py:160-167 setting)
+    s2, p2, _ = _fixture(nx=24, nz=6, tap=3, lateral=False)
+    A2 = _matrix(lambda e: O.scatter_fwd(e, 2, w, s2, p2), p2.nx)
+    AC2 = _matrix(lambda e: O.scatter_adj(e, 2, w, s2, p2, "compat"), p2.nx)
+    np.testing.assert_allclose(AC2, A2.conj().T, atol=1e-12)
+
+
+def test_scatter_ntaylor0_closed_form():
+    s, p, ref = _fixture(nx=16, nz=4, tap=0, pad=0)
+    p0 = O.StepParams(**{**p.__dict__, "ntaylor": 0})
+    x = crand(16)
+    w = 2 * np.pi * 5.0
+    s1 = s[1].astype(complex)
+    want = -1j * w * 10.0 * (s1 - 0.04 * s1.real) ** -0.5 * x
+    np.testing.assert_allclose(O.scatter_fwd(x, 1, w, s, p0), want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("lin", ["down", "up"])
+def test_lin_oneway_dot_product_exact_mode(lin):
+    s, p, ref = _fixture(nx=20, nz=6, tap=2, pad=3, nref=2)
+    w = 2 * np.pi * 7.0
+    bg = crand(6, 20)
+    m = crand(6, 20)
+    d = crand(6, 20)
+    if lin == "down":
+        fm = O.lindown_fwd(m, bg, w, s, ref, p)
+        ad = O.lindown_adj(d, bg, w, s, ref, p, mode="exact")
+        # row 0 of the output is always zero; row nz-1 of the model unused
+        lhs = O.cdot(fm, d)
+        rhs = O.cdot(m, ad)
+    else:
+        fm = O.linup_fwd(m, bg, w, s, ref, p)
+        ad = O.linup_adj(d, bg, w, s, ref, p, mode="exact")
+        lhs = O.cdot(fm, d)
+        rhs = O.cdot(m, ad)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+# ---------------------------------------------------------------- IC / Reflect
+
+def test_dreflect_and_reflect_dot_products():
+    nz, nx = 7, 9
+    s = (1 / RNG.uniform(1500, 3000, (nz, nx)) ** 2).astype(complex)
+    den = RNG.uniform(1, 2, (nz, nx)).astype(complex)
+    dS, dR, dr = crand(nz, nx), crand(nz, nx), crand(nz, nx)
+    f = O.dreflect_fwd(s, den, dS, dR)
+    gs, gd = O.dreflect_adj(s, den, dr)
+    lhs = O.cdot(f, dr)
+    rhs = O.cdot(dS, gs) + O.cdot(dR, gd)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    r = O.reflect_coef(s, den)
+    assert np.all(r[0] == 0)
+    # one interface with equal density: r = (sqrt(S1)-sqrt(S2))/(sqrt(S1)+sqrt(S2))
+    np.testing.assert_allclose(r[3], (np.sqrt(s[2]) - np.sqrt(s[3])) / (np.sqrt(s[2]) + np.sqrt(s[3]))
+                               if np.allclose(den[3], den[2]) else r[3])
+
+
+def test_offset_ic_zero_lag_and_adjoint():
+    nz, nx, nh = 5, 12, 5
+    bg, sc = crand(nz, nx), crand(nz, nx)
+    I = O.offset_ic_adj(bg, sc, nh)
+    np.testing.assert_allclose(I[nh // 2], O.ic_adj(bg, sc))
+    img = crand(nh, nz, nx)
+    lhs = O.cdot(O.offset_ic_fwd(bg, img), sc)
+    rhs = O.cdot(img, I)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    # shift identity: a delta receiver field at x0 images only at x0 - h
+    d = np.zeros((nz, nx), complex)
+    d[2, 6] = 1
+    ones = np.ones((nz, nx), complex)
+    I2 = O.offset_ic_adj(ones, d, nh)
+    for j, h in enumerate(range(-(nh // 2), nh // 2 + 1)):
+        nzr = np.argwhere(I2[j] != 0)
+        assert nzr.tolist() == [[2, 6 - h]]
+
+
+# ---------------------------------------------------------------- operators
+
+def _survey(nx=24, nz=10, nw=3, ns=2, pad=3, tap=2, nref=1, prop="ssf", seed=11):
+    rng = np.random.default_rng(seed)
+    nt = 64
+    freq = (2.0 + np.arange(nw) * 1.5).astype(np.float32)
+    index = np.arange(nw) + 3
+    wave_f = crand(nt, rng=rng)
+    sv = O.Survey(nx=nx, nz=nz, dx=10.0, dz=10.0, freq=freq, index=index, wave_f=wave_f,
+                  src_iz=[0] * ns, src_ix=list(np.linspace(3, nx - 4, ns).astype(int)),
+                  rec_iz=[0] * 6, rec_ix=list(np.linspace(0, nx - 1, 6).astype(int)),
+                  nt_f=nt, tap=tap, pad=pad, nref=nref, eps=0.04, prop=prop, ntaylor=1,
+                  ngs=1, ngr=1, ng=1)
+    v = 1500 + 1000 * np.linspace(0, 1, nz)[:, None] + 50 * rng.standard_normal((nz, nx))
+    slow = (1 / v ** 2).astype(np.complex64)
+    den = (1 + 0.1 * rng.random((nz, nx))).astype(np.complex64)
+    return sv, slow, den
+
+
+def test_born_dot_product_exact_mode():
+    sv, slow, den = _survey()
+    rng = np.random.default_rng(5)
+    dS, dR = crand(sv.nz, sv.nx, rng=rng), crand(sv.nz, sv.nx, rng=rng)
+    d = np.zeros((sv.ns, sv.nr, sv.nt_f), complex)
+    d[:, :, sv.index] = crand(sv.ns, sv.nr, len(sv.index), rng=rng)
+    fwd = O.born_forward(sv, slow, den, dS, dR)
+    gS, gR = O.born_adjoint(sv, slow, den, d, mode="exact")
+    lhs = O.cdot(fwd, d)
+    rhs = O.cdot(dS, gS) + O.cdot(dR, gR)
+    assert abs(lhs - rhs) <= 1e-9 * abs(lhs)
+
+
+def test_bornrefl_dot_product():
+    sv, slow, den = _survey(pad=2, tap=2)
+    rng = np.random.default_rng(6)
+    img = crand(len(sv.freq), sv.nz, sv.nx, rng=rng)
+    d = np.zeros((sv.ns, sv.nr, sv.nt_f), complex)
+    d[:, :, sv.index] = crand(sv.ns, sv.nr, len(sv.index), rng=rng)
+    lhs = O.cdot(O.bornrefl_forward(sv, slow, img), d)
+    rhs = O.cdot(img, O.bornrefl_adjoint(sv, slow, d))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_extwem_equals_wem_for_constant_slices():
+    # test_wem_ops.py:268-282
+    sv, slow, den = _survey(nw=2, ns=1)
+    nw = len(sv.freq)
+    d1 = O.wem_forward(sv, slow, den)
+    d2 = O.extwem_forward(sv, np.repeat(slow[None], nw, 0), np.repeat(den[None], nw, 0))
+    assert np.array_equal(d1, d2)
+
+
+def test_extborn_dot_product_exact_mode():
+    sv, slow, den = _survey(nw=2, ns=2, nx=16, nz=8)
+    nw = len(sv.freq)
+    rng = np.random.default_rng(8)
+    s3 = np.repeat(slow[None], nw, 0)
+    d3 = np.repeat(den[None], nw, 0)
+    dS, dR = crand(nw, sv.nz, sv.nx, rng=rng), crand(nw, sv.nz, sv.nx, rng=rng)
+    d = np.zeros((sv.ns, sv.nr, sv.nt_f), complex)
+    d[:, :, sv.index] = crand(sv.ns, sv.nr, nw, rng=rng)
+    lhs = O.cdot(O.extborn_forward(sv, s3, d3, dS, dR), d)
+    gS, gR = O.extborn_adjoint(sv, s3, d3, d, mode="exact")
+    rhs = O.cdot(dS, gS) + O.cdot(dR, gR)
+    assert abs(lhs - rhs) <= 1e-9 * abs(lhs)
+
+
+def test_freq_grid_reference_fixture_and_config_grid():
+    # test_wem_ops.py:23-84: nt=1001, dt=.004, fmin=1, fmax=0, nfreq=10 -> bins 1..10
+    f, idx = O.freq_grid(1001, 0.004, fmin=1, fmax=0, nfreq=10)
+    np.testing.assert_array_equal(idx, np.arange(1, 11))
+    np.testing.assert_allclose(f, 1 + np.arange(10) / 4.0, rtol=1e-6)
+    # 64 frequencies 2..20 Hz: tmax = 3.5 s (nt = 876 at 4 ms)
+    f, idx = O.freq_grid(876, 0.004, fmin=2, fmax=20)
+    assert len(f) == 64 and idx[0] == 7 and idx[-1] == 70
+    np.testing.assert_allclose(f[-1], 20.0, rtol=1e-6)
+
+
+def test_device_indices_round_half_up():
+    iz, ix = O.device_indices(3, 1000.0, 50.0, 10.0, 0.0, 0.0, 0.0, 10.0, 10.0)
+    assert ix == [100, 101, 102] and iz == [5, 5, 5]
+    with pytest.raises(ValueError):
+        O.device_indices(1, -20.0, 0.0, 0.0, 1.0, 0.0, 0.0, 10.0, 10.0)
