@@ -1,0 +1,413 @@
+// Host layer of libwemfwi: RefSampler, background models, phase tables and
+// the C ABI declared in include/wemfwi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/wemfwi.h"
+#include "host.h"
+#include "kernels.h"
+
+using cf = std::complex<float>;
+using cd = std::complex<double>;
+
+namespace wemfwi {
+
+static thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(WEMFWI_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ RefSampler
+
+int refsampler_host(int nz, int nx, const cf* slow, int nref, cf* sref, int* refid) {
+  if (nz < 1 || nx < 2 || nref < 1) return fail(WEMFWI_ERR_ARG, "refsampler: need nz>=1, nx>=2, nref>=1");
+  // uniform_sample (RefSampler.cpp:28-39)
+  const int o = nx / (2 * nref);
+  for (int iz = 0; iz < nz; ++iz)
+    for (int r = 0; r < nref; ++r) {
+      const int off = o + r * nx / nref;
+      sref[(size_t)iz * nref + r] = slow[(size_t)iz * nx + off];
+      if (sref[(size_t)iz * nref + r].real() <= 0.f)
+        return fail(WEMFWI_ERR_REFERENCE, "Ref velocity " + std::to_string(sref[(size_t)iz * nref + r].real()) +
+                                              " at the layer " + std::to_string(iz));
+    }
+  // getIndexes (RefSampler.cpp:41-84).  The reference buffer has nx entries of
+  // which only nref are refreshed per depth; the whole buffer is sorted.
+  std::vector<float> buf(nx, 0.f), s(nx);
+  std::vector<int> is(nx), isr(nref);
+  std::vector<std::pair<int, int>> lists(nref);
+  for (int iz = 0; iz < nz; ++iz) {
+    for (int x = 0; x < nx; ++x) s[x] = slow[(size_t)iz * nx + x].real();
+    for (int r = 0; r < nref; ++r) buf[r] = sref[(size_t)iz * nref + r].real();
+    std::iota(is.begin(), is.end(), 0);
+    std::iota(isr.begin(), isr.end(), 0);
+    std::stable_sort(is.begin(), is.end(), [&](int a, int b) { return s[a] < s[b]; });
+    std::stable_sort(isr.begin(), isr.end(), [&](int a, int b) { return buf[a] < buf[b]; });
+    std::sort(s.begin(), s.end());
+    std::sort(buf.begin(), buf.end());
+    for (auto& l : lists) l = {0, 0};
+    int off = 0;
+    while (off < nx && (s[off] - buf[0]) < (buf[1] - s[off])) ++off;
+    lists[isr[0]] = {0, off};
+    int off_beg = off;
+    for (int ir = 1; ir < nref - 1; ++ir) {
+      while (off < nx && (s[off] - buf[ir]) < (buf[ir + 1] - s[off])) ++off;
+      lists[isr[ir]] = {off_beg, off};
+      off_beg = off;
+    }
+    lists[isr[nref - 1]] = {off, nx};
+    int* row = refid + (size_t)iz * nx;
+    std::fill(row, row + nx, -1);
+    for (int r = 0; r < nref; ++r)
+      for (int i = lists[r].first; i < lists[r].second; ++i) row[is[i]] = r;
+  }
+  return WEMFWI_OK;
+}
+
+// ------------------------------------------------------------------ device helpers
+
+template <typename T>
+int dev_alloc(T** p, size_t n, const char* what) {
+  *p = nullptr;
+  if (n == 0) return WEMFWI_OK;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+  }
+  return WEMFWI_OK;
+}
+
+template <typename T>
+int dev_upload(T** p, const T* src, size_t n, const char* what) {
+  int rc = dev_alloc(p, n, what);
+  if (rc) return rc;
+  if (n == 0) return WEMFWI_OK;
+  hipError_t e = hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return WEMFWI_OK;
+}
+
+void Workspace::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+int Workspace::reserve(size_t bytes, const char* what) {
+  if (bytes <= cap) return WEMFWI_OK;
+  release();
+  hipError_t e = hipMalloc(&ptr, bytes);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+  }
+  cap = bytes;
+  return WEMFWI_OK;
+}
+
+static std::map<int, float2*>& twiddle_cache() {
+  static std::map<int, float2*> c;
+  return c;
+}
+
+int twiddles(int n, const float2** out) {
+  auto& c = twiddle_cache();
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int key = n * 64 + dev;
+  auto it = c.find(key);
+  if (it != c.end()) {
+    *out = it->second;
+    return WEMFWI_OK;
+  }
+  std::vector<float2> h(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)n;
+    h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2* d = nullptr;
+  int rc = dev_upload(&d, h.data(), h.size(), "twiddle table");
+  if (rc) return rc;
+  c[key] = d;
+  *out = d;
+  return WEMFWI_OK;
+}
+
+// ------------------------------------------------------------------ model
+
+Model::~Model() {
+  void* ptrs[] = {d_slot_begin, d_slot_ref, d_refid, d_phi, d_taper, d_slot_sref, d_tables};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  ptab.release();
+  wlist.release();
+  rows.release();
+}
+
+int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_h) {
+  d = desc;
+  if (d.nz < 2 || d.nx < 2 || d.nref < 1 || d.tap < 0 || d.taper < 0 || d.ntaylor < 0 ||
+      d.ntaylor > 4 || !(d.dx > 0) || !(d.dz > 0))
+    return fail(WEMFWI_ERR_ARG, "model: invalid grid or parameters");
+  if (d.taper == 1) return fail(WEMFWI_ERR_ARG, "model: taper width pad=1 is undefined (Taper.h divides by pad-1)");
+  if (2 * d.taper > d.nx) return fail(WEMFWI_ERR_ARG, "model: taper wider than half the row");
+  N = d.nx + 2 * d.tap;
+  if (!fft_size_supported(N))
+    return fail(WEMFWI_ERR_UNSUPPORTED, "model: FFT length nx+2*tap=" + std::to_string(N) +
+                                            " not supported (built: 128, 256, 512, 1024, 2048)");
+  const size_t nzx = (size_t)d.nz * d.nx;
+  slow.assign(slow_h, slow_h + nzx);
+  if (den_h) den.assign(den_h, den_h + nzx);
+  else den.assign(nzx, cf(1.f, 0.f));
+  sref.resize((size_t)d.nz * d.nref);
+  refid.resize(nzx);
+  int rc = refsampler_host(d.nz, d.nx, slow.data(), d.nref, sref.data(), refid.data());
+  if (rc) return rc;
+
+  // active (depth, reference) slots: references selecting no x are skipped
+  slot_begin.assign(d.nz + 1, 0);
+  slot_ref.clear();
+  std::vector<double2> ssref;
+  std::vector<char> used(d.nref);
+  for (int iz = 0; iz < d.nz; ++iz) {
+    slot_begin[iz] = (int)slot_ref.size();
+    std::fill(used.begin(), used.end(), 0);
+    for (int x = 0; x < d.nx; ++x) {
+      const int r = refid[(size_t)iz * d.nx + x];
+      if (r >= 0) used[r] = 1;
+    }
+    for (int r = 0; r < d.nref; ++r)
+      if (used[r]) {
+        slot_ref.push_back(r);
+        const cf s = sref[(size_t)iz * d.nref + r];
+        double2 v;
+        v.x = s.real();
+        v.y = s.imag();
+        ssref.push_back(v);
+      }
+  }
+  slot_begin[d.nz] = (int)slot_ref.size();
+  nslots = (int)slot_ref.size();
+
+  // split-step exponent phi = dz (S - Sref)/sqrt(Sref) of x's reference, so
+  // that SplitStep.cpp:13 is exp(-i w phi); zero for PSPI.
+  std::vector<float2> phi(nzx, make_float2(0.f, 0.f));
+  std::vector<signed char> rid8(nzx);
+  for (size_t i = 0; i < nzx; ++i) {
+    const int r = refid[i];
+    rid8[i] = (signed char)r;
+    if (r < 0 || d.pspi) continue;
+    const size_t iz = i / d.nx;
+    const cd sr(sref[iz * d.nref + r].real(), sref[iz * d.nref + r].imag());
+    const cd sv(slow[i].real(), slow[i].imag());
+    const cd p = (double)d.dz * (sv - sr) / std::sqrt(sr);
+    phi[i] = make_float2((float)p.real(), (float)p.imag());
+  }
+  // Taper.h:11-23
+  std::vector<float> taper(d.nx, 1.f);
+  for (int i = 0; i < d.taper; ++i) {
+    taper[i] = (float)(0.5 * (1.0 - std::cos(M_PI * i / (double)(d.taper - 1))));
+    taper[d.nx - 1 - i] = taper[i];
+  }
+
+  if ((rc = dev_upload(&d_slot_begin, slot_begin.data(), slot_begin.size(), "slot_begin"))) return rc;
+  if ((rc = dev_upload(&d_slot_ref, slot_ref.data(), slot_ref.size(), "slot_ref"))) return rc;
+  if ((rc = dev_upload(&d_refid, rid8.data(), rid8.size(), "refid"))) return rc;
+  if ((rc = dev_upload(&d_phi, phi.data(), phi.size(), "phi"))) return rc;
+  if ((rc = dev_upload(&d_taper, taper.data(), taper.size(), "taper"))) return rc;
+  if ((rc = dev_upload(&d_slot_sref, ssref.data(), ssref.size(), "slot_sref"))) return rc;
+  DepthTables t;
+  t.slot_begin = d_slot_begin;
+  t.slot_ref = d_slot_ref;
+  t.refid = d_refid;
+  t.phi = d_phi;
+  t.taper = d_taper;
+  if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
+  if ((rc = twiddles(N, &d_tw))) return rc;
+  return WEMFWI_OK;
+}
+
+StepGeom Model::geom() const {
+  StepGeom g;
+  g.nx = d.nx;
+  g.tap = d.tap;
+  g.ssf = d.pspi ? 0 : 1;
+  return g;
+}
+
+// Builds the phase table of the distinct frequencies of freq_hz; fills
+// wrow[i] (angular frequency) and prow[i] (table of row i).
+int Model::phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow,
+                        std::vector<const float2*>& prow, hipStream_t s) {
+  std::vector<float> uniq(freq_hz, freq_hz + nrows);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  std::vector<double>& wl = h_wlist;  // member: outlives the asynchronous upload
+  wl.resize(uniq.size());
+  for (size_t i = 0; i < uniq.size(); ++i) wl[i] = 2.0 * M_PI * (double)uniq[i];
+  const size_t per = (size_t)std::max(nslots, 1) * N;
+  int rc = ptab.reserve(uniq.size() * per * sizeof(float2), "phase table");
+  if (rc) return rc;
+  if ((rc = wlist.reserve(wl.size() * sizeof(double), "frequency list"))) return rc;
+  hipError_t e = hipMemcpyAsync(wlist.ptr, wl.data(), wl.size() * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "frequency upload");
+  const double dk = 2.0 * M_PI / ((double)d.dx * (double)(N - 1));
+  e = launch_phase_table((float2*)ptab.ptr, (const double*)wlist.ptr, (int)wl.size(), d_slot_sref, nslots,
+                         N, dk, (double)d.dz, (double)d.eps, s);
+  if (e != hipSuccess) return hip_fail(e, "phase table kernel");
+  wrow.resize(nrows);
+  prow.resize(nrows);
+  for (int i = 0; i < nrows; ++i) {
+    const size_t k = std::lower_bound(uniq.begin(), uniq.end(), freq_hz[i]) - uniq.begin();
+    wrow[i] = (float)wl[k];
+    prow[i] = (const float2*)ptab.ptr + k * per;
+  }
+  return WEMFWI_OK;
+}
+
+int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf* in, cf* out, int add,
+                  hipStream_t s) {
+  if (nrows < 0 || !freq_hz || !in || !out) return fail(WEMFWI_ERR_ARG, "oneway: null argument");
+  if (nrows == 0) return WEMFWI_OK;
+  const size_t nzx = (size_t)d.nz * d.nx;
+  hipError_t e;
+  if (in == out && !add) {
+    // OneWay.cpp:34: the shared vector is zeroed before the recursion reads it
+    e = hipMemsetAsync(out, 0, nrows * nzx * sizeof(cf), s);
+    if (e != hipSuccess) return hip_fail(e, "memset");
+    e = hipStreamSynchronize(s);
+    return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "oneway");
+  }
+  std::vector<float> wrow;
+  std::vector<const float2*> prow;
+  int rc = phase_tables(nrows, freq_hz, wrow, prow, s);
+  if (rc) return rc;
+  h_sweep.resize(nrows);
+  std::vector<SweepRow>& h = h_sweep;
+  for (int i = 0; i < nrows; ++i) {
+    h[i].in = (const float2*)(in + i * nzx);
+    h[i].out = (float2*)(out + i * nzx);
+    h[i].ptab = prow[i];
+    h[i].w = wrow[i];
+    h[i].model = 0;
+  }
+  if ((rc = rows.reserve(h.size() * sizeof(SweepRow), "row descriptors"))) return rc;
+  e = hipMemcpyAsync(rows.ptr, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "row upload");
+  const bool down = !up;
+  const int step = (down != (adjoint != 0)) ? 1 : -1;
+  e = launch_oneway(N, (const SweepRow*)rows.ptr, nrows, d_tables, geom(), d.nz, step, adjoint != 0, d_tw, s);
+  if (e != hipSuccess) return hip_fail(e, "oneway kernel");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "oneway");
+  return WEMFWI_OK;
+}
+
+int Model::onestep(int adjoint, int nrows, const float* freq_hz, const int* depth, const cf* in, cf* out,
+                   int add, hipStream_t s) {
+  if (nrows < 0 || !freq_hz || !depth || !in || !out) return fail(WEMFWI_ERR_ARG, "onestep: null argument");
+  if (nrows == 0) return WEMFWI_OK;
+  for (int i = 0; i < nrows; ++i)
+    if (depth[i] < 0 || depth[i] >= d.nz) return fail(WEMFWI_ERR_ARG, "onestep: depth out of range");
+  std::vector<float> wrow;
+  std::vector<const float2*> prow;
+  int rc = phase_tables(nrows, freq_hz, wrow, prow, s);
+  if (rc) return rc;
+  h_step.resize(nrows);
+  std::vector<StepRow>& h = h_step;
+  for (int i = 0; i < nrows; ++i) {
+    h[i].in = (const float2*)(in + (size_t)i * d.nx);
+    h[i].out = (float2*)(out + (size_t)i * d.nx);
+    h[i].ptab = prow[i];
+    h[i].w = wrow[i];
+    h[i].d = depth[i];
+    h[i].model = 0;
+    h[i].add = add ? 1 : 0;
+  }
+  if ((rc = rows.reserve(h.size() * sizeof(StepRow), "row descriptors"))) return rc;
+  hipError_t e = hipMemcpyAsync(rows.ptr, h.data(), h.size() * sizeof(StepRow), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "row upload");
+  e = launch_onestep(N, (const StepRow*)rows.ptr, nrows, d_tables, geom(), adjoint != 0, d_tw, s);
+  if (e != hipSuccess) return hip_fail(e, "onestep kernel");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "onestep");
+  return WEMFWI_OK;
+}
+
+}  // namespace wemfwi
+
+// ------------------------------------------------------------------ C ABI
+
+using namespace wemfwi;
+
+struct wemfwi_model {
+  Model m;
+};
+
+extern "C" {
+
+const char* wemfwi_last_error(void) { return g_err.c_str(); }
+
+int wemfwi_version(void) { return 100; }
+
+int wemfwi_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int wemfwi_refsampler(int nz, int nx, const wemfwi_complex* slow, int nref, wemfwi_complex* sref,
+                      int* refid) {
+  if (!slow || !sref || !refid) return fail(WEMFWI_ERR_ARG, "refsampler: null argument");
+  return refsampler_host(nz, nx, (const cf*)slow, nref, (cf*)sref, refid);
+}
+
+int wemfwi_model_create(const wemfwi_model_desc* desc, const wemfwi_complex* slow, const wemfwi_complex* den,
+                        wemfwi_model** out) {
+  if (!desc || !slow || !out) return fail(WEMFWI_ERR_ARG, "model_create: null argument");
+  *out = nullptr;
+  wemfwi_model* h = new (std::nothrow) wemfwi_model();
+  if (!h) return fail(WEMFWI_ERR_ALLOC, "model_create: out of host memory");
+  int rc = h->m.build(*desc, (const cf*)slow, (const cf*)den);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return WEMFWI_OK;
+}
+
+int wemfwi_model_destroy(wemfwi_model* model) {
+  delete model;
+  return WEMFWI_OK;
+}
+
+int wemfwi_oneway(wemfwi_model* model, int up, int adjoint, int nrows, const float* freq_hz,
+                  const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream) {
+  if (!model) return fail(WEMFWI_ERR_ARG, "oneway: null model");
+  return model->m.oneway(up, adjoint, nrows, freq_hz, (const cf*)in, (cf*)out, add, (hipStream_t)stream);
+}
+
+int wemfwi_onestep(wemfwi_model* model, int adjoint, int nrows, const float* freq_hz, const int* depth,
+                   const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream) {
+  if (!model) return fail(WEMFWI_ERR_ARG, "onestep: null model");
+  return model->m.onestep(adjoint, nrows, freq_hz, depth, (const cf*)in, (cf*)out, add, (hipStream_t)stream);
+}
+
+}  // extern "C"

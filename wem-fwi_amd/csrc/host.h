@@ -1,0 +1,62 @@
+// Host-side model object behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "../../include/wemfwi.h"
+#include "kernels.h"
+
+namespace wemfwi {
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int refsampler_host(int nz, int nx, const std::complex<float>* slow, int nref, std::complex<float>* sref,
+                    int* refid);
+int twiddles(int n, const float2** out);
+
+// Grow-only device allocation.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes, const char* what);
+  void release();
+};
+
+struct Model {
+  wemfwi_model_desc d{};
+  int N = 0;         // FFT length nx + 2*tap
+  int nslots = 0;    // active (depth, reference) pairs
+  std::vector<std::complex<float>> slow, den, sref;
+  std::vector<int> refid, slot_begin, slot_ref;
+
+  int* d_slot_begin = nullptr;
+  int* d_slot_ref = nullptr;
+  signed char* d_refid = nullptr;
+  float2* d_phi = nullptr;
+  float* d_taper = nullptr;
+  double2* d_slot_sref = nullptr;
+  DepthTables* d_tables = nullptr;
+  const float2* d_tw = nullptr;
+
+  Workspace ptab, wlist, rows;
+  // host staging of asynchronous uploads (kept alive until the call syncs)
+  std::vector<double> h_wlist;
+  std::vector<SweepRow> h_sweep;
+  std::vector<StepRow> h_step;
+
+  ~Model();
+  int build(const wemfwi_model_desc& desc, const std::complex<float>* slow, const std::complex<float>* den);
+  StepGeom geom() const;
+  int phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow, std::vector<const float2*>& prow,
+                   hipStream_t s);
+  int oneway(int up, int adjoint, int nrows, const float* freq_hz, const std::complex<float>* in,
+             std::complex<float>* out, int add, hipStream_t s);
+  int onestep(int adjoint, int nrows, const float* freq_hz, const int* depth, const std::complex<float>* in,
+              std::complex<float>* out, int add, hipStream_t s);
+};
+
+}  // namespace wemfwi

@@ -1,0 +1,40 @@
+// Device-side descriptors shared by the kernels and the host layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sweep.hpp"
+
+namespace wemfwi {
+
+// One (shot, frequency) row of a Down/Up sweep.
+struct SweepRow {
+  const float2* in;      // [nz][nx] source terms (model of OneWay::forward/adjoint)
+  float2* out;           // [nz][nx] result; == in for the in-place recursion
+  const float2* ptab;    // [nslots][N] phase table of this row's frequency
+  float w;               // angular frequency
+  int model;             // index into the DepthTables array
+};
+
+// One independent depth step.
+struct StepRow {
+  const float2* in;      // [nx]
+  float2* out;           // [nx]
+  const float2* ptab;    // [nslots][N]
+  float w;
+  int d;                 // depth of the step
+  int model;
+  int add;               // 1: out += step(in)
+};
+
+bool fft_size_supported(int n);
+
+hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const double2* sref,
+                              int nslots, int N, double dk, double dz, double eps,
+                              hipStream_t stream);
+hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
+                         StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s);
+hipError_t launch_onestep(int N, const StepRow* rows, int nrows, const DepthTables* models,
+                          StepGeom g, bool adj, const float2* tw, hipStream_t s);
+
+}  // namespace wemfwi

@@ -1,0 +1,215 @@
+// HIP kernels of the one-way WEM hot path for gfx950, and their launchers.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "kernels.h"
+#include "sweep.hpp"
+
+namespace wemfwi {
+
+// ------------------------------------------------------------------ tables
+
+// Phase-shift multipliers exp(-i kz dz)/N per (frequency, active slot, k),
+// evaluated in double precision (phase_shift.ispc:8-14 with the k grid of
+// Phshift_cmplx.cpp:12-24).  kz = re + i im with im <= 0; Im is formed as
+// -|b|/(2 re) (or Re as |b|/(2|im|) when a < 0) -- equal to the reference's
+// -sqrt((c-a)/2) since (c-a)(c+a) = b^2, without its cancellation.
+__global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __restrict__ wlist,
+                                   int nw, const double2* __restrict__ sref, int nslots, int N,
+                                   double dk, double dz, double eps) {
+  const size_t total = (size_t)nw * nslots * N;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % N);
+    const int s = (int)((i / N) % nslots);
+    const int iw = (int)(i / ((size_t)N * nslots));
+    const int h = N / 2;
+    double kk = 0.0;
+    if (k < h) kk = k * dk;
+    else if (N - k < h) kk = -(N - k) * dk;
+    const double w = wlist[iw];
+    const double2 sr = sref[s];
+    const double a = w * w * sr.x - kk * kk;
+    const double b = w * w * (sr.y - eps * sr.x);
+    const double c = hypot(a, b);
+    const double big = sqrt(0.5 * (c + fabs(a)));
+    const double small = big > 0.0 ? fabs(b) / (2.0 * big) : 0.0;
+    const double re = a >= 0.0 ? big : small;
+    const double im = a >= 0.0 ? -small : -big;
+    const double amp = exp(im * dz) / (double)N;
+    double sn, cs;
+    sincos(re * dz, &sn, &cs);
+    ptab[i] = make_float2((float)(amp * cs), (float)(-amp * sn));
+  }
+}
+
+// ------------------------------------------------------------------ rows
+
+template <int N>
+__device__ __forceinline__ void load_row(float2 (&v)[N / 64], const float2* __restrict__ p, int nx,
+                                         int tap, int t) {
+#pragma unroll
+  for (int e = 0; e < N / 64; ++e) {
+    const int x = t + 64 * e - tap;
+    const bool ok = in_row(x, nx);
+    const float2 a = ldg(p, ok ? x : 0);
+    v[e] = ok ? a : make_float2(0.f, 0.f);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void store_row(float2* __restrict__ p, const float2 (&v)[N / 64], int nx,
+                                          int tap, int t) {
+#pragma unroll
+  for (int e = 0; e < N / 64; ++e) {
+    const int x = t + 64 * e - tap;
+    if (in_row(x, nx)) stg(p, x, v[e]);
+  }
+}
+
+constexpr int kWavesPerBlock = 4;
+
+// In-place or out-of-place Down/Up recursion (OneWay.cpp:32-109) of one
+// (shot, frequency) row per wavefront.  step = +1 runs iz = 0..nz-2 (Down
+// forward, Up adjoint), step = -1 runs iz = nz-1..1 (Down adjoint, Up forward).
+// Forward steps use depth cur, adjoint steps depth next.
+// Waves per SIMD the register allocator targets for an FFT length.
+template <int N>
+struct Occupancy {
+  static constexpr int value = N <= 256 ? 4 : (N <= 1024 ? 2 : 1);
+};
+
+template <int N, bool ADJ>
+__global__ void __launch_bounds__(64 * kWavesPerBlock)
+    __attribute__((amdgpu_waves_per_eu(Occupancy<N>::value, 8))) oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
+                  StepGeom g, int nz, int step, const float2* __restrict__ twg) {
+  constexpr int E = N / 64;
+  __shared__ float2 tw[N];
+  __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
+  const int t = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = ldg(twg, i);
+  __syncthreads();
+  const int row = blockIdx.x * kWavesPerBlock + wv;
+  if (row >= nrows) return;
+  const SweepRow R = ldg(rows, row);
+  const DepthTables dt = ldg(models, R.model);
+  float2* buf = bufs[wv];
+  const int nx = g.nx;
+  const bool inplace = (R.in == R.out);
+  const int first = step > 0 ? 0 : nz - 1;
+  float2 st[E];
+  load_row<N>(st, R.in + (size_t)first * nx, nx, g.tap, t);
+  if (!inplace) store_row<N>(R.out + (size_t)first * nx, st, nx, g.tap, t);
+  for (int k = 0; k < nz - 1; ++k) {
+    const int cur = first + k * step;
+    const int nxt = cur + step;
+    if (!inplace && k > 0) load_row<N>(st, R.in + (size_t)cur * nx, nx, g.tap, t);
+    if constexpr (ADJ) step_adjoint<N>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t);
+    else step_forward<N>(st, cur, R.w, dt, g, R.ptab, buf, tw, t);
+    float2 src[E];
+    load_row<N>(src, R.in + (size_t)nxt * nx, nx, g.tap, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
+    store_row<N>(R.out + (size_t)nxt * nx, st, nx, g.tap, t);
+  }
+}
+
+// Independent single depth steps (SSF::forward/adjoint as a unit operator).
+template <int N, bool ADJ>
+__global__ void __launch_bounds__(64 * kWavesPerBlock)
+    __attribute__((amdgpu_waves_per_eu(Occupancy<N>::value, 8))) onestep_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
+                   StepGeom g, const float2* __restrict__ twg) {
+  constexpr int E = N / 64;
+  __shared__ float2 tw[N];
+  __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
+  const int t = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = ldg(twg, i);
+  __syncthreads();
+  const int row = blockIdx.x * kWavesPerBlock + wv;
+  if (row >= nrows) return;
+  const StepRow R = ldg(rows, row);
+  const DepthTables dt = ldg(models, R.model);
+  float2 st[E];
+  load_row<N>(st, R.in, g.nx, g.tap, t);
+  if constexpr (ADJ) step_adjoint<N>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
+  else step_forward<N>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
+  if (R.add) {
+    float2 o[E];
+    load_row<N>(o, R.out, g.nx, g.tap, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = cadd(st[e], o[e]);
+  }
+  store_row<N>(R.out, st, g.nx, g.tap, t);
+}
+
+// ------------------------------------------------------------------ launchers
+
+bool fft_size_supported(int n) {
+  return n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
+}
+
+hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const double2* sref,
+                              int nslots, int N, double dk, double dz, double eps,
+                              hipStream_t stream) {
+  const size_t total = (size_t)nw * nslots * N;
+  if (total == 0) return hipSuccess;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(phase_table_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, ptab, wlist, nw,
+                     sref, nslots, N, dk, dz, eps);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTables* models,
+                                  StepGeom g, int nz, int step, bool adj, const float2* tw,
+                                  hipStream_t s) {
+  dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+  if (adj)
+    hipLaunchKernelGGL((oneway_kernel<N, true>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
+  else
+    hipLaunchKernelGGL((oneway_kernel<N, false>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
+                         StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s) {
+  if (nrows <= 0 || nz < 2) return hipSuccess;
+  switch (N) {
+    case 128: return launch_oneway_n<128>(rows, nrows, models, g, nz, step, adj, tw, s);
+    case 256: return launch_oneway_n<256>(rows, nrows, models, g, nz, step, adj, tw, s);
+    case 512: return launch_oneway_n<512>(rows, nrows, models, g, nz, step, adj, tw, s);
+    case 1024: return launch_oneway_n<1024>(rows, nrows, models, g, nz, step, adj, tw, s);
+    case 2048: return launch_oneway_n<2048>(rows, nrows, models, g, nz, step, adj, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N>
+static hipError_t launch_onestep_n(const StepRow* rows, int nrows, const DepthTables* models,
+                                   StepGeom g, bool adj, const float2* tw, hipStream_t s) {
+  dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+  if (adj)
+    hipLaunchKernelGGL((onestep_kernel<N, true>), grid, block, 0, s, rows, nrows, models, g, tw);
+  else
+    hipLaunchKernelGGL((onestep_kernel<N, false>), grid, block, 0, s, rows, nrows, models, g, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_onestep(int N, const StepRow* rows, int nrows, const DepthTables* models,
+                          StepGeom g, bool adj, const float2* tw, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  switch (N) {
+    case 128: return launch_onestep_n<128>(rows, nrows, models, g, adj, tw, s);
+    case 256: return launch_onestep_n<256>(rows, nrows, models, g, adj, tw, s);
+    case 512: return launch_onestep_n<512>(rows, nrows, models, g, adj, tw, s);
+    case 1024: return launch_onestep_n<1024>(rows, nrows, models, g, adj, tw, s);
+    case 2048: return launch_onestep_n<2048>(rows, nrows, models, g, adj, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace wemfwi

@@ -1,0 +1,10 @@
+"""wemfwi -- MI355X-native one-way wave-equation propagator (HIP, gfx950).
+
+The hot path (SSF/PSPI depth stepping, Born operators, imaging condition) runs
+in libwemfwi.so; this package is the Python side of its C ABI.
+"""
+
+from ._lib import LIB_PATH, WemfwiError, lib  # noqa: F401
+from .core import Model, refsampler  # noqa: F401
+
+__all__ = ["Model", "refsampler", "WemfwiError", "LIB_PATH"]

@@ -1,0 +1,79 @@
+"""ctypes binding of libwemfwi.so (the C ABI of include/wemfwi.h).
+
+The library is built in-tree (wem-fwi_amd/build.py).  Importing this module
+never falls back to another implementation: a missing or unloadable library
+raises ImportError, and every failing call raises WemfwiError.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libwemfwi.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "wemfwi.h")
+
+OK = 0
+ERR_ARG = 1
+ERR_HIP = 2
+ERR_UNSUPPORTED = 3
+ERR_REFERENCE = 4
+ERR_ALLOC = 5
+
+
+class WemfwiError(RuntimeError):
+    """Error returned through the C ABI (message from wemfwi_last_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"wemfwi error {code}: {msg}")
+        self.code = code
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("nz", C.c_int), ("nx", C.c_int), ("dx", C.c_float), ("dz", C.c_float),
+        ("tap", C.c_int), ("taper", C.c_int), ("nref", C.c_int), ("pspi", C.c_int),
+        ("eps", C.c_float), ("ntaylor", C.c_int),
+    ]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); every symbol declared in include/wemfwi.h
+SIGNATURES = {
+    "wemfwi_last_error": (C.c_char_p, []),
+    "wemfwi_version": (_i, []),
+    "wemfwi_device_count": (_i, []),
+    "wemfwi_refsampler": (_i, [_i, _i, _vp, _i, _vp, _ip]),
+    "wemfwi_model_create": (_i, [C.POINTER(ModelDesc), _vp, _vp, C.POINTER(_vp)]),
+    "wemfwi_model_destroy": (_i, [_vp]),
+    "wemfwi_oneway": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _i, _vp]),
+    "wemfwi_onestep": (_i, [_vp, _i, _i, _fp, _ip, _vp, _vp, _i, _vp]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libwemfwi.so not built at {LIB_PATH} (run wem-fwi_amd/build.py)")
+    try:
+        lib = C.CDLL(LIB_PATH)
+    except OSError as e:  # pragma: no cover - environment specific
+        raise ImportError(f"cannot load {LIB_PATH}: {e}") from e
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        msg = lib.wemfwi_last_error()
+        raise WemfwiError(rc, msg.decode() if msg else "")

@@ -1,0 +1,115 @@
+"""Device-level wrappers over the C ABI: background models and the batched
+Down/Up and single-step entry points on torch tensors resident on the GPU.
+
+torch only provides device memory and the stream here; every operation is a
+call into libwemfwi.so.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from ._lib import ModelDesc, check, lib
+
+
+def _stream_ptr(stream: Optional[torch.cuda.Stream]) -> C.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _c64_host(a) -> np.ndarray:
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex64))
+
+
+def _dev_c64(t: torch.Tensor, name: str) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.complex64 or not t.is_cuda:
+        raise TypeError(f"{name} must be a complex64 CUDA tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+def refsampler(slow, nref: int):
+    """RefSampler through the C ABI (host): returns (sref[nz][nref] complex64,
+    refid[nz][nx] int32)."""
+    s = _c64_host(slow)
+    nz, nx = s.shape
+    sref = np.zeros((nz, nref), dtype=np.complex64)
+    refid = np.zeros((nz, nx), dtype=np.int32)
+    check(lib.wemfwi_refsampler(nz, nx, s.ctypes.data, nref, sref.ctypes.data,
+                                refid.ctypes.data_as(C.POINTER(C.c_int))))
+    return sref, refid
+
+
+class Model:
+    """Background slowness model of the one-way operators on the propagation
+    grid (the state OneWay/OneStep/RefSampler derive from 'slow' and 'par')."""
+
+    def __init__(self, slow, dx: float, dz: float, *, tap: int = 0, pad: int = 0,
+                 nref: int = 1, prop: str = "ssf", eps: float = 0.04, ntaylor: int = 1,
+                 den=None):
+        s = _c64_host(slow)
+        nz, nx = s.shape
+        d = None if den is None else _c64_host(den)
+        self.desc = ModelDesc(nz=nz, nx=nx, dx=dx, dz=dz, tap=tap, taper=pad, nref=nref,
+                              pspi=1 if prop == "pspi" else 0, eps=eps, ntaylor=ntaylor)
+        h = C.c_void_p()
+        check(lib.wemfwi_model_create(C.byref(self.desc), s.ctypes.data,
+                                      None if d is None else d.ctypes.data, C.byref(h)))
+        self._h = h
+        self.nz, self.nx = nz, nx
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.wemfwi_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -------------------------------------------------------------- operators
+
+    def oneway(self, up: bool, adjoint: bool, freq_hz: Sequence[float], inp: torch.Tensor,
+               out: Optional[torch.Tensor] = None, add: bool = True,
+               stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+        """Down (up=False) / Up (up=True) over a batch [nrows][nz][nx]; out=None
+        or out is inp runs the in-place recursion."""
+        _dev_c64(inp, "inp")
+        if out is None:
+            out = inp
+        _dev_c64(out, "out")
+        nrows = inp.shape[0] if inp.dim() == 3 else 1
+        if tuple(inp.shape[-2:]) != (self.nz, self.nx) or out.shape != inp.shape:
+            raise ValueError("oneway: wavefield shape does not match the model")
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(freq_hz, dtype=np.float32), (nrows,)))
+        check(lib.wemfwi_oneway(self._h, int(up), int(adjoint), nrows,
+                                f.ctypes.data_as(C.POINTER(C.c_float)),
+                                C.c_void_p(inp.data_ptr()), C.c_void_p(out.data_ptr()),
+                                int(bool(add)), _stream_ptr(stream)))
+        return out
+
+    def onestep(self, adjoint: bool, freq_hz, depth, inp: torch.Tensor, out: torch.Tensor,
+                add: bool = False, stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+        """SSF/PSPI step on rows [nrows][nx] at depth[i], freq_hz[i]."""
+        _dev_c64(inp, "inp")
+        _dev_c64(out, "out")
+        nrows = inp.shape[0] if inp.dim() == 2 else 1
+        if inp.shape[-1] != self.nx or out.shape != inp.shape:
+            raise ValueError("onestep: row shape does not match the model")
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(freq_hz, dtype=np.float32), (nrows,)))
+        dd = np.ascontiguousarray(np.broadcast_to(np.asarray(depth, dtype=np.int32), (nrows,)))
+        check(lib.wemfwi_onestep(self._h, int(adjoint), nrows,
+                                 f.ctypes.data_as(C.POINTER(C.c_float)),
+                                 dd.ctypes.data_as(C.POINTER(C.c_int)),
+                                 C.c_void_p(inp.data_ptr()), C.c_void_p(out.data_ptr()),
+                                 int(bool(add)), _stream_ptr(stream)))
+        return out
