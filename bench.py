@@ -1,0 +1,197 @@
+#!/usr/bin/env python
+"""Throughput of the fused one-way depth step on one MI355X.
+
+Workload (default): 2-D PSPI downward continuation with 4 reference
+slownesses per depth, nx = 1024, nz = 500, dx = dz = 10 m, 256 frequencies
+2-30 Hz, 16 shots at ix = 32 + 64*s, complex64; velocity = 6 seeded random flat
+layers 1500-4500 m/s plus a smooth +-10% perturbation (seed 2).  All
+frequency x shot rows go to the GPU in one launch per sweep.
+
+One step = the phase tables of the 256 frequencies (Phshift::setFreq for all
+of them) + Down forward over all 4096 rows (sources -> wavefield) + Down
+adjoint over all rows (in place), i.e. 2 sweeps of 1024 x 499 x 4096 wavefield
+updates.  The metric is wavefield updates per second (x.z.w, summed over the
+shots), the reference's unit of work.
+
+Prints one JSON line.  --cpu-sample 0 skips the CPU baseline.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "wem-fwi_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+BYTES_PER_UPDATE = 16     # read W[iz] c64 + write W[iz+1] c64 (slowness row amortised)
+
+CONFIGS = {
+    # PSPI downward continuation (the headline configuration)
+    "pspi1024": dict(nx=1024, nz=500, nw=256, fmin=2.0, fmax=30.0, ns=16, nref=4, prop="pspi",
+                     seed=2, model="layers"),
+    # split-step Fourier, single shot (the CPU-runnable configuration)
+    "ssf256": dict(nx=256, nz=200, nw=64, fmin=2.0, fmax=20.0, ns=1, nref=1, prop="ssf",
+                   seed=1, model="gradient"),
+}
+
+
+def velocity(cfg) -> np.ndarray:
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(cfg["seed"])
+    nz, nx = cfg["nz"], cfg["nx"]
+    if cfg["model"] == "layers":
+        tops = np.sort(rng.integers(1, nz - 1, size=5))
+        vel = rng.uniform(1500.0, 4500.0, size=6)
+        v = np.empty((nz, nx))
+        bounds = [0, *tops.tolist(), nz]
+        for i in range(6):
+            v[bounds[i]:bounds[i + 1]] = vel[i]
+        pert = gaussian_filter(rng.standard_normal((nz, nx)), 10.0)
+        v *= 1.0 + 0.10 * pert / np.max(np.abs(pert))
+    else:
+        z = np.arange(nz)[:, None] * 10.0
+        v = 1500.0 + 2000.0 * z / ((nz - 1) * 10.0) + np.zeros((nz, nx))
+        pert = gaussian_filter(rng.standard_normal((nz, nx)), 5.0)
+        v = v + 0.05 * v * pert / np.max(np.abs(pert))
+    return v
+
+
+def ricker_spectrum(freqs, f0=10.0, t0=0.1):
+    f = np.asarray(freqs, dtype=np.float64)
+    amp = 2.0 / math.sqrt(math.pi) * f ** 2 / f0 ** 3 * np.exp(-(f / f0) ** 2)
+    return (amp * np.exp(-2j * math.pi * f * t0)).astype(np.complex64)
+
+
+def cpu_baseline(cfg, slow, freqs, src_ix, nrows_sample):
+    """FP32 C restatement (oracle/cpu_wem.c) on a bounded sample of rows."""
+    from oracle import cpu_wem
+    from oracle import wem_oracle as O
+    nz, nx = cfg["nz"], cfg["nx"]
+    p = O.StepParams(nx=nx, dx=10.0, dz=10.0, nref=cfg["nref"], prop=cfg["prop"], eps=0.04)
+    ref = O.refsampler(slow, cfg["nref"])
+    nw = len(freqs)
+    stride = max(1, (nw * len(src_ix)) // nrows_sample)
+    rows = [(s, w) for s in range(len(src_ix)) for w in range(nw)][::stride][:nrows_sample]
+    W = np.zeros((len(rows), nz, nx), dtype=np.complex64)
+    spec = ricker_spectrum(freqs)
+    fr = np.array([freqs[w] for _, w in rows], dtype=np.float32)
+    for i, (s, w) in enumerate(rows):
+        W[i, 0, src_ix[s]] = spec[w]
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    t0 = time.perf_counter()
+    cpu_wem.oneway(W, fr, slow, ref, p, up=False, adjoint=False, nthreads=cores)
+    cpu_wem.oneway(W, fr, slow, ref, p, up=False, adjoint=True, nthreads=cores)
+    dt = time.perf_counter() - t0
+    upd = 2.0 * nx * (nz - 1) * len(rows)
+    return {"value": upd / dt / 1e9, "unit": "Gupdates/s", "cores": cores, "kind": "port",
+            "sample": f"{len(rows)} of {nw * len(src_ix)} (shot, frequency) rows, Down forward + "
+                      f"adjoint, FP32 C restatement (oracle/cpu_wem.c, OpenMP, radix-2 FFT), "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="pspi1024", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-sample", type=int, default=1024,
+                    help="rows of the CPU baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the (shot, frequency) rows of this path are "
+              "independent and would be split across GPUs with one final image reduction; "
+              "only --gpus 1 is run here.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import wemfwi
+
+    cfg = CONFIGS[args.config]
+    nx, nz, nw, ns = cfg["nx"], cfg["nz"], cfg["nw"], cfg["ns"]
+    freqs = (cfg["fmin"] + np.arange(nw) * (cfg["fmax"] - cfg["fmin"]) / (nw - 1)).astype(np.float32)
+    src_ix = [32 + 64 * s for s in range(ns)] if ns > 1 else [nx // 2]
+    v = velocity(cfg)
+    slow = (1.0 / v ** 2).astype(np.complex64)
+    model = wemfwi.Model(slow, 10.0, 10.0, nref=cfg["nref"], prop=cfg["prop"], eps=0.04)
+
+    dev = torch.device("cuda:0")
+    nrows = ns * nw
+    # rows ordered (shot, frequency); sources injected at the surface
+    f_rows = np.tile(freqs, ns)
+    S = torch.zeros((nrows, nz, nx), dtype=torch.complex64, device=dev)
+    spec = torch.from_numpy(np.tile(ricker_spectrum(freqs), ns)).to(dev)
+    ix = torch.tensor(np.repeat(src_ix, nw), device=dev)
+    S[torch.arange(nrows, device=dev), 0, ix] = spec
+    W = torch.empty_like(S)
+
+    def step():
+        model.set_freqs(freqs)
+        model.oneway_recursive(False, False, f_rows, S, W)
+        model.oneway(False, True, f_rows, W)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    wemfwi.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wemfwi.timing(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    upd_sweep = nx * (nz - 1) * nrows
+    value = 2 * upd_sweep / (ms_step / 1e3) / 1e9
+    k_ms, k_n = wemfwi.kernel_time("oneway_kernel")
+    t_ms, t_n = wemfwi.kernel_time("phase_table_kernel")
+    avg = k_ms / max(k_n, 1)
+    achieved = BYTES_PER_UPDATE * upd_sweep / (avg / 1e3) / 1e9
+    if not np.isfinite(float(W.abs().max())):
+        print("non-finite wavefield", file=sys.stderr)
+        sys.exit(1)
+
+    line = {
+        "metric": "Gupdates/s (x.z.w wavefield updates, Down forward + adjoint)",
+        "value": round(value, 3),
+        "unit": "Gupdates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded velocity model, Ricker sources)",
+        "config": {"workload": f"{cfg['prop']} downward continuation nref={cfg['nref']} "
+                               f"nx={nx} nz={nz} nw={nw} {cfg['fmin']:g}-{cfg['fmax']:g} Hz "
+                               f"ns={ns}",
+                   "rows": nrows, "updates_per_step": 2 * upd_sweep},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "oneway_kernel", "kernel_ms": round(avg, 3),
+                     "phase_table_ms": round(t_ms / max(t_n, 1), 3)},
+    }
+    if args.cpu_sample > 0:
+        line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

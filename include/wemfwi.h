@@ -73,6 +73,14 @@ int wemfwi_model_create(const wemfwi_model_desc* desc, const wemfwi_complex* slo
                         const wemfwi_complex* den, wemfwi_model** out);
 int wemfwi_model_destroy(wemfwi_model* model);
 
+/* Frequency set of the model: evaluates the phase-shift multipliers
+ * (phase_shift.ispc:8-14) of every active reference at every depth for the
+ * given frequencies, in double precision -- the per-frequency part of
+ * Phshift::setFreq (Phshift.h:24-25) hoisted out of the depth loop.
+ * wemfwi_oneway / wemfwi_onestep rebuild it themselves when a row's
+ * frequency is not in the current set. */
+int wemfwi_model_set_freqs(wemfwi_model* model, int nfreq, const float* freq_hz, void* stream);
+
 /* Down / Up (OneWay.cpp:32-109) for nrows independent (shot, frequency) rows
  * of [nz][nx] each, row i at angular frequency 2*pi*freq_hz[i].
  * up = 0: Down, 1: Up.  adjoint = 0: forward, 1: adjoint.
@@ -83,12 +91,108 @@ int wemfwi_model_destroy(wemfwi_model* model);
 int wemfwi_oneway(wemfwi_model* model, int up, int adjoint, int nrows, const float* freq_hz,
                   const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
 
+/* Recursive Down / Up with separate source and output buffers:
+ * out = (I - L)^-1 src, i.e. the in-place recursion of wemfwi_oneway applied
+ * to a copy of src, without the copy.  No single reference call corresponds;
+ * it is the in-place call of OneWay.cpp:32-109 on a vector that was set to
+ * the sources. */
+int wemfwi_oneway_recursive(wemfwi_model* model, int up, int adjoint, int nrows,
+                            const float* freq_hz, const wemfwi_complex* src, wemfwi_complex* out,
+                            void* stream);
+
 /* SSF / PSPI single depth step (SSF.cpp:8-60, PSPI.cpp:8-53) on nrows
  * independent rows of nx samples: row i at depth depth[i] and frequency
  * freq_hz[i].  add = 1 accumulates into out. */
 int wemfwi_onestep(wemfwi_model* model, int adjoint, int nrows, const float* freq_hz,
                    const int* depth, const wemfwi_complex* in, wemfwi_complex* out, int add,
                    void* stream);
+
+/* Acquisition and frequency grid of the shot x frequency operators: what
+ * PropParam (propagator/src/PropParam.h:45-85), Device (Device.h:39-63) and
+ * the Injection objects (Injection.h:19-65) derive from the wavelet, the
+ * model hypercube and the parameter object.  Every (shot, frequency) pair is
+ * one row of the batched operators. */
+typedef struct wemfwi_survey_desc {
+  int ns;                         /* shots */
+  const int* src_iz;              /* [ns] source depth index */
+  const int* src_ix;              /* [ns] source x index (model grid) */
+  int nr;                         /* receivers */
+  const int* rec_iz;              /* [nr] */
+  const int* rec_ix;              /* [nr] */
+  int nw;                         /* frequencies */
+  const float* freq_hz;           /* [nw] freq[i] (PropParam.h:76) */
+  const int* bin;                 /* [nw] data bin index[i] (PropParam.h:77) */
+  const wemfwi_complex* wavelet;  /* [nw] wavelet spectrum at bin[i] */
+  int nt;                         /* bins per trace of the data [ns][nr][nt] */
+  int ngs, ngr;                   /* cos^2 spread half-widths (par "ngs", "ngr" / "ng") */
+  int offset;                     /* device x offset on the propagation grid (par "pad") */
+} wemfwi_survey_desc;
+
+typedef struct wemfwi_survey wemfwi_survey;
+
+int wemfwi_survey_create(const wemfwi_survey_desc* desc, wemfwi_survey** out);
+int wemfwi_survey_destroy(wemfwi_survey* survey);
+
+/* WEM::forward into complex data (propagator/src/WEMnl.cpp:67-82,
+ * wem_prop.cpp:9-78).  model: background slowness and density on the
+ * propagation grid (edge padded by "pad", wem_prop.cpp:26-33).  data
+ * [ns][nr][nt] device; add = 0 zeroes data first. */
+int wemfwi_wem_forward(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* data, int add,
+                       void* stream);
+
+/* BornFull::forward / adjoint into / from complex data (BornFull.cpp:64-117,
+ * born_prop.cpp:11-235).  dslow, dden: perturbations [nz][nx] on the
+ * propagation grid (zero padded, born_prop.cpp:52-58); the adjoint sums
+ * every (shot, frequency) contribution in a fixed order in double precision.
+ * mode 0 applies Scatter::adjoint in the reference's operator order
+ * (Scatter.cpp:52-66); mode 1 applies the exact adjoint of Scatter::forward. */
+int wemfwi_born_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* dslow,
+                        const wemfwi_complex* dden, wemfwi_complex* data, int add, void* stream);
+int wemfwi_born_adjoint(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* dslow,
+                        wemfwi_complex* dden, const wemfwi_complex* data, int add, int mode,
+                        void* stream);
+
+/* extWEM / extBornFull (ext_wem_prop.cpp:8-59, ext_born_prop.cpp:8-194): one
+ * background model per frequency (models[nw]); perturbations
+ * [nw][nz][nx].  extborn_adjoint mode 0 reproduces the reference's serial
+ * shot loop, in which LinDown::adjoint clears the last depth row of the
+ * shared slice (LinOneWay.cpp:64-65): that row keeps the last shot's LinUp
+ * term only.  mode 1 is the exact adjoint. */
+int wemfwi_extwem_forward(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* data,
+                          int add, void* stream);
+int wemfwi_extborn_forward(wemfwi_model* const* models, wemfwi_survey* survey,
+                           const wemfwi_complex* dslow, const wemfwi_complex* dden,
+                           wemfwi_complex* data, int add, void* stream);
+int wemfwi_extborn_adjoint(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* dslow,
+                           wemfwi_complex* dden, const wemfwi_complex* data, int add, int mode,
+                           void* stream);
+
+/* BornRefl (ext_born_refl.cpp:8-110): frequency-extended reflectivity Born
+ * operator / migration on the unpadded grid; image [nw][nz][nx] device.  The
+ * adjoint is the zero-lag imaging condition image[w] += sum_s conj(D) U
+ * (ic.cpp:17-28). */
+int wemfwi_bornrefl_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* image,
+                            wemfwi_complex* data, int add, void* stream);
+int wemfwi_bornrefl_adjoint(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* image,
+                            const wemfwi_complex* data, int add, void* stream);
+
+/* Subsurface-offset extended imaging (no counterpart in the reference; its
+ * h = 0 slice is the zero-lag image summed over frequencies):
+ * I[h][z][x] = sum_{w,s} conj(D(z, x-h)) U(z, x+h), h = -(nh/2)..nh/2, nh odd;
+ * image [nh][nz][nx].  The forward is its adjoint with respect to U. */
+int wemfwi_offset_image_forward(wemfwi_model* model, wemfwi_survey* survey, int nh,
+                                const wemfwi_complex* image, wemfwi_complex* data, int add,
+                                void* stream);
+int wemfwi_offset_image_adjoint(wemfwi_model* model, wemfwi_survey* survey, int nh,
+                                wemfwi_complex* image, const wemfwi_complex* data, int add,
+                                void* stream);
+
+/* Kernel timing with HIP events on the launch stream (off by default).
+ * wemfwi_timing(1) clears and enables; wemfwi_timing_get returns the summed
+ * duration (ms) and launch count of a kernel by name ("oneway_kernel",
+ * "phase_table_kernel", ...). */
+int wemfwi_timing(int enable);
+int wemfwi_timing_get(const char* kernel, double* total_ms, long* count);
 
 #ifdef __cplusplus
 }

@@ -124,3 +124,19 @@ def test_oneway_pspi_reference_compat_partition_deep():
     for i in range(2):
         want = O.down_fwd(W[i].astype(complex), 2 * np.pi * float(freqs[i]), s, ref, p)
         assert rel(got[i], want) < TOL
+
+
+@pytest.mark.parametrize("up,adjoint", [(False, False), (True, False), (False, True), (True, True)])
+def test_oneway_recursive_equals_inplace_on_copy(up, adjoint):
+    nx, nz = 512, 24
+    torch, m, s, p, ref = _setup(nx, 0, 0, 3, "pspi", nz, seed=7)
+    rng = np.random.default_rng(8)
+    W = (rng.standard_normal((4, nz, nx)) + 1j * rng.standard_normal((4, nz, nx))).astype(np.complex64)
+    freqs = np.array([2.0, 7.0, 15.0, 29.0], dtype=np.float32)
+    src = torch.from_numpy(W).cuda()
+    out = torch.zeros_like(src)
+    m.oneway_recursive(up, adjoint, freqs, src, out)
+    inpl = src.clone()
+    m.oneway(up, adjoint, freqs, inpl)
+    assert torch.equal(out, inpl)
+    assert torch.equal(src, torch.from_numpy(W).cuda())   # sources untouched

@@ -1,0 +1,62 @@
+// Descriptors and launchers of the Born / imaging kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace wemfwi {
+
+// One (shot, frequency) row of a LinDown / LinUp application.
+struct LinRow {
+  const float2* bg;      // [nz][nx] background wavefield
+  const float2* m;       // forward: perturbation [nz][nx]; adjoint: wavefield [nz][nx]
+  float2* out;           // forward: wavefield (+=); adjoint: model contribution (+=)
+  const float2* ptab;    // phase table of the row's frequency
+  float w;
+  int model;
+};
+
+// Injection of many devices as a gather per touched cell (CSR by cell).
+struct InjPlan {
+  int ncell;
+  const long long* cell;  // [ncell] offset z*nx + x
+  const int* ptr;         // [ncell+1]
+  const int* dev;         // [nc] device of each contribution
+  const float* wgt;       // [nc] cos^2 weight
+};
+
+// Recording of many devices (CSR by device).
+struct RecPlan {
+  int ndev;
+  const int* ptr;         // [ndev+1]
+  const long long* cell;  // [nc] offset z*nx + x of each contribution
+  const float* wgt;       // [nc]
+};
+
+hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                      int dir, bool adj, bool exact, const float2* tw, hipStream_t s);
+hipError_t launch_scatter(int N, const StepRow* rows, int nrows, const DepthTables* models, StepGeom g, bool adj,
+                          bool exact, const float2* tw, hipStream_t s);
+hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, const float2* B, long long sb,
+                      long long n, int nrows, bool conjA, bool acc, hipStream_t s);
+hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, bool acc,
+                              hipStream_t s);
+hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s);
+hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hipStream_t s);
+hipError_t launch_inject_point(float2* W, long long wstride, int nx, const int* iz, const int* ix,
+                               const float2* val, const float* gw, int ng, int tap, int nrows, hipStream_t s);
+hipError_t launch_inject_csr(float2* W, long long wstride, InjPlan p, const float2* val, const long long* vbase,
+                             long long vdev, int nrows, hipStream_t s);
+hipError_t launch_record(float2* val, const long long* vbase, long long vdev, const float2* W, long long wstride,
+                         RecPlan p, int nrows, hipStream_t s);
+hipError_t launch_dreflect_fwd(float2* dr, const float2* coef, const float2* dS, const float2* dR, int nz, int nx,
+                               bool acc, hipStream_t s);
+hipError_t launch_dreflect_adj(float2* gS, float2* gR, const float2* coef, const float2* dr, int nz, int nx,
+                               hipStream_t s);
+hipError_t launch_offset_ic_adj(double2* img, const float2* const* bg, const float2* const* sc, int nrows, int nh,
+                                int nz, int nx, hipStream_t s);
+hipError_t launch_offset_ic_fwd(float2* src, const float2* bg, const float2* img, int nh, int nz, int nx,
+                                hipStream_t s);
+
+}  // namespace wemfwi

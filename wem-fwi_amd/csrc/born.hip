@@ -1,0 +1,449 @@
+// HIP kernels of the linearised (Born) operators, imaging conditions,
+// injection/recording and the deterministic frequency/shot reduction.
+#include <hip/hip_runtime.h>
+
+#include "born.h"
+#include "sweep.hpp"
+
+namespace wemfwi {
+
+constexpr int kLinWaves = 4;
+
+template <int N>
+__device__ __forceinline__ void lrow(float2 (&v)[N / 64], const float2* __restrict__ p, const StepGeom& g,
+                                     int t) {
+  const float2* b = p + (t - g.tap);
+#pragma unroll
+  for (int e = 0; e < N / 64; ++e) {
+    v[e] = make_float2(0.f, 0.f);
+    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) v[e] = ldg(b, (unsigned)(64 * e));
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void acc_row(float2* __restrict__ p, const float2 (&v)[N / 64], const StepGeom& g,
+                                        int t) {
+  float2* b = p + (t - g.tap);
+#pragma unroll
+  for (int e = 0; e < N / 64; ++e)
+    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) {
+      const float2 o = ldg(b, (unsigned)(64 * e));
+      stg(b, (unsigned)(64 * e), cadd(o, v[e]));
+    }
+}
+
+// LinDown / LinUp (LinOneWay.cpp:33-142) as independent (row, depth) items.
+// dir = +1 (LinDown): depths d = 0..nz-2, neighbour d+1; dir = -1 (LinUp):
+// d = 1..nz-1, neighbour d-1.
+//   forward: out[d+dir] += P_d Sc_d (bg[d] * m[d])
+//   adjoint: out[d]     += conj(bg[d]) * Sc^H_d P^H_d (in[d+dir])
+template <int N, bool ADJ, bool EXACT>
+__global__ void __launch_bounds__(64 * kLinWaves)
+    lin_kernel(const LinRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models, StepGeom g,
+               int nz, int dir, const float2* __restrict__ twg) {
+  constexpr int E = N / 64;
+  __shared__ float2 tw[plan_tw_size<N>()];
+  __shared__ float2 bufs[kLinWaves][LdsSize<N>::value];
+  const int t = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
+  __syncthreads();
+  const int item = blockIdx.x * kLinWaves + wv;
+  const int nd = nz - 1;
+  if (item >= nrows * nd) return;
+  const int row = item / nd;
+  const int j = item - row * nd;
+  const int d = dir > 0 ? j : j + 1;
+  const LinRow R = rows[row];
+  const DepthTables dt = models[R.model];
+  float2* buf = bufs[wv];
+  const size_t nx = g.nx;
+  float2 st[E];
+  if constexpr (!ADJ) {
+    float2 b[E];
+    lrow<N>(st, R.m + d * nx, g, t);
+    lrow<N>(b, R.bg + d * nx, g, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = cmul(b[e], st[e]);
+    scatter_row<N, false, false>(st, d, R.w, dt, g, buf, tw, t);
+    step_forward<N>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
+    acc_row<N>(R.out + (d + dir) * nx, st, g, t);
+  } else {
+    lrow<N>(st, R.m + (d + dir) * nx, g, t);
+    step_adjoint<N>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
+    scatter_row<N, true, EXACT>(st, d, R.w, dt, g, buf, tw, t);
+    float2 b[E];
+    lrow<N>(b, R.bg + d * nx, g, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], b[e]);
+    acc_row<N>(R.out + d * nx, st, g, t);
+  }
+}
+
+// Scatter as a unit operator on independent rows: out (+)= Sc_d(in).
+template <int N, bool ADJ, bool EXACT>
+__global__ void __launch_bounds__(64 * kLinWaves)
+    scatter_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
+                   StepGeom g, const float2* __restrict__ twg) {
+  constexpr int E = N / 64;
+  __shared__ float2 tw[plan_tw_size<N>()];
+  __shared__ float2 bufs[kLinWaves][LdsSize<N>::value];
+  const int t = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
+  __syncthreads();
+  const int row = blockIdx.x * kLinWaves + wv;
+  if (row >= nrows) return;
+  const StepRow R = rows[row];
+  const DepthTables dt = models[R.model];
+  float2 st[E];
+  lrow<N>(st, R.in, g, t);
+  scatter_row<N, ADJ, EXACT>(st, R.d, R.w, dt, g, bufs[wv], tw, t);
+  if (R.add) {
+    acc_row<N>(R.out, st, g, t);
+  } else {
+    float2* b = R.out + (t - g.tap);
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (g.full || in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), st[e]);
+  }
+}
+
+// out[r so + i] (= or +=) op(A[r sa + i]) * B[r sb + i]; op = identity or conj
+__global__ void mul_kernel(float2* __restrict__ out, long long so, const float2* __restrict__ A, long long sa,
+                           const float2* __restrict__ B, long long sb, long long n, int nrows, int conjA,
+                           int acc) {
+  const long long total = n * nrows;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long r = i / n, k = i - r * n;
+    const float2 a = ldg(A, (size_t)(r * sa + k));
+    const float2 b = ldg(B, (size_t)(r * sb + k));
+    float2 v = conjA ? cmulc(b, a) : cmul(a, b);
+    float2* o = out + r * so + k;
+    if (acc) v = cadd(*o, v);
+    *o = v;
+  }
+}
+
+// out[i] (+)= sum_r in[r stride + i] in double precision, rows in order
+__global__ void reduce_rows_kernel(double2* __restrict__ out, const float2* __restrict__ in, long long stride,
+                                   long long n, int nrows, int acc) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double sx = 0.0, sy = 0.0;
+    if (acc) {
+      sx = out[i].x;
+      sy = out[i].y;
+    }
+    for (int r = 0; r < nrows; ++r) {
+      const float2 v = ldg(in, (size_t)(r * stride + i));
+      sx += v.x;
+      sy += v.y;
+    }
+    out[i] = make_double2(sx, sy);
+  }
+}
+
+// out += in
+__global__ void add_kernel(float2* __restrict__ out, const float2* __restrict__ in, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = cadd(out[i], in[i]);
+}
+
+// out (= or +=) (float2) in
+__global__ void d2f_kernel(float2* __restrict__ out, const double2* __restrict__ in, long long n, int acc) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double x = in[i].x, y = in[i].y;
+    if (acc) {
+      x += out[i].x;
+      y += out[i].y;
+    }
+    out[i] = make_float2((float)x, (float)y);
+  }
+}
+
+// Point source of each row (Injection.cpp:8-19): W[row][iz][ix+ig+tap] +=
+// g[ig] * val[row], ig = -ng..ng, skipping cells outside [0,nx).
+__global__ void inject_point_kernel(float2* __restrict__ W, long long wstride, int nx, const int* __restrict__ iz,
+                                    const int* __restrict__ ix, const float2* __restrict__ val,
+                                    const float* __restrict__ gw, int ng, int tap, int nrows) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int span = 2 * ng + 1;
+  if (i >= nrows * span) return;
+  const int row = i / span, ig = i - row * span - ng;
+  const int off = ix[row] + ig + tap;
+  if (off < 0 || off >= nx) return;
+  float2* p = W + row * wstride + (long long)iz[row] * nx + off;
+  *p = cadd(*p, cscale(val[row], gw[ig + ng]));
+}
+
+// Receiver injection (Injection.cpp:56-67) as a gather over cells:
+// W[row][cell] += sum_c w_c * val[vbase[row] + dev_c * vdev].
+__global__ void inject_csr_kernel(float2* __restrict__ W, long long wstride, InjPlan p,
+                                  const float2* __restrict__ val, const long long* __restrict__ vbase,
+                                  long long vdev, int nrows) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nrows * p.ncell) return;
+  const int row = (int)(i / p.ncell), cell = (int)(i - (long long)row * p.ncell);
+  float sx = 0.f, sy = 0.f;
+  for (int c = p.ptr[cell]; c < p.ptr[cell + 1]; ++c) {
+    const float2 v = val[vbase[row] + (long long)p.dev[c] * vdev];
+    sx += p.wgt[c] * v.x;
+    sy += p.wgt[c] * v.y;
+  }
+  float2* q = W + row * wstride + p.cell[cell];
+  *q = make_float2(q->x + sx, q->y + sy);
+}
+
+// Receiver recording (Injection.cpp:69-80): val[vbase[row] + i*vdev] +=
+// sum_ig g[ig] W[row][iz_i][ix_i+ig+tap].
+__global__ void record_kernel(float2* __restrict__ val, const long long* __restrict__ vbase, long long vdev,
+                              const float2* __restrict__ W, long long wstride, RecPlan p, int nrows) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nrows * p.ndev) return;
+  const int row = (int)(i / p.ndev), dev = (int)(i - (long long)row * p.ndev);
+  float sx = 0.f, sy = 0.f;
+  for (int c = p.ptr[dev]; c < p.ptr[dev + 1]; ++c) {
+    const float2 v = ldg(W, (size_t)(row * wstride + p.cell[c]));
+    sx += p.wgt[c] * v.x;
+    sy += p.wgt[c] * v.y;
+  }
+  float2* q = val + vbase[row] + dev * vdev;
+  *q = make_float2(q->x + sx, q->y + sy);
+}
+
+// dReflect::forward (dReflect.h:29-56) with precomputed coefficients
+// a1 = c s1s2sq, b1 = s1s2sq / c, a2 = rho[iz-1] r1r2sq, b2 = rho[iz] r1r2sq:
+// dr[iz] = a1 dS[iz-1] - b1 dS[iz] + a2 dR[iz] - b2 dR[iz-1], iz = 1..nz-2.
+__global__ void dreflect_fwd_kernel(float2* __restrict__ dr, const float2* __restrict__ coef,
+                                    const float2* __restrict__ dS, const float2* __restrict__ dR, int nz, int nx,
+                                    int acc) {
+  const long long n = (long long)nz * nx;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int iz = (int)(i / nx);
+    float2 v = make_float2(0.f, 0.f);
+    if (iz >= 1 && iz <= nz - 2) {
+      const float2 a1 = coef[i], b1 = coef[n + i], a2 = coef[2 * n + i], b2 = coef[3 * n + i];
+      v = csub(cmul(a1, dS[i - nx]), cmul(b1, dS[i]));
+      v = cadd(v, csub(cmul(a2, dR[i]), cmul(b2, dR[i - nx])));
+    }
+    if (acc) v = cadd(dr[i], v);
+    dr[i] = v;
+  }
+}
+
+// dReflect::adjoint (dReflect.h:58-84) as a gather: gS[z] += conj(a1[z+1])
+// dr[z+1] - conj(b1[z]) dr[z]; gR[z] += conj(a2[z]) dr[z] - conj(b2[z+1]) dr[z+1]
+// (terms only where the row index lies in 1..nz-2).
+__global__ void dreflect_adj_kernel(float2* __restrict__ gS, float2* __restrict__ gR,
+                                    const float2* __restrict__ coef, const float2* __restrict__ dr, int nz,
+                                    int nx) {
+  const long long n = (long long)nz * nx;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(i / nx);
+    float2 s = gS[i], r = gR[i];
+    if (z + 1 >= 1 && z + 1 <= nz - 2) {
+      const long long j = i + nx;
+      s = cadd(s, cmulc(dr[j], coef[j]));
+      r = csub(r, cmulc(dr[j], coef[3 * n + j]));
+    }
+    if (z >= 1 && z <= nz - 2) {
+      s = csub(s, cmulc(dr[i], coef[n + i]));
+      r = cadd(r, cmulc(dr[i], coef[2 * n + i]));
+    }
+    gS[i] = s;
+    gR[i] = r;
+  }
+}
+
+// Subsurface-offset imaging condition (no reference counterpart; h = 0 is
+// IC::adjoint, ic.cpp:17-28):
+//   adjoint: I[h][z][x] += sum_rows conj(bg[z][x-h]) sc[z][x+h]
+//   forward: src[z][x+h] += bg[z][x-h] I[h][z][x]
+// h = -(nh/2)..nh/2; terms with x-h or x+h outside [0,nx) vanish.
+// The adjoint sums rows in order per (h, z, x) in double precision.
+__global__ void offset_ic_adj_kernel(double2* __restrict__ img, const float2* const* __restrict__ bg,
+                                     const float2* const* __restrict__ sc, int nrows, int nh, int nz, int nx) {
+  const long long n = (long long)nh * nz * nx;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nx);
+    const int z = (int)((i / nx) % nz);
+    const int h = (int)(i / ((long long)nz * nx)) - nh / 2;
+    if (x - h < 0 || x - h >= nx || x + h < 0 || x + h >= nx) continue;
+    double sx = img[i].x, sy = img[i].y;
+    for (int r = 0; r < nrows; ++r) {
+      const float2 b = bg[r][(long long)z * nx + x - h];
+      const float2 s = sc[r][(long long)z * nx + x + h];
+      const float2 v = cmulc(s, b);
+      sx += v.x;
+      sy += v.y;
+    }
+    img[i] = make_double2(sx, sy);
+  }
+}
+
+__global__ void offset_ic_fwd_kernel(float2* __restrict__ src, const float2* __restrict__ bg,
+                                     const float2* __restrict__ img, int nh, int nz, int nx) {
+  const long long n = (long long)nz * nx;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int xo = (int)(i % nx);   // output position x + h
+    const long long zrow = i - xo;
+    float2 s = src[i];
+    for (int j = 0; j < nh; ++j) {
+      const int h = j - nh / 2;
+      const int x = xo - h;
+      if (x - h < 0 || x - h >= nx || x < 0 || x >= nx) continue;
+      s = cadd(s, cmul(bg[zrow + x - h], img[(long long)j * n + zrow + x]));
+    }
+    src[i] = s;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+static dim3 grid1d(long long n, int block = 256) {
+  long long b = (n + block - 1) / block;
+  if (b > 1048576) b = 1048576;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+template <int N>
+static hipError_t launch_lin_n(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                               int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
+  const long long items = (long long)nrows * (nz - 1);
+  dim3 grid((unsigned)((items + kLinWaves - 1) / kLinWaves)), block(64 * kLinWaves);
+  if (!adj)
+    hipLaunchKernelGGL((lin_kernel<N, false, false>), grid, block, 0, s, rows, nrows, models, g, nz, dir, tw);
+  else if (exact)
+    hipLaunchKernelGGL((lin_kernel<N, true, true>), grid, block, 0, s, rows, nrows, models, g, nz, dir, tw);
+  else
+    hipLaunchKernelGGL((lin_kernel<N, true, false>), grid, block, 0, s, rows, nrows, models, g, nz, dir, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                      int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
+  if (nrows <= 0 || nz < 2) return hipSuccess;
+  switch (N) {
+    case 128: return launch_lin_n<128>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
+    case 256: return launch_lin_n<256>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
+    case 512: return launch_lin_n<512>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
+    case 1024: return launch_lin_n<1024>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
+    case 2048: return launch_lin_n<2048>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N>
+static hipError_t launch_scatter_n(const StepRow* rows, int nrows, const DepthTables* models, StepGeom g,
+                                   bool adj, bool exact, const float2* tw, hipStream_t s) {
+  dim3 grid((nrows + kLinWaves - 1) / kLinWaves), block(64 * kLinWaves);
+  if (!adj)
+    hipLaunchKernelGGL((scatter_kernel<N, false, false>), grid, block, 0, s, rows, nrows, models, g, tw);
+  else if (exact)
+    hipLaunchKernelGGL((scatter_kernel<N, true, true>), grid, block, 0, s, rows, nrows, models, g, tw);
+  else
+    hipLaunchKernelGGL((scatter_kernel<N, true, false>), grid, block, 0, s, rows, nrows, models, g, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(int N, const StepRow* rows, int nrows, const DepthTables* models, StepGeom g, bool adj,
+                          bool exact, const float2* tw, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  switch (N) {
+    case 128: return launch_scatter_n<128>(rows, nrows, models, g, adj, exact, tw, s);
+    case 256: return launch_scatter_n<256>(rows, nrows, models, g, adj, exact, tw, s);
+    case 512: return launch_scatter_n<512>(rows, nrows, models, g, adj, exact, tw, s);
+    case 1024: return launch_scatter_n<1024>(rows, nrows, models, g, adj, exact, tw, s);
+    case 2048: return launch_scatter_n<2048>(rows, nrows, models, g, adj, exact, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, const float2* B, long long sb,
+                      long long n, int nrows, bool conjA, bool acc, hipStream_t s) {
+  if (n <= 0 || nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mul_kernel, grid1d(n * nrows), dim3(256), 0, s, out, so, A, sa, B, sb, n, nrows,
+                     conjA ? 1 : 0, acc ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, bool acc,
+                              hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_rows_kernel, grid1d(n), dim3(256), 0, s, out, in, stride, n, nrows, acc ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(add_kernel, grid1d(n), dim3(256), 0, s, out, in, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(d2f_kernel, grid1d(n), dim3(256), 0, s, out, in, n, acc ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_inject_point(float2* W, long long wstride, int nx, const int* iz, const int* ix,
+                               const float2* val, const float* gw, int ng, int tap, int nrows, hipStream_t s) {
+  const long long n = (long long)nrows * (2 * ng + 1);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(inject_point_kernel, grid1d(n), dim3(256), 0, s, W, wstride, nx, iz, ix, val, gw, ng, tap,
+                     nrows);
+  return hipGetLastError();
+}
+
+hipError_t launch_inject_csr(float2* W, long long wstride, InjPlan p, const float2* val, const long long* vbase,
+                             long long vdev, int nrows, hipStream_t s) {
+  const long long n = (long long)nrows * p.ncell;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(inject_csr_kernel, grid1d(n), dim3(256), 0, s, W, wstride, p, val, vbase, vdev, nrows);
+  return hipGetLastError();
+}
+
+hipError_t launch_record(float2* val, const long long* vbase, long long vdev, const float2* W, long long wstride,
+                         RecPlan p, int nrows, hipStream_t s) {
+  const long long n = (long long)nrows * p.ndev;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(record_kernel, grid1d(n), dim3(256), 0, s, val, vbase, vdev, W, wstride, p, nrows);
+  return hipGetLastError();
+}
+
+hipError_t launch_dreflect_fwd(float2* dr, const float2* coef, const float2* dS, const float2* dR, int nz, int nx,
+                               bool acc, hipStream_t s) {
+  hipLaunchKernelGGL(dreflect_fwd_kernel, grid1d((long long)nz * nx), dim3(256), 0, s, dr, coef, dS, dR, nz, nx,
+                     acc ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_dreflect_adj(float2* gS, float2* gR, const float2* coef, const float2* dr, int nz, int nx,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(dreflect_adj_kernel, grid1d((long long)nz * nx), dim3(256), 0, s, gS, gR, coef, dr, nz, nx);
+  return hipGetLastError();
+}
+
+hipError_t launch_offset_ic_adj(double2* img, const float2* const* bg, const float2* const* sc, int nrows, int nh,
+                                int nz, int nx, hipStream_t s) {
+  hipLaunchKernelGGL(offset_ic_adj_kernel, grid1d((long long)nh * nz * nx), dim3(256), 0, s, img, bg, sc, nrows,
+                     nh, nz, nx);
+  return hipGetLastError();
+}
+
+hipError_t launch_offset_ic_fwd(float2* src, const float2* bg, const float2* img, int nh, int nz, int nx,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(offset_ic_fwd_kernel, grid1d((long long)nz * nx), dim3(256), 0, s, src, bg, img, nh, nz, nx);
+  return hipGetLastError();
+}
+
+}  // namespace wemfwi

@@ -6,13 +6,16 @@
 // (Govindaraju et al., "High performance discrete Fourier transforms on
 // graphics processors", SC 2008): pass p with sub-transform length NS reads
 // butterfly j's inputs at j + r*N/R and writes its outputs at
-// (j/NS)*NS*R + j%NS + r*NS.  When R divides E the first pass reads and the
-// last pass writes the strided natural layout directly, so only the interior
-// passes exchange data through LDS.  LDS addresses are padded by one complex
-// every 16 to keep the scattered first-pass writes bank-conflict free.
+// (j/NS)*NS*R + j%NS + r*NS.  Every radix divides E, so the first pass reads
+// and the last pass writes the strided natural layout directly and only the
+// interior passes exchange data through LDS.  LDS addresses are padded by one
+// complex every 16 to keep the scattered first-pass writes bank-conflict
+// free.  All LDS addresses of a pass are one per-lane base register plus
+// compile-time offsets (ds_read/ds_write immediate offsets).
 //
-// Twiddles come from a per-workgroup LDS table tw[m] = exp(-2*pi*i*m/N)
-// computed in double precision on the host.
+// Twiddles: a per-pass table in LDS, r-major, entry (r-1)*NS + j%NS =
+// exp(-2*pi*i*(j%NS)*r/(NS*R)), written once per launch from a
+// double-precision table.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,7 +47,7 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
 }
 
 // padded LDS index (one pad slot every 16 complex)
-__device__ __forceinline__ int lidx(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lidx(int i) { return i + (i >> 4); }
 
 template <int N>
 struct LdsSize {
@@ -83,7 +86,6 @@ __device__ __forceinline__ float2 tw16(float2 a) {
   else if constexpr (m == 8) return make_float2(-a.x, -a.y);
   else if constexpr (m == 12) return mul_mi<!INV>(a);
   else {
-    // cos/sin of 2*pi*m/16 from the first-octant constants
     constexpr float c[16] = {1.f, kCos8, kSqrtHalf, kSin8, 0.f, -kSin8, -kSqrtHalf, -kCos8,
                              -1.f, -kCos8, -kSqrtHalf, -kSin8, 0.f, kSin8, kSqrtHalf, kCos8};
     constexpr float s[16] = {0.f, kSin8, kSqrtHalf, kCos8, 1.f, kCos8, kSqrtHalf, kSin8,
@@ -118,7 +120,6 @@ __device__ __forceinline__ void dft16(float2 (&v)[16]) {
   // y[q1 + 4 q2] = sum_n2 W16^(n2 q1) W4^(n2 q2) sum_n1 v[4 n1 + n2] W4^(n1 q1)
 #pragma unroll
   for (int n2 = 0; n2 < 4; ++n2) dft4<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
-  // now v[n2 + 4 q1] = B[n2][q1]
   v[5] = tw16<INV, 1>(v[5]);
   v[6] = tw16<INV, 2>(v[6]);
   v[7] = tw16<INV, 3>(v[7]);
@@ -130,7 +131,6 @@ __device__ __forceinline__ void dft16(float2 (&v)[16]) {
   v[15] = tw16<INV, 9>(v[15]);
 #pragma unroll
   for (int q1 = 0; q1 < 4; ++q1) dft4<INV>(v[4 * q1 + 0], v[4 * q1 + 1], v[4 * q1 + 2], v[4 * q1 + 3]);
-  // v[4 q1 + q2] = y[q1 + 4 q2]: transpose
   float2 t[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) t[i] = v[i];
@@ -149,6 +149,55 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
   else static_assert(R == 2, "unsupported radix");
 }
 
+// ---------------------------------------------------------------- plans
+
+// Radix plans (every radix divides E = N/64; at most 4 passes).
+template <int N>
+struct FftPlan;
+template <>
+struct FftPlan<128> {
+  static constexpr int P = 7;
+  static constexpr int R[P] = {2, 2, 2, 2, 2, 2, 2};
+};
+template <>
+struct FftPlan<256> {
+  static constexpr int P = 4;
+  static constexpr int R[P] = {4, 4, 4, 4};
+};
+template <>
+struct FftPlan<512> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {8, 8, 8};
+};
+template <>
+struct FftPlan<1024> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {16, 4, 16};
+};
+template <>
+struct FftPlan<2048> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {16, 8, 16};
+};
+
+template <int N>
+__host__ __device__ constexpr int plan_ns(int p) {
+  int ns = 1;
+  for (int q = 0; q < p; ++q) ns *= FftPlan<N>::R[q];
+  return ns;
+}
+// offset of pass p's twiddles in the concatenated table
+template <int N>
+__host__ __device__ constexpr int plan_tw_off(int p) {
+  int off = 0;
+  for (int q = 0; q < p; ++q) off += (FftPlan<N>::R[q] - 1) * plan_ns<N>(q);
+  return off;
+}
+template <int N>
+__host__ __device__ constexpr int plan_tw_size() {
+  return plan_tw_off<N>(FftPlan<N>::P);
+}
+
 // ---------------------------------------------------------------- passes
 
 __device__ __forceinline__ void wave_lds_order() {
@@ -158,36 +207,81 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int N, int E, int R, int NS, bool INV, bool FROM_LDS, bool TO_LDS>
-__device__ __forceinline__ void stockham_pass(float2 (&x)[E], float2* __restrict__ buf,
+// Pass p of the plan.  FROM_LDS: inputs come from LDS (else from x in the
+// strided natural layout); TO_LDS: outputs go to LDS (else to x).
+template <int N, int p, bool INV, bool FROM_LDS, bool TO_LDS>
+__device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __restrict__ buf,
                                               const float2* __restrict__ tw, int t) {
+  constexpr int E = N / 64;
+  constexpr int R = FftPlan<N>::R[p];
+  constexpr int NS = plan_ns<N>(p);
   constexpr int NB = N / R;
   constexpr int BPL = E / R;
   static_assert(BPL * R == E, "radix must divide the per-lane count");
+  static_assert(NS <= 64 || 64 * (BPL - 1) < NS, "twiddle index wraps");
   if constexpr (FROM_LDS) {
+    // element t + 64 b + r NB: lidx = (t + t/16) + 68 b + r (NB + NB/16)
+    const float2* base = buf + (t + (t >> 4));
 #pragma unroll
     for (int b = 0; b < BPL; ++b)
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[b + r * BPL] = buf[lidx(t + 64 * b + r * NB)];
+      for (int r = 0; r < R; ++r) x[b + r * BPL] = base[68 * b + r * lidx(NB)];
     wave_lds_order();
+  }
+  // twiddle base: entry (r-1)*NS + (j & (NS-1)), j = t + 64 b
+  const float2* twb = tw + plan_tw_off<N>(p) + (NS >= 64 ? t : (t & (NS - 1)));
+  // output base of butterfly j = t + 64 b
+  int obase;
+  if constexpr (NS == 1) obase = R * t + ((R * t) >> 4);
+  else if constexpr (NS >= 64) obase = t + (t >> 4);
+  else if constexpr (NS % 16 == 0) {
+    const int u = (t / NS) * NS * R + (t & (NS - 1));
+    obase = u + (u >> 4);
+  } else {
+    obase = 0;
   }
 #pragma unroll
   for (int b = 0; b < BPL; ++b) {
-    const int j = t + 64 * b;
     float2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = x[b + r * BPL];
     if constexpr (NS > 1) {
-      const int jm = j & (NS - 1);
-      constexpr int STRIDE = N / (NS * R);
+      constexpr int boff = NS >= 64 ? 64 : 0;  // (j & (NS-1)) advances by 64 per b when NS > 64
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[r] = cmul_tw<INV>(v[r], tw[jm * r * STRIDE]);
+      for (int r = 1; r < R; ++r) v[r] = cmul_tw<INV>(v[r], twb[(r - 1) * NS + (NS > 64 ? b * boff : 0)]);
     }
     dft<R, INV>(v);
     if constexpr (TO_LDS) {
-      const int base = (j / NS) * NS * R + (j & (NS - 1));
+      // output index (j/NS)*NS*R + j%NS + r NS, j = t + 64 b
+      if constexpr (NS == 1) {
+        // j R + r = R t + 64 R b + r  ->  lidx = obase + 68 R b + r
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lidx(base + r * NS)] = v[r];
+        for (int r = 0; r < R; ++r) buf[obase + 68 * R * b + r] = v[r];
+      } else if constexpr (NS >= 64) {
+        // NS multiple of 64: j/NS = (64 b)/NS, j%NS = t + 64 b - NS (j/NS)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int q = (64 * b) / NS;            // compile-time per b
+          const int c = q * NS * R + (64 * b - q * NS) + r * NS;
+          buf[obase + c + (c >> 4)] = v[r];
+        }
+      } else if constexpr (NS % 16 == 0) {
+        // 16 <= NS < 64: j/NS = t/NS + 64 b/NS, j%NS = t%NS; offsets are
+        // multiples of 16, so the padding splits off the lane base
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int c = (64 * b / NS) * NS * R + r * NS;
+          buf[obase + c + (c >> 4)] = v[r];
+        }
+      } else {
+        // NS < 16: full padded index per output
+        const int u = (t / NS) * NS * R + (t & (NS - 1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = u + (64 * b / NS) * NS * R + r * NS;
+          buf[i + (i >> 4)] = v[r];
+        }
+      }
     } else {
 #pragma unroll
       for (int r = 0; r < R; ++r) x[b + r * BPL] = v[r];
@@ -196,66 +290,39 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[E], float2* __restrict
   if constexpr (TO_LDS) wave_lds_order();
 }
 
-// Radix plans.  Every radix divides E = N/64 so that the first pass reads and
-// the last pass writes registers; all plans have at most 4 passes.
-template <int N>
-struct FftPlan;
-template <>
-struct FftPlan<256> {  // E = 4
-  template <bool INV>
-  __device__ static void run(float2 (&x)[4], float2* buf, const float2* tw, int t) {
-    stockham_pass<256, 4, 4, 1, INV, false, true>(x, buf, tw, t);
-    stockham_pass<256, 4, 4, 4, INV, true, true>(x, buf, tw, t);
-    stockham_pass<256, 4, 4, 16, INV, true, true>(x, buf, tw, t);
-    stockham_pass<256, 4, 4, 64, INV, true, false>(x, buf, tw, t);
+template <int N, bool INV, int p>
+__device__ __forceinline__ void fft_passes(float2 (&x)[N / 64], float2* buf, const float2* tw, int t) {
+  constexpr int P = FftPlan<N>::P;
+  if constexpr (p < P) {
+    stockham_pass<N, p, INV, (p > 0), (p < P - 1)>(x, buf, tw, t);
+    fft_passes<N, INV, p + 1>(x, buf, tw, t);
   }
-};
-template <>
-struct FftPlan<512> {  // E = 8
-  template <bool INV>
-  __device__ static void run(float2 (&x)[8], float2* buf, const float2* tw, int t) {
-    stockham_pass<512, 8, 8, 1, INV, false, true>(x, buf, tw, t);
-    stockham_pass<512, 8, 8, 8, INV, true, true>(x, buf, tw, t);
-    stockham_pass<512, 8, 8, 64, INV, true, false>(x, buf, tw, t);
-  }
-};
-template <>
-struct FftPlan<1024> {  // E = 16
-  template <bool INV>
-  __device__ static void run(float2 (&x)[16], float2* buf, const float2* tw, int t) {
-    stockham_pass<1024, 16, 16, 1, INV, false, true>(x, buf, tw, t);
-    stockham_pass<1024, 16, 4, 16, INV, true, true>(x, buf, tw, t);
-    stockham_pass<1024, 16, 16, 64, INV, true, false>(x, buf, tw, t);
-  }
-};
-template <>
-struct FftPlan<2048> {  // E = 32
-  template <bool INV>
-  __device__ static void run(float2 (&x)[32], float2* buf, const float2* tw, int t) {
-    stockham_pass<2048, 32, 16, 1, INV, false, true>(x, buf, tw, t);
-    stockham_pass<2048, 32, 8, 16, INV, true, true>(x, buf, tw, t);
-    stockham_pass<2048, 32, 16, 128, INV, true, false>(x, buf, tw, t);
-  }
-};
-template <>
-struct FftPlan<128> {  // E = 2
-  template <bool INV>
-  __device__ static void run(float2 (&x)[2], float2* buf, const float2* tw, int t) {
-    stockham_pass<128, 2, 2, 1, INV, false, true>(x, buf, tw, t);
-    stockham_pass<128, 2, 2, 2, INV, true, true>(x, buf, tw, t);
-    stockham_pass<128, 2, 2, 4, INV, true, true>(x, buf, tw, t);
-    stockham_pass<128, 2, 2, 8, INV, true, true>(x, buf, tw, t);
-    stockham_pass<128, 2, 2, 16, INV, true, true>(x, buf, tw, t);
-    stockham_pass<128, 2, 2, 32, INV, true, true>(x, buf, tw, t);
-    stockham_pass<128, 2, 2, 64, INV, true, false>(x, buf, tw, t);
-  }
-};
+}
 
 // Unnormalised DFT (forward: exp(-2 pi i jk/N); inverse: exp(+2 pi i jk/N))
 // of the row held in x (strided natural layout in and out).
 template <int N, bool INV>
 __device__ __forceinline__ void fft_row(float2 (&x)[N / 64], float2* buf, const float2* tw, int t) {
-  FftPlan<N>::template run<INV>(x, buf, tw, t);
+  fft_passes<N, INV, 0>(x, buf, tw, t);
+}
+
+// Fills the concatenated per-pass twiddle table of FftPlan<N> (double
+// precision, rounded once).
+template <int N>
+__global__ void twiddle_table_kernel(float2* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= plan_tw_size<N>()) return;
+  int p = 0;
+  while (p + 1 < FftPlan<N>::P && plan_tw_off<N>(p + 1) <= i) ++p;
+  const int R = FftPlan<N>::R[p];
+  const int NS = plan_ns<N>(p);
+  const int k = i - plan_tw_off<N>(p);
+  const int r = k / NS + 1;
+  const int jm = k % NS;
+  const double a = -2.0 * 3.14159265358979323846 * (double)(jm * r) / (double)(NS * R);
+  double s, c;
+  sincos(a, &s, &c);
+  out[i] = make_float2((float)c, (float)s);
 }
 
 }  // namespace wemfwi

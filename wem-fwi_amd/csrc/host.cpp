@@ -32,6 +32,46 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(WEMFWI_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// ------------------------------------------------------------------ kernel timing
+
+struct TimingEntry {
+  double ms = 0.0;
+  long count = 0;
+};
+static bool g_timing = false;
+static std::map<std::string, TimingEntry>& timing_table() {
+  static std::map<std::string, TimingEntry> t;
+  return t;
+}
+
+// HIP events around one kernel launch on its stream; folded into the table
+// after the stream has been synchronised.
+struct KernelTimer {
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  explicit KernelTimer(const char* n) : name(n) {}
+  void begin(hipStream_t s) {
+    if (!g_timing) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(a, s);
+  }
+  void end(hipStream_t s) {
+    if (a && b) (void)hipEventRecord(b, s);
+  }
+  ~KernelTimer() {
+    if (a && b) {
+      float ms = 0.f;
+      if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+        auto& e = timing_table()[name];
+        e.ms += ms;
+        e.count += 1;
+      }
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
 // ------------------------------------------------------------------ RefSampler
 
 int refsampler_host(int nz, int nx, const cf* slow, int nref, cf* sref, int* refid) {
@@ -136,14 +176,12 @@ int twiddles(int n, const float2** out) {
     *out = it->second;
     return WEMFWI_OK;
   }
-  std::vector<float2> h(n);
-  for (int m = 0; m < n; ++m) {
-    const double a = -2.0 * M_PI * (double)m / (double)n;
-    h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
-  }
   float2* d = nullptr;
-  int rc = dev_upload(&d, h.data(), h.size(), "twiddle table");
+  int rc = dev_alloc(&d, (size_t)twiddle_table_size(n), "twiddle table");
   if (rc) return rc;
+  hipError_t e = launch_twiddles(n, d, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "twiddle table");
   c[key] = d;
   *out = d;
   return WEMFWI_OK;
@@ -152,7 +190,8 @@ int twiddles(int n, const float2** out) {
 // ------------------------------------------------------------------ model
 
 Model::~Model() {
-  void* ptrs[] = {d_slot_begin, d_slot_ref, d_refid, d_phi, d_taper, d_slot_sref, d_tables};
+  void* ptrs[] = {d_slot_begin, d_slot_ref, d_refid, d_phi, d_taper, d_slot_sref, d_tables,
+                  d_scoef, d_refl, d_drefl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ptab.release();
@@ -232,12 +271,43 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   if ((rc = dev_upload(&d_phi, phi.data(), phi.size(), "phi"))) return rc;
   if ((rc = dev_upload(&d_taper, taper.data(), taper.size(), "taper"))) return rc;
   if ((rc = dev_upload(&d_slot_sref, ssref.data(), ssref.size(), "slot_sref"))) return rc;
-  DepthTables t;
+  // Scatterer coefficients coef[it] (S - eps Re S)^(-1/2-it) (Scatter.cpp:82)
+  static const double coef[5] = {1., 1. / 2., 3. / 8., 5. / 16., 35. / 128.};
+  std::vector<float2> sc((size_t)(d.ntaylor + 1) * nzx);
+  for (int it = 0; it <= d.ntaylor; ++it)
+    for (size_t i = 0; i < nzx; ++i) {
+      const cd s(slow[i].real(), slow[i].imag());
+      const cd v = coef[it] * std::pow(s - (double)d.eps * s.real(), -0.5 - it);
+      sc[(size_t)it * nzx + i] = make_float2((float)v.real(), (float)v.imag());
+    }
+  // Reflect.h:19-26 and dReflect.h:18-27,40-47 (coefficients as written)
+  std::vector<float2> rf(nzx, make_float2(0.f, 0.f)), dr(4 * nzx, make_float2(0.f, 0.f));
+  for (int iz = 1; iz < d.nz; ++iz)
+    for (int x = 0; x < d.nx; ++x) {
+      const size_t i = (size_t)iz * d.nx + x, j = i - d.nx;
+      const cd s1(slow[j].real(), slow[j].imag()), s2(slow[i].real(), slow[i].imag());
+      const cd r1(den[j].real(), den[j].imag()), r2(den[i].real(), den[i].imag());
+      const cd a = r2 * std::sqrt(s1), b = r1 * std::sqrt(s2);
+      const cd r = (a - b) / (a + b);
+      rf[i] = make_float2((float)r.real(), (float)r.imag());
+      const cd v = 1.0 / ((a + b) * (a + b));
+      const cd s1s2sq = r1 * r2 / v, r1r2sq = 2.0 * s1 * s2 / v;
+      const cd c = std::sqrt(s2 / s1);
+      const cd q[4] = {c * s1s2sq, 1.0 / c * s1s2sq, r1 * r1r2sq, r2 * r1r2sq};
+      for (int k = 0; k < 4; ++k) dr[(size_t)k * nzx + i] = make_float2((float)q[k].real(), (float)q[k].imag());
+    }
+  if ((rc = dev_upload(&d_scoef, sc.data(), sc.size(), "scatter coefficients"))) return rc;
+  if ((rc = dev_upload(&d_refl, rf.data(), rf.size(), "reflection coefficients"))) return rc;
+  if ((rc = dev_upload(&d_drefl, dr.data(), dr.size(), "dReflect coefficients"))) return rc;
+
+  DepthTables& t = h_tables;
   t.slot_begin = d_slot_begin;
   t.slot_ref = d_slot_ref;
   t.refid = d_refid;
   t.phi = d_phi;
   t.taper = d_taper;
+  t.scoef = d_scoef;
+  t.scoef_stride = (long long)nzx;
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
   if ((rc = twiddles(N, &d_tw))) return rc;
   return WEMFWI_OK;
@@ -248,41 +318,62 @@ StepGeom Model::geom() const {
   g.nx = d.nx;
   g.tap = d.tap;
   g.ssf = d.pspi ? 0 : 1;
+  g.full = (d.tap == 0 && d.nx == N) ? 1 : 0;
+  g.dz = d.dz;
+  g.dk = (float)(2.0 * M_PI / ((double)d.dx * (double)(N - 1)));
+  g.ntaylor = d.ntaylor;
   return g;
 }
 
-// Builds the phase table of the distinct frequencies of freq_hz; fills
-// wrow[i] (angular frequency) and prow[i] (table of row i).
-int Model::phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow,
-                        std::vector<const float2*>& prow, hipStream_t s) {
-  std::vector<float> uniq(freq_hz, freq_hz + nrows);
+// Phase table of a set of frequencies: exp(-i kz dz)/N per (frequency,
+// active slot, k <= N/2) -- the k grid is symmetric, so P[N-k] = P[k].
+int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
+  std::vector<float> uniq(freq_hz, freq_hz + n);
   std::sort(uniq.begin(), uniq.end());
   uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
   std::vector<double>& wl = h_wlist;  // member: outlives the asynchronous upload
   wl.resize(uniq.size());
   for (size_t i = 0; i < uniq.size(); ++i) wl[i] = 2.0 * M_PI * (double)uniq[i];
-  const size_t per = (size_t)std::max(nslots, 1) * N;
+  const size_t per = (size_t)std::max(nslots, 1) * NH();
   int rc = ptab.reserve(uniq.size() * per * sizeof(float2), "phase table");
   if (rc) return rc;
   if ((rc = wlist.reserve(wl.size() * sizeof(double), "frequency list"))) return rc;
   hipError_t e = hipMemcpyAsync(wlist.ptr, wl.data(), wl.size() * sizeof(double), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "frequency upload");
   const double dk = 2.0 * M_PI / ((double)d.dx * (double)(N - 1));
+  KernelTimer kt("phase_table_kernel");
+  kt.begin(s);
   e = launch_phase_table((float2*)ptab.ptr, (const double*)wlist.ptr, (int)wl.size(), d_slot_sref, nslots,
-                         N, dk, (double)d.dz, (double)d.eps, s);
+                         N, NH(), dk, (double)d.dz, (double)d.eps, s);
+  kt.end(s);
   if (e != hipSuccess) return hip_fail(e, "phase table kernel");
+  freqs = uniq;
+  return WEMFWI_OK;
+}
+
+// Maps rows to the current frequency set, rebuilding it when a row's
+// frequency is not in it.
+int Model::phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow,
+                        std::vector<const float2*>& prow, hipStream_t s) {
+  bool have = !freqs.empty();
+  for (int i = 0; i < nrows && have; ++i) have = std::binary_search(freqs.begin(), freqs.end(), freq_hz[i]);
+  if (!have) {
+    int rc = set_freqs(nrows, freq_hz, s);
+    if (rc) return rc;
+  }
+  const size_t per = (size_t)std::max(nslots, 1) * NH();
   wrow.resize(nrows);
   prow.resize(nrows);
   for (int i = 0; i < nrows; ++i) {
-    const size_t k = std::lower_bound(uniq.begin(), uniq.end(), freq_hz[i]) - uniq.begin();
-    wrow[i] = (float)wl[k];
+    const size_t k = std::lower_bound(freqs.begin(), freqs.end(), freq_hz[i]) - freqs.begin();
+    wrow[i] = (float)h_wlist[k];
     prow[i] = (const float2*)ptab.ptr + k * per;
   }
   return WEMFWI_OK;
 }
 
 int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf* in, cf* out, int add,
-                  hipStream_t s) {
+                  int recursive, hipStream_t s) {
   if (nrows < 0 || !freq_hz || !in || !out) return fail(WEMFWI_ERR_ARG, "oneway: null argument");
   if (nrows == 0) return WEMFWI_OK;
   const size_t nzx = (size_t)d.nz * d.nx;
@@ -306,14 +397,20 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
     h[i].ptab = prow[i];
     h[i].w = wrow[i];
     h[i].model = 0;
+    h[i].recursive = recursive ? 1 : 0;
   }
   if ((rc = rows.reserve(h.size() * sizeof(SweepRow), "row descriptors"))) return rc;
   e = hipMemcpyAsync(rows.ptr, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "row upload");
   const bool down = !up;
   const int step = (down != (adjoint != 0)) ? 1 : -1;
-  e = launch_oneway(N, (const SweepRow*)rows.ptr, nrows, d_tables, geom(), d.nz, step, adjoint != 0, d_tw, s);
-  if (e != hipSuccess) return hip_fail(e, "oneway kernel");
+  {
+    KernelTimer kt("oneway_kernel");
+    kt.begin(s);
+    e = launch_oneway(N, (const SweepRow*)rows.ptr, nrows, d_tables, geom(), d.nz, step, adjoint != 0, d_tw, s);
+    kt.end(s);
+    if (e != hipSuccess) return hip_fail(e, "oneway kernel");
+  }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "oneway");
   return WEMFWI_OK;
@@ -356,9 +453,6 @@ int Model::onestep(int adjoint, int nrows, const float* freq_hz, const int* dept
 
 using namespace wemfwi;
 
-struct wemfwi_model {
-  Model m;
-};
 
 extern "C" {
 
@@ -401,7 +495,36 @@ int wemfwi_model_destroy(wemfwi_model* model) {
 int wemfwi_oneway(wemfwi_model* model, int up, int adjoint, int nrows, const float* freq_hz,
                   const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream) {
   if (!model) return fail(WEMFWI_ERR_ARG, "oneway: null model");
-  return model->m.oneway(up, adjoint, nrows, freq_hz, (const cf*)in, (cf*)out, add, (hipStream_t)stream);
+  return model->m.oneway(up, adjoint, nrows, freq_hz, (const cf*)in, (cf*)out, add, in == out,
+                         (hipStream_t)stream);
+}
+
+int wemfwi_oneway_recursive(wemfwi_model* model, int up, int adjoint, int nrows, const float* freq_hz,
+                            const wemfwi_complex* src, wemfwi_complex* out, void* stream) {
+  if (!model) return fail(WEMFWI_ERR_ARG, "oneway_recursive: null model");
+  return model->m.oneway(up, adjoint, nrows, freq_hz, (const cf*)src, (cf*)out, 1, 1, (hipStream_t)stream);
+}
+
+int wemfwi_model_set_freqs(wemfwi_model* model, int nfreq, const float* freq_hz, void* stream) {
+  if (!model || !freq_hz || nfreq <= 0) return fail(WEMFWI_ERR_ARG, "set_freqs: invalid argument");
+  int rc = model->m.set_freqs(nfreq, freq_hz, (hipStream_t)stream);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "set_freqs");
+}
+
+int wemfwi_timing(int enable) {
+  g_timing = enable != 0;
+  if (enable) timing_table().clear();
+  return WEMFWI_OK;
+}
+
+int wemfwi_timing_get(const char* kernel, double* total_ms, long* count) {
+  if (!kernel || !total_ms || !count) return fail(WEMFWI_ERR_ARG, "timing_get: null argument");
+  auto it = timing_table().find(kernel);
+  *total_ms = it == timing_table().end() ? 0.0 : it->second.ms;
+  *count = it == timing_table().end() ? 0 : it->second.count;
+  return WEMFWI_OK;
 }
 
 int wemfwi_onestep(wemfwi_model* model, int adjoint, int nrows, const float* freq_hz, const int* depth,

@@ -39,7 +39,11 @@ struct Model {
   float2* d_phi = nullptr;
   float* d_taper = nullptr;
   double2* d_slot_sref = nullptr;
+  float2* d_scoef = nullptr;   // [ntaylor+1][nz][nx] scatterer coefficients
+  float2* d_refl = nullptr;    // [nz][nx] reflection coefficient (Reflect.h:19-26)
+  float2* d_drefl = nullptr;   // [4][nz][nx] a1, b1, a2, b2 of dReflect.h:18-27,40-47
   DepthTables* d_tables = nullptr;
+  DepthTables h_tables{};
   const float2* d_tw = nullptr;
 
   Workspace ptab, wlist, rows;
@@ -51,12 +55,19 @@ struct Model {
   ~Model();
   int build(const wemfwi_model_desc& desc, const std::complex<float>* slow, const std::complex<float>* den);
   StepGeom geom() const;
+  std::vector<float> freqs;   // frequency set of the current phase table (sorted)
+  int NH() const { return N / 2 + 1; }
+  int set_freqs(int n, const float* freq_hz, hipStream_t s);
   int phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow, std::vector<const float2*>& prow,
                    hipStream_t s);
   int oneway(int up, int adjoint, int nrows, const float* freq_hz, const std::complex<float>* in,
-             std::complex<float>* out, int add, hipStream_t s);
+             std::complex<float>* out, int add, int recursive, hipStream_t s);
   int onestep(int adjoint, int nrows, const float* freq_hz, const int* depth, const std::complex<float>* in,
               std::complex<float>* out, int add, hipStream_t s);
 };
 
 }  // namespace wemfwi
+
+struct wemfwi_model {
+  wemfwi::Model m;
+};

@@ -11,16 +11,18 @@ namespace wemfwi {
 struct SweepRow {
   const float2* in;      // [nz][nx] source terms (model of OneWay::forward/adjoint)
   float2* out;           // [nz][nx] result; == in for the in-place recursion
-  const float2* ptab;    // [nslots][N] phase table of this row's frequency
+  const float2* ptab;    // [nslots][N/2+1] phase table of this row's frequency
   float w;               // angular frequency
   int model;             // index into the DepthTables array
+  int recursive;         // 1: W[next] = P W[cur] + in[next] with W the output rows
+                         // 0: out[next] = P in[cur] + in[next] (one bidiagonal step)
 };
 
 // One independent depth step.
 struct StepRow {
   const float2* in;      // [nx]
   float2* out;           // [nx]
-  const float2* ptab;    // [nslots][N]
+  const float2* ptab;    // [nslots][N/2+1]
   float w;
   int d;                 // depth of the step
   int model;
@@ -28,9 +30,11 @@ struct StepRow {
 };
 
 bool fft_size_supported(int n);
+int twiddle_table_size(int N);
+hipError_t launch_twiddles(int N, float2* out, hipStream_t s);
 
 hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const double2* sref,
-                              int nslots, int N, double dk, double dz, double eps,
+                              int nslots, int N, int NH, double dk, double dz, double eps,
                               hipStream_t stream);
 hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
                          StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s);

@@ -15,19 +15,19 @@ namespace wemfwi {
 // Phshift_cmplx.cpp:12-24).  kz = re + i im with im <= 0; Im is formed as
 // -|b|/(2 re) (or Re as |b|/(2|im|) when a < 0) -- equal to the reference's
 // -sqrt((c-a)/2) since (c-a)(c+a) = b^2, without its cancellation.
+// Only k = 0..N/2 is stored: the grid satisfies k[N-i] = -k[i], so the
+// multiplier of index k is entry min(k, N-k).
 __global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __restrict__ wlist,
                                    int nw, const double2* __restrict__ sref, int nslots, int N,
-                                   double dk, double dz, double eps) {
-  const size_t total = (size_t)nw * nslots * N;
+                                   int NH, double dk, double dz, double eps) {
+  const size_t total = (size_t)nw * nslots * NH;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
-    const int k = (int)(i % N);
-    const int s = (int)((i / N) % nslots);
-    const int iw = (int)(i / ((size_t)N * nslots));
+    const int k = (int)(i % NH);
+    const int s = (int)((i / NH) % nslots);
+    const int iw = (int)(i / ((size_t)NH * nslots));
     const int h = N / 2;
-    double kk = 0.0;
-    if (k < h) kk = k * dk;
-    else if (N - k < h) kk = -(N - k) * dk;
+    const double kk = k < h ? k * dk : 0.0;
     const double w = wlist[iw];
     const double2 sr = sref[s];
     const double a = w * w * sr.x - kk * kk;
@@ -47,25 +47,23 @@ __global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __re
 // ------------------------------------------------------------------ rows
 
 template <int N>
-__device__ __forceinline__ void load_row(float2 (&v)[N / 64], const float2* __restrict__ p, int nx,
-                                         int tap, int t) {
+__device__ __forceinline__ void load_row(float2 (&v)[N / 64], const float2* __restrict__ p,
+                                         const StepGeom& g, int t) {
+  const float2* b = p + (t - g.tap);
 #pragma unroll
   for (int e = 0; e < N / 64; ++e) {
-    const int x = t + 64 * e - tap;
-    const bool ok = in_row(x, nx);
-    const float2 a = ldg(p, ok ? x : 0);
-    v[e] = ok ? a : make_float2(0.f, 0.f);
+    v[e] = make_float2(0.f, 0.f);
+    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) v[e] = ldg(b, (unsigned)(64 * e));
   }
 }
 
 template <int N>
-__device__ __forceinline__ void store_row(float2* __restrict__ p, const float2 (&v)[N / 64], int nx,
-                                          int tap, int t) {
+__device__ __forceinline__ void store_row(float2* __restrict__ p, const float2 (&v)[N / 64],
+                                          const StepGeom& g, int t) {
+  float2* b = p + (t - g.tap);
 #pragma unroll
-  for (int e = 0; e < N / 64; ++e) {
-    const int x = t + 64 * e - tap;
-    if (in_row(x, nx)) stg(p, x, v[e]);
-  }
+  for (int e = 0; e < N / 64; ++e)
+    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), v[e]);
 }
 
 constexpr int kWavesPerBlock = 4;
@@ -85,34 +83,34 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock)
     __attribute__((amdgpu_waves_per_eu(Occupancy<N>::value, 8))) oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                   StepGeom g, int nz, int step, const float2* __restrict__ twg) {
   constexpr int E = N / 64;
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[plan_tw_size<N>()];
   __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
   const int t = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = ldg(twg, i);
+  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kWavesPerBlock + wv;
   if (row >= nrows) return;
-  const SweepRow R = ldg(rows, row);
-  const DepthTables dt = ldg(models, R.model);
+  const SweepRow R = rows[row];
+  const DepthTables dt = models[R.model];
   float2* buf = bufs[wv];
   const int nx = g.nx;
-  const bool inplace = (R.in == R.out);
+  const bool recursive = R.recursive != 0;
   const int first = step > 0 ? 0 : nz - 1;
   float2 st[E];
-  load_row<N>(st, R.in + (size_t)first * nx, nx, g.tap, t);
-  if (!inplace) store_row<N>(R.out + (size_t)first * nx, st, nx, g.tap, t);
+  load_row<N>(st, R.in + (size_t)first * nx, g, t);
+  if (R.in != R.out) store_row<N>(R.out + (size_t)first * nx, st, g, t);
   for (int k = 0; k < nz - 1; ++k) {
     const int cur = first + k * step;
     const int nxt = cur + step;
-    if (!inplace && k > 0) load_row<N>(st, R.in + (size_t)cur * nx, nx, g.tap, t);
+    if (!recursive && k > 0) load_row<N>(st, R.in + (size_t)cur * nx, g, t);
     if constexpr (ADJ) step_adjoint<N>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t);
     else step_forward<N>(st, cur, R.w, dt, g, R.ptab, buf, tw, t);
     float2 src[E];
-    load_row<N>(src, R.in + (size_t)nxt * nx, nx, g.tap, t);
+    load_row<N>(src, R.in + (size_t)nxt * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
-    store_row<N>(R.out + (size_t)nxt * nx, st, nx, g.tap, t);
+    store_row<N>(R.out + (size_t)nxt * nx, st, g, t);
   }
 }
 
@@ -122,44 +120,73 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock)
     __attribute__((amdgpu_waves_per_eu(Occupancy<N>::value, 8))) onestep_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                    StepGeom g, const float2* __restrict__ twg) {
   constexpr int E = N / 64;
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[plan_tw_size<N>()];
   __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
   const int t = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = ldg(twg, i);
+  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kWavesPerBlock + wv;
   if (row >= nrows) return;
-  const StepRow R = ldg(rows, row);
-  const DepthTables dt = ldg(models, R.model);
+  const StepRow R = rows[row];
+  const DepthTables dt = models[R.model];
   float2 st[E];
-  load_row<N>(st, R.in, g.nx, g.tap, t);
+  load_row<N>(st, R.in, g, t);
   if constexpr (ADJ) step_adjoint<N>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
   else step_forward<N>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
   if (R.add) {
     float2 o[E];
-    load_row<N>(o, R.out, g.nx, g.tap, t);
+    load_row<N>(o, R.out, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cadd(st[e], o[e]);
   }
-  store_row<N>(R.out, st, g.nx, g.tap, t);
+  store_row<N>(R.out, st, g, t);
 }
 
 // ------------------------------------------------------------------ launchers
+
+template <int N>
+static hipError_t launch_twiddles_n(float2* out, hipStream_t s) {
+  const int n = plan_tw_size<N>();
+  hipLaunchKernelGGL((twiddle_table_kernel<N>), dim3((n + 255) / 256), dim3(256), 0, s, out);
+  return hipGetLastError();
+}
+
+int twiddle_table_size(int N) {
+  switch (N) {
+    case 128: return plan_tw_size<128>();
+    case 256: return plan_tw_size<256>();
+    case 512: return plan_tw_size<512>();
+    case 1024: return plan_tw_size<1024>();
+    case 2048: return plan_tw_size<2048>();
+    default: return 0;
+  }
+}
+
+hipError_t launch_twiddles(int N, float2* out, hipStream_t s) {
+  switch (N) {
+    case 128: return launch_twiddles_n<128>(out, s);
+    case 256: return launch_twiddles_n<256>(out, s);
+    case 512: return launch_twiddles_n<512>(out, s);
+    case 1024: return launch_twiddles_n<1024>(out, s);
+    case 2048: return launch_twiddles_n<2048>(out, s);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 bool fft_size_supported(int n) {
   return n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
 }
 
 hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const double2* sref,
-                              int nslots, int N, double dk, double dz, double eps,
+                              int nslots, int N, int NH, double dk, double dz, double eps,
                               hipStream_t stream) {
-  const size_t total = (size_t)nw * nslots * N;
+  const size_t total = (size_t)nw * nslots * NH;
   if (total == 0) return hipSuccess;
   size_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(phase_table_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, ptab, wlist, nw,
-                     sref, nslots, N, dk, dz, eps);
+                     sref, nslots, N, NH, dk, dz, eps);
   return hipGetLastError();
 }
 

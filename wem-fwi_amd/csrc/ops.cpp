@@ -1,0 +1,853 @@
+// Shot x frequency operators of the reference (WEM, BornFull, BornRefl, the
+// frequency-extended variants) and the remaining unit operators, built from
+// the batched sweep / step / elementwise kernels.  Every (shot, frequency)
+// pair is one row; rows are processed in chunks that fit device memory and
+// reductions over rows are deterministic (fixed order, double precision).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/wemfwi.h"
+#include "born.h"
+#include "host.h"
+#include "kernels.h"
+
+using cf = std::complex<float>;
+
+namespace wemfwi {
+
+// ------------------------------------------------------------------ survey
+
+static std::vector<float> cos2_window(int ng) {
+  // Injection.h:53-65 (cos^2 spread; float arithmetic as the reference)
+  std::vector<float> g(2 * ng + 1, 1.f);
+  if (ng == 0) return g;
+  const float pi = (float)(4.0 * std::atan(1.0));
+  const int size = 2 * ng + 1;
+  for (int i = 0; i < size / 2; ++i) g[i] = (-std::cos(pi / ng * i) + 1) / 2;
+  for (int i = size / 2; i < size; ++i) g[i] = (std::cos(pi / ng * (i - size + ng + 1)) + 1) / 2;
+  return g;
+}
+
+struct DevPlan {
+  int nx = -1, ng = -1, offset = -1;
+  InjPlan inj{};
+  RecPlan rec{};
+  std::vector<void*> allocs;
+  void release() {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    nx = -1;
+  }
+};
+
+struct Survey {
+  wemfwi_survey_desc d{};
+  std::vector<int> src_iz, src_ix, rec_iz, rec_ix, bin;
+  std::vector<float> freq;
+  std::vector<cf> wave;
+  DevPlan rplan;  // receivers on the current grid
+  ~Survey() { rplan.release(); }
+};
+
+template <typename T>
+static int upload(std::vector<void*>& allocs, T** p, const std::vector<T>& h, const char* what) {
+  *p = nullptr;
+  if (h.empty()) return WEMFWI_OK;
+  hipError_t e = hipMalloc((void**)p, h.size() * sizeof(T));
+  if (e != hipSuccess) return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what);
+  allocs.push_back(*p);
+  e = hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return WEMFWI_OK;
+}
+
+// Receiver injection / recording plans of a device list on an nx-wide grid.
+static int build_plan(DevPlan& P, const std::vector<int>& iz, const std::vector<int>& ix, int ng, int offset,
+                      int nx, int nz) {
+  if (P.nx == nx && P.ng == ng && P.offset == offset) return WEMFWI_OK;
+  P.release();
+  const std::vector<float> g = cos2_window(ng);
+  // recording: contributions by device, in the order of Injection.cpp:69-80
+  std::vector<int> rptr(1, 0);
+  std::vector<long long> rcell;
+  std::vector<float> rw;
+  // injection: gather by cell; contributions listed in device order per cell
+  std::vector<std::vector<std::pair<int, float>>> bycell;
+  std::vector<long long> cells;
+  std::vector<int> cell_of;  // temp map
+  std::vector<long long> keys;
+  for (size_t i = 0; i < ix.size(); ++i) {
+    if (iz[i] < 0 || iz[i] >= nz) return fail(WEMFWI_ERR_ARG, "device depth index outside the model");
+    for (int ig = -ng; ig <= ng; ++ig) {
+      const int off = ix[i] + ig + offset;
+      if (off < 0 || off >= nx) continue;
+      const long long c = (long long)iz[i] * nx + off;
+      rcell.push_back(c);
+      rw.push_back(g[ig + ng]);
+      keys.push_back(c);
+    }
+    rptr.push_back((int)rcell.size());
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  bycell.assign(keys.size(), {});
+  for (size_t i = 0; i < ix.size(); ++i)
+    for (int c = rptr[i]; c < rptr[i + 1]; ++c) {
+      const size_t k = std::lower_bound(keys.begin(), keys.end(), rcell[c]) - keys.begin();
+      bycell[k].push_back({(int)i, rw[c]});
+    }
+  std::vector<int> iptr(1, 0), idev;
+  std::vector<float> iw;
+  for (auto& v : bycell) {
+    for (auto& pr : v) {
+      idev.push_back(pr.first);
+      iw.push_back(pr.second);
+    }
+    iptr.push_back((int)idev.size());
+  }
+  int rc;
+  long long* d_cells;
+  int *d_iptr, *d_idev, *d_rptr;
+  float *d_iw, *d_rw;
+  long long* d_rcell;
+  if ((rc = upload(P.allocs, &d_cells, keys, "injection cells"))) return rc;
+  if ((rc = upload(P.allocs, &d_iptr, iptr, "injection ptr"))) return rc;
+  if ((rc = upload(P.allocs, &d_idev, idev, "injection devices"))) return rc;
+  if ((rc = upload(P.allocs, &d_iw, iw, "injection weights"))) return rc;
+  if ((rc = upload(P.allocs, &d_rptr, rptr, "recording ptr"))) return rc;
+  if ((rc = upload(P.allocs, &d_rcell, rcell, "recording cells"))) return rc;
+  if ((rc = upload(P.allocs, &d_rw, rw, "recording weights"))) return rc;
+  P.inj = InjPlan{(int)keys.size(), d_cells, d_iptr, d_idev, d_iw};
+  P.rec = RecPlan{(int)ix.size(), d_rptr, d_rcell, d_rw};
+  P.nx = nx;
+  P.ng = ng;
+  P.offset = offset;
+  return WEMFWI_OK;
+}
+
+// ------------------------------------------------------------------ batches
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes, const char* what) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = bytes;
+    if (bytes == 0) return WEMFWI_OK;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + " (" + std::to_string(bytes) +
+                                        " bytes)");
+    }
+    return WEMFWI_OK;
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+template <typename T>
+static int put(DevBuf& b, const std::vector<T>& h, hipStream_t s, const char* what) {
+  if (b.n < h.size() * sizeof(T)) {
+    int rc = b.alloc(h.size() * sizeof(T), what);
+    if (rc) return rc;
+  }
+  if (h.empty()) return WEMFWI_OK;
+  hipError_t e = hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, what);
+}
+
+#define HIPCHK(expr, what)                              \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return hip_fail(_e, (what));  \
+  } while (0)
+
+#define RCCHK(expr)        \
+  do {                     \
+    int _rc = (expr);      \
+    if (_rc) return _rc;   \
+  } while (0)
+
+// A chunk of (shot, frequency) rows with their per-row models.
+struct Rows {
+  std::vector<int> shot, fi;           // shot and frequency index of each row
+  std::vector<Model*> model;           // background model of each row
+};
+
+static size_t chunk_rows(size_t per_row_bytes, size_t nrows) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr == 0) fr = (size_t)8 << 30;
+  const size_t budget = fr / 2;
+  size_t c = per_row_bytes ? budget / per_row_bytes : nrows;
+  if (c < 1) c = 1;
+  return std::min(c, nrows);
+}
+
+// Context of one operator call: geometry shared by all models, device-side
+// descriptor tables for the sweep / lin kernels.
+struct Ctx {
+  Model* m0;            // reference model (geometry)
+  Survey* sv;
+  hipStream_t s;
+  int nz, nx, N;
+  size_t nzx;
+  std::vector<Model*> models;  // distinct models of the call
+  DevBuf d_models;             // DepthTables[nmodels]
+  DevBuf d_rows;
+  std::vector<SweepRow> hsw;
+  std::vector<LinRow> hlin;
+  std::vector<float2*> hp1, hp2, hp3;
+  DevBuf dp1, dp2, dp3;
+  std::vector<long long> hvb;
+  DevBuf dvb;
+  std::vector<int> hiz, hix;
+  std::vector<float2> hval;
+  DevBuf diz, dix, dval, dgw;
+  std::vector<float> hgw;
+
+  int init(const std::vector<Model*>& ms, Survey* survey, hipStream_t st) {
+    models = ms;
+    m0 = ms[0];
+    sv = survey;
+    s = st;
+    nz = m0->d.nz;
+    nx = m0->d.nx;
+    N = m0->N;
+    nzx = (size_t)nz * nx;
+    for (Model* m : ms)
+      if (m->d.nz != nz || m->d.nx != nx || m->N != N || m->d.tap != m0->d.tap || m->d.pspi != m0->d.pspi ||
+          m->d.taper != m0->d.taper || m->d.ntaylor != m0->d.ntaylor)
+        return fail(WEMFWI_ERR_ARG, "models of one call must share grid and parameters");
+    std::vector<DepthTables> t;
+    for (Model* m : ms) t.push_back(m->h_tables);
+    RCCHK(put(d_models, t, s, "model tables"));
+    // frequency sets: every model holds the survey's frequencies
+    for (Model* m : ms) {
+      bool have = m->freqs.size() == sv->freq.size();
+      if (have) {
+        std::vector<float> f = sv->freq;
+        std::sort(f.begin(), f.end());
+        f.erase(std::unique(f.begin(), f.end()), f.end());
+        have = f == m->freqs;
+      }
+      if (!have) RCCHK(m->set_freqs((int)sv->freq.size(), sv->freq.data(), s));
+    }
+    return WEMFWI_OK;
+  }
+  int model_index(Model* m) const {
+    return (int)(std::find(models.begin(), models.end(), m) - models.begin());
+  }
+  const float2* ptab(Model* m, int fi) const {
+    const size_t k = std::lower_bound(m->freqs.begin(), m->freqs.end(), sv->freq[fi]) - m->freqs.begin();
+    return (const float2*)m->ptab.ptr + k * (size_t)std::max(m->nslots, 1) * m->NH();
+  }
+  float omega(Model* m, int fi) const {
+    const size_t k = std::lower_bound(m->freqs.begin(), m->freqs.end(), sv->freq[fi]) - m->freqs.begin();
+    return (float)m->h_wlist[k];
+  }
+
+  // in-place / recursive sweep of W (rows x nzx) with sources src
+  int sweep(const Rows& r, int up, int adjoint, const float2* src, float2* W) {
+    const int n = (int)r.shot.size();
+    hsw.resize(n);
+    for (int i = 0; i < n; ++i) {
+      hsw[i].in = src + (size_t)i * nzx;
+      hsw[i].out = W + (size_t)i * nzx;
+      hsw[i].ptab = ptab(r.model[i], r.fi[i]);
+      hsw[i].w = omega(r.model[i], r.fi[i]);
+      hsw[i].model = model_index(r.model[i]);
+      hsw[i].recursive = 1;
+    }
+    RCCHK(put(d_rows, hsw, s, "sweep rows"));
+    const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
+    HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, step,
+                         adjoint != 0, m0->d_tw, s),
+           "oneway kernel");
+    return sync();
+  }
+
+  // LinDown (dir=+1) / LinUp (dir=-1)
+  int lin(const Rows& r, int dir, int adjoint, int exact, const float2* bg, const float2* m, size_t m_stride,
+          float2* out) {
+    const int n = (int)r.shot.size();
+    hlin.resize(n);
+    for (int i = 0; i < n; ++i) {
+      hlin[i].bg = bg + (size_t)i * nzx;
+      hlin[i].m = m + (adjoint ? (size_t)i * nzx : (size_t)0) + (adjoint ? 0 : m_stride * (size_t)r.fi[i]);
+      hlin[i].out = out + (size_t)i * nzx;
+      hlin[i].ptab = ptab(r.model[i], r.fi[i]);
+      hlin[i].w = omega(r.model[i], r.fi[i]);
+      hlin[i].model = model_index(r.model[i]);
+    }
+    RCCHK(put(d_rows, hlin, s, "lin rows"));
+    HIPCHK(launch_lin(N, d_rows.as<LinRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, dir, adjoint != 0,
+                      exact != 0, m0->d_tw, s),
+           "lin kernel");
+    return sync();
+  }
+
+  // rows: out_i (= or +=) op(A_i) * B_i, per-row pointers
+  int mul(int n, const std::vector<float2*>& out, const std::vector<const float2*>& A,
+          const std::vector<const float2*>& B, bool conjA, bool acc, size_t len) {
+    for (int i = 0; i < n; ++i)
+      HIPCHK(launch_mul(out[i], 0, A[i], 0, B[i], 0, (long long)len, 1, conjA, acc, s), "mul kernel");
+    return WEMFWI_OK;
+  }
+
+  // sources of the rows' shots, value = wavelet of the row's frequency
+  int inject_sources(const Rows& r, float2* W, int ng, int offset) {
+    const int n = (int)r.shot.size();
+    hiz.resize(n);
+    hix.resize(n);
+    hval.resize(n);
+    for (int i = 0; i < n; ++i) {
+      hiz[i] = sv->src_iz[r.shot[i]];
+      hix[i] = sv->src_ix[r.shot[i]];
+      const cf v = sv->wave[r.fi[i]];
+      hval[i] = make_float2(v.real(), v.imag());
+      if (hiz[i] < 0 || hiz[i] >= nz) return fail(WEMFWI_ERR_ARG, "source depth index outside the model");
+    }
+    hgw = cos2_window(ng);
+    RCCHK(put(diz, hiz, s, "source iz"));
+    RCCHK(put(dix, hix, s, "source ix"));
+    RCCHK(put(dval, hval, s, "source values"));
+    RCCHK(put(dgw, hgw, s, "source spread"));
+    HIPCHK(launch_inject_point(W, (long long)nzx, nx, diz.as<int>(), dix.as<int>(), dval.as<float2>(),
+                               dgw.as<float>(), ng, offset, n, s),
+           "inject sources");
+    return WEMFWI_OK;
+  }
+
+  int vbase(const Rows& r) {
+    const int n = (int)r.shot.size();
+    hvb.resize(n);
+    for (int i = 0; i < n; ++i)
+      hvb[i] = (long long)r.shot[i] * sv->d.nr * sv->d.nt + sv->bin[r.fi[i]];
+    return put(dvb, hvb, s, "data offsets");
+  }
+
+  // W_i += R^H d (receivers of the rows' traces)
+  int inject_receivers(const Rows& r, float2* W, const float2* data, int ng, int offset) {
+    RCCHK(build_plan(sv->rplan, sv->rec_iz, sv->rec_ix, ng, offset, nx, nz));
+    RCCHK(vbase(r));
+    HIPCHK(launch_inject_csr(W, (long long)nzx, sv->rplan.inj, data, dvb.as<long long>(), sv->d.nt,
+                             (int)r.shot.size(), s),
+           "inject receivers");
+    return WEMFWI_OK;
+  }
+
+  // data += R W_i
+  int record(const Rows& r, float2* data, const float2* W, int ng, int offset) {
+    RCCHK(build_plan(sv->rplan, sv->rec_iz, sv->rec_ix, ng, offset, nx, nz));
+    RCCHK(vbase(r));
+    HIPCHK(launch_record(data, dvb.as<long long>(), sv->d.nt, W, (long long)nzx, sv->rplan.rec,
+                         (int)r.shot.size(), s),
+           "record receivers");
+    return sync();
+  }
+
+  // W_i *= (conj) a_i, a_i = per-row shared array (e.g. the row model's r)
+  int scale_rows(const Rows& r, float2* W, const std::vector<const float2*>& a, bool conjA) {
+    const int n = (int)r.shot.size();
+    for (int i = 0; i < n; ++i)
+      HIPCHK(launch_mul(W + (size_t)i * nzx, 0, a[i], 0, W + (size_t)i * nzx, 0, (long long)nzx, 1, conjA, false,
+                        s),
+             "scale rows");
+    return WEMFWI_OK;
+  }
+
+  int zero(void* p, size_t bytes) {
+    HIPCHK(hipMemsetAsync(p, 0, bytes, s), "memset");
+    return WEMFWI_OK;
+  }
+  int sync() {
+    HIPCHK(hipStreamSynchronize(s), "stream sync");
+    return WEMFWI_OK;
+  }
+};
+
+static std::vector<Rows> make_chunks(int ns, int nw, const std::vector<Model*>& per_freq_model, size_t per_row,
+                                     bool freq_major) {
+  std::vector<std::pair<int, int>> all;
+  if (freq_major) {
+    for (int i = 0; i < nw; ++i)
+      for (int s = 0; s < ns; ++s) all.push_back({s, i});
+  } else {
+    for (int s = 0; s < ns; ++s)
+      for (int i = 0; i < nw; ++i) all.push_back({s, i});
+  }
+  const size_t c = chunk_rows(per_row, all.size());
+  std::vector<Rows> out;
+  for (size_t b = 0; b < all.size(); b += c) {
+    Rows r;
+    for (size_t k = b; k < std::min(all.size(), b + c); ++k) {
+      r.shot.push_back(all[k].first);
+      r.fi.push_back(all[k].second);
+      r.model.push_back(per_freq_model[all[k].second]);
+    }
+    out.push_back(r);
+  }
+  return out;
+}
+
+static int check_survey(Survey* sv, Model* m) {
+  if (!sv || !m) return fail(WEMFWI_ERR_ARG, "null survey or model");
+  return WEMFWI_OK;
+}
+
+// ------------------------------------------------------------------ WEM
+
+// WEM::full_fwd (wem_prop.cpp:52-74) for every (shot, frequency) row:
+// inject, Down, record, r*W, Up, record.
+static int wem_forward(const std::vector<Model*>& fm, Survey* sv, float2* data, int add, hipStream_t s) {
+  RCCHK(check_survey(sv, fm[0]));
+  Ctx c;
+  std::vector<Model*> uniq;
+  for (Model* m : fm)
+    if (std::find(uniq.begin(), uniq.end(), m) == uniq.end()) uniq.push_back(m);
+  RCCHK(c.init(uniq, sv, s));
+  const size_t dbytes = (size_t)sv->d.ns * sv->d.nr * sv->d.nt * sizeof(float2);
+  if (!add) RCCHK(c.zero(data, dbytes));
+  auto chunks = make_chunks(sv->d.ns, sv->d.nw, fm, c.nzx * sizeof(float2), false);
+  DevBuf W;
+  RCCHK(W.alloc(chunks[0].shot.size() * c.nzx * sizeof(float2), "wavefield"));
+  float2* w = W.as<float2>();
+  for (const Rows& r : chunks) {
+    const int n = (int)r.shot.size();
+    RCCHK(c.zero(w, n * c.nzx * sizeof(float2)));
+    RCCHK(c.inject_sources(r, w, sv->d.ngs, sv->d.offset));
+    RCCHK(c.sweep(r, 0, 0, w, w));
+    RCCHK(c.record(r, data, w, sv->d.ngr, sv->d.offset));
+    std::vector<const float2*> refl(n);
+    for (int i = 0; i < n; ++i) refl[i] = r.model[i]->d_refl;
+    RCCHK(c.scale_rows(r, w, refl, false));
+    RCCHK(c.sweep(r, 1, 0, w, w));
+    RCCHK(c.record(r, data, w, sv->d.ngr, sv->d.offset));
+  }
+  return c.sync();
+}
+
+// ------------------------------------------------------------------ Born
+
+// dReflect of the perturbation of each model (one per frequency for the
+// extended operator): dr_m = dReflect(dS_m, dR_m).
+// BornFull::full_fwd (born_prop.cpp:75-113) per (shot, frequency) row:
+//   D  = Down(inj_src)
+//   s1 = Down(LinDown(dS; D)),                          record s1
+//   s2 = Up(D * dr + r * s1),                           record s2
+//        (Born_refl::forward and the down-scattering term share one Up)
+//   U  = Up(r * D)
+//   s3 = Up(LinUp(dS; U)),                              record s3
+static int born_forward(const std::vector<Model*>& fm, Survey* sv, const float2* dS, const float2* dR,
+                        size_t mstride, float2* data, int add, hipStream_t s) {
+  RCCHK(check_survey(sv, fm[0]));
+  std::vector<Model*> uniq;
+  for (Model* m : fm)
+    if (std::find(uniq.begin(), uniq.end(), m) == uniq.end()) uniq.push_back(m);
+  Ctx c;
+  RCCHK(c.init(uniq, sv, s));
+  const size_t dbytes = (size_t)sv->d.ns * sv->d.nr * sv->d.nt * sizeof(float2);
+  if (!add) RCCHK(c.zero(data, dbytes));
+  const int nw = sv->d.nw;
+  // reflectivity perturbation of every frequency's model slice
+  const size_t nslice = mstride ? (size_t)nw : 1;
+  DevBuf drb;
+  RCCHK(drb.alloc(nslice * c.nzx * sizeof(float2), "dr"));
+  for (size_t k = 0; k < nslice; ++k) {
+    Model* m = fm[k];
+    HIPCHK(launch_dreflect_fwd(drb.as<float2>() + k * c.nzx, m->d_drefl, dS + k * mstride, dR + k * mstride, c.nz,
+                               c.nx, false, s),
+           "dreflect");
+  }
+  auto chunks = make_chunks(sv->d.ns, nw, fm, 3 * c.nzx * sizeof(float2), false);
+  const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
+  DevBuf B, S1, T;
+  RCCHK(B.alloc(cap, "background"));
+  RCCHK(S1.alloc(cap, "scattered"));
+  RCCHK(T.alloc(cap, "scattered"));
+  float2 *bg = B.as<float2>(), *s1 = S1.as<float2>(), *tt = T.as<float2>();
+  for (const Rows& r : chunks) {
+    const int n = (int)r.shot.size();
+    const size_t bytes = n * c.nzx * sizeof(float2);
+    RCCHK(c.zero(bg, bytes));
+    RCCHK(c.inject_sources(r, bg, sv->d.ngs, sv->d.offset));
+    RCCHK(c.sweep(r, 0, 0, bg, bg));
+    RCCHK(c.zero(s1, bytes));
+    RCCHK(c.lin(r, +1, 0, 0, bg, dS, mstride, s1));
+    RCCHK(c.sweep(r, 0, 0, s1, s1));
+    RCCHK(c.record(r, data, s1, sv->d.ngr, sv->d.offset));
+    for (int i = 0; i < n; ++i) {
+      const size_t o = (size_t)i * c.nzx;
+      const float2* dri = drb.as<float2>() + (mstride ? (size_t)r.fi[i] * c.nzx : 0);
+      HIPCHK(launch_mul(tt + o, 0, bg + o, 0, dri, 0, (long long)c.nzx, 1, false, false, s), "ic");
+      HIPCHK(launch_mul(tt + o, 0, r.model[i]->d_refl, 0, s1 + o, 0, (long long)c.nzx, 1, false, true, s),
+             "reflect");
+      HIPCHK(launch_mul(bg + o, 0, r.model[i]->d_refl, 0, bg + o, 0, (long long)c.nzx, 1, false, false, s),
+             "reflect");
+    }
+    RCCHK(c.sweep(r, 1, 0, tt, tt));
+    RCCHK(c.record(r, data, tt, sv->d.ngr, sv->d.offset));
+    RCCHK(c.sweep(r, 1, 0, bg, bg));
+    RCCHK(c.zero(s1, bytes));
+    RCCHK(c.lin(r, -1, 0, 0, bg, dS, mstride, s1));
+    RCCHK(c.sweep(r, 1, 0, s1, s1));
+    RCCHK(c.record(r, data, s1, sv->d.ngr, sv->d.offset));
+  }
+  return c.sync();
+}
+
+// BornFull::full_adj (born_prop.cpp:178-232) per row, summed over rows in a
+// fixed order in double precision:
+//   D = Down(inj_src),  R = Up^H(R^H d)
+//   dr    += conj(D) * R                              (Born_refl::adjoint)
+//   Q     = Down^H(R^H d + conj(r) * R)               (both LinDown^H terms)
+//   dS    += LinDown^H(Q; D)
+//   U     = Up(r * D),  dS += LinUp^H(R; U)
+//   (dS, dR) += dReflect^H(dr)
+// ext_rowlast: reproduce ext_born_prop.cpp's serial shot loop, in which each
+// LinDown::adjoint zeroes the last depth row (LinOneWay.cpp:64-65) so only
+// the last shot's LinUp contribution remains there.
+static int born_adjoint(const std::vector<Model*>& fm, Survey* sv, float2* gS, float2* gR, size_t mstride,
+                        const float2* data, int add, int exact, bool ext_rowlast, hipStream_t s) {
+  RCCHK(check_survey(sv, fm[0]));
+  std::vector<Model*> uniq;
+  for (Model* m : fm)
+    if (std::find(uniq.begin(), uniq.end(), m) == uniq.end()) uniq.push_back(m);
+  Ctx c;
+  RCCHK(c.init(uniq, sv, s));
+  const int nw = sv->d.nw, ns = sv->d.ns;
+  const size_t nslice = mstride ? (size_t)nw : 1;
+  // FP64 accumulators per model slice: dr, dS
+  DevBuf accDR, accS;
+  RCCHK(accDR.alloc(nslice * c.nzx * sizeof(double2), "dr accumulator"));
+  RCCHK(accS.alloc(nslice * c.nzx * sizeof(double2), "dS accumulator"));
+  RCCHK(c.zero(accDR.p, accDR.n));
+  RCCHK(c.zero(accS.p, accS.n));
+  // frequency-major rows so that the shots of one frequency are contiguous
+  auto chunks = make_chunks(ns, nw, fm, 5 * c.nzx * sizeof(float2), true);
+  const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
+  DevBuf B, Rb, Qb, CS, CR;
+  RCCHK(B.alloc(cap, "background"));
+  RCCHK(Rb.alloc(cap, "receiver wavefield"));
+  RCCHK(Qb.alloc(cap, "receiver wavefield"));
+  RCCHK(CS.alloc(cap, "dS contributions"));
+  RCCHK(CR.alloc(cap, "dr contributions"));
+  float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>(),
+         *cr = CR.as<float2>();
+  for (const Rows& r : chunks) {
+    const int n = (int)r.shot.size();
+    const size_t bytes = n * c.nzx * sizeof(float2);
+    RCCHK(c.zero(bg, bytes));
+    RCCHK(c.inject_sources(r, bg, sv->d.ngs, sv->d.offset));
+    RCCHK(c.sweep(r, 0, 0, bg, bg));
+    RCCHK(c.zero(R, bytes));
+    RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset));
+    RCCHK(c.sweep(r, 1, 1, R, R));
+    RCCHK(c.zero(Q, bytes));
+    RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset));
+    for (int i = 0; i < n; ++i) {
+      const size_t o = (size_t)i * c.nzx;
+      HIPCHK(launch_mul(cr + o, 0, bg + o, 0, R + o, 0, (long long)c.nzx, 1, true, false, s), "ic adjoint");
+      HIPCHK(launch_mul(Q + o, 0, r.model[i]->d_refl, 0, R + o, 0, (long long)c.nzx, 1, true, true, s),
+             "reflect adjoint");
+    }
+    RCCHK(c.sweep(r, 0, 1, Q, Q));
+    RCCHK(c.zero(cs, bytes));
+    RCCHK(c.lin(r, +1, 1, exact, bg, Q, 0, cs));
+    for (int i = 0; i < n; ++i) {
+      const size_t o = (size_t)i * c.nzx;
+      HIPCHK(launch_mul(bg + o, 0, r.model[i]->d_refl, 0, bg + o, 0, (long long)c.nzx, 1, false, false, s),
+             "reflect");
+    }
+    RCCHK(c.sweep(r, 1, 0, bg, bg));
+    RCCHK(c.lin(r, -1, 1, exact, bg, R, 0, cs));
+    if (ext_rowlast) {
+      for (int i = 0; i < n; ++i)
+        if (r.shot[i] != ns - 1)
+          HIPCHK(hipMemsetAsync(cs + (size_t)i * c.nzx + (size_t)(c.nz - 1) * c.nx, 0, c.nx * sizeof(float2), s),
+                 "memset");
+    }
+    // deterministic reduction: consecutive rows of one frequency (or all rows)
+    int i0 = 0;
+    while (i0 < n) {
+      int i1 = i0 + 1;
+      if (mstride)
+        while (i1 < n && r.fi[i1] == r.fi[i0]) ++i1;
+      else
+        i1 = n;
+      const size_t slot = mstride ? (size_t)r.fi[i0] : 0;
+      HIPCHK(launch_reduce_rows(accS.as<double2>() + slot * c.nzx, cs + (size_t)i0 * c.nzx, (long long)c.nzx,
+                                (long long)c.nzx, i1 - i0, true, s),
+             "reduce");
+      HIPCHK(launch_reduce_rows(accDR.as<double2>() + slot * c.nzx, cr + (size_t)i0 * c.nzx, (long long)c.nzx,
+                                (long long)c.nzx, i1 - i0, true, s),
+             "reduce");
+      i0 = i1;
+    }
+    RCCHK(c.sync());
+  }
+  // dReflect^H of the summed dr, then write the model outputs
+  DevBuf fS, fR, fdr;
+  RCCHK(fS.alloc(c.nzx * sizeof(float2), "tmp"));
+  RCCHK(fR.alloc(c.nzx * sizeof(float2), "tmp"));
+  RCCHK(fdr.alloc(c.nzx * sizeof(float2), "tmp"));
+  for (size_t k = 0; k < nslice; ++k) {
+    Model* m = fm[k];
+    float2* oS = gS + k * mstride;
+    float2* oR = gR + k * mstride;
+    HIPCHK(launch_d2f(fS.as<float2>(), accS.as<double2>() + k * c.nzx, (long long)c.nzx, false, s), "d2f");
+    HIPCHK(launch_d2f(fdr.as<float2>(), accDR.as<double2>() + k * c.nzx, (long long)c.nzx, false, s), "d2f");
+    RCCHK(c.zero(fR.p, c.nzx * sizeof(float2)));
+    HIPCHK(launch_dreflect_adj(fS.as<float2>(), fR.as<float2>(), m->d_drefl, fdr.as<float2>(), c.nz, c.nx, s),
+           "dreflect adjoint");
+    if (!add) {
+      HIPCHK(hipMemcpyAsync(oS, fS.p, c.nzx * sizeof(float2), hipMemcpyDeviceToDevice, s), "copy");
+      HIPCHK(hipMemcpyAsync(oR, fR.p, c.nzx * sizeof(float2), hipMemcpyDeviceToDevice, s), "copy");
+    } else {
+      HIPCHK(launch_add(oS, fS.as<float2>(), (long long)c.nzx, s), "add");
+      HIPCHK(launch_add(oR, fR.as<float2>(), (long long)c.nzx, s), "add");
+    }
+  }
+  return c.sync();
+}
+
+// ------------------------------------------------------------------ BornRefl
+
+// ext_born_refl.cpp:8-56 (forward) and 60-110 (adjoint) on the unpadded
+// grid: source spread ng without offset, receivers without spread.
+static int bornrefl_forward(Model* m, Survey* sv, const float2* image, int nh, float2* data, int add,
+                            hipStream_t s) {
+  RCCHK(check_survey(sv, m));
+  Ctx c;
+  RCCHK(c.init({m}, sv, s));
+  if (!add) RCCHK(c.zero(data, (size_t)sv->d.ns * sv->d.nr * sv->d.nt * sizeof(float2)));
+  std::vector<Model*> fm(sv->d.nw, m);
+  auto chunks = make_chunks(sv->d.ns, sv->d.nw, fm, 2 * c.nzx * sizeof(float2), false);
+  const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
+  DevBuf B, S;
+  RCCHK(B.alloc(cap, "background"));
+  RCCHK(S.alloc(cap, "scattered"));
+  float2 *bg = B.as<float2>(), *sc = S.as<float2>();
+  for (const Rows& r : chunks) {
+    const int n = (int)r.shot.size();
+    const size_t bytes = n * c.nzx * sizeof(float2);
+    RCCHK(c.zero(bg, bytes));
+    RCCHK(c.inject_sources(r, bg, sv->d.ngs, 0));
+    RCCHK(c.sweep(r, 0, 0, bg, bg));
+    for (int i = 0; i < n; ++i) {
+      const size_t o = (size_t)i * c.nzx;
+      if (nh <= 0) {
+        HIPCHK(launch_mul(sc + o, 0, bg + o, 0, image + (size_t)r.fi[i] * c.nzx, 0, (long long)c.nzx, 1, false,
+                          false, s),
+               "ic");
+      } else {
+        HIPCHK(hipMemsetAsync(sc + o, 0, c.nzx * sizeof(float2), s), "memset");
+        HIPCHK(launch_offset_ic_fwd(sc + o, bg + o, image, nh, c.nz, c.nx, s), "offset ic");
+      }
+    }
+    RCCHK(c.sweep(r, 1, 0, sc, sc));
+    RCCHK(c.record(r, data, sc, 0, 0));
+  }
+  return c.sync();
+}
+
+static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const float2* data, int add,
+                            hipStream_t s) {
+  RCCHK(check_survey(sv, m));
+  Ctx c;
+  RCCHK(c.init({m}, sv, s));
+  const int nw = sv->d.nw;
+  const size_t nimg = nh > 0 ? (size_t)nh : (size_t)nw;
+  DevBuf acc;
+  RCCHK(acc.alloc(nimg * c.nzx * sizeof(double2), "image accumulator"));
+  RCCHK(c.zero(acc.p, acc.n));
+  std::vector<Model*> fm(nw, m);
+  auto chunks = make_chunks(sv->d.ns, nw, fm, 3 * c.nzx * sizeof(float2), true);
+  const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
+  DevBuf B, S, C;
+  RCCHK(B.alloc(cap, "background"));
+  RCCHK(S.alloc(cap, "receiver wavefield"));
+  RCCHK(C.alloc(cap, "image contributions"));
+  float2 *bg = B.as<float2>(), *sc = S.as<float2>(), *cb = C.as<float2>();
+  DevBuf pb, ps;
+  for (const Rows& r : chunks) {
+    const int n = (int)r.shot.size();
+    const size_t bytes = n * c.nzx * sizeof(float2);
+    RCCHK(c.zero(bg, bytes));
+    RCCHK(c.inject_sources(r, bg, sv->d.ngs, 0));
+    RCCHK(c.sweep(r, 0, 0, bg, bg));
+    RCCHK(c.zero(sc, bytes));
+    RCCHK(c.inject_receivers(r, sc, data, 0, 0));
+    RCCHK(c.sweep(r, 1, 1, sc, sc));
+    if (nh <= 0) {
+      for (int i = 0; i < n; ++i) {
+        const size_t o = (size_t)i * c.nzx;
+        HIPCHK(launch_mul(cb + o, 0, bg + o, 0, sc + o, 0, (long long)c.nzx, 1, true, false, s), "ic adjoint");
+      }
+      int i0 = 0;
+      while (i0 < n) {
+        int i1 = i0 + 1;
+        while (i1 < n && r.fi[i1] == r.fi[i0]) ++i1;
+        HIPCHK(launch_reduce_rows(acc.as<double2>() + (size_t)r.fi[i0] * c.nzx, cb + (size_t)i0 * c.nzx,
+                                  (long long)c.nzx, (long long)c.nzx, i1 - i0, true, s),
+               "reduce");
+        i0 = i1;
+      }
+    } else {
+      std::vector<const float2*> hb(n), hs(n);
+      for (int i = 0; i < n; ++i) {
+        hb[i] = bg + (size_t)i * c.nzx;
+        hs[i] = sc + (size_t)i * c.nzx;
+      }
+      RCCHK(put(pb, hb, s, "row pointers"));
+      RCCHK(put(ps, hs, s, "row pointers"));
+      HIPCHK(launch_offset_ic_adj(acc.as<double2>(), pb.as<const float2*>(), ps.as<const float2*>(), n, nh, c.nz,
+                                  c.nx, s),
+             "offset ic adjoint");
+    }
+    RCCHK(c.sync());
+  }
+  HIPCHK(launch_d2f(image, acc.as<double2>(), (long long)(nimg * c.nzx), add != 0, s), "d2f");
+  return c.sync();
+}
+
+}  // namespace wemfwi
+
+// ------------------------------------------------------------------ C ABI
+
+using namespace wemfwi;
+
+struct wemfwi_survey {
+  Survey s;
+};
+
+extern "C" {
+
+int wemfwi_survey_create(const wemfwi_survey_desc* d, wemfwi_survey** out) {
+  if (!d || !out) return fail(WEMFWI_ERR_ARG, "survey_create: null argument");
+  *out = nullptr;
+  if (d->ns < 1 || d->nr < 1 || d->nw < 1 || d->nt < 1 || !d->src_iz || !d->src_ix || !d->rec_iz || !d->rec_ix ||
+      !d->freq_hz || !d->bin || !d->wavelet || d->ngs < 0 || d->ngr < 0 || d->ngs > 31 || d->ngr > 31)
+    return fail(WEMFWI_ERR_ARG, "survey_create: invalid survey");
+  for (int i = 0; i < d->nw; ++i)
+    if (d->bin[i] < 0 || d->bin[i] >= d->nt) return fail(WEMFWI_ERR_ARG, "survey_create: frequency bin outside trace");
+  wemfwi_survey* h = new (std::nothrow) wemfwi_survey();
+  if (!h) return fail(WEMFWI_ERR_ALLOC, "survey_create: out of host memory");
+  Survey& s = h->s;
+  s.d = *d;
+  s.src_iz.assign(d->src_iz, d->src_iz + d->ns);
+  s.src_ix.assign(d->src_ix, d->src_ix + d->ns);
+  s.rec_iz.assign(d->rec_iz, d->rec_iz + d->nr);
+  s.rec_ix.assign(d->rec_ix, d->rec_ix + d->nr);
+  s.freq.assign(d->freq_hz, d->freq_hz + d->nw);
+  s.bin.assign(d->bin, d->bin + d->nw);
+  for (int i = 0; i < d->nw; ++i) s.wave.push_back(cf(d->wavelet[i].re, d->wavelet[i].im));
+  s.d.src_iz = s.src_iz.data();
+  s.d.src_ix = s.src_ix.data();
+  s.d.rec_iz = s.rec_iz.data();
+  s.d.rec_ix = s.rec_ix.data();
+  s.d.freq_hz = s.freq.data();
+  s.d.bin = s.bin.data();
+  s.d.wavelet = nullptr;
+  *out = h;
+  return WEMFWI_OK;
+}
+
+int wemfwi_survey_destroy(wemfwi_survey* s) {
+  delete s;
+  return WEMFWI_OK;
+}
+
+int wemfwi_wem_forward(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* data, int add, void* stream) {
+  if (!model || !survey || !data) return fail(WEMFWI_ERR_ARG, "wem_forward: null argument");
+  std::vector<Model*> fm(survey->s.d.nw, &model->m);
+  return wem_forward(fm, &survey->s, (float2*)data, add, (hipStream_t)stream);
+}
+
+int wemfwi_extwem_forward(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* data, int add,
+                          void* stream) {
+  if (!models || !survey || !data) return fail(WEMFWI_ERR_ARG, "extwem_forward: null argument");
+  std::vector<Model*> fm;
+  for (int i = 0; i < survey->s.d.nw; ++i) {
+    if (!models[i]) return fail(WEMFWI_ERR_ARG, "extwem_forward: null model");
+    fm.push_back(&models[i]->m);
+  }
+  return wem_forward(fm, &survey->s, (float2*)data, add, (hipStream_t)stream);
+}
+
+int wemfwi_born_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* dslow,
+                        const wemfwi_complex* dden, wemfwi_complex* data, int add, void* stream) {
+  if (!model || !survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "born_forward: null argument");
+  std::vector<Model*> fm(survey->s.d.nw, &model->m);
+  return born_forward(fm, &survey->s, (const float2*)dslow, (const float2*)dden, 0, (float2*)data, add,
+                      (hipStream_t)stream);
+}
+
+int wemfwi_born_adjoint(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* dslow, wemfwi_complex* dden,
+                        const wemfwi_complex* data, int add, int mode, void* stream) {
+  if (!model || !survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "born_adjoint: null argument");
+  std::vector<Model*> fm(survey->s.d.nw, &model->m);
+  return born_adjoint(fm, &survey->s, (float2*)dslow, (float2*)dden, 0, (const float2*)data, add, mode != 0, false,
+                      (hipStream_t)stream);
+}
+
+int wemfwi_extborn_forward(wemfwi_model* const* models, wemfwi_survey* survey, const wemfwi_complex* dslow,
+                           const wemfwi_complex* dden, wemfwi_complex* data, int add, void* stream) {
+  if (!models || !survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "extborn_forward: null argument");
+  std::vector<Model*> fm;
+  for (int i = 0; i < survey->s.d.nw; ++i) fm.push_back(&models[i]->m);
+  const size_t ms = (size_t)fm[0]->d.nz * fm[0]->d.nx;
+  return born_forward(fm, &survey->s, (const float2*)dslow, (const float2*)dden, ms, (float2*)data, add,
+                      (hipStream_t)stream);
+}
+
+int wemfwi_extborn_adjoint(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* dslow,
+                           wemfwi_complex* dden, const wemfwi_complex* data, int add, int mode, void* stream) {
+  if (!models || !survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "extborn_adjoint: null argument");
+  std::vector<Model*> fm;
+  for (int i = 0; i < survey->s.d.nw; ++i) fm.push_back(&models[i]->m);
+  const size_t ms = (size_t)fm[0]->d.nz * fm[0]->d.nx;
+  return born_adjoint(fm, &survey->s, (float2*)dslow, (float2*)dden, ms, (const float2*)data, add, mode != 0,
+                      mode == 0, (hipStream_t)stream);
+}
+
+int wemfwi_bornrefl_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* image,
+                            wemfwi_complex* data, int add, void* stream) {
+  if (!model || !survey || !image || !data) return fail(WEMFWI_ERR_ARG, "bornrefl_forward: null argument");
+  return bornrefl_forward(&model->m, &survey->s, (const float2*)image, 0, (float2*)data, add, (hipStream_t)stream);
+}
+
+int wemfwi_bornrefl_adjoint(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* image,
+                            const wemfwi_complex* data, int add, void* stream) {
+  if (!model || !survey || !image || !data) return fail(WEMFWI_ERR_ARG, "bornrefl_adjoint: null argument");
+  return bornrefl_adjoint(&model->m, &survey->s, (float2*)image, 0, (const float2*)data, add, (hipStream_t)stream);
+}
+
+int wemfwi_offset_image_forward(wemfwi_model* model, wemfwi_survey* survey, int nh, const wemfwi_complex* image,
+                                wemfwi_complex* data, int add, void* stream) {
+  if (!model || !survey || !image || !data || nh < 1 || nh % 2 == 0)
+    return fail(WEMFWI_ERR_ARG, "offset_image_forward: invalid argument (nh must be odd)");
+  return bornrefl_forward(&model->m, &survey->s, (const float2*)image, nh, (float2*)data, add, (hipStream_t)stream);
+}
+
+int wemfwi_offset_image_adjoint(wemfwi_model* model, wemfwi_survey* survey, int nh, wemfwi_complex* image,
+                                const wemfwi_complex* data, int add, void* stream) {
+  if (!model || !survey || !image || !data || nh < 1 || nh % 2 == 0)
+    return fail(WEMFWI_ERR_ARG, "offset_image_adjoint: invalid argument (nh must be odd)");
+  return bornrefl_adjoint(&model->m, &survey->s, (float2*)image, nh, (const float2*)data, add, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -25,17 +25,29 @@ namespace wemfwi {
 
 // Loads through the global address space (saddr + 32-bit offset addressing
 // instead of 64-bit flat addresses per lane).
-template <typename T>
-__device__ __forceinline__ T ldg(const T* p, int i) {
-  return ((const WEMFWI_GLOBAL T*)p)[i];
-}
+typedef float wemfwi_f2v __attribute__((ext_vector_type(2)));
+
 template <typename T>
 __device__ __forceinline__ T ldg(const T* p, size_t i) {
   return ((const WEMFWI_GLOBAL T*)p)[i];
 }
 template <typename T>
-__device__ __forceinline__ void stg(T* p, int i, T v) {
-  ((WEMFWI_GLOBAL T*)p)[i] = v;
+__device__ __forceinline__ T ldg(const T* p, unsigned i) {
+  return ((const WEMFWI_GLOBAL T*)p)[i];
+}
+__device__ __forceinline__ float2 ldg(const float2* p, size_t i) {
+  const wemfwi_f2v v = ((const WEMFWI_GLOBAL wemfwi_f2v*)p)[i];
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ldg(const float2* p, unsigned i) {
+  const wemfwi_f2v v = ((const WEMFWI_GLOBAL wemfwi_f2v*)p)[i];
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stg(float2* p, unsigned i, float2 v) {
+  wemfwi_f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  ((WEMFWI_GLOBAL wemfwi_f2v*)p)[i] = w;
 }
 
 // Per-model depth tables (device pointers).
@@ -45,12 +57,18 @@ struct DepthTables {
   const signed char* refid;     // [nz][nx] reference selecting x (-1: none)
   const float2* phi;            // [nz][nx] dz*(S - Sref)/sqrt(Sref) of x's reference (0 for PSPI)
   const float* taper;           // [nx]
+  const float2* scoef;          // [ntaylor+1][nz][nx] scatterer coefficients C_it
+  long long scoef_stride;       // nz*nx
 };
 
 struct StepGeom {
   int nx;       // propagation width
   int tap;      // FFT zero padding per side
   int ssf;      // 1: split-step correction on
+  int full;     // 1: tap == 0 and nx == N (every lane sample is a model sample)
+  float dz;     // depth step
+  float dk;     // wavenumber step of the phase-shift / scatterer grid
+  int ntaylor;  // Taylor order of the scatterer
 };
 
 // sin and cos of x for |x| < 1e5: Cody-Waite reduction by a three-part pi/2
@@ -80,19 +98,32 @@ __device__ __forceinline__ void sincos_cw(float x, float* s, float* c) {
 
 __device__ __forceinline__ bool in_row(int x, int nx) { return (unsigned)x < (unsigned)nx; }
 
-// taper[x] * exp(-i w phi[x]) for the sample this lane holds, 0 outside [0,nx)
-__device__ __forceinline__ float2 ss_taper(const DepthTables& dt, const StepGeom& g, int d, int x,
-                                           float w) {
-  const bool ok = in_row(x, g.nx);
-  const int xc = ok ? x : 0;
-  const float tp = ok ? ldg(dt.taper, xc) : 0.f;
-  if (!g.ssf) return make_float2(tp, 0.f);
-  const float2 ph = ldg(dt.phi, (size_t)d * g.nx + xc);
-  // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr))
-  float sn, cs;
-  sincos_cw(w * ph.x, &sn, &cs);
-  const float a = tp * exp2f(w * ph.y * 1.4426950408889634f);
-  return make_float2(a * cs, -a * sn);
+// taper[x] * exp(-i w phi[x]) of the E samples of this lane (0 outside [0,nx))
+template <int N>
+__device__ __forceinline__ void ss_taper_row(float2 (&f)[N / 64], const DepthTables& dt,
+                                             const StepGeom& g, int d, float w, int t) {
+  constexpr int E = N / 64;
+  const float* tp = dt.taper + (t - g.tap);
+  const float2* ph = dt.phi + (size_t)d * g.nx + (t - g.tap);
+  float tv[E];
+  float2 pv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    tv[e] = 0.f;
+    pv[e] = make_float2(0.f, 0.f);
+    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) {
+      tv[e] = ldg(tp, (unsigned)(64 * e));
+      if (g.ssf) pv[e] = ldg(ph, (unsigned)(64 * e));
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr))
+    float sn, cs;
+    sincos_cw(w * pv[e].x, &sn, &cs);
+    const float a = tv[e] * exp2f(w * pv[e].y * 1.4426950408889634f);
+    f[e] = make_float2(a * cs, -a * sn);
+  }
 }
 
 // reference ids of the E samples of this lane, packed 4 per 32-bit register
@@ -100,20 +131,27 @@ template <int N>
 __device__ __forceinline__ void load_refid(const DepthTables& dt, const StepGeom& g, int d, int t,
                                            unsigned (&rid)[(N / 64 + 3) / 4]) {
   constexpr int E = N / 64;
-  const signed char* row = dt.refid + (size_t)d * g.nx;
+  const signed char* row = dt.refid + (size_t)d * g.nx + (t - g.tap);
 #pragma unroll
   for (int i = 0; i < (E + 3) / 4; ++i) rid[i] = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int x = t + 64 * e - g.tap;
-    const bool ok = in_row(x, g.nx);
-    const int v = ok ? (int)ldg(row, ok ? x : 0) : -1;
+    int v = -1;
+    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) v = (int)ldg(row, (unsigned)(64 * e));
     rid[e / 4] |= (unsigned)(v & 0xff) << (8 * (e % 4));
   }
 }
 
 __device__ __forceinline__ int rid_at(const unsigned* rid, int e) {
   return (int)(signed char)((rid[e / 4] >> (8 * (e % 4))) & 0xff);
+}
+
+// phase-table entry of wavenumber index k = t + 64 e (the table holds
+// k <= N/2; entry N-k for the upper half)
+template <int N>
+__device__ __forceinline__ float2 ptab_at(const float2* P, int t, int e) {
+  if (e < N / 128) return ldg(P, (unsigned)(t + 64 * e));
+  return ldg(P, (unsigned)(N - t - 64 * e));  // 1 <= N - t - 64 e <= N/2
 }
 
 // out = P_d(in)   (in and out in padded strided layout; out is 0 outside [0,nx))
@@ -126,13 +164,13 @@ __device__ __forceinline__ void step_forward(float2 (&st)[N / 64], int d, float 
   fft_row<N, false>(st, buf, tw, t);
   unsigned rid[(E + 3) / 4];
   load_refid<N>(dt, g, d, t, rid);
-  const int s0 = ldg(dt.slot_begin, d), s1 = ldg(dt.slot_begin, d + 1);
+  const int s0 = dt.slot_begin[d], s1 = dt.slot_begin[d + 1];
   if (s1 - s0 == 1) {
     // one active reference: transform in place
-    const int r = ldg(dt.slot_ref, s0);
-    const float2* P = ptab + (size_t)s0 * N;
+    const int r = dt.slot_ref[s0];
+    const float2* P = ptab + (size_t)s0 * (N / 2 + 1);
 #pragma unroll
-    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], ldg(P, t + 64 * e));
+    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], ptab_at<N>(P, t, e));
     fft_row<N, true>(st, buf, tw, t);
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -145,19 +183,21 @@ __device__ __forceinline__ void step_forward(float2 (&st)[N / 64], int d, float 
       st[e] = make_float2(0.f, 0.f);
     }
     for (int s = s0; s < s1; ++s) {
-      const int r = ldg(dt.slot_ref, s);
-      const float2* P = ptab + (size_t)s * N;
+      const int r = dt.slot_ref[s];
+      const float2* P = ptab + (size_t)s * (N / 2 + 1);
       float2 Y[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) Y[e] = cmul(X[e], ldg(P, t + 64 * e));
+      for (int e = 0; e < E; ++e) Y[e] = cmul(X[e], ptab_at<N>(P, t, e));
       fft_row<N, true>(Y, buf, tw, t);
 #pragma unroll
       for (int e = 0; e < E; ++e)
         if (rid_at(rid, e) == r) st[e] = Y[e];
     }
   }
+  float2 f[E];
+  ss_taper_row<N>(f, dt, g, d, w, t);
 #pragma unroll
-  for (int e = 0; e < E; ++e) st[e] = cmul(st[e], ss_taper(dt, g, d, t + 64 * e - g.tap, w));
+  for (int e = 0; e < E; ++e) st[e] = cmul(st[e], f[e]);
 }
 
 // out = P_d^H(in)
@@ -170,31 +210,35 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[N / 64], int d, float 
   unsigned rid[(E + 3) / 4];
   load_refid<N>(dt, g, d, t, rid);
   // SplitStep adjoint factor exp(+i w conj(phi)) = conj(exp(-i w phi)), taper self-adjoint
+  {
+    float2 f[E];
+    ss_taper_row<N>(f, dt, g, d, w, t);
 #pragma unroll
-  for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], ss_taper(dt, g, d, t + 64 * e - g.tap, w));
-  const int s0 = ldg(dt.slot_begin, d), s1 = ldg(dt.slot_begin, d + 1);
+    for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], f[e]);
+  }
+  const int s0 = dt.slot_begin[d], s1 = dt.slot_begin[d + 1];
   if (s1 - s0 == 1) {
-    const int r = ldg(dt.slot_ref, s0);
-    const float2* P = ptab + (size_t)s0 * N;
+    const int r = dt.slot_ref[s0];
+    const float2* P = ptab + (size_t)s0 * (N / 2 + 1);
 #pragma unroll
     for (int e = 0; e < E; ++e)
       if (rid_at(rid, e) != r) st[e] = make_float2(0.f, 0.f);
     fft_row<N, false>(st, buf, tw, t);
 #pragma unroll
-    for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], ldg(P, t + 64 * e));
+    for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], ptab_at<N>(P, t, e));
   } else {
     float2 acc[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = make_float2(0.f, 0.f);
     for (int s = s0; s < s1; ++s) {
-      const int r = ldg(dt.slot_ref, s);
-      const float2* P = ptab + (size_t)s * N;
+      const int r = dt.slot_ref[s];
+      const float2* P = ptab + (size_t)s * (N / 2 + 1);
       float2 Z[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) Z[e] = (rid_at(rid, e) == r) ? st[e] : make_float2(0.f, 0.f);
       fft_row<N, false>(Z, buf, tw, t);
 #pragma unroll
-      for (int e = 0; e < E; ++e) acc[e] = cadd(acc[e], cmulc(Z[e], ldg(P, t + 64 * e)));
+      for (int e = 0; e < E; ++e) acc[e] = cadd(acc[e], cmulc(Z[e], ptab_at<N>(P, t, e)));
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = acc[e];
@@ -202,8 +246,71 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[N / 64], int d, float 
   fft_row<N, true>(st, buf, tw, t);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int x = t + 64 * e - g.tap;
-    if (!in_row(x, g.nx)) st[e] = make_float2(0.f, 0.f);
+    if (!g.full && !in_row(t + 64 * e - g.tap, g.nx)) st[e] = make_float2(0.f, 0.f);
+  }
+}
+
+// Scatterer (Scatter.cpp:35-111) on one row in padded strided layout:
+//   forward:        y = (-i w dz) sum_it C_it * (M_it x)
+//   adjoint compat: y = (+i w dz) sum_it conj(C_it) * (M_it d)   (Scatter.cpp:52-66)
+//   adjoint exact:  y = (+i w dz) sum_it M_it (conj(C_it) * d)
+// with C_it = coef[it] (S - eps Re S)^(-1/2-it) and M_it = Pad^H F^H (k/w)^(2 it)
+// F Pad (M_0 = identity on the row).
+template <int N, bool ADJ, bool EXACT>
+__device__ __forceinline__ void scatter_row(float2 (&st)[N / 64], int d, float w, const DepthTables& dt,
+                                            const StepGeom& g, float2* buf, const float2* tw, int t) {
+  constexpr int E = N / 64;
+  const float2* C0 = dt.scoef + (size_t)d * g.nx + (t - g.tap);
+  float2 acc[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    float2 c = make_float2(0.f, 0.f);
+    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) c = ldg(C0, (unsigned)(64 * e));
+    acc[e] = ADJ ? cmulc(st[e], c) : cmul(st[e], c);
+  }
+  // (k/w)^2 / N^(1/it) pieces: q[e] = kk^2 / w^2
+  const float iw2 = 1.f / (w * w);
+  for (int it = 1; it <= g.ntaylor; ++it) {
+    const float2* Ci = C0 + (size_t)it * dt.scoef_stride;
+    float2 Y[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (ADJ && EXACT) {
+        float2 c = make_float2(0.f, 0.f);
+        if (g.full || in_row(t + 64 * e - g.tap, g.nx)) c = ldg(Ci, (unsigned)(64 * e));
+        Y[e] = cmulc(st[e], c);
+      } else {
+        Y[e] = st[e];
+      }
+    }
+    fft_row<N, false>(Y, buf, tw, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + 64 * e;
+      const int kh = k < N / 2 ? k : ((N - k) < N / 2 ? N - k : 0);
+      const float kk = (float)kh * g.dk;
+      const float q = kk * kk * iw2;
+      float qi = q;
+      for (int j = 1; j < it; ++j) qi *= q;
+      Y[e] = cscale(Y[e], qi * (1.f / (float)N));
+    }
+    fft_row<N, true>(Y, buf, tw, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (!(g.full || in_row(t + 64 * e - g.tap, g.nx))) continue;
+      if (ADJ && EXACT) {
+        acc[e] = cadd(acc[e], Y[e]);
+      } else {
+        const float2 c = ldg(Ci, (unsigned)(64 * e));
+        acc[e] = cadd(acc[e], ADJ ? cmulc(Y[e], c) : cmul(Y[e], c));
+      }
+    }
+  }
+  const float s = w * g.dz;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    // forward: (-i s) acc ; adjoint: (+i s) acc
+    st[e] = ADJ ? make_float2(-acc[e].y * s, acc[e].x * s) : make_float2(acc[e].y * s, -acc[e].x * s);
   }
 }
 

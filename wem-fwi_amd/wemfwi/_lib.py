@@ -53,6 +53,22 @@ SIGNATURES = {
     "wemfwi_model_destroy": (_i, [_vp]),
     "wemfwi_oneway": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _i, _vp]),
     "wemfwi_onestep": (_i, [_vp, _i, _i, _fp, _ip, _vp, _vp, _i, _vp]),
+    "wemfwi_oneway_recursive": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _vp]),
+    "wemfwi_model_set_freqs": (_i, [_vp, _i, _fp, _vp]),
+    "wemfwi_survey_create": (_i, [_vp, C.POINTER(_vp)]),
+    "wemfwi_survey_destroy": (_i, [_vp]),
+    "wemfwi_wem_forward": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "wemfwi_born_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_born_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "wemfwi_extwem_forward": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "wemfwi_extborn_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_extborn_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "wemfwi_bornrefl_forward": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_bornrefl_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_offset_image_forward": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
+    "wemfwi_offset_image_adjoint": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
+    "wemfwi_timing": (_i, [_i]),
+    "wemfwi_timing_get": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
 }
 
 

@@ -97,6 +97,29 @@ class Model:
                                 int(bool(add)), _stream_ptr(stream)))
         return out
 
+    def oneway_recursive(self, up: bool, adjoint: bool, freq_hz: Sequence[float],
+                         src: torch.Tensor, out: torch.Tensor,
+                         stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+        """out = (I - L)^-1 src: the in-place recursion on a copy of src."""
+        _dev_c64(src, "src")
+        _dev_c64(out, "out")
+        nrows = src.shape[0] if src.dim() == 3 else 1
+        if tuple(src.shape[-2:]) != (self.nz, self.nx) or out.shape != src.shape:
+            raise ValueError("oneway_recursive: wavefield shape does not match the model")
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(freq_hz, dtype=np.float32), (nrows,)))
+        check(lib.wemfwi_oneway_recursive(self._h, int(up), int(adjoint), nrows,
+                                          f.ctypes.data_as(C.POINTER(C.c_float)),
+                                          C.c_void_p(src.data_ptr()), C.c_void_p(out.data_ptr()),
+                                          _stream_ptr(stream)))
+        return out
+
+    def set_freqs(self, freq_hz: Sequence[float], stream: Optional[torch.cuda.Stream] = None):
+        """Builds the phase tables of a frequency set (Phshift::setFreq for all
+        frequencies at once)."""
+        f = np.ascontiguousarray(np.asarray(freq_hz, dtype=np.float32).ravel())
+        check(lib.wemfwi_model_set_freqs(self._h, len(f), f.ctypes.data_as(C.POINTER(C.c_float)),
+                                         _stream_ptr(stream)))
+
     def onestep(self, adjoint: bool, freq_hz, depth, inp: torch.Tensor, out: torch.Tensor,
                 add: bool = False, stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
         """SSF/PSPI step on rows [nrows][nx] at depth[i], freq_hz[i]."""
@@ -113,3 +136,16 @@ class Model:
                                  C.c_void_p(inp.data_ptr()), C.c_void_p(out.data_ptr()),
                                  int(bool(add)), _stream_ptr(stream)))
         return out
+
+
+def timing(enable: bool) -> None:
+    """Enables (and clears) or disables HIP-event kernel timing in the library."""
+    check(lib.wemfwi_timing(int(enable)))
+
+
+def kernel_time(name: str):
+    """(total milliseconds, launch count) of a library kernel since timing(True)."""
+    ms = C.c_double()
+    n = C.c_long()
+    check(lib.wemfwi_timing_get(name.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value
