@@ -103,15 +103,48 @@ def cpu_baseline(cfg, slow, freqs, src_ix, nrows_sample):
                       f"{dt:.2f} s"}
 
 
+def measured_traffic(config: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this configuration (mean over its launches, each
+    counter scaled by its copy-kernel calibration factor), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{config}_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for ln in f:
+            if not ln.startswith("bench-traffic:"):
+                continue
+            kv = dict(tok.split("=", 1) for tok in ln.split()[1:])
+            if kv.get("kernel") != kernel or kv.get("config") != config:
+                continue
+            fetch = [float(x) for x in kv["fetch_kib"].split(",")]
+            write = [float(x) for x in kv["write_kib"].split(",")]
+            ff, wf = float(kv["fetch_factor"]), float(kv["write_factor"])
+            per = [1024.0 * (ff * a + wf * b) for a, b in zip(fetch, write)]
+            return sum(per) / len(per), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="pspi1024", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-sample", type=int, default=1024,
+    ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="rows of the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the (shot, frequency) rows of this path are "
               "independent and would be split across GPUs with one final image reduction; "
@@ -166,6 +199,7 @@ def main():
         print("non-finite wavefield", file=sys.stderr)
         sys.exit(1)
 
+    traffic, traffic_src = measured_traffic(args.config, "oneway_kernel")
     line = {
         "metric": "Gupdates/s (x.z.w wavefield updates, Down forward + adjoint)",
         "value": round(value, 3),
@@ -184,7 +218,10 @@ def main():
                                f"ns={ns}",
                    "rows": nrows, "updates_per_step": 2 * upd_sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic),
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes": BYTES_PER_UPDATE * upd_sweep,
                      "kernel": "oneway_kernel", "kernel_ms": round(avg, 3),
                      "phase_table_ms": round(t_ms / max(t_n, 1), 3)},
     }

@@ -9,24 +9,20 @@ namespace wemfwi {
 
 constexpr int kLinWaves = 4;
 
-template <int N>
-__device__ __forceinline__ void lrow(float2 (&v)[N / 64], const float2* __restrict__ p, const StepGeom& g,
-                                     int t) {
-  const float2* b = p + (t - g.tap);
+// row loads/accumulates of the general variant (padding allowed)
+template <class V>
+__device__ __forceinline__ void lrow(float2 (&v)[V::E], const float2* __restrict__ p, const StepGeom& g, int t) {
 #pragma unroll
-  for (int e = 0; e < N / 64; ++e) {
-    v[e] = make_float2(0.f, 0.f);
-    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) v[e] = ldg(b, (unsigned)(64 * e));
-  }
+  for (int e = 0; e < V::E; ++e) v[e] = row_sample<V>(p, g, t, e);
 }
 
-template <int N>
-__device__ __forceinline__ void acc_row(float2* __restrict__ p, const float2 (&v)[N / 64], const StepGeom& g,
+template <class V>
+__device__ __forceinline__ void acc_row(float2* __restrict__ p, const float2 (&v)[V::E], const StepGeom& g,
                                         int t) {
   float2* b = p + (t - g.tap);
 #pragma unroll
-  for (int e = 0; e < N / 64; ++e)
-    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) {
+  for (int e = 0; e < V::E; ++e)
+    if (in_row(t + 64 * e - g.tap, g.nx)) {
       const float2 o = ldg(b, (unsigned)(64 * e));
       stg(b, (unsigned)(64 * e), cadd(o, v[e]));
     }
@@ -39,13 +35,15 @@ __device__ __forceinline__ void acc_row(float2* __restrict__ p, const float2 (&v
 //   adjoint: out[d]     += conj(bg[d]) * Sc^H_d P^H_d (in[d+dir])
 template <int N, bool ADJ, bool EXACT>
 __global__ void __launch_bounds__(64 * kLinWaves)
+    __attribute__((amdgpu_waves_per_eu(N <= 256 ? 4 : 2, 8)))
     lin_kernel(const LinRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models, StepGeom g,
                int nz, int dir, const float2* __restrict__ twg) {
+  using V = Var<N, false, true>;
   constexpr int E = N / 64;
   __shared__ float2 tw[plan_tw_size<N>()];
   __shared__ float2 bufs[kLinWaves][LdsSize<N>::value];
   const int t = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int item = blockIdx.x * kLinWaves + wv;
@@ -54,29 +52,29 @@ __global__ void __launch_bounds__(64 * kLinWaves)
   const int row = item / nd;
   const int j = item - row * nd;
   const int d = dir > 0 ? j : j + 1;
-  const LinRow R = rows[row];
-  const DepthTables dt = models[R.model];
+  const LinRow R = ldu_struct(rows, row);
+  const DepthTables dt = ldu_struct(models, R.model);
   float2* buf = bufs[wv];
   const size_t nx = g.nx;
   float2 st[E];
   if constexpr (!ADJ) {
     float2 b[E];
-    lrow<N>(st, R.m + d * nx, g, t);
-    lrow<N>(b, R.bg + d * nx, g, t);
+    lrow<V>(st, R.m + d * nx, g, t);
+    lrow<V>(b, R.bg + d * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmul(b[e], st[e]);
-    scatter_row<N, false, false>(st, d, R.w, dt, g, buf, tw, t);
-    step_forward<N>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
-    acc_row<N>(R.out + (d + dir) * nx, st, g, t);
+    scatter_row<V, false, false>(st, d, R.w, dt, g, buf, tw, t);
+    step_forward<V>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
+    acc_row<V>(R.out + (d + dir) * nx, st, g, t);
   } else {
-    lrow<N>(st, R.m + (d + dir) * nx, g, t);
-    step_adjoint<N>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
-    scatter_row<N, true, EXACT>(st, d, R.w, dt, g, buf, tw, t);
+    lrow<V>(st, R.m + (d + dir) * nx, g, t);
+    step_adjoint<V>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
+    scatter_row<V, true, EXACT>(st, d, R.w, dt, g, buf, tw, t);
     float2 b[E];
-    lrow<N>(b, R.bg + d * nx, g, t);
+    lrow<V>(b, R.bg + d * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], b[e]);
-    acc_row<N>(R.out + d * nx, st, g, t);
+    acc_row<V>(R.out + d * nx, st, g, t);
   }
 }
 
@@ -85,27 +83,28 @@ template <int N, bool ADJ, bool EXACT>
 __global__ void __launch_bounds__(64 * kLinWaves)
     scatter_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                    StepGeom g, const float2* __restrict__ twg) {
+  using V = Var<N, false, true>;
   constexpr int E = N / 64;
   __shared__ float2 tw[plan_tw_size<N>()];
   __shared__ float2 bufs[kLinWaves][LdsSize<N>::value];
   const int t = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kLinWaves + wv;
   if (row >= nrows) return;
-  const StepRow R = rows[row];
-  const DepthTables dt = models[R.model];
+  const StepRow R = ldu_struct(rows, row);
+  const DepthTables dt = ldu_struct(models, R.model);
   float2 st[E];
-  lrow<N>(st, R.in, g, t);
-  scatter_row<N, ADJ, EXACT>(st, R.d, R.w, dt, g, bufs[wv], tw, t);
+  lrow<V>(st, R.in, g, t);
+  scatter_row<V, ADJ, EXACT>(st, R.d, R.w, dt, g, bufs[wv], tw, t);
   if (R.add) {
-    acc_row<N>(R.out, st, g, t);
+    acc_row<V>(R.out, st, g, t);
   } else {
     float2* b = R.out + (t - g.tap);
 #pragma unroll
     for (int e = 0; e < E; ++e)
-      if (g.full || in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), st[e]);
+      if (in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), st[e]);
   }
 }
 

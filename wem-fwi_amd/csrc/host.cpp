@@ -190,7 +190,7 @@ int twiddles(int n, const float2** out) {
 // ------------------------------------------------------------------ model
 
 Model::~Model() {
-  void* ptrs[] = {d_slot_begin, d_slot_ref, d_refid, d_phi, d_taper, d_slot_sref, d_tables,
+  void* ptrs[] = {d_slot_begin, d_slot_ref, d_rid_lane, d_one_ref, d_phi_lane, d_taper_lane, d_slot_sref, d_tables,
                   d_scoef, d_refl, d_drefl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -267,9 +267,33 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
 
   if ((rc = dev_upload(&d_slot_begin, slot_begin.data(), slot_begin.size(), "slot_begin"))) return rc;
   if ((rc = dev_upload(&d_slot_ref, slot_ref.data(), slot_ref.size(), "slot_ref"))) return rc;
-  if ((rc = dev_upload(&d_refid, rid8.data(), rid8.size(), "refid"))) return rc;
-  if ((rc = dev_upload(&d_phi, phi.data(), phi.size(), "phi"))) return rc;
-  if ((rc = dev_upload(&d_taper, taper.data(), taper.size(), "taper"))) return rc;
+  // lane layout of the FFT row: entry t*E + e is padded sample t + 64 e
+  const int E = N / 64;
+  std::vector<signed char> rid_lane((size_t)d.nz * N, (signed char)-1);
+  std::vector<float2> phi_lane((size_t)d.nz * N, make_float2(0.f, 0.f));
+  std::vector<float> taper_lane(N, 0.f);
+  std::vector<int> one_ref(d.nz, 0);
+  for (int tl = 0; tl < 64; ++tl)
+    for (int e = 0; e < E; ++e) {
+      const int x = tl + 64 * e - d.tap;
+      if (x < 0 || x >= d.nx) continue;
+      taper_lane[tl * E + e] = taper[x];
+      for (int iz = 0; iz < d.nz; ++iz) {
+        rid_lane[(size_t)iz * N + tl * E + e] = rid8[(size_t)iz * d.nx + x];
+        phi_lane[(size_t)iz * N + tl * E + e] = phi[(size_t)iz * d.nx + x];
+      }
+    }
+  for (int iz = 0; iz < d.nz; ++iz) {
+    if (slot_begin[iz + 1] - slot_begin[iz] != 1) continue;
+    const int r = slot_ref[slot_begin[iz]];
+    bool all = true;
+    for (int x = 0; x < d.nx && all; ++x) all = refid[(size_t)iz * d.nx + x] == r;
+    one_ref[iz] = all ? 1 : 0;
+  }
+  if ((rc = dev_upload(&d_rid_lane, rid_lane.data(), rid_lane.size(), "refid"))) return rc;
+  if ((rc = dev_upload(&d_one_ref, one_ref.data(), one_ref.size(), "one_ref"))) return rc;
+  if ((rc = dev_upload(&d_phi_lane, phi_lane.data(), phi_lane.size(), "phi"))) return rc;
+  if ((rc = dev_upload(&d_taper_lane, taper_lane.data(), taper_lane.size(), "taper"))) return rc;
   if ((rc = dev_upload(&d_slot_sref, ssref.data(), ssref.size(), "slot_sref"))) return rc;
   // Scatterer coefficients coef[it] (S - eps Re S)^(-1/2-it) (Scatter.cpp:82)
   static const double coef[5] = {1., 1. / 2., 3. / 8., 5. / 16., 35. / 128.};
@@ -303,9 +327,10 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   DepthTables& t = h_tables;
   t.slot_begin = d_slot_begin;
   t.slot_ref = d_slot_ref;
-  t.refid = d_refid;
-  t.phi = d_phi;
-  t.taper = d_taper;
+  t.rid_lane = d_rid_lane;
+  t.one_ref = d_one_ref;
+  t.phi_lane = d_phi_lane;
+  t.taper_lane = d_taper_lane;
   t.scoef = d_scoef;
   t.scoef_stride = (long long)nzx;
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
@@ -322,6 +347,7 @@ StepGeom Model::geom() const {
   g.dz = d.dz;
   g.dk = (float)(2.0 * M_PI / ((double)d.dx * (double)(N - 1)));
   g.ntaylor = d.ntaylor;
+  g.taper = d.taper > 0 ? 1 : 0;
   return g;
 }
 

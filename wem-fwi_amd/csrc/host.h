@@ -35,9 +35,10 @@ struct Model {
 
   int* d_slot_begin = nullptr;
   int* d_slot_ref = nullptr;
-  signed char* d_refid = nullptr;
-  float2* d_phi = nullptr;
-  float* d_taper = nullptr;
+  signed char* d_rid_lane = nullptr;     // [nz][N] lane layout
+  int* d_one_ref = nullptr;             // [nz]
+  float2* d_phi_lane = nullptr;         // [nz][N] lane layout
+  float* d_taper_lane = nullptr;        // [N] lane layout
   double2* d_slot_sref = nullptr;
   float2* d_scoef = nullptr;   // [ntaylor+1][nz][nx] scatterer coefficients
   float2* d_refl = nullptr;    // [nz][nx] reflection coefficient (Reflect.h:19-26)

@@ -46,27 +46,34 @@ __global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __re
 
 // ------------------------------------------------------------------ rows
 
-template <int N>
-__device__ __forceinline__ void load_row(float2 (&v)[N / 64], const float2* __restrict__ p,
-                                         const StepGeom& g, int t) {
-  const float2* b = p + (t - g.tap);
+template <class V>
+__device__ __forceinline__ void load_row(float2 (&v)[V::E], const float2* __restrict__ p, const StepGeom& g,
+                                         int t) {
+  load_samples<V>(v, p, g, t);
+}
+
+template <class V>
+__device__ __forceinline__ void store_row(float2* __restrict__ p, const float2 (&v)[V::E], const StepGeom& g,
+                                          int t) {
+  if constexpr (V::FULL) {
+    float2* b = p + t;
 #pragma unroll
-  for (int e = 0; e < N / 64; ++e) {
-    v[e] = make_float2(0.f, 0.f);
-    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) v[e] = ldg(b, (unsigned)(64 * e));
+    for (int e = 0; e < V::E; ++e) stg(b, (unsigned)(64 * e), v[e]);
+  } else {
+    float2* b = p + (t - g.tap);
+#pragma unroll
+    for (int e = 0; e < V::E; ++e)
+      if (in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), v[e]);
   }
 }
 
+// Waves (rows) per workgroup: the twiddle table in LDS is shared by the
+// waves of a workgroup; 8 waves keep 4 workgroups' worth of LDS (16 waves)
+// per CU for N = 1024.
 template <int N>
-__device__ __forceinline__ void store_row(float2* __restrict__ p, const float2 (&v)[N / 64],
-                                          const StepGeom& g, int t) {
-  float2* b = p + (t - g.tap);
-#pragma unroll
-  for (int e = 0; e < N / 64; ++e)
-    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), v[e]);
-}
-
-constexpr int kWavesPerBlock = 4;
+struct WavesPerBlock {
+  static constexpr int value = 4;
+};
 
 // In-place or out-of-place Down/Up recursion (OneWay.cpp:32-109) of one
 // (shot, frequency) row per wavefront.  step = +1 runs iz = 0..nz-2 (Down
@@ -75,72 +82,78 @@ constexpr int kWavesPerBlock = 4;
 // Waves per SIMD the register allocator targets for an FFT length.
 template <int N>
 struct Occupancy {
-  static constexpr int value = N <= 256 ? 4 : (N <= 1024 ? 2 : 1);
+  static constexpr int value = N <= 512 ? 4 : (N <= 1024 ? 3 : 2);
 };
 
-template <int N, bool ADJ>
-__global__ void __launch_bounds__(64 * kWavesPerBlock)
-    __attribute__((amdgpu_waves_per_eu(Occupancy<N>::value, 8))) oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
+template <class V, bool ADJ>
+__global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
+    __attribute__((amdgpu_waves_per_eu(Occupancy<V::n>::value, 8)))
+    oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                   StepGeom g, int nz, int step, const float2* __restrict__ twg) {
-  constexpr int E = N / 64;
+  constexpr int N = V::n;
+  constexpr int E = V::E;
+  constexpr int kWavesPerBlock = WavesPerBlock<N>::value;
   __shared__ float2 tw[plan_tw_size<N>()];
   __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
   const int t = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kWavesPerBlock + wv;
   if (row >= nrows) return;
-  const SweepRow R = rows[row];
-  const DepthTables dt = models[R.model];
+  const SweepRow R = ldu_struct(rows, row);
+  const DepthTables dt = ldu_struct(models, R.model);
   float2* buf = bufs[wv];
   const int nx = g.nx;
   const bool recursive = R.recursive != 0;
   const int first = step > 0 ? 0 : nz - 1;
   float2 st[E];
-  load_row<N>(st, R.in + (size_t)first * nx, g, t);
-  if (R.in != R.out) store_row<N>(R.out + (size_t)first * nx, st, g, t);
+  load_row<V>(st, R.in + (size_t)first * nx, g, t);
+  if (R.in != R.out) store_row<V>(R.out + (size_t)first * nx, st, g, t);
   for (int k = 0; k < nz - 1; ++k) {
     const int cur = first + k * step;
     const int nxt = cur + step;
-    if (!recursive && k > 0) load_row<N>(st, R.in + (size_t)cur * nx, g, t);
-    if constexpr (ADJ) step_adjoint<N>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t);
-    else step_forward<N>(st, cur, R.w, dt, g, R.ptab, buf, tw, t);
+    if (!recursive && k > 0) load_row<V>(st, R.in + (size_t)cur * nx, g, t);
+    if constexpr (ADJ) step_adjoint<V>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t);
+    else step_forward<V>(st, cur, R.w, dt, g, R.ptab, buf, tw, t);
     float2 src[E];
-    load_row<N>(src, R.in + (size_t)nxt * nx, g, t);
+    load_row<V>(src, R.in + (size_t)nxt * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
-    store_row<N>(R.out + (size_t)nxt * nx, st, g, t);
+    store_row<V>(R.out + (size_t)nxt * nx, st, g, t);
   }
 }
 
 // Independent single depth steps (SSF::forward/adjoint as a unit operator).
-template <int N, bool ADJ>
-__global__ void __launch_bounds__(64 * kWavesPerBlock)
-    __attribute__((amdgpu_waves_per_eu(Occupancy<N>::value, 8))) onestep_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
+template <class V, bool ADJ>
+__global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
+    __attribute__((amdgpu_waves_per_eu(Occupancy<V::n>::value, 8)))
+    onestep_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                    StepGeom g, const float2* __restrict__ twg) {
-  constexpr int E = N / 64;
+  constexpr int N = V::n;
+  constexpr int E = V::E;
+  constexpr int kWavesPerBlock = WavesPerBlock<N>::value;
   __shared__ float2 tw[plan_tw_size<N>()];
   __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
   const int t = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kWavesPerBlock + wv;
   if (row >= nrows) return;
-  const StepRow R = rows[row];
-  const DepthTables dt = models[R.model];
+  const StepRow R = ldu_struct(rows, row);
+  const DepthTables dt = ldu_struct(models, R.model);
   float2 st[E];
-  load_row<N>(st, R.in, g, t);
-  if constexpr (ADJ) step_adjoint<N>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
-  else step_forward<N>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
+  load_row<V>(st, R.in, g, t);
+  if constexpr (ADJ) step_adjoint<V>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
+  else step_forward<V>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
   if (R.add) {
     float2 o[E];
-    load_row<N>(o, R.out, g, t);
+    load_row<V>(o, R.out, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cadd(st[e], o[e]);
   }
-  store_row<N>(R.out, st, g, t);
+  store_row<V>(R.out, st, g, t);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -190,16 +203,25 @@ hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const d
   return hipGetLastError();
 }
 
-template <int N>
-static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTables* models,
-                                  StepGeom g, int nz, int step, bool adj, const float2* tw,
-                                  hipStream_t s) {
+template <class V>
+static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                                  int step, bool adj, const float2* tw, hipStream_t s) {
+  constexpr int kWavesPerBlock = WavesPerBlock<V::n>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   if (adj)
-    hipLaunchKernelGGL((oneway_kernel<N, true>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, true>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
   else
-    hipLaunchKernelGGL((oneway_kernel<N, false>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, false>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
   return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                                  int step, bool adj, const float2* tw, hipStream_t s) {
+  if (g.full && g.ssf) return launch_oneway_v<Var<N, true, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  if (g.full) return launch_oneway_v<Var<N, true, false>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  if (g.ssf) return launch_oneway_v<Var<N, false, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  return launch_oneway_v<Var<N, false, false>>(rows, nrows, models, g, nz, step, adj, tw, s);
 }
 
 hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
@@ -215,14 +237,17 @@ hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTabl
   }
 }
 
+// single steps use the general variant (padding and split step allowed)
 template <int N>
 static hipError_t launch_onestep_n(const StepRow* rows, int nrows, const DepthTables* models,
                                    StepGeom g, bool adj, const float2* tw, hipStream_t s) {
+  using V = Var<N, false, true>;
+  constexpr int kWavesPerBlock = WavesPerBlock<N>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   if (adj)
-    hipLaunchKernelGGL((onestep_kernel<N, true>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((onestep_kernel<V, true>), grid, block, 0, s, rows, nrows, models, g, tw);
   else
-    hipLaunchKernelGGL((onestep_kernel<N, false>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((onestep_kernel<V, false>), grid, block, 0, s, rows, nrows, models, g, tw);
   return hipGetLastError();
 }
 

@@ -43,6 +43,30 @@ __device__ __forceinline__ float2 ldg(const float2* p, unsigned i) {
   const wemfwi_f2v v = ((const WEMFWI_GLOBAL wemfwi_f2v*)p)[i];
   return make_float2(v.x, v.y);
 }
+// Wave-uniform loads through the constant address space: the tables and
+// row descriptors are read-only for the whole launch, so these become scalar
+// (SMEM) loads into SGPRs instead of per-lane loads into VGPRs.  The index
+// must be the same on every lane of the wave.
+#define WEMFWI_CONST __attribute__((address_space(4)))
+__device__ __forceinline__ int ldu(const int* p, int i) { return ((const WEMFWI_CONST int*)p)[i]; }
+
+// a whole descriptor struct (size a multiple of 4 bytes) by scalar loads
+template <class T>
+__device__ __forceinline__ T ldu_struct(const T* p, int i) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor size must be a multiple of 4 bytes");
+  constexpr int n = (int)(sizeof(T) / 4);
+  const WEMFWI_CONST unsigned* q = (const WEMFWI_CONST unsigned*)(p + i);
+  unsigned w[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) w[k] = q[k];
+  T v;
+  __builtin_memcpy(&v, w, sizeof(T));
+  return v;
+}
+
+// index of this wave in its workgroup, as a wave-uniform value
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 __device__ __forceinline__ void stg(float2* p, unsigned i, float2 v) {
   wemfwi_f2v w;
   w.x = v.x;
@@ -50,13 +74,17 @@ __device__ __forceinline__ void stg(float2* p, unsigned i, float2 v) {
   ((WEMFWI_GLOBAL wemfwi_f2v*)p)[i] = w;
 }
 
-// Per-model depth tables (device pointers).
+// Per-model depth tables (device pointers).  The *_lane tables are in the
+// padded lane layout of one FFT row: entry t*E + e holds padded sample
+// p = t + 64 e (model sample x = p - tap), so each lane reads its E values
+// contiguously; padding entries hold refid -1, taper 0 and phi 0.
 struct DepthTables {
   const int* slot_begin;        // [nz+1] first active (depth, reference) slot of a depth
   const int* slot_ref;          // [nslots] reference index of each slot
-  const signed char* refid;     // [nz][nx] reference selecting x (-1: none)
-  const float2* phi;            // [nz][nx] dz*(S - Sref)/sqrt(Sref) of x's reference (0 for PSPI)
-  const float* taper;           // [nx]
+  const signed char* rid_lane;  // [nz][N] reference selecting each sample (-1: none)
+  const int* one_ref;           // [nz] 1: a single reference selects every x of the depth
+  const float2* phi_lane;       // [nz][N] dz*(S - Sref)/sqrt(Sref) of the sample's reference
+  const float* taper_lane;      // [N] taper (Taper.h:11-23)
   const float2* scoef;          // [ntaylor+1][nz][nx] scatterer coefficients C_it
   long long scoef_stride;       // nz*nx
 };
@@ -69,6 +97,17 @@ struct StepGeom {
   float dz;     // depth step
   float dk;     // wavenumber step of the phase-shift / scatterer grid
   int ntaylor;  // Taylor order of the scatterer
+  int taper;    // 1: a taper is applied (width > 0)
+};
+
+// Compile-time variants of the step: FULL (no padding samples) and SSF
+// (split-step correction).
+template <int N, bool FULL_, bool SSF_>
+struct Var {
+  static constexpr int n = N;
+  static constexpr int E = N / 64;
+  static constexpr bool FULL = FULL_;
+  static constexpr bool SSF = SSF_;
 };
 
 // sin and cos of x for |x| < 1e5: Cody-Waite reduction by a three-part pi/2
@@ -98,47 +137,65 @@ __device__ __forceinline__ void sincos_cw(float x, float* s, float* c) {
 
 __device__ __forceinline__ bool in_row(int x, int nx) { return (unsigned)x < (unsigned)nx; }
 
-// taper[x] * exp(-i w phi[x]) of the E samples of this lane (0 outside [0,nx))
-template <int N>
-__device__ __forceinline__ void ss_taper_row(float2 (&f)[N / 64], const DepthTables& dt,
-                                             const StepGeom& g, int d, float w, int t) {
-  constexpr int E = N / 64;
-  const float* tp = dt.taper + (t - g.tap);
-  const float2* ph = dt.phi + (size_t)d * g.nx + (t - g.tap);
+// Sample x = t + 64 e - tap of a row of nx values starting at p, 0 for
+// padding samples.  The load index is x itself or 0 (never negative), so the
+// load stays inside the row on every lane.
+template <class V>
+__device__ __forceinline__ float2 row_sample(const float2* p, const StepGeom& g, int t, int e) {
+  if constexpr (V::FULL) return ldg(p + t, (unsigned)(64 * e));
+  const int x = t + 64 * e - g.tap;
+  const bool ok = in_row(x, g.nx);
+  const float2 a = ldg(p, (unsigned)(ok ? x : 0));
+  return ok ? a : make_float2(0.f, 0.f);
+}
+
+// taper * exp(-i w phi) of the E samples of this lane (0 on padding samples
+// when a taper or the split step is on; 1 otherwise)
+template <class V>
+__device__ __forceinline__ void ss_taper_row(float2 (&f)[V::E], const DepthTables& dt, const StepGeom& g, int d,
+                                             float w, int t) {
+  constexpr int E = V::E;
   float tv[E];
-  float2 pv[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    tv[e] = 0.f;
-    pv[e] = make_float2(0.f, 0.f);
-    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) {
-      tv[e] = ldg(tp, (unsigned)(64 * e));
-      if (g.ssf) pv[e] = ldg(ph, (unsigned)(64 * e));
-    }
+  for (int e = 0; e < E; ++e) tv[e] = 1.f;
+  if (g.taper) {
+    const float* tp = dt.taper_lane + t * E;
+#pragma unroll
+    for (int e = 0; e < E; ++e) tv[e] = ldg(tp, (unsigned)e);
   }
+  if constexpr (V::SSF) {
+    const float2* ph = dt.phi_lane + (size_t)d * V::n + t * E;
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr))
-    float sn, cs;
-    sincos_cw(w * pv[e].x, &sn, &cs);
-    const float a = tv[e] * exp2f(w * pv[e].y * 1.4426950408889634f);
-    f[e] = make_float2(a * cs, -a * sn);
+    for (int e = 0; e < E; ++e) {
+      const float2 pv = ldg(ph, (unsigned)e);
+      // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr))
+      float sn, cs;
+      sincos_cw(w * pv.x, &sn, &cs);
+      const float a = tv[e] * exp2f(w * pv.y * 1.4426950408889634f);
+      f[e] = make_float2(a * cs, -a * sn);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) f[e] = make_float2(tv[e], 0.f);
   }
 }
 
 // reference ids of the E samples of this lane, packed 4 per 32-bit register
-template <int N>
-__device__ __forceinline__ void load_refid(const DepthTables& dt, const StepGeom& g, int d, int t,
-                                           unsigned (&rid)[(N / 64 + 3) / 4]) {
-  constexpr int E = N / 64;
-  const signed char* row = dt.refid + (size_t)d * g.nx + (t - g.tap);
+template <class V>
+__device__ __forceinline__ void load_refid(const DepthTables& dt, int d, int t, unsigned (&rid)[(V::E + 3) / 4]) {
+  constexpr int E = V::E;
+  const signed char* row = dt.rid_lane + (size_t)d * V::n + t * E;
+  if constexpr (E >= 4) {
+    const unsigned* w = (const unsigned*)row;
 #pragma unroll
-  for (int i = 0; i < (E + 3) / 4; ++i) rid[i] = 0;
+    for (int i = 0; i < E / 4; ++i) rid[i] = ldg(w, (unsigned)i);
+  } else {
+    rid[0] = 0xffffffffu;
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    int v = -1;
-    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) v = (int)ldg(row, (unsigned)(64 * e));
-    rid[e / 4] |= (unsigned)(v & 0xff) << (8 * (e % 4));
+    for (int e = 0; e < E; ++e) {
+      const unsigned v = (unsigned)(unsigned char)ldg(row, (unsigned)e);
+      rid[0] = (rid[0] & ~(0xffu << (8 * e))) | (v << (8 * e));
+    }
   }
 }
 
@@ -154,28 +211,42 @@ __device__ __forceinline__ float2 ptab_at(const float2* P, int t, int e) {
   return ldg(P, (unsigned)(N - t - 64 * e));  // 1 <= N - t - 64 e <= N/2
 }
 
-// out = P_d(in)   (in and out in padded strided layout; out is 0 outside [0,nx))
-template <int N>
-__device__ __forceinline__ void step_forward(float2 (&st)[N / 64], int d, float w,
-                                             const DepthTables& dt, const StepGeom& g,
-                                             const float2* __restrict__ ptab, float2* buf,
+// row of samples x = t + 64 e - tap (0 on padding samples)
+template <class V>
+__device__ __forceinline__ void load_samples(float2 (&v)[V::E], const float2* p, const StepGeom& g, int t) {
+#pragma unroll
+  for (int e = 0; e < V::E; ++e) v[e] = row_sample<V>(p, g, t, e);
+}
+
+// out = P_d(in)   (in and out in padded strided layout; out is 0 outside
+// [0,nx)).  Loads are issued where their values are first needed: holding
+// them across an FFT costs more registers (and waves per SIMD) than their
+// latency costs.
+template <class V>
+__device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+                                             const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                              const float2* tw, int t) {
-  constexpr int E = N / 64;
+  constexpr int N = V::n;
+  constexpr int E = V::E;
   fft_row<N, false>(st, buf, tw, t);
-  unsigned rid[(E + 3) / 4];
-  load_refid<N>(dt, g, d, t, rid);
-  const int s0 = dt.slot_begin[d], s1 = dt.slot_begin[d + 1];
+  const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
   if (s1 - s0 == 1) {
     // one active reference: transform in place
-    const int r = dt.slot_ref[s0];
+    const int r = ldu(dt.slot_ref, s0);
     const float2* P = ptab + (size_t)s0 * (N / 2 + 1);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmul(st[e], ptab_at<N>(P, t, e));
     fft_row<N, true>(st, buf, tw, t);
+    if (!(V::FULL && ldu(dt.one_ref, d))) {
+      unsigned rid[(E + 3) / 4];
+      load_refid<V>(dt, d, t, rid);
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      if (rid_at(rid, e) != r) st[e] = make_float2(0.f, 0.f);
+      for (int e = 0; e < E; ++e)
+        if (rid_at(rid, e) != r) st[e] = make_float2(0.f, 0.f);
+    }
   } else {
+    unsigned rid[(E + 3) / 4];
+    load_refid<V>(dt, d, t, rid);
     float2 X[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -183,7 +254,7 @@ __device__ __forceinline__ void step_forward(float2 (&st)[N / 64], int d, float 
       st[e] = make_float2(0.f, 0.f);
     }
     for (int s = s0; s < s1; ++s) {
-      const int r = dt.slot_ref[s];
+      const int r = ldu(dt.slot_ref, s);
       const float2* P = ptab + (size_t)s * (N / 2 + 1);
       float2 Y[E];
 #pragma unroll
@@ -194,44 +265,50 @@ __device__ __forceinline__ void step_forward(float2 (&st)[N / 64], int d, float 
         if (rid_at(rid, e) == r) st[e] = Y[e];
     }
   }
-  float2 f[E];
-  ss_taper_row<N>(f, dt, g, d, w, t);
+  if (V::SSF || g.taper) {
+    float2 f[E];
+    ss_taper_row<V>(f, dt, g, d, w, t);
 #pragma unroll
-  for (int e = 0; e < E; ++e) st[e] = cmul(st[e], f[e]);
+    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], f[e]);
+  }
 }
 
 // out = P_d^H(in)
-template <int N>
-__device__ __forceinline__ void step_adjoint(float2 (&st)[N / 64], int d, float w,
-                                             const DepthTables& dt, const StepGeom& g,
-                                             const float2* __restrict__ ptab, float2* buf,
+template <class V>
+__device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+                                             const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                              const float2* tw, int t) {
-  constexpr int E = N / 64;
-  unsigned rid[(E + 3) / 4];
-  load_refid<N>(dt, g, d, t, rid);
+  constexpr int N = V::n;
+  constexpr int E = V::E;
+  const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
   // SplitStep adjoint factor exp(+i w conj(phi)) = conj(exp(-i w phi)), taper self-adjoint
-  {
+  if (V::SSF || g.taper) {
     float2 f[E];
-    ss_taper_row<N>(f, dt, g, d, w, t);
+    ss_taper_row<V>(f, dt, g, d, w, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], f[e]);
   }
-  const int s0 = dt.slot_begin[d], s1 = dt.slot_begin[d + 1];
   if (s1 - s0 == 1) {
-    const int r = dt.slot_ref[s0];
+    const int r = ldu(dt.slot_ref, s0);
     const float2* P = ptab + (size_t)s0 * (N / 2 + 1);
+    if (!(V::FULL && ldu(dt.one_ref, d))) {
+      unsigned rid[(E + 3) / 4];
+      load_refid<V>(dt, d, t, rid);
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      if (rid_at(rid, e) != r) st[e] = make_float2(0.f, 0.f);
+      for (int e = 0; e < E; ++e)
+        if (rid_at(rid, e) != r) st[e] = make_float2(0.f, 0.f);
+    }
     fft_row<N, false>(st, buf, tw, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], ptab_at<N>(P, t, e));
   } else {
+    unsigned rid[(E + 3) / 4];
+    load_refid<V>(dt, d, t, rid);
     float2 acc[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = make_float2(0.f, 0.f);
     for (int s = s0; s < s1; ++s) {
-      const int r = dt.slot_ref[s];
+      const int r = ldu(dt.slot_ref, s);
       const float2* P = ptab + (size_t)s * (N / 2 + 1);
       float2 Z[E];
 #pragma unroll
@@ -244,10 +321,18 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[N / 64], int d, float 
     for (int e = 0; e < E; ++e) st[e] = acc[e];
   }
   fft_row<N, true>(st, buf, tw, t);
+  if constexpr (!V::FULL) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    if (!g.full && !in_row(t + 64 * e - g.tap, g.nx)) st[e] = make_float2(0.f, 0.f);
+    for (int e = 0; e < E; ++e)
+      if (!in_row(t + 64 * e - g.tap, g.nx)) st[e] = make_float2(0.f, 0.f);
   }
+}
+
+// scatterer coefficient C_it of this lane's sample e (0 on padding samples);
+// C is the start of the coefficient row
+template <class V>
+__device__ __forceinline__ float2 scoef_at(const float2* C, const StepGeom& g, int t, int e) {
+  return row_sample<V>(C, g, t, e);
 }
 
 // Scatterer (Scatter.cpp:35-111) on one row in padded strided layout:
@@ -256,36 +341,28 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[N / 64], int d, float 
 //   adjoint exact:  y = (+i w dz) sum_it M_it (conj(C_it) * d)
 // with C_it = coef[it] (S - eps Re S)^(-1/2-it) and M_it = Pad^H F^H (k/w)^(2 it)
 // F Pad (M_0 = identity on the row).
-template <int N, bool ADJ, bool EXACT>
-__device__ __forceinline__ void scatter_row(float2 (&st)[N / 64], int d, float w, const DepthTables& dt,
+template <class V, bool ADJ, bool EXACT>
+__device__ __forceinline__ void scatter_row(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                             const StepGeom& g, float2* buf, const float2* tw, int t) {
-  constexpr int E = N / 64;
-  const float2* C0 = dt.scoef + (size_t)d * g.nx + (t - g.tap);
+  constexpr int N = V::n;
+  constexpr int E = V::E;
+  const float2* C0 = dt.scoef + (size_t)d * g.nx;
   float2 acc[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    float2 c = make_float2(0.f, 0.f);
-    if (g.full || in_row(t + 64 * e - g.tap, g.nx)) c = ldg(C0, (unsigned)(64 * e));
+    const float2 c = scoef_at<V>(C0, g, t, e);
     acc[e] = ADJ ? cmulc(st[e], c) : cmul(st[e], c);
   }
-  // (k/w)^2 / N^(1/it) pieces: q[e] = kk^2 / w^2
   const float iw2 = 1.f / (w * w);
   for (int it = 1; it <= g.ntaylor; ++it) {
     const float2* Ci = C0 + (size_t)it * dt.scoef_stride;
     float2 Y[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (ADJ && EXACT) {
-        float2 c = make_float2(0.f, 0.f);
-        if (g.full || in_row(t + 64 * e - g.tap, g.nx)) c = ldg(Ci, (unsigned)(64 * e));
-        Y[e] = cmulc(st[e], c);
-      } else {
-        Y[e] = st[e];
-      }
-    }
+    for (int e = 0; e < E; ++e) Y[e] = (ADJ && EXACT) ? cmulc(st[e], scoef_at<V>(Ci, g, t, e)) : st[e];
     fft_row<N, false>(Y, buf, tw, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
+      // (k/w)^(2 it) / N on the grid of Scatter.cpp:24-31
       const int k = t + 64 * e;
       const int kh = k < N / 2 ? k : ((N - k) < N / 2 ? N - k : 0);
       const float kk = (float)kh * g.dk;
@@ -297,11 +374,10 @@ __device__ __forceinline__ void scatter_row(float2 (&st)[N / 64], int d, float w
     fft_row<N, true>(Y, buf, tw, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      if (!(g.full || in_row(t + 64 * e - g.tap, g.nx))) continue;
       if (ADJ && EXACT) {
-        acc[e] = cadd(acc[e], Y[e]);
+        if (V::FULL || in_row(t + 64 * e - g.tap, g.nx)) acc[e] = cadd(acc[e], Y[e]);
       } else {
-        const float2 c = ldg(Ci, (unsigned)(64 * e));
+        const float2 c = scoef_at<V>(Ci, g, t, e);
         acc[e] = cadd(acc[e], ADJ ? cmulc(Y[e], c) : cmul(Y[e], c));
       }
     }
