@@ -57,6 +57,44 @@ int wemfwi_version(void);
 /* Number of visible devices (0 without a GPU). */
 int wemfwi_device_count(void);
 
+/* Frequency grid and wavelet spectrum of PropParam::calcPropParam
+ * (propagator/src/PropParam.h:45-85), host only, float32 arithmetic as the
+ * reference: the float wavelet wave[nt] (time origin ot, step dt) is
+ * zero-padded by tpad samples; wave_f[nt + tpad] receives its unitary forward
+ * FFT (FFT1_r.cpp:74-83); freq[i] = fmin + i/tmax and index[i] = i + imin with
+ * tmax = ot + dt*(nt + tpad - 1), imin = (int)(fmin*tmax),
+ * imax = (int)(fmax*tmax + 0.5) -- or, when fmax == 0, imin = 1 and
+ * imax = nfreq (nfreq == 0: (nt + tpad)/2 + 1).  *nw receives imax - imin + 1;
+ * freq/index must hold cap entries (WEMFWI_ERR_ARG when cap < *nw; freq, index
+ * and wave_f may be NULL to query *nw only). */
+int wemfwi_prop_param(int nt, float ot, float dt, const float* wave, int tpad, float fmin,
+                      float fmax, int nfreq, int cap, int* nw, float* freq, int* index,
+                      wemfwi_complex* wave_f);
+
+/* Float time traces <-> complex frequency data, the float-data boundary of
+ * WEM::forward (WEMnl.cpp:39-65), BornFull::forward/adjoint
+ * (BornFull.cpp:39-62,80-102) and BornRefl (BornRefl.cpp:13-49): traces are
+ * zero-padded by tpad samples (n = nt + tpad, PropParam.h:49) and transformed
+ * with the unitary FFT1 (operator/src/FFT1_r.cpp:74-96), restricted to the nw
+ * survey bins (the only bins the propagation reads or writes).
+ * trace_to_freq: data_f[tr][bins[i]] = n^-1/2 sum_t data_t[tr][t] e^{-2 pi i bins[i] t/n};
+ *   add == 0 zeroes data_f[ntr][nbins] first.
+ * freq_to_trace: data_t[tr][t] (+)= Re(n^-1/2 sum_i data_f[tr][bins[i]] e^{+2 pi i bins[i] t/n}),
+ *   t < nt (the crop of Pad::adjoint).
+ * data_t, data_f are device pointers; bins is a host array, 0 <= bins[i] < min(nbins, n). */
+int wemfwi_trace_to_freq(int ntr, int nt, int tpad, const float* data_t, int nw, const int* bins,
+                         int nbins, wemfwi_complex* data_f, int add, void* stream);
+int wemfwi_freq_to_trace(int ntr, int nt, int tpad, const wemfwi_complex* data_f, int nbins, int nw,
+                         const int* bins, float* data_t, int add, void* stream);
+
+/* Regular device geometry of Device::createRegGeom (propagator/src/Device.h:39-63):
+ * device i at x = o_x + i*d_x, z = o_z + i*d_z on the model grid (origin ox, oz,
+ * spacing dxv, dzv) gets ix = (int)((x - ox)/dxv + 0.5f), iz likewise, in
+ * float32.  WEMFWI_ERR_ARG for n == 0, d_x == d_z == 0, a start outside the
+ * model, or a negative index (the reference throws SEPException). */
+int wemfwi_device_indices(int n, float o_x, float o_z, float d_x, float d_z, float ox, float oz,
+                          float dxv, float dzv, int* iz, int* ix);
+
 /* RefSampler (propagator/src/RefSampler.cpp:10-91), host only: reference
  * slownesses sref[nz][nref] and the reference selecting each x,
  * refid[nz][nx] (-1: none), including the stale reference-buffer behaviour of

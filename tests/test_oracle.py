@@ -358,3 +358,132 @@ def test_device_indices_round_half_up():
     assert ix == [100, 101, 102] and iz == [5, 5, 5]
     with pytest.raises(ValueError):
         O.device_indices(1, -20.0, 0.0, 0.0, 1.0, 0.0, 0.0, 10.0, 10.0)
+
+
+# ---------------------------------------------------------------------------
+# closed forms of the injection, the wavelet spectrum and the end-to-end
+# operators in a homogeneous medium (reflection coefficient 0), computed here
+# from the mathematics and not from the oracle's own helpers
+# ---------------------------------------------------------------------------
+
+def test_injection_point_source_and_cos2_spread():
+    nz, nx, tap = 6, 40, 3
+    # ng = 0: a single sample at (iz, ix + tap)
+    W = O.inj_wavelet_fwd(2.0 - 1.0j, [2], [7], 0, tap, nz, nx)
+    assert W[2, 10] == 2.0 - 1.0j and np.count_nonzero(W) == 1
+    # ng = 4: 2 ng + 1 samples, zero end weights, unit centre, weights summing
+    # to ng (sum_{k=-ng..ng} (1 + cos(pi k/ng))/2 = ng)
+    ng = 4
+    W = O.inj_wavelet_fwd(1.0, [1], [20], ng, tap, nz, nx)
+    row = W[1, 20 + tap - ng: 20 + tap + ng + 1].real
+    assert row[0] == 0 and row[-1] == 0 and row[ng] == 1.0
+    assert abs(row.sum() - ng) < 1e-12
+    want = 0.5 * (1 + np.cos(np.pi * np.arange(-ng, ng + 1) / ng))
+    assert np.allclose(row, want, atol=1e-12)
+    # samples falling outside the row are dropped (Injection.cpp in-bounds test)
+    W = O.inj_wavelet_fwd(1.0, [0], [0], ng, 0, nz, nx)
+    assert np.count_nonzero(W) == ng  # x = 0..ng-1 kept, weights at x>=1 nonzero
+    # recording is the adjoint of receiver injection
+    rng = np.random.default_rng(5)
+    iz, ix = [0, 2, 5, 5], [1, 13, 30, 38]
+    d = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+    Wr = rng.standard_normal((nz, nx)) + 1j * rng.standard_normal((nz, nx))
+    lhs = np.vdot(Wr, O.inj_rec_fwd(d, iz, ix, 2, tap, nz, nx))
+    rhs = np.vdot(O.inj_rec_adj(Wr, iz, ix, 2, tap), d)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_wavelet_spectrum_is_unitary_dft_of_padded_trace():
+    rng = np.random.default_rng(6)
+    wave, tpad = rng.standard_normal(37), 11
+    n = len(wave) + tpad
+    x = np.concatenate([wave, np.zeros(tpad)])
+    k = np.arange(n)
+    brute = np.exp(-2j * np.pi * np.outer(k, k) / n) @ x / np.sqrt(n)
+    got = O.wavelet_spectrum(wave, tpad)
+    assert np.max(np.abs(got - brute)) < 1e-12
+    assert abs(np.linalg.norm(got) - np.linalg.norm(wave)) < 1e-12
+
+
+def _homog_step(n, dx, dz, w, s0, eps):
+    """One homogeneous depth step as a circulant matrix: F^H diag(e^{-i kz dz}) F
+    with the reference k grid (dk = 2 pi/(dx (n-1)), k[n/2] = 0) and
+    kz = principal sqrt(w^2 s0 (1 - i eps) - k^2), Im kz <= 0."""
+    dk = 2 * np.pi / (dx * (n - 1))
+    j = np.arange(n)
+    k = np.where(j < n // 2, j * dk, -(n - j) * dk)
+    k[n // 2] = 0.0
+    if n % 2:
+        k[n // 2 + 1] = 0.0
+    kz = np.sqrt(w * w * s0 * (1 - 1j * eps) - k * k + 0j)
+    F = np.exp(-2j * np.pi * np.outer(j, j) / n) / np.sqrt(n)
+    return F.conj().T @ np.diag(np.exp(-1j * kz * dz)) @ F
+
+
+def _homog_survey(nx, nz, nw, ntaylor=1, rec_iz=None, rec_ix=None):
+    freq = (3.0 + 4.5 * np.arange(nw)).astype(np.float32)
+    nt = 16
+    index = np.arange(nw) + 2
+    rng = np.random.default_rng(8)
+    wave_f = rng.standard_normal(nt) + 1j * rng.standard_normal(nt)
+    return O.Survey(nx=nx, nz=nz, dx=12.0, dz=8.0, freq=freq, index=index, wave_f=wave_f,
+                    src_iz=[0], src_ix=[nx // 2], rec_iz=rec_iz, rec_ix=rec_ix, nt_f=nt,
+                    tap=0, pad=0, nref=2, eps=0.04, prop="ssf", ntaylor=ntaylor,
+                    ngs=0, ngr=0, ng=0)
+
+
+def test_wem_forward_homogeneous_closed_form():
+    nx, nz, nw, s0 = 24, 9, 2, 1.0 / 2000.0 ** 2
+    rec_iz, rec_ix = [0, 3, 3, 8], [2, 12, 20, 7]
+    sv = _homog_survey(nx, nz, nw, rec_iz=rec_iz, rec_ix=rec_ix)
+    slow = np.full((nz, nx), s0, dtype=np.complex64)
+    den = np.ones((nz, nx), dtype=np.complex64)
+    got = O.wem_forward(sv, slow, den)
+    s0f = float(np.complex64(s0).real)
+    for i in range(nw):
+        w = 2 * np.pi * float(sv.freq[i])
+        P = _homog_step(nx, sv.dx, sv.dz, w, s0f, sv.eps)
+        D = np.zeros((nz, nx), complex)
+        D[0, nx // 2] = sv.wave_f[sv.index[i]]
+        for iz in range(nz - 1):
+            D[iz + 1] = P @ D[iz]
+        want = np.array([D[z, x] for z, x in zip(rec_iz, rec_ix)])
+        g = got[0, :, sv.index[i]]
+        assert np.linalg.norm(g - want) < 1e-10 * np.linalg.norm(want), i
+    # only the survey's bins are filled
+    mask = np.ones(sv.nt_f, bool)
+    mask[sv.index] = False
+    assert np.all(got[:, :, mask] == 0)
+
+
+def test_born_forward_homogeneous_point_scatterer_closed_form():
+    # r = 0 in a homogeneous background, so receivers deeper than z0 + 1 see
+    # only the down-going scattered field Down(LinDown(dS; D)); with
+    # ntaylor = 0 the scatterer is (-i w dz) (S - eps Re S)^(-1/2) pointwise.
+    nx, nz, nw, s0 = 24, 10, 2, 1.0 / 2500.0 ** 2
+    z0, x0 = 3, 9
+    rec_iz, rec_ix = [6, 6, 9, 9], [4, 11, 9, 20]
+    sv = _homog_survey(nx, nz, nw, ntaylor=0, rec_iz=rec_iz, rec_ix=rec_ix)
+    slow = np.full((nz, nx), s0, dtype=np.complex64)
+    den = np.ones((nz, nx), dtype=np.complex64)
+    dS = np.zeros((nz, nx), dtype=np.complex64)
+    dS[z0, x0] = 3e-9
+    got = O.born_forward(sv, slow, den, dS, np.zeros_like(dS))
+    s0f = float(np.complex64(s0).real)
+    c0 = (s0f * (1 - sv.eps)) ** -0.5
+    for i in range(nw):
+        w = 2 * np.pi * float(sv.freq[i])
+        P = _homog_step(nx, sv.dx, sv.dz, w, s0f, sv.eps)
+        D = np.zeros((nz, nx), complex)
+        D[0, nx // 2] = sv.wave_f[sv.index[i]]
+        for iz in range(nz - 1):
+            D[iz + 1] = P @ D[iz]
+        sc = np.zeros(nx, complex)
+        sc[x0] = -1j * w * sv.dz * c0 * D[z0, x0] * float(dS[z0, x0].real)
+        S1 = np.zeros((nz, nx), complex)
+        S1[z0 + 1] = P @ sc
+        for iz in range(z0 + 1, nz - 1):
+            S1[iz + 1] = P @ S1[iz]
+        want = np.array([S1[z, x] for z, x in zip(rec_iz, rec_ix)])
+        g = got[0, :, sv.index[i]]
+        assert np.linalg.norm(g - want) < 1e-9 * np.linalg.norm(want), (i, np.linalg.norm(g - want) / np.linalg.norm(want))

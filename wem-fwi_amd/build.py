@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libwemfwi.so")
-SOURCES = ["kernels.hip", "born.hip", "host.cpp", "ops.cpp"]
+SOURCES = ["kernels.hip", "born.hip", "host.cpp", "ops.cpp", "params.cpp", "traces.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -59,4 +59,10 @@ hipError_t launch_offset_ic_adj(double2* img, const float2* const* bg, const flo
 hipError_t launch_offset_ic_fwd(float2* src, const float2* bg, const float2* img, int nh, int nz, int nx,
                                 hipStream_t s);
 
+hipError_t launch_trace_twiddles(double2* tw, int n, hipStream_t s);
+hipError_t launch_trace_to_freq(const float* x, int ntr, int nt, int n, const int* bins, int nw, const double2* tw,
+                                float2* f, int nbins, hipStream_t s);
+hipError_t launch_freq_to_trace(const float2* f, int ntr, int nbins, const int* bins, int nw, int nt, int n,
+                                const double2* tw, float* x, int add, hipStream_t s);
+
 }  // namespace wemfwi

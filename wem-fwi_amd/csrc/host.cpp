@@ -121,28 +121,6 @@ int refsampler_host(int nz, int nx, const cf* slow, int nref, cf* sref, int* ref
 
 // ------------------------------------------------------------------ device helpers
 
-template <typename T>
-int dev_alloc(T** p, size_t n, const char* what) {
-  *p = nullptr;
-  if (n == 0) return WEMFWI_OK;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
-  }
-  return WEMFWI_OK;
-}
-
-template <typename T>
-int dev_upload(T** p, const T* src, size_t n, const char* what) {
-  int rc = dev_alloc(p, n, what);
-  if (rc) return rc;
-  if (n == 0) return WEMFWI_OK;
-  hipError_t e = hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return hip_fail(e, what);
-  return WEMFWI_OK;
-}
-
 void Workspace::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;

@@ -18,6 +18,30 @@ int refsampler_host(int nz, int nx, const std::complex<float>* slow, int nref, s
                     int* refid);
 int twiddles(int n, const float2** out);
 
+// Device allocation of n elements (WEMFWI_ERR_ALLOC on failure).
+template <typename T>
+int dev_alloc(T** p, size_t n, const char* what) {
+  *p = nullptr;
+  if (n == 0) return WEMFWI_OK;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+  }
+  return WEMFWI_OK;
+}
+
+// Device allocation initialised from host memory.
+template <typename T>
+int dev_upload(T** p, const T* src, size_t n, const char* what) {
+  int rc = dev_alloc(p, n, what);
+  if (rc) return rc;
+  if (n == 0) return WEMFWI_OK;
+  hipError_t e = hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return WEMFWI_OK;
+}
+
 // Grow-only device allocation.
 struct Workspace {
   void* ptr = nullptr;

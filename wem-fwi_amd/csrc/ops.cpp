@@ -165,7 +165,10 @@ static int put(DevBuf& b, const std::vector<T>& h, hipStream_t s, const char* wh
     if (rc) return rc;
   }
   if (h.empty()) return WEMFWI_OK;
+  // the copy completes before returning, so the host vector may be a
+  // temporary of the caller (descriptor tables are small)
   hipError_t e = hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? WEMFWI_OK : hip_fail(e, what);
 }
 
@@ -205,6 +208,7 @@ struct Ctx {
   int nz, nx, N;
   size_t nzx;
   std::vector<Model*> models;  // distinct models of the call
+  std::vector<DepthTables> htables;  // host staging of d_models (alive until the call ends)
   DevBuf d_models;             // DepthTables[nmodels]
   DevBuf d_rows;
   std::vector<SweepRow> hsw;
@@ -231,9 +235,9 @@ struct Ctx {
       if (m->d.nz != nz || m->d.nx != nx || m->N != N || m->d.tap != m0->d.tap || m->d.pspi != m0->d.pspi ||
           m->d.taper != m0->d.taper || m->d.ntaylor != m0->d.ntaylor)
         return fail(WEMFWI_ERR_ARG, "models of one call must share grid and parameters");
-    std::vector<DepthTables> t;
-    for (Model* m : ms) t.push_back(m->h_tables);
-    RCCHK(put(d_models, t, s, "model tables"));
+    htables.clear();
+    for (Model* m : ms) htables.push_back(m->h_tables);
+    RCCHK(put(d_models, htables, s, "model tables"));
     // frequency sets: every model holds the survey's frequencies
     for (Model* m : ms) {
       bool have = m->freqs.size() == sv->freq.size();

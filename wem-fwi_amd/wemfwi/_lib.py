@@ -48,6 +48,12 @@ SIGNATURES = {
     "wemfwi_last_error": (C.c_char_p, []),
     "wemfwi_version": (_i, []),
     "wemfwi_device_count": (_i, []),
+    "wemfwi_prop_param": (_i, [_i, C.c_float, C.c_float, _fp, _i, C.c_float, C.c_float, _i, _i, _ip,
+                               _fp, _ip, _vp]),
+    "wemfwi_trace_to_freq": (_i, [_i, _i, _i, _vp, _i, _ip, _i, _vp, _i, _vp]),
+    "wemfwi_freq_to_trace": (_i, [_i, _i, _i, _vp, _i, _i, _ip, _vp, _i, _vp]),
+    "wemfwi_device_indices": (_i, [_i, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, C.c_float, C.c_float, _ip, _ip]),
     "wemfwi_refsampler": (_i, [_i, _i, _vp, _i, _vp, _ip]),
     "wemfwi_model_create": (_i, [C.POINTER(ModelDesc), _vp, _vp, C.POINTER(_vp)]),
     "wemfwi_model_destroy": (_i, [_vp]),
