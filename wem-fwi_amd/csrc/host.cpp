@@ -175,6 +175,7 @@ Model::~Model() {
   ptab.release();
   wlist.release();
   rows.release();
+  scratch.release();
 }
 
 int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_h) {
@@ -311,6 +312,8 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   t.taper_lane = d_taper_lane;
   t.scoef = d_scoef;
   t.scoef_stride = (long long)nzx;
+  t.nz = d.nz;
+  t.n = N;
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
   if ((rc = twiddles(N, &d_tw))) return rc;
   return WEMFWI_OK;
@@ -403,6 +406,8 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
     h[i].model = 0;
     h[i].recursive = recursive ? 1 : 0;
   }
+  if ((rc = scratch.reserve((size_t)nrows * N * sizeof(float2), "scratch rows"))) return rc;
+  for (int i = 0; i < nrows; ++i) h[i].scratch = (float2*)scratch.ptr + (size_t)i * N;
   if ((rc = rows.reserve(h.size() * sizeof(SweepRow), "row descriptors"))) return rc;
   e = hipMemcpyAsync(rows.ptr, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "row upload");
@@ -441,6 +446,8 @@ int Model::onestep(int adjoint, int nrows, const float* freq_hz, const int* dept
     h[i].model = 0;
     h[i].add = add ? 1 : 0;
   }
+  if ((rc = scratch.reserve((size_t)nrows * N * sizeof(float2), "scratch rows"))) return rc;
+  for (int i = 0; i < nrows; ++i) h[i].scratch = (float2*)scratch.ptr + (size_t)i * N;
   if ((rc = rows.reserve(h.size() * sizeof(StepRow), "row descriptors"))) return rc;
   hipError_t e = hipMemcpyAsync(rows.ptr, h.data(), h.size() * sizeof(StepRow), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "row upload");

@@ -72,6 +72,7 @@ struct Model {
   const float2* d_tw = nullptr;
 
   Workspace ptab, wlist, rows;
+  Workspace scratch;  // [nrows][N] per-row scratch rows of the sweep / step kernels
   // host staging of asynchronous uploads (kept alive until the call syncs)
   std::vector<double> h_wlist;
   std::vector<SweepRow> h_sweep;

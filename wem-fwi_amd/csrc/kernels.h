@@ -16,6 +16,7 @@ struct SweepRow {
   int model;             // index into the DepthTables array
   int recursive;         // 1: W[next] = P W[cur] + in[next] with W the output rows
                          // 0: out[next] = P in[cur] + in[next] (one bidiagonal step)
+  float2* scratch;       // [N] row of this wave for depths with several references
 };
 
 // One independent depth step.
@@ -27,6 +28,7 @@ struct StepRow {
   int d;                 // depth of the step
   int model;
   int add;               // 1: out += step(in)
+  float2* scratch;       // [N]
 };
 
 bool fft_size_supported(int n);

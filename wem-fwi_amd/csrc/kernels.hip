@@ -55,24 +55,17 @@ __device__ __forceinline__ void load_row(float2 (&v)[V::E], const float2* __rest
 template <class V>
 __device__ __forceinline__ void store_row(float2* __restrict__ p, const float2 (&v)[V::E], const StepGeom& g,
                                           int t) {
-  if constexpr (V::FULL) {
-    float2* b = p + t;
 #pragma unroll
-    for (int e = 0; e < V::E; ++e) stg(b, (unsigned)(64 * e), v[e]);
-  } else {
-    float2* b = p + (t - g.tap);
-#pragma unroll
-    for (int e = 0; e < V::E; ++e)
-      if (in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), v[e]);
-  }
+  for (int e = 0; e < V::E; ++e) row_store<V>(p, v[e], g, t, e);
 }
 
 // Waves (rows) per workgroup: the twiddle table in LDS is shared by the
-// waves of a workgroup; 8 waves keep 4 workgroups' worth of LDS (16 waves)
-// per CU for N = 1024.
+// waves of a workgroup.  For N = 1024 two workgroups of 8 waves (2 x 77.6 KB
+// of LDS) give 16 waves per CU, i.e. 4 per SIMD; for N = 2048 one workgroup
+// of 8 waves (155 KB) gives 2 per SIMD.
 template <int N>
 struct WavesPerBlock {
-  static constexpr int value = 4;
+  static constexpr int value = N >= 1024 ? 8 : 4;
 };
 
 // In-place or out-of-place Down/Up recursion (OneWay.cpp:32-109) of one
@@ -82,7 +75,7 @@ struct WavesPerBlock {
 // Waves per SIMD the register allocator targets for an FFT length.
 template <int N>
 struct Occupancy {
-  static constexpr int value = N <= 512 ? 4 : (N <= 1024 ? 3 : 2);
+  static constexpr int value = N <= 256 ? 8 : (N == 512 ? 6 : (N == 1024 ? 4 : 2));
 };
 
 template <class V, bool ADJ>
@@ -114,8 +107,8 @@ __global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
     const int cur = first + k * step;
     const int nxt = cur + step;
     if (!recursive && k > 0) load_row<V>(st, R.in + (size_t)cur * nx, g, t);
-    if constexpr (ADJ) step_adjoint<V>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t);
-    else step_forward<V>(st, cur, R.w, dt, g, R.ptab, buf, tw, t);
+    if constexpr (ADJ) step_adjoint<V, true>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
+    else step_forward<V, true>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
     float2 src[E];
     load_row<V>(src, R.in + (size_t)nxt * nx, g, t);
 #pragma unroll
@@ -145,8 +138,8 @@ __global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
   const DepthTables dt = ldu_struct(models, R.model);
   float2 st[E];
   load_row<V>(st, R.in, g, t);
-  if constexpr (ADJ) step_adjoint<V>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
-  else step_forward<V>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t);
+  if constexpr (ADJ) step_adjoint<V, true>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t, R.scratch);
+  else step_forward<V, true>(st, R.d, R.w, dt, g, R.ptab, bufs[wv], tw, t, R.scratch);
   if (R.add) {
     float2 o[E];
     load_row<V>(o, R.out, g, t);

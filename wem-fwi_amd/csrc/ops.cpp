@@ -210,6 +210,7 @@ struct Ctx {
   std::vector<Model*> models;  // distinct models of the call
   std::vector<DepthTables> htables;  // host staging of d_models (alive until the call ends)
   DevBuf d_models;             // DepthTables[nmodels]
+  DevBuf scr;                  // [rows][N] scratch rows of the sweep kernel
   DevBuf d_rows;
   std::vector<SweepRow> hsw;
   std::vector<LinRow> hlin;
@@ -275,6 +276,8 @@ struct Ctx {
       hsw[i].model = model_index(r.model[i]);
       hsw[i].recursive = 1;
     }
+    if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
+    for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
     RCCHK(put(d_rows, hsw, s, "sweep rows"));
     const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
     HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, step,

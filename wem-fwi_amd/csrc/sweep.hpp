@@ -74,6 +74,34 @@ __device__ __forceinline__ void stg(float2* p, unsigned i, float2 v) {
   ((WEMFWI_GLOBAL wemfwi_f2v*)p)[i] = w;
 }
 
+// Buffer-resource access: a wave-uniform descriptor (SGPRs) over a range of
+// bytes, a per-lane byte offset (VGPR) and a wave-uniform byte offset
+// (SGPR / immediate).  Offsets past the range read 0 and drop stores (the
+// range check applies to the per-lane offset plus the immediate), which is
+// how padding samples of a row are zero.  Word 3 0x00020000: raw buffer,
+// 32-bit data format (gfx9 family).
+typedef __amdgpu_buffer_rsrc_t brsrc;
+// the 64-bit payload type of the b64 buffer builtins (two 32-bit lanes);
+// values are moved in and out of it by bit casts only
+typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(brsrc(), 0, 0, 0)) wemfwi_b64;
+
+__device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 bld2(brsrc r, unsigned voff, unsigned soff) {
+  const unsigned long long u =
+      __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+  return make_float2(__builtin_bit_cast(float, (unsigned)u), __builtin_bit_cast(float, (unsigned)(u >> 32)));
+}
+__device__ __forceinline__ void bst2(brsrc r, float2 a, unsigned voff, unsigned soff) {
+  const unsigned long long u = ((unsigned long long)__builtin_bit_cast(unsigned, a.y) << 32) |
+                               (unsigned long long)__builtin_bit_cast(unsigned, a.x);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wemfwi_b64, u), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ unsigned bld1(brsrc r, unsigned voff, unsigned soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+
 // Per-model depth tables (device pointers).  The *_lane tables are in the
 // padded lane layout of one FFT row: entry t*E + e holds padded sample
 // p = t + 64 e (model sample x = p - tap), so each lane reads its E values
@@ -87,6 +115,8 @@ struct DepthTables {
   const float* taper_lane;      // [N] taper (Taper.h:11-23)
   const float2* scoef;          // [ntaylor+1][nz][nx] scatterer coefficients C_it
   long long scoef_stride;       // nz*nx
+  int nz;                       // depth samples
+  int n;                        // FFT length N (lane-table row length)
 };
 
 struct StepGeom {
@@ -142,11 +172,19 @@ __device__ __forceinline__ bool in_row(int x, int nx) { return (unsigned)x < (un
 // load stays inside the row on every lane.
 template <class V>
 __device__ __forceinline__ float2 row_sample(const float2* p, const StepGeom& g, int t, int e) {
-  if constexpr (V::FULL) return ldg(p + t, (unsigned)(64 * e));
-  const int x = t + 64 * e - g.tap;
-  const bool ok = in_row(x, g.nx);
-  const float2 a = ldg(p, (unsigned)(ok ? x : 0));
-  return ok ? a : make_float2(0.f, 0.f);
+  const brsrc r = make_rsrc(p, (unsigned)g.nx * 8u);
+  if constexpr (V::FULL) return bld2(r, (unsigned)t * 8u, 512u * (unsigned)e);
+  // x < 0 wraps to a huge offset: out of range like x >= nx, and reads 0
+  return bld2(r, (unsigned)(t + 64 * e - g.tap) * 8u, 0u);
+}
+
+// stores sample x = t + 64 e - tap into a row of nx values (padding samples
+// fall outside the range and are dropped)
+template <class V>
+__device__ __forceinline__ void row_store(float2* p, float2 v, const StepGeom& g, int t, int e) {
+  const brsrc r = make_rsrc(p, (unsigned)g.nx * 8u);
+  if constexpr (V::FULL) bst2(r, v, (unsigned)t * 8u, 512u * (unsigned)e);
+  else bst2(r, v, (unsigned)(t + 64 * e - g.tap) * 8u, 0u);
 }
 
 // taper * exp(-i w phi) of the E samples of this lane (0 on padding samples
@@ -159,15 +197,15 @@ __device__ __forceinline__ void ss_taper_row(float2 (&f)[V::E], const DepthTable
 #pragma unroll
   for (int e = 0; e < E; ++e) tv[e] = 1.f;
   if (g.taper) {
-    const float* tp = dt.taper_lane + t * E;
+    const brsrc r = make_rsrc(dt.taper_lane, (unsigned)V::n * 4u);
 #pragma unroll
-    for (int e = 0; e < E; ++e) tv[e] = ldg(tp, (unsigned)e);
+    for (int e = 0; e < E; ++e) tv[e] = __builtin_bit_cast(float, bld1(r, (unsigned)(t * E + e) * 4u, 0u));
   }
   if constexpr (V::SSF) {
-    const float2* ph = dt.phi_lane + (size_t)d * V::n + t * E;
+    const brsrc r = make_rsrc(dt.phi_lane + (size_t)d * V::n, (unsigned)V::n * 8u);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const float2 pv = ldg(ph, (unsigned)e);
+      const float2 pv = bld2(r, (unsigned)(t * E + e) * 8u, 0u);
       // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr))
       float sn, cs;
       sincos_cw(w * pv.x, &sn, &cs);
@@ -186,9 +224,9 @@ __device__ __forceinline__ void load_refid(const DepthTables& dt, int d, int t, 
   constexpr int E = V::E;
   const signed char* row = dt.rid_lane + (size_t)d * V::n + t * E;
   if constexpr (E >= 4) {
-    const unsigned* w = (const unsigned*)row;
+    const brsrc r = make_rsrc(dt.rid_lane + (size_t)d * V::n, (unsigned)V::n);
 #pragma unroll
-    for (int i = 0; i < E / 4; ++i) rid[i] = ldg(w, (unsigned)i);
+    for (int i = 0; i < E / 4; ++i) rid[i] = bld1(r, (unsigned)(t * E), 4u * (unsigned)i);
   } else {
     rid[0] = 0xffffffffu;
 #pragma unroll
@@ -207,8 +245,10 @@ __device__ __forceinline__ int rid_at(const unsigned* rid, int e) {
 // k <= N/2; entry N-k for the upper half)
 template <int N>
 __device__ __forceinline__ float2 ptab_at(const float2* P, int t, int e) {
-  if (e < N / 128) return ldg(P, (unsigned)(t + 64 * e));
-  return ldg(P, (unsigned)(N - t - 64 * e));  // 1 <= N - t - 64 e <= N/2
+  const brsrc r = make_rsrc(P, (unsigned)(N / 2 + 1) * 8u);
+  if (e < N / 128) return bld2(r, (unsigned)t * 8u, 512u * (unsigned)e);
+  // N - t - 64 e = (64 - t) + 64 (E - 1 - e), in 1..N/2
+  return bld2(r, (unsigned)(64 - t) * 8u, 512u * (unsigned)(N / 64 - 1 - e));
 }
 
 // row of samples x = t + 64 e - tap (0 on padding samples)
@@ -223,9 +263,23 @@ __device__ __forceinline__ void load_samples(float2 (&v)[V::E], const float2* p,
 // them across an FFT costs more registers (and waves per SIMD) than their
 // latency costs.
 template <class V>
+__device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+                                                 const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
+                                                 const float2* tw, int t, float2* scr);
+
+//
+// SCR: one code path for every depth (step_forward_scr): at depths with
+// several active references the spectrum is parked in the wave's scratch row
+// (global memory, written and re-read by the same lane) instead of registers,
+// which keeps the step within 128 VGPRs.
+template <class V, bool SCR = false>
 __device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                              const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                             const float2* tw, int t) {
+                                             const float2* tw, int t, float2* scr = nullptr) {
+  if constexpr (SCR) {
+    step_forward_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr);
+    return;
+  }
   constexpr int N = V::n;
   constexpr int E = V::E;
   fft_row<N, false>(st, buf, tw, t);
@@ -253,6 +307,7 @@ __device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w,
       X[e] = st[e];
       st[e] = make_float2(0.f, 0.f);
     }
+#pragma unroll 1
     for (int s = s0; s < s1; ++s) {
       const int r = ldu(dt.slot_ref, s);
       const float2* P = ptab + (size_t)s * (N / 2 + 1);
@@ -273,11 +328,21 @@ __device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w,
   }
 }
 
-// out = P_d^H(in)
+// out = P_d^H(in); SCR as in step_forward (the split-step-corrected input row
+// is parked in the scratch row at depths with several references)
 template <class V>
+__device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+                                                 const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
+                                                 const float2* tw, int t, float2* scr);
+
+template <class V, bool SCR = false>
 __device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                              const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                             const float2* tw, int t) {
+                                             const float2* tw, int t, float2* scr = nullptr) {
+  if constexpr (SCR) {
+    step_adjoint_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr);
+    return;
+  }
   constexpr int N = V::n;
   constexpr int E = V::E;
   const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
@@ -307,6 +372,7 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w,
     float2 acc[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = make_float2(0.f, 0.f);
+#pragma unroll 1
     for (int s = s0; s < s1; ++s) {
       const int r = ldu(dt.slot_ref, s);
       const float2* P = ptab + (size_t)s * (N / 2 + 1);
@@ -319,6 +385,121 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w,
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = acc[e];
+  }
+  fft_row<N, true>(st, buf, tw, t);
+  if constexpr (!V::FULL) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (!in_row(t + 64 * e - g.tap, g.nx)) st[e] = make_float2(0.f, 0.f);
+  }
+}
+
+// One loop over the active references of the depth for both the single-
+// and the multi-reference case (two separate paths in one sweep loop cost the
+// register allocator more than either path alone).
+//   forward: X = F in; for each reference r: Y = F^H (P_r X), out[x in r] = Y[x]
+template <class V>
+__device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+                                                 const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
+                                                 const float2* tw, int t, float2* scr) {
+  constexpr int N = V::n;
+  constexpr int E = V::E;
+  fft_row<N, false>(st, buf, tw, t);
+  const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
+  const bool multi = s1 - s0 > 1;
+  const bool all = V::FULL && ldu(dt.one_ref, d);
+  const brsrc xs = make_rsrc(scr, (unsigned)N * 8u);  // the wave's scratch row, lane index t + 64 e
+  if (multi) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) bst2(xs, st[e], (unsigned)t * 8u, 512u * (unsigned)e);
+  }
+  unsigned rid[(E + 3) / 4];
+  if (!all) load_refid<V>(dt, d, t, rid);
+#pragma unroll 1
+  for (int s = s0; s < s1; ++s) {
+    const int r = ldu(dt.slot_ref, s);
+    const float2* P = ptab + (size_t)s * (N / 2 + 1);
+    float2 Y[E];
+    // in two halves, so that only half of the spectrum / phase loads are in
+    // flight next to the merged output
+    if (multi) {
+#pragma unroll
+      for (int e = 0; e < E / 2; ++e) Y[e] = cmul(bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e), ptab_at<N>(P, t, e));
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = E / 2; e < E; ++e) Y[e] = cmul(bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e), ptab_at<N>(P, t, e));
+    } else {
+#pragma unroll
+      for (int e = 0; e < E / 2; ++e) Y[e] = cmul(st[e], ptab_at<N>(P, t, e));
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = E / 2; e < E; ++e) Y[e] = cmul(st[e], ptab_at<N>(P, t, e));
+    }
+    fft_row<N, true>(Y, buf, tw, t);
+    if (all) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) st[e] = Y[e];
+    } else {
+      const bool first = s == s0;
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        st[e] = (rid_at(rid, e) == r) ? Y[e] : (first ? make_float2(0.f, 0.f) : st[e]);
+    }
+  }
+  if (V::SSF || g.taper) {
+    float2 f[E];
+    ss_taper_row<V>(f, dt, g, d, w, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], f[e]);
+  }
+}
+
+//   adjoint: u = conj(f) in; acc = sum_r conj(P_r) F (u restricted to r); out = F^H acc
+template <class V>
+__device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+                                                 const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
+                                                 const float2* tw, int t, float2* scr) {
+  constexpr int N = V::n;
+  constexpr int E = V::E;
+  if (V::SSF || g.taper) {
+    float2 f[E];
+    ss_taper_row<V>(f, dt, g, d, w, t);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], f[e]);
+  }
+  const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
+  const bool multi = s1 - s0 > 1;
+  const bool all = V::FULL && ldu(dt.one_ref, d);
+  const brsrc xs = make_rsrc(scr, (unsigned)N * 8u);  // the wave's scratch row, lane index t + 64 e
+  if (multi) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) bst2(xs, st[e], (unsigned)t * 8u, 512u * (unsigned)e);
+  }
+  unsigned rid[(E + 3) / 4];
+  if (!all) load_refid<V>(dt, d, t, rid);
+#pragma unroll 1
+  for (int s = s0; s < s1; ++s) {
+    const int r = ldu(dt.slot_ref, s);
+    const float2* P = ptab + (size_t)s * (N / 2 + 1);
+    float2 Z[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 v = multi ? bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e) : st[e];
+      Z[e] = (all || rid_at(rid, e) == r) ? v : make_float2(0.f, 0.f);
+    }
+    fft_row<N, false>(Z, buf, tw, t);
+    const bool first = s == s0;
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) {
+      const float2 z = cmulc(Z[e], ptab_at<N>(P, t, e));
+      st[e] = first ? z : cadd(st[e], z);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int e = E / 2; e < E; ++e) {
+      const float2 z = cmulc(Z[e], ptab_at<N>(P, t, e));
+      st[e] = first ? z : cadd(st[e], z);
+    }
   }
   fft_row<N, true>(st, buf, tw, t);
   if constexpr (!V::FULL) {
