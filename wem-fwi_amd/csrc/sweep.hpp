@@ -88,15 +88,22 @@ typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(brsrc(), 0, 0, 0)) wemfwi_
 __device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
+// AUX: cache policy bits of the access; kStream (bit 1, non-temporal) marks
+// data read or written once per sweep (wavefield rows), so that it does not
+// evict the phase tables and scratch rows that are re-read from L2.
+constexpr int kStream = 2;
+
+template <int AUX = 0>
 __device__ __forceinline__ float2 bld2(brsrc r, unsigned voff, unsigned soff) {
   const unsigned long long u =
-      __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+      __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
   return make_float2(__builtin_bit_cast(float, (unsigned)u), __builtin_bit_cast(float, (unsigned)(u >> 32)));
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst2(brsrc r, float2 a, unsigned voff, unsigned soff) {
   const unsigned long long u = ((unsigned long long)__builtin_bit_cast(unsigned, a.y) << 32) |
                                (unsigned long long)__builtin_bit_cast(unsigned, a.x);
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wemfwi_b64, u), r, (int)voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wemfwi_b64, u), r, (int)voff, (int)soff, AUX);
 }
 __device__ __forceinline__ unsigned bld1(brsrc r, unsigned voff, unsigned soff) {
   return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
@@ -173,9 +180,9 @@ __device__ __forceinline__ bool in_row(int x, int nx) { return (unsigned)x < (un
 template <class V>
 __device__ __forceinline__ float2 row_sample(const float2* p, const StepGeom& g, int t, int e) {
   const brsrc r = make_rsrc(p, (unsigned)g.nx * 8u);
-  if constexpr (V::FULL) return bld2(r, (unsigned)t * 8u, 512u * (unsigned)e);
+  if constexpr (V::FULL) return bld2<kStream>(r, (unsigned)t * 8u, 512u * (unsigned)e);
   // x < 0 wraps to a huge offset: out of range like x >= nx, and reads 0
-  return bld2(r, (unsigned)(t + 64 * e - g.tap) * 8u, 0u);
+  return bld2<kStream>(r, (unsigned)(t + 64 * e - g.tap) * 8u, 0u);
 }
 
 // stores sample x = t + 64 e - tap into a row of nx values (padding samples
@@ -183,8 +190,8 @@ __device__ __forceinline__ float2 row_sample(const float2* p, const StepGeom& g,
 template <class V>
 __device__ __forceinline__ void row_store(float2* p, float2 v, const StepGeom& g, int t, int e) {
   const brsrc r = make_rsrc(p, (unsigned)g.nx * 8u);
-  if constexpr (V::FULL) bst2(r, v, (unsigned)t * 8u, 512u * (unsigned)e);
-  else bst2(r, v, (unsigned)(t + 64 * e - g.tap) * 8u, 0u);
+  if constexpr (V::FULL) bst2<kStream>(r, v, (unsigned)t * 8u, 512u * (unsigned)e);
+  else bst2<kStream>(r, v, (unsigned)(t + 64 * e - g.tap) * 8u, 0u);
 }
 
 // taper * exp(-i w phi) of the E samples of this lane (0 on padding samples
