@@ -13,7 +13,14 @@ adjoint over all rows (in place), i.e. 2 sweeps of 1024 x 499 x 4096 wavefield
 updates.  The metric is wavefield updates per second (x.z.w, summed over the
 shots), the reference's unit of work.
 
-Prints one JSON line.  --cpu-sample 0 skips the CPU baseline.
+Also reported in the same line: forward and adjoint sweep rates on their own,
+and the FWI gradient (WEM forward + residual + exact Born adjoint, 512 x 300,
+128 frequencies, 32 shots) with all rows batched against the same library
+called once per (shot, frequency) row (a timed sample of rows, scaled), with
+the relative difference of the two gradients on the sampled rows.
+
+Prints one JSON line.  --cpu-sample 0 skips the CPU baseline, --fwi-sample 0
+the gradient comparison.
 """
 
 from __future__ import annotations
@@ -103,6 +110,119 @@ def cpu_baseline(cfg, slow, freqs, src_ix, nrows_sample):
                       f"{dt:.2f} s"}
 
 
+# FWI gradient configuration: 512 x 300 grid, 128 frequencies 2-15 Hz, 32 shots,
+# SSF; true model = linear gradient 1500 -> 3500 m/s with a seeded +-5%
+# smooth field regridded to 512 x 300 plus two Gaussian anomalies of +-300 m/s
+# (radius 200 m); start model = the true model smoothed over 10 cells.
+FWI = dict(nx=512, nz=300, nw=128, fmin=2.0, fmax=15.0, ns=32, seed=4)
+
+
+def fwi_models():
+    from scipy.ndimage import gaussian_filter
+    c = FWI
+    nz, nx = c["nz"], c["nx"]
+    rng = np.random.default_rng(c["seed"])
+    z = np.arange(nz)[:, None] * 10.0
+    v = 1500.0 + 2000.0 * z / ((nz - 1) * 10.0) + np.zeros((nz, nx))
+    pert = gaussian_filter(rng.standard_normal((nz, nx)), 5.0)
+    v = v + 0.05 * v * pert / np.max(np.abs(pert))
+    X, Z = np.meshgrid(np.arange(nx) * 10.0, np.arange(nz) * 10.0)
+    for (x0, z0, a) in ((0.3 * nx * 10, 0.5 * nz * 10, 300.0), (0.7 * nx * 10, 0.6 * nz * 10, -300.0)):
+        v = v + a * np.exp(-((X - x0) ** 2 + (Z - z0) ** 2) / (200.0 ** 2))
+    v0 = gaussian_filter(v, 10.0)
+    return (1 / v ** 2).astype(np.complex64), (1 / v0 ** 2).astype(np.complex64)
+
+
+def fwi_gradient_bench(sample_rows: int):
+    """Wall time of one FWI gradient (WEM forward of the start model, residual,
+    exact Born adjoint) with all (shot, frequency) rows batched, against the
+    same library called once per (shot, frequency) row on a sample of rows."""
+    import torch
+    import wemfwi
+    from wemfwi import ops
+    c = FWI
+    nx, nz, nw, ns = c["nx"], c["nz"], c["nw"], c["ns"]
+    freqs = (c["fmin"] + np.arange(nw) * (c["fmax"] - c["fmin"]) / (nw - 1)).astype(np.float32)
+    wav = ricker_spectrum(freqs)
+    src_ix = [int(round(8 + s * (nx - 16) / (ns - 1))) for s in range(ns)]
+    rec_ix = list(range(nx))
+    s_true, s_start = fwi_models()
+    den = np.ones((nz, nx), np.complex64)
+    bins = np.arange(nw)
+
+    def survey(shots, fidx):
+        return ops.Survey([0] * len(shots), [src_ix[s] for s in shots], [0] * nx, rec_ix, freqs[fidx],
+                          np.arange(len(fidx)), wav[fidx], len(fidx))
+
+    m_true = wemfwi.Model(s_true, 10.0, 10.0, den=den)
+    m0 = wemfwi.Model(s_start, 10.0, 10.0, den=den)
+    sv = survey(list(range(ns)), bins)
+    dev = torch.device("cuda:0")
+    obs = torch.zeros(sv.data_shape, dtype=torch.complex64, device=dev)
+    ops.wem_forward(m_true, sv, obs)
+    pred = torch.zeros_like(obs)
+    gS = torch.zeros((nz, nx), dtype=torch.complex64, device=dev)
+    gR = torch.zeros_like(gS)
+
+    def batched():
+        ops.wem_forward(m0, sv, pred)
+        pred.sub_(obs)
+        ops.born_adjoint(m0, sv, gS, gR, pred, exact=True)
+
+    batched()  # warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batched()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    # plain loop over a sample of rows (one survey per row, built beforehand)
+    total = ns * nw
+    stride = max(1, total // sample_rows)
+    rows = [(s, i) for s in range(ns) for i in range(nw)][::stride][:sample_rows]
+    row_sv = [survey([s], [i]) for s, i in rows]
+    row_obs = [obs[s:s + 1, :, i:i + 1].contiguous() for s, i in rows]
+    d1 = torch.zeros((1, nx, 1), dtype=torch.complex64, device=dev)
+    pS = torch.zeros_like(gS)
+    pR = torch.zeros_like(gS)
+
+    def plain():
+        pS.zero_()
+        pR.zero_()
+        for k in range(len(rows)):
+            ops.wem_forward(m0, row_sv[k], d1)
+            d1.sub_(row_obs[k])
+            ops.born_adjoint(m0, row_sv[k], pS, pR, d1, add=True, exact=True)
+
+    plain()  # warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain()
+    torch.cuda.synchronize()
+    t_plain_sample = time.perf_counter() - t0
+    # the batched gradient restricted to the same rows, for the accuracy check
+    bS = torch.zeros_like(gS)
+    bR = torch.zeros_like(gS)
+    for s in sorted(set(s for s, _ in rows)):
+        idx = [i for (ss, i) in rows if ss == s]
+        svs = survey([s], idx)
+        d = torch.zeros(svs.data_shape, dtype=torch.complex64, device=dev)
+        ops.wem_forward(m0, svs, d)
+        d.sub_(obs[s:s + 1, :, idx].contiguous())
+        ops.born_adjoint(m0, svs, bS, bR, d, add=True, exact=True)
+    torch.cuda.synchronize()
+    rel = float(torch.linalg.vector_norm((pS - bS).to(torch.complex128)) /
+                torch.linalg.vector_norm(bS.to(torch.complex128)))
+    t_plain = t_plain_sample * total / len(rows)
+    upd = 12.0 * nx * (nz - 1) * total
+    return {"config": f"ssf nx={nx} nz={nz} nw={nw} {c['fmin']:g}-{c['fmax']:g} Hz ns={ns} nr={nx}",
+            "batched_ms": round(t_batched * 1e3, 2),
+            "batched_gupdates_per_s": round(upd / t_batched / 1e9, 2),
+            "plain_loop_ms": round(t_plain * 1e3, 1),
+            "plain_loop_sample": f"{len(rows)} of {total} rows timed ({t_plain_sample:.2f} s), scaled to all rows",
+            "speedup": round(t_plain / t_batched, 2),
+            "rel_err_plain_vs_batched": rel}
+
+
 def measured_traffic(config: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this configuration (mean over its launches, each
@@ -133,6 +253,8 @@ def main():
     ap.add_argument("--config", default="pspi1024", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="rows of the CPU baseline sample (0: skip)")
+    ap.add_argument("--fwi-sample", type=int, default=64,
+                    help="rows of the plain per-row loop in the FWI gradient comparison (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -199,9 +321,20 @@ def main():
         print("non-finite wavefield", file=sys.stderr)
         sys.exit(1)
 
+    # forward and adjoint sweeps timed on their own (phase tables in place)
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / reps
+    reps = max(1, args.steps)
+    t_fwd = timed(lambda: model.oneway_recursive(False, False, f_rows, S, W), reps)
+    t_adj = timed(lambda: model.oneway(False, True, f_rows, W), reps)
     traffic, traffic_src = measured_traffic(args.config, "oneway_kernel")
     line = {
-        "metric": "Gupdates/s (x.z.w wavefield updates, Down forward + adjoint)",
+        "metric": "Gupdates/s (x·z·ω) fwd+adj",
         "value": round(value, 3),
         "unit": "Gupdates/s",
         "n_gpus": 1,
@@ -225,6 +358,10 @@ def main():
                      "kernel": "oneway_kernel", "kernel_ms": round(avg, 3),
                      "phase_table_ms": round(t_ms / max(t_n, 1), 3)},
     }
+    line["fwd_gupdates_per_s"] = round(upd_sweep / t_fwd / 1e9, 3)
+    line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
+    if args.fwi_sample > 0:
+        line["fwi_gradient"] = fwi_gradient_bench(args.fwi_sample)
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)
     print(json.dumps(line), flush=True)
