@@ -38,6 +38,14 @@ hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* m
                       int dir, bool adj, bool exact, const float2* tw, hipStream_t s);
 hipError_t launch_scatter(int N, const StepRow* rows, int nrows, const DepthTables* models, StepGeom g, bool adj,
                           bool exact, const float2* tw, hipStream_t s);
+// One row of a batched product out (=|+=) op(a) * b over n samples.
+struct MulRow {
+  float2* out;
+  const float2* a;
+  const float2* b;
+};
+
+hipError_t launch_mul_rows(const MulRow* rows, int nrows, long long n, bool conjA, bool acc, hipStream_t s);
 hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, const float2* B, long long sb,
                       long long n, int nrows, bool conjA, bool acc, hipStream_t s);
 hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, bool acc,

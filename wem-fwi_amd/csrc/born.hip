@@ -125,6 +125,22 @@ __global__ void mul_kernel(float2* __restrict__ out, long long so, const float2*
   }
 }
 
+// Batched rows of out = op(a) * b (or +=): x over the samples of a row, y over
+// rows (strided when there are more rows than the grid's y extent).
+__global__ void mul_rows_kernel(const MulRow* __restrict__ rows, int nrows, long long n, int conjA, int acc) {
+  for (int r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const MulRow R = rows[r];
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+      const float2 a = ldg(R.a, (size_t)k);
+      const float2 b = ldg(R.b, (size_t)k);
+      float2 v = conjA ? cmulc(b, a) : cmul(a, b);
+      if (acc) v = cadd(R.out[k], v);
+      R.out[k] = v;
+    }
+  }
+}
+
 // out[i] (+)= sum_r in[r stride + i] in double precision, rows in order
 __global__ void reduce_rows_kernel(double2* __restrict__ out, const float2* __restrict__ in, long long stride,
                                    long long n, int nrows, int acc) {
@@ -365,6 +381,16 @@ hipError_t launch_scatter(int N, const StepRow* rows, int nrows, const DepthTabl
     case 2048: return launch_scatter_n<2048>(rows, nrows, models, g, adj, exact, tw, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_mul_rows(const MulRow* rows, int nrows, long long n, bool conjA, bool acc, hipStream_t s) {
+  if (n <= 0 || nrows <= 0) return hipSuccess;
+  long long bx = (n + 255) / 256;
+  if (bx > 64) bx = 64;
+  const int by = nrows < 65535 ? nrows : 65535;
+  hipLaunchKernelGGL(mul_rows_kernel, dim3((unsigned)bx, (unsigned)by), dim3(256), 0, s, rows, nrows, n,
+                     conjA ? 1 : 0, acc ? 1 : 0);
+  return hipGetLastError();
 }
 
 hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, const float2* B, long long sb,

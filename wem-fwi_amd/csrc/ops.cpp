@@ -211,6 +211,8 @@ struct Ctx {
   std::vector<DepthTables> htables;  // host staging of d_models (alive until the call ends)
   DevBuf d_models;             // DepthTables[nmodels]
   DevBuf scr;                  // [rows][N] scratch rows of the sweep kernel
+  std::vector<MulRow> hmul;    // batched products (host staging)
+  DevBuf dmul;
   DevBuf d_rows;
   std::vector<SweepRow> hsw;
   std::vector<LinRow> hlin;
@@ -306,11 +308,12 @@ struct Ctx {
     return sync();
   }
 
-  // rows: out_i (= or +=) op(A_i) * B_i, per-row pointers
-  int mul(int n, const std::vector<float2*>& out, const std::vector<const float2*>& A,
-          const std::vector<const float2*>& B, bool conjA, bool acc, size_t len) {
-    for (int i = 0; i < n; ++i)
-      HIPCHK(launch_mul(out[i], 0, A[i], 0, B[i], 0, (long long)len, 1, conjA, acc, s), "mul kernel");
+  // rows: out_i (= or +=) op(A_i) * B_i over len samples, one launch
+  int mul_rows(const std::vector<MulRow>& rows, size_t len, bool conjA, bool acc) {
+    if (rows.empty()) return WEMFWI_OK;
+    hmul = rows;
+    RCCHK(put(dmul, hmul, s, "product rows"));
+    HIPCHK(launch_mul_rows(dmul.as<MulRow>(), (int)hmul.size(), (long long)len, conjA, acc, s), "mul rows");
     return WEMFWI_OK;
   }
 
@@ -369,11 +372,9 @@ struct Ctx {
   // W_i *= (conj) a_i, a_i = per-row shared array (e.g. the row model's r)
   int scale_rows(const Rows& r, float2* W, const std::vector<const float2*>& a, bool conjA) {
     const int n = (int)r.shot.size();
-    for (int i = 0; i < n; ++i)
-      HIPCHK(launch_mul(W + (size_t)i * nzx, 0, a[i], 0, W + (size_t)i * nzx, 0, (long long)nzx, 1, conjA, false,
-                        s),
-             "scale rows");
-    return WEMFWI_OK;
+    std::vector<MulRow> m(n);
+    for (int i = 0; i < n; ++i) m[i] = MulRow{W + (size_t)i * nzx, a[i], W + (size_t)i * nzx};
+    return mul_rows(m, nzx, conjA, false);
   }
 
   int zero(void* p, size_t bytes) {
@@ -496,14 +497,18 @@ static int born_forward(const std::vector<Model*>& fm, Survey* sv, const float2*
     RCCHK(c.lin(r, +1, 0, 0, bg, dS, mstride, s1));
     RCCHK(c.sweep(r, 0, 0, s1, s1));
     RCCHK(c.record(r, data, s1, sv->d.ngr, sv->d.offset));
-    for (int i = 0; i < n; ++i) {
-      const size_t o = (size_t)i * c.nzx;
-      const float2* dri = drb.as<float2>() + (mstride ? (size_t)r.fi[i] * c.nzx : 0);
-      HIPCHK(launch_mul(tt + o, 0, bg + o, 0, dri, 0, (long long)c.nzx, 1, false, false, s), "ic");
-      HIPCHK(launch_mul(tt + o, 0, r.model[i]->d_refl, 0, s1 + o, 0, (long long)c.nzx, 1, false, true, s),
-             "reflect");
-      HIPCHK(launch_mul(bg + o, 0, r.model[i]->d_refl, 0, bg + o, 0, (long long)c.nzx, 1, false, false, s),
-             "reflect");
+    {
+      std::vector<MulRow> ic(n), rs1(n), rbg(n);
+      for (int i = 0; i < n; ++i) {
+        const size_t o = (size_t)i * c.nzx;
+        const float2* dri = drb.as<float2>() + (mstride ? (size_t)r.fi[i] * c.nzx : 0);
+        ic[i] = MulRow{tt + o, bg + o, dri};
+        rs1[i] = MulRow{tt + o, r.model[i]->d_refl, s1 + o};
+        rbg[i] = MulRow{bg + o, r.model[i]->d_refl, bg + o};
+      }
+      RCCHK(c.mul_rows(ic, c.nzx, false, false));   // D * dr      (Born_refl::forward)
+      RCCHK(c.mul_rows(rs1, c.nzx, false, true));   // + r * s1    (down-scattering)
+      RCCHK(c.mul_rows(rbg, c.nzx, false, false));  // r * D       (background up-going source)
     }
     RCCHK(c.sweep(r, 1, 0, tt, tt));
     RCCHK(c.record(r, data, tt, sv->d.ngr, sv->d.offset));
@@ -565,19 +570,26 @@ static int born_adjoint(const std::vector<Model*>& fm, Survey* sv, float2* gS, f
     RCCHK(c.sweep(r, 1, 1, R, R));
     RCCHK(c.zero(Q, bytes));
     RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset));
-    for (int i = 0; i < n; ++i) {
-      const size_t o = (size_t)i * c.nzx;
-      HIPCHK(launch_mul(cr + o, 0, bg + o, 0, R + o, 0, (long long)c.nzx, 1, true, false, s), "ic adjoint");
-      HIPCHK(launch_mul(Q + o, 0, r.model[i]->d_refl, 0, R + o, 0, (long long)c.nzx, 1, true, true, s),
-             "reflect adjoint");
+    {
+      std::vector<MulRow> ic(n), rq(n);
+      for (int i = 0; i < n; ++i) {
+        const size_t o = (size_t)i * c.nzx;
+        ic[i] = MulRow{cr + o, bg + o, R + o};
+        rq[i] = MulRow{Q + o, r.model[i]->d_refl, R + o};
+      }
+      RCCHK(c.mul_rows(ic, c.nzx, true, false));  // conj(D) * R   (Born_refl::adjoint)
+      RCCHK(c.mul_rows(rq, c.nzx, true, true));   // Q += conj(r) R
     }
     RCCHK(c.sweep(r, 0, 1, Q, Q));
     RCCHK(c.zero(cs, bytes));
     RCCHK(c.lin(r, +1, 1, exact, bg, Q, 0, cs));
-    for (int i = 0; i < n; ++i) {
-      const size_t o = (size_t)i * c.nzx;
-      HIPCHK(launch_mul(bg + o, 0, r.model[i]->d_refl, 0, bg + o, 0, (long long)c.nzx, 1, false, false, s),
-             "reflect");
+    {
+      std::vector<MulRow> rbg(n);
+      for (int i = 0; i < n; ++i) {
+        const size_t o = (size_t)i * c.nzx;
+        rbg[i] = MulRow{bg + o, r.model[i]->d_refl, bg + o};
+      }
+      RCCHK(c.mul_rows(rbg, c.nzx, false, false));  // r * D
     }
     RCCHK(c.sweep(r, 1, 0, bg, bg));
     RCCHK(c.lin(r, -1, 1, exact, bg, R, 0, cs));
@@ -654,13 +666,17 @@ static int bornrefl_forward(Model* m, Survey* sv, const float2* image, int nh, f
     RCCHK(c.zero(bg, bytes));
     RCCHK(c.inject_sources(r, bg, sv->d.ngs, 0));
     RCCHK(c.sweep(r, 0, 0, bg, bg));
-    for (int i = 0; i < n; ++i) {
-      const size_t o = (size_t)i * c.nzx;
-      if (nh <= 0) {
-        HIPCHK(launch_mul(sc + o, 0, bg + o, 0, image + (size_t)r.fi[i] * c.nzx, 0, (long long)c.nzx, 1, false,
-                          false, s),
-               "ic");
-      } else {
+    if (nh <= 0) {
+      std::vector<MulRow> ic(n);
+      for (int i = 0; i < n; ++i) {
+        const size_t o = (size_t)i * c.nzx;
+        ic[i] = MulRow{sc + o, bg + o, image + (size_t)r.fi[i] * c.nzx};
+      }
+      RCCHK(c.mul_rows(ic, c.nzx, false, false));
+    }
+    if (nh > 0) {
+      for (int i = 0; i < n; ++i) {
+        const size_t o = (size_t)i * c.nzx;
         HIPCHK(hipMemsetAsync(sc + o, 0, c.nzx * sizeof(float2), s), "memset");
         HIPCHK(launch_offset_ic_fwd(sc + o, bg + o, image, nh, c.nz, c.nx, s), "offset ic");
       }
@@ -700,10 +716,12 @@ static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const f
     RCCHK(c.inject_receivers(r, sc, data, 0, 0));
     RCCHK(c.sweep(r, 1, 1, sc, sc));
     if (nh <= 0) {
+      std::vector<MulRow> ic(n);
       for (int i = 0; i < n; ++i) {
         const size_t o = (size_t)i * c.nzx;
-        HIPCHK(launch_mul(cb + o, 0, bg + o, 0, sc + o, 0, (long long)c.nzx, 1, true, false, s), "ic adjoint");
+        ic[i] = MulRow{cb + o, bg + o, sc + o};
       }
+      RCCHK(c.mul_rows(ic, c.nzx, true, false));
       int i0 = 0;
       while (i0 < n) {
         int i1 = i0 + 1;
