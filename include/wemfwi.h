@@ -225,6 +225,45 @@ int wemfwi_offset_image_adjoint(wemfwi_model* model, wemfwi_survey* survey, int 
                                 wemfwi_complex* image, const wemfwi_complex* data, int add,
                                 void* stream);
 
+/* ---------------------------------------------------------------- unit operators
+ * Row-batched forms of the reference's unit operators (py_wem_ops.cpp:25-206),
+ * for operator tests and custom pipelines.  Wavefields are device arrays;
+ * add == 0 overwrites the output (the reference's add=false). */
+
+/* Scatter (Scatter.cpp:35-111) on nrows rows of nx samples, row i at depth[i]
+ * and frequency freq_hz[i]; mode 0 keeps Scatter::adjoint's operator order
+ * (Scatter.cpp:52-66), mode 1 is the exact adjoint. */
+int wemfwi_scatter(wemfwi_model* model, int adjoint, int mode, int nrows, const float* freq_hz,
+                   const int* depth, const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
+
+/* LinDown (up = 0) / LinUp (up = 1) (LinOneWay.cpp:33-142) on [nrows][nz][nx]
+ * with background wavefields bg[nrows][nz][nx]: forward
+ * out[iz+-1] += P_iz Scatter_iz(bg[iz] * in[iz]); adjoint the transpose. */
+int wemfwi_lin_oneway(wemfwi_model* model, int up, int adjoint, int mode, int nrows,
+                      const float* freq_hz, const wemfwi_complex* bg, const wemfwi_complex* in,
+                      wemfwi_complex* out, int add, void* stream);
+
+/* Reflect (Reflect.h:12-48) on [nrows][nz][nx]: out = r * in (adjoint conj(r) * in). */
+int wemfwi_reflect(wemfwi_model* model, int adjoint, int nrows, const wemfwi_complex* in,
+                   wemfwi_complex* out, int add, void* stream);
+
+/* dReflect (dReflect.h:11-84) of the model's background on [nz][nx]:
+ * forward dr = dReflect(dslow, dden); adjoint (dslow, dden) = dReflect^H(dr). */
+int wemfwi_dreflect(wemfwi_model* model, int adjoint, wemfwi_complex* dslow, wemfwi_complex* dden,
+                    wemfwi_complex* dr, int add, void* stream);
+
+/* IC (ic.cpp:5-45) on n samples: forward d = bg * m; adjoint m = conj(bg) * d. */
+int wemfwi_ic(int adjoint, long long n, const wemfwi_complex* bg, wemfwi_complex* m, wemfwi_complex* d,
+              int add, void* stream);
+
+/* Injection (Injection.h:19-65, Injection.cpp:8-80) of ndev devices at (iz, ix)
+ * (host arrays) with cos^2 spread ng and x offset tap, on nrows wavefields
+ * [nrows][nz][nx]: forward wfld[iz][ix+ig+tap] += g[ig] vals[row][dev];
+ * adjoint vals[row][dev] = sum g[ig] wfld[iz][ix+ig+tap]. */
+int wemfwi_injection(int nz, int nx, int ndev, const int* iz, const int* ix, int ng, int tap,
+                     int adjoint, int nrows, wemfwi_complex* vals, wemfwi_complex* wfld, int add,
+                     void* stream);
+
 /* Kernel timing with HIP events on the launch stream (off by default).
  * wemfwi_timing(1) clears and enables; wemfwi_timing_get returns the summed
  * duration (ms) and launch count of a kernel by name ("oneway_kernel",

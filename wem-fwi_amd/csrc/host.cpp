@@ -186,9 +186,9 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   if (d.taper == 1) return fail(WEMFWI_ERR_ARG, "model: taper width pad=1 is undefined (Taper.h divides by pad-1)");
   if (2 * d.taper > d.nx) return fail(WEMFWI_ERR_ARG, "model: taper wider than half the row");
   N = d.nx + 2 * d.tap;
-  if (!fft_size_supported(N))
-    return fail(WEMFWI_ERR_UNSUPPORTED, "model: FFT length nx+2*tap=" + std::to_string(N) +
-                                            " not supported (built: 128, 256, 512, 1024, 2048)");
+  // the background tables (Reflect, dReflect, scatterer coefficients,
+  // RefSampler) need no FFT; the propagating entry points check fft_ok
+  fft_ok = fft_size_supported(N);
   const size_t nzx = (size_t)d.nz * d.nx;
   slow.assign(slow_h, slow_h + nzx);
   if (den_h) den.assign(den_h, den_h + nzx);
@@ -315,7 +315,7 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   t.nz = d.nz;
   t.n = N;
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
-  if ((rc = twiddles(N, &d_tw))) return rc;
+  if (fft_ok && (rc = twiddles(N, &d_tw))) return rc;
   return WEMFWI_OK;
 }
 
@@ -334,7 +334,14 @@ StepGeom Model::geom() const {
 
 // Phase table of a set of frequencies: exp(-i kz dz)/N per (frequency,
 // active slot, k <= N/2) -- the k grid is symmetric, so P[N-k] = P[k].
+int Model::require_fft() const {
+  if (fft_ok) return WEMFWI_OK;
+  return fail(WEMFWI_ERR_UNSUPPORTED, "model: FFT length nx+2*tap=" + std::to_string(N) +
+                                          " not supported (built: 128, 256, 512, 1024, 2048)");
+}
+
 int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
+  if (int rc = require_fft()) return rc;
   std::vector<float> uniq(freq_hz, freq_hz + n);
   std::sort(uniq.begin(), uniq.end());
   uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
@@ -362,6 +369,7 @@ int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
 // frequency is not in it.
 int Model::phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow,
                         std::vector<const float2*>& prow, hipStream_t s) {
+  if (int rc = require_fft()) return rc;
   bool have = !freqs.empty();
   for (int i = 0; i < nrows && have; ++i) have = std::binary_search(freqs.begin(), freqs.end(), freq_hz[i]);
   if (!have) {

@@ -53,6 +53,7 @@ struct Workspace {
 struct Model {
   wemfwi_model_desc d{};
   int N = 0;         // FFT length nx + 2*tap
+  bool fft_ok = false;  // N is one of the built FFT lengths (propagation possible)
   int nslots = 0;    // active (depth, reference) pairs
   std::vector<std::complex<float>> slow, den, sref;
   std::vector<int> refid, slot_begin, slot_ref;
@@ -84,6 +85,7 @@ struct Model {
   std::vector<float> freqs;   // frequency set of the current phase table (sorted)
   int NH() const { return N / 2 + 1; }
   int set_freqs(int n, const float* freq_hz, hipStream_t s);
+  int require_fft() const;  // WEMFWI_ERR_UNSUPPORTED unless fft_ok
   int phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow, std::vector<const float2*>& prow,
                    hipStream_t s);
   int oneway(int up, int adjoint, int nrows, const float* freq_hz, const std::complex<float>* in,

@@ -234,6 +234,7 @@ struct Ctx {
     nx = m0->d.nx;
     N = m0->N;
     nzx = (size_t)nz * nx;
+    RCCHK(m0->require_fft());
     for (Model* m : ms)
       if (m->d.nz != nz || m->d.nx != nx || m->N != N || m->d.tap != m0->d.tap || m->d.pspi != m0->d.pspi ||
           m->d.taper != m0->d.taper || m->d.ntaylor != m0->d.ntaylor)
@@ -873,6 +874,183 @@ int wemfwi_offset_image_adjoint(wemfwi_model* model, wemfwi_survey* survey, int 
   if (!model || !survey || !image || !data || nh < 1 || nh % 2 == 0)
     return fail(WEMFWI_ERR_ARG, "offset_image_adjoint: invalid argument (nh must be odd)");
   return bornrefl_adjoint(&model->m, &survey->s, (float2*)image, nh, (const float2*)data, add, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ unit operators
+
+namespace wemfwi {
+
+// per-row frequency tables and zero/validation helpers of the unit operators
+static int unit_rows(Model& m, int nrows, const float* freq_hz, std::vector<float>& w,
+                     std::vector<const float2*>& p, hipStream_t s) {
+  if (nrows < 0 || (nrows > 0 && !freq_hz)) return fail(WEMFWI_ERR_ARG, "unit operator: invalid rows");
+  if (nrows == 0) return WEMFWI_OK;
+  return m.phase_tables(nrows, freq_hz, w, p, s);
+}
+
+static int unit_scatter(Model& m, int adjoint, int mode, int nrows, const float* freq_hz, const int* depth,
+                        const float2* in, float2* out, int add, hipStream_t s) {
+  if (!in || !out || (nrows > 0 && !depth)) return fail(WEMFWI_ERR_ARG, "scatter: null argument");
+  for (int i = 0; i < nrows; ++i)
+    if (depth[i] < 0 || depth[i] >= m.d.nz) return fail(WEMFWI_ERR_ARG, "scatter: depth out of range");
+  std::vector<float> w;
+  std::vector<const float2*> p;
+  RCCHK(unit_rows(m, nrows, freq_hz, w, p, s));
+  if (nrows == 0) return WEMFWI_OK;
+  std::vector<StepRow> h(nrows);
+  for (int i = 0; i < nrows; ++i)
+    h[i] = StepRow{in + (size_t)i * m.d.nx, out + (size_t)i * m.d.nx, p[i], w[i], depth[i], 0, add ? 1 : 0,
+                   nullptr};
+  DevBuf rows;
+  RCCHK(put(rows, h, s, "scatter rows"));
+  HIPCHK(launch_scatter(m.N, rows.as<StepRow>(), nrows, m.d_tables, m.geom(), adjoint != 0, mode != 0, m.d_tw, s),
+         "scatter kernel");
+  HIPCHK(hipStreamSynchronize(s), "scatter");
+  return WEMFWI_OK;
+}
+
+static int unit_lin(Model& m, int up, int adjoint, int mode, int nrows, const float* freq_hz, const float2* bg,
+                    const float2* in, float2* out, int add, hipStream_t s) {
+  if (!bg || !in || !out) return fail(WEMFWI_ERR_ARG, "lin_oneway: null argument");
+  std::vector<float> w;
+  std::vector<const float2*> p;
+  RCCHK(unit_rows(m, nrows, freq_hz, w, p, s));
+  if (nrows == 0) return WEMFWI_OK;
+  const size_t nzx = (size_t)m.d.nz * m.d.nx;
+  if (!add) {
+    HIPCHK(hipMemsetAsync(out, 0, nrows * nzx * sizeof(float2), s), "memset");
+  } else {
+    // the row the operator overwrites with zeros even when adding: LinDown
+    // forward row 0 (LinOneWay.cpp:33-58), LinDown adjoint and LinUp forward
+    // row nz-1 (LinOneWay.cpp:60-115); the kernel writes every other row
+    const int zrow = (!up && !adjoint) ? 0 : ((!up || !adjoint) ? m.d.nz - 1 : -1);
+    if (zrow >= 0)
+      HIPCHK(hipMemset2DAsync(out + (size_t)zrow * m.d.nx, nzx * sizeof(float2), 0, m.d.nx * sizeof(float2),
+                              nrows, s),
+             "memset");
+  }
+  std::vector<LinRow> h(nrows);
+  for (int i = 0; i < nrows; ++i)
+    h[i] = LinRow{bg + i * nzx, in + i * nzx, out + i * nzx, p[i], w[i], 0};
+  DevBuf rows;
+  RCCHK(put(rows, h, s, "lin rows"));
+  HIPCHK(launch_lin(m.N, rows.as<LinRow>(), nrows, m.d_tables, m.geom(), m.d.nz, up ? -1 : +1, adjoint != 0,
+                    mode != 0, m.d_tw, s),
+         "lin kernel");
+  HIPCHK(hipStreamSynchronize(s), "lin_oneway");
+  return WEMFWI_OK;
+}
+
+static int unit_injection(int nz, int nx, int ndev, const int* iz, const int* ix, int ng, int tap, int adjoint,
+                          int nrows, float2* vals, float2* wfld, int add, hipStream_t s) {
+  if (nz < 1 || nx < 1 || ndev < 1 || !iz || !ix || ng < 0 || tap < 0 || nrows < 0 || !vals || !wfld)
+    return fail(WEMFWI_ERR_ARG, "injection: invalid argument");
+  if (nrows == 0) return WEMFWI_OK;
+  DevPlan P;
+  std::vector<int> viz(iz, iz + ndev), vix(ix, ix + ndev);
+  int rc = build_plan(P, viz, vix, ng, tap, nx, nz);
+  if (rc) {
+    P.release();
+    return rc;
+  }
+  std::vector<long long> vb(nrows);
+  for (int i = 0; i < nrows; ++i) vb[i] = (long long)i * ndev;
+  DevBuf dvb;
+  const size_t nzx = (size_t)nz * nx;
+  hipError_t e = hipSuccess;
+  if ((rc = put(dvb, vb, s, "device offsets")) == WEMFWI_OK) {
+    if (!adjoint) {
+      if (!add) e = hipMemsetAsync(wfld, 0, nrows * nzx * sizeof(float2), s);
+      if (e == hipSuccess) e = launch_inject_csr(wfld, (long long)nzx, P.inj, vals, dvb.as<long long>(), 1, nrows, s);
+    } else {
+      if (!add) e = hipMemsetAsync(vals, 0, (size_t)nrows * ndev * sizeof(float2), s);
+      if (e == hipSuccess) e = launch_record(vals, dvb.as<long long>(), 1, wfld, (long long)nzx, P.rec, nrows, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "injection");
+  }
+  P.release();
+  return rc;
+}
+
+}  // namespace wemfwi
+
+extern "C" {
+
+int wemfwi_scatter(wemfwi_model* model, int adjoint, int mode, int nrows, const float* freq_hz, const int* depth,
+                   const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream) {
+  if (!model) return fail(WEMFWI_ERR_ARG, "scatter: null model");
+  return unit_scatter(model->m, adjoint, mode, nrows, freq_hz, depth, (const float2*)in, (float2*)out, add,
+                      (hipStream_t)stream);
+}
+
+int wemfwi_lin_oneway(wemfwi_model* model, int up, int adjoint, int mode, int nrows, const float* freq_hz,
+                      const wemfwi_complex* bg, const wemfwi_complex* in, wemfwi_complex* out, int add,
+                      void* stream) {
+  if (!model) return fail(WEMFWI_ERR_ARG, "lin_oneway: null model");
+  return unit_lin(model->m, up, adjoint, mode, nrows, freq_hz, (const float2*)bg, (const float2*)in, (float2*)out,
+                  add, (hipStream_t)stream);
+}
+
+int wemfwi_reflect(wemfwi_model* model, int adjoint, int nrows, const wemfwi_complex* in, wemfwi_complex* out,
+                   int add, void* stream) {
+  if (!model || !in || !out || nrows < 0) return fail(WEMFWI_ERR_ARG, "reflect: invalid argument");
+  if (nrows == 0) return WEMFWI_OK;
+  Model& m = model->m;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nzx = (size_t)m.d.nz * m.d.nx;
+  std::vector<MulRow> h(nrows);
+  for (int i = 0; i < nrows; ++i)
+    h[i] = MulRow{(float2*)out + i * nzx, m.d_refl, (const float2*)in + i * nzx};
+  DevBuf rows;
+  RCCHK(put(rows, h, s, "reflect rows"));
+  HIPCHK(launch_mul_rows(rows.as<MulRow>(), nrows, (long long)nzx, adjoint != 0, add != 0, s), "reflect");
+  HIPCHK(hipStreamSynchronize(s), "reflect");
+  return WEMFWI_OK;
+}
+
+int wemfwi_dreflect(wemfwi_model* model, int adjoint, wemfwi_complex* dslow, wemfwi_complex* dden,
+                    wemfwi_complex* dr, int add, void* stream) {
+  if (!model || !dslow || !dden || !dr) return fail(WEMFWI_ERR_ARG, "dreflect: null argument");
+  Model& m = model->m;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nzx = (size_t)m.d.nz * m.d.nx;
+  if (!adjoint) {
+    HIPCHK(launch_dreflect_fwd((float2*)dr, m.d_drefl, (const float2*)dslow, (const float2*)dden, m.d.nz, m.d.nx,
+                               add != 0, s),
+           "dreflect");
+  } else {
+    if (!add) {
+      HIPCHK(hipMemsetAsync(dslow, 0, nzx * sizeof(float2), s), "memset");
+      HIPCHK(hipMemsetAsync(dden, 0, nzx * sizeof(float2), s), "memset");
+    }
+    HIPCHK(launch_dreflect_adj((float2*)dslow, (float2*)dden, m.d_drefl, (const float2*)dr, m.d.nz, m.d.nx, s),
+           "dreflect adjoint");
+  }
+  HIPCHK(hipStreamSynchronize(s), "dreflect");
+  return WEMFWI_OK;
+}
+
+int wemfwi_ic(int adjoint, long long n, const wemfwi_complex* bg, wemfwi_complex* m, wemfwi_complex* d, int add,
+              void* stream) {
+  if (n < 0 || !bg || !m || !d) return fail(WEMFWI_ERR_ARG, "ic: invalid argument");
+  if (n == 0) return WEMFWI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (!adjoint)
+    HIPCHK(launch_mul((float2*)d, 0, (const float2*)bg, 0, (const float2*)m, 0, n, 1, false, add != 0, s), "ic");
+  else
+    HIPCHK(launch_mul((float2*)m, 0, (const float2*)bg, 0, (const float2*)d, 0, n, 1, true, add != 0, s),
+           "ic adjoint");
+  HIPCHK(hipStreamSynchronize(s), "ic");
+  return WEMFWI_OK;
+}
+
+int wemfwi_injection(int nz, int nx, int ndev, const int* iz, const int* ix, int ng, int tap, int adjoint, int nrows,
+                     wemfwi_complex* vals, wemfwi_complex* wfld, int add, void* stream) {
+  return unit_injection(nz, nx, ndev, iz, ix, ng, tap, adjoint, nrows, (float2*)vals, (float2*)wfld, add,
+                        (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -73,6 +73,12 @@ SIGNATURES = {
     "wemfwi_bornrefl_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_offset_image_forward": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
     "wemfwi_offset_image_adjoint": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
+    "wemfwi_scatter": (_i, [_vp, _i, _i, _i, _fp, _ip, _vp, _vp, _i, _vp]),
+    "wemfwi_lin_oneway": (_i, [_vp, _i, _i, _i, _i, _fp, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_reflect": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp]),
+    "wemfwi_dreflect": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_ic": (_i, [_i, C.c_longlong, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_injection": (_i, [_i, _i, _i, _ip, _ip, _i, _i, _i, _i, _vp, _vp, _i, _vp]),
     "wemfwi_timing": (_i, [_i]),
     "wemfwi_timing_get": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
 }
