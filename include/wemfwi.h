@@ -95,6 +95,13 @@ int wemfwi_freq_to_trace(int ntr, int nt, int tpad, const wemfwi_complex* data_f
 int wemfwi_device_indices(int n, float o_x, float o_z, float d_x, float d_z, float ox, float oz,
                           float dxv, float dzv, int* iz, int* ix);
 
+/* Irregular device geometry of Device::createIrregGeom (Device.h:65-85): device
+ * i at coordinates (x[i], z[i]) -- the "<name>x" / "<name>z" arrays of the JSON
+ * file named by the "coord" parameter -- gets ix = (int)((x[i] - ox)/dxv + 0.5f),
+ * iz likewise, in float32.  WEMFWI_ERR_ARG for a negative index. */
+int wemfwi_device_coords(int n, const float* x, const float* z, float ox, float oz, float dxv,
+                         float dzv, int* iz, int* ix);
+
 /* RefSampler (propagator/src/RefSampler.cpp:10-91), host only: reference
  * slownesses sref[nz][nref] and the reference selecting each x,
  * refid[nz][nx] (-1: none), including the stale reference-buffer behaviour of
@@ -110,6 +117,16 @@ int wemfwi_refsampler(int nz, int nx, const wemfwi_complex* slow, int nref,
 int wemfwi_model_create(const wemfwi_model_desc* desc, const wemfwi_complex* slow,
                         const wemfwi_complex* den, wemfwi_model** out);
 int wemfwi_model_destroy(wemfwi_model* model);
+
+/* Background model whose Reflect coefficients (Reflect.h:19-26) come from a
+ * different slowness than everything else: the conformal-map path builds
+ * Reflect from the mapped, padded slowness before scaling it by the map's
+ * Jacobian (wem_prop.cpp:35-38, born_prop.cpp:45-47), while RefSampler, the
+ * one-way steps, the scatterer and dReflect (built later, Born_refl.cpp:11)
+ * see the scaled one.  slow: the scaled slowness; slow_refl: the unscaled. */
+int wemfwi_model_create_refl(const wemfwi_model_desc* desc, const wemfwi_complex* slow,
+                             const wemfwi_complex* den, const wemfwi_complex* slow_refl,
+                             wemfwi_model** out);
 
 /* Frequency set of the model: evaluates the phase-shift multipliers
  * (phase_shift.ispc:8-14) of every active reference at every depth for the
@@ -204,6 +221,61 @@ int wemfwi_extborn_forward(wemfwi_model* const* models, wemfwi_survey* survey,
 int wemfwi_extborn_adjoint(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* dslow,
                            wemfwi_complex* dden, const wemfwi_complex* data, int add, int mode,
                            void* stream);
+
+/* ---------------------------------------------------------------- conformal maps
+ * The per-shot grids of PropParam::prepareConformalMaps (PropParam.h:89-131,
+ * conformal/src) and the shot x frequency operators on them
+ * (wem_prop.cpp:20-45, born_prop.cpp:28-70,136-174,219-231). */
+
+/* Map tables of every shot, host only.  kind 0: identity; 1: ShiftAndScale
+ * (source to the origin, farthest corner to |z| = 1; the scale carries over
+ * from shot to shot as PropParam.h:91 declares it outside the loop) followed
+ * by ComplexLog(eps_log); 2: Rotation(angles[s]) (the chain built there is
+ * unused, PropParam.h:123).  The model grid has nz x nx samples, origin
+ * (oz, ox), spacing (dz, dx); sources src_*[ns], receivers rec_*[nr].
+ * Outputs per shot s (n = nz*nx):
+ *   axes[4s..4s+3]  = o_x, d_x, o_z, d_z of the propagation grid;
+ *   fwd[s][n]       model -> grid gather (-1: zero) of cmap->forward(model, grid),
+ *                   including ComplexLog's uncleared row buffer;
+ *   inv[s][n]       grid -> model gather (-1: zero) of cmap->inverse (Born adjoint);
+ *   jac[s][2][n]    the two Jacobian factors of scaleByJacobian, in order;
+ *   nsrc[s], siz/six[s][nx]: mapped source devices (cmap 1: the first row of
+ *                   the log grid, ComplexLog.cpp:167-190);
+ *   riz/rix[s][nr]: mapped receivers.
+ * A mapped device outside the grid is WEMFWI_ERR_ARG (the reference would
+ * index outside its arrays). */
+int wemfwi_cmap_plan(int kind, int nz, int nx, float oz, float ox, float dz, float dx, int ns,
+                     const int* src_iz, const int* src_ix, int nr, const int* rec_iz, const int* rec_ix,
+                     float eps_log, const float* angles, float* axes, int* fwd, int* inv, float* jac,
+                     int* nsrc, int* siz, int* six, int* riz, int* rix);
+
+/* Per-shot device lists: shot `shot` injects its source at the nsrc points
+ * (src_iz, src_ix) and records its nr traces at (rec_iz, rec_ix); shots not
+ * set keep the survey's point source and shared receivers. */
+int wemfwi_survey_set_shot_devices(wemfwi_survey* survey, int shot, int nsrc, const int* src_iz,
+                                   const int* src_ix, const int* rec_iz, const int* rec_ix);
+
+/* WEM / BornFull with one background model per shot (models[ns], all of one
+ * size; grid spacing may differ): WEM::full_fwd and BornFull::full_fwd /
+ * full_adj on the mapped grids.  Perturbations are per shot,
+ * dslow/dden[ns][nz][nx] on the padded grids (the mapped model perturbation
+ * zero-padded, born_prop.cpp:50-58); the adjoint writes each shot's slice
+ * (before the inverse map of born_prop.cpp:217-231), summed over
+ * frequencies in a fixed order in double precision; mode as born_adjoint. */
+int wemfwi_shotwem_forward(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* data,
+                           int add, void* stream);
+int wemfwi_shotborn_forward(wemfwi_model* const* models, wemfwi_survey* survey,
+                            const wemfwi_complex* dslow, const wemfwi_complex* dden,
+                            wemfwi_complex* data, int add, void* stream);
+int wemfwi_shotborn_adjoint(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* dslow,
+                            wemfwi_complex* dden, const wemfwi_complex* data, int add, int mode,
+                            void* stream);
+
+/* out[j] (+)= sum_{c = ptr[j]}^{ptr[j+1]-1} in[src[c]] (device arrays; double
+ * accumulation, list order; empty lists give 0 or leave out[j] with add): the
+ * model <-> grid gathers of the conformal maps and their sums over shots. */
+int wemfwi_gather_sum(long long nout, const long long* ptr, const long long* src,
+                      const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
 
 /* BornRefl (ext_born_refl.cpp:8-110): frequency-extended reflectivity Born
  * operator / migration on the unpadded grid; image [nw][nz][nx] device.  The

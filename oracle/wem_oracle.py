@@ -40,6 +40,9 @@ reference's propagator (paths are relative to the reference repository):
   extborn_forward / _adjoint   propagator/src/ext_born_prop.cpp:8-194
   offset_ic_fwd / _adj         no reference counterpart (subsurface-offset
                                imaging condition, extends ic.cpp)
+  mapped_models,               propagator/src/wem_prop.cpp:20-45 and
+  wem_forward_mapped,          born_prop.cpp:28-70,136-174,217-231 (conformal
+  born_*_mapped                maps; tables from oracle/cmap_oracle.py)
   ===========================  ===============================================
 
 Quirks of the reference are reproduced, not fixed: the k grid with
@@ -690,6 +693,14 @@ class Survey:
     ngs: int = 0
     ngr: int = 0
     ng: int = 0              # BornRefl source spread
+    shot_src: Optional[list] = None   # per-shot (iz list, ix list) of mapped sources
+    shot_rec: Optional[list] = None   # per-shot (iz list, ix list) of mapped receivers
+
+    def sources(self, s: int):
+        return self.shot_src[s] if self.shot_src else ([self.src_iz[s]], [self.src_ix[s]])
+
+    def receivers(self, s: int):
+        return self.shot_rec[s] if self.shot_rec else (self.rec_iz, self.rec_ix)
 
     def step_params(self, padded: bool = True) -> StepParams:
         nxp = self.nx + 2 * self.pad if padded else self.nx
@@ -726,23 +737,27 @@ def wem_forward(sv: Survey, slow, den) -> np.ndarray:
     for s in range(sv.ns):
         for i in range(len(sv.freq)):
             w = sv.omega(i)
-            W = inj_wavelet_fwd(sv.wave_f[sv.index[i]], [sv.src_iz[s]], [sv.src_ix[s]],
-                                sv.ngs, sv.pad, sv.nz, p.nx)
-            W = down_fwd(W, w, sp, ref, p)
-            data[s, :, sv.index[i]] += inj_rec_adj(W, sv.rec_iz, sv.rec_ix, sv.ngr, sv.pad)
-            W = r * W
-            W = up_fwd(W, w, sp, ref, p)
-            data[s, :, sv.index[i]] += inj_rec_adj(W, sv.rec_iz, sv.rec_ix, sv.ngr, sv.pad)
+            _wem_unit(sv, s, i, sp, ref, r, p, data)
     return data
+
+
+def _wem_unit(sv, s, i, sp, ref, r, p, data):
+    """Per-(shot, frequency) body of WEM::full_fwd (wem_prop.cpp:52-72)."""
+    w = sv.omega(i)
+    W = inj_wavelet_fwd(sv.wave_f[sv.index[i]], *sv.sources(s), sv.ngs, sv.pad, sv.nz, p.nx)
+    W = down_fwd(W, w, sp, ref, p)
+    data[s, :, sv.index[i]] += inj_rec_adj(W, *sv.receivers(s), sv.ngr, sv.pad)
+    W = r * W
+    W = up_fwd(W, w, sp, ref, p)
+    data[s, :, sv.index[i]] += inj_rec_adj(W, *sv.receivers(s), sv.ngr, sv.pad)
 
 
 def _born_unit_fwd(sv, s, i, sp, dp, ref, r, dS, dR, p, data):
     """Per-(shot, frequency) body of BornFull::full_fwd (born_prop.cpp:75-113)."""
     w = sv.omega(i)
     bi = sv.index[i]
-    rec = (sv.rec_iz, sv.rec_ix, sv.ngr, sv.pad)
-    bg = inj_wavelet_fwd(sv.wave_f[bi], [sv.src_iz[s]], [sv.src_ix[s]], sv.ngs, sv.pad,
-                         sv.nz, p.nx)
+    rec = (*sv.receivers(s), sv.ngr, sv.pad)
+    bg = inj_wavelet_fwd(sv.wave_f[bi], *sv.sources(s), sv.ngs, sv.pad, sv.nz, p.nx)
     bg = down_fwd(bg, w, sp, ref, p)
     # forward scattering
     sc = lindown_fwd(dS, bg, w, sp, ref, p)
@@ -772,11 +787,10 @@ def _born_unit_adj(sv, s, i, sp, dp, ref, r, data, p, mode):
     bi = sv.index[i]
     nz, nxp = sv.nz, p.nx
     tr = data[s, :, bi]
-    rec = (sv.rec_iz, sv.rec_ix, sv.ngr, sv.pad, nz, nxp)
+    rec = (*sv.receivers(s), sv.ngr, sv.pad, nz, nxp)
     gS = np.zeros((nz, nxp), dtype=C128)
     gR = np.zeros((nz, nxp), dtype=C128)
-    bg = inj_wavelet_fwd(sv.wave_f[bi], [sv.src_iz[s]], [sv.src_ix[s]], sv.ngs, sv.pad,
-                         nz, nxp)
+    bg = inj_wavelet_fwd(sv.wave_f[bi], *sv.sources(s), sv.ngs, sv.pad, nz, nxp)
     bg = down_fwd(bg, w, sp, ref, p)
     # Born_refl::adjoint (Born_refl.cpp:29-40)
     sc = up_adj(inj_rec_fwd(tr, *rec), w, sp, ref, p)
@@ -978,6 +992,106 @@ def extborn_adjoint(sv: Survey, slow3, den3, data, mode: str = "compat"):
             gR[i] += b
     sl = slice(sv.pad, sv.pad + sv.nx)
     return gS[:, :, sl], gR[:, :, sl]
+
+
+# --------------------------------------------------------------------------
+# conformal-map grids (one model per shot)
+# --------------------------------------------------------------------------
+
+def mapped_models(slow, den, fwd, jac, pad: int):
+    """wem_prop.cpp:24-40 for every shot: the model gathered onto the shot's
+    grid (fwd[s], -1 = 0), edge-padded; Reflect's slowness is that padded map,
+    the propagation slowness the same scaled in complex64 by jac[s][0] then
+    jac[s][1], twice, over the first nz*nx samples of the padded array.
+    Returns [(scaled slowness, padded density, Reflect slowness)] in C128."""
+    s0 = np.asarray(slow, dtype=np.complex64).ravel()
+    d0 = np.asarray(den, dtype=np.complex64).ravel()
+    nz, nx = np.asarray(slow).shape
+    n = nz * nx
+    out = []
+    for k in range(fwd.shape[0]):
+        f = fwd[k]
+        g = np.where(f >= 0, s0[np.maximum(f, 0)], 0).astype(np.complex64).reshape(nz, nx)
+        gd = np.where(f >= 0, d0[np.maximum(f, 0)], 0).astype(np.complex64).reshape(nz, nx)
+        sp = np.pad(g, [(0, 0), (pad, pad)], mode="edge")
+        dp = np.pad(gd, [(0, 0), (pad, pad)], mode="edge")
+        refl = sp.copy()
+        flat = sp.reshape(-1)
+        for _ in range(2):
+            for j in (jac[k, 0], jac[k, 1]):
+                flat[:n] = (flat[:n].real * j + 1j * (flat[:n].imag * j)).astype(np.complex64)
+        out.append((sp.astype(C128), dp.astype(C128), refl.astype(C128)))
+    return out
+
+
+def _shot_params(sv: Survey, axes_s) -> StepParams:
+    return StepParams(nx=sv.nx + 2 * sv.pad, dx=float(axes_s[1]), dz=float(axes_s[3]), tap=sv.tap, pad=sv.pad,
+                      nref=sv.nref, eps=sv.eps, prop=sv.prop, ntaylor=sv.ntaylor)
+
+
+def wem_forward_mapped(sv: Survey, slow, den, fwd, jac, axes) -> np.ndarray:
+    """WEM::full_fwd on the per-shot grids (sv carries the mapped devices)."""
+    data = np.zeros((sv.ns, sv.nr, sv.nt_f), dtype=C128)
+    for s, (sp, dp, rs) in enumerate(mapped_models(slow, den, fwd, jac, sv.pad)):
+        p = _shot_params(sv, axes[s])
+        ref = refsampler(sp, sv.nref)
+        r = reflect_coef(rs, dp)
+        for i in range(len(sv.freq)):
+            _wem_unit(sv, s, i, sp, ref, r, p, data)
+    return data
+
+
+def _gather_pad(m, f, pad):
+    """cmap->forward of a perturbation then Pad (born_prop.cpp:50-58)."""
+    m0 = np.asarray(m, dtype=C128).ravel()
+    nz, nx = np.asarray(m).shape
+    g = np.where(f >= 0, m0[np.maximum(f, 0)], 0).reshape(nz, nx)
+    return zero_pad(g, pad)
+
+
+def born_forward_mapped(sv: Survey, slow, den, dslow, dden, fwd, jac, axes) -> np.ndarray:
+    """BornFull::full_fwd on the per-shot grids (born_prop.cpp:28-113)."""
+    data = np.zeros((sv.ns, sv.nr, sv.nt_f), dtype=C128)
+    for s, (sp, dp, rs) in enumerate(mapped_models(slow, den, fwd, jac, sv.pad)):
+        p = _shot_params(sv, axes[s])
+        ref = refsampler(sp, sv.nref)
+        r = reflect_coef(rs, dp)
+        dS = _gather_pad(dslow, fwd[s], sv.pad)
+        dR = _gather_pad(dden, fwd[s], sv.pad)
+        for i in range(len(sv.freq)):
+            _born_unit_fwd(sv, s, i, sp, dp, ref, r, dS, dR, p, data)
+    return data
+
+
+def born_adjoint_mapped(sv: Survey, slow, den, data, fwd, inv, jac, axes, mode: str = "compat",
+                        map_adjoint: str = "inverse"):
+    """BornFull::full_adj on the per-shot grids (born_prop.cpp:136-231): each
+    shot's (dS, dR), summed over frequencies, is truncated (Pad::adjoint) and
+    carried back by the inverse map (map_adjoint='inverse', born_prop.cpp:220,227)
+    or by the transpose of the forward gather ('transpose'), then summed over
+    shots."""
+    nz, nx = sv.nz, sv.nx
+    gS = np.zeros(nz * nx, dtype=C128)
+    gR = np.zeros(nz * nx, dtype=C128)
+    sl = slice(sv.pad, sv.pad + nx)
+    for s, (sp, dp, rs) in enumerate(mapped_models(slow, den, fwd, jac, sv.pad)):
+        p = _shot_params(sv, axes[s])
+        ref = refsampler(sp, sv.nref)
+        r = reflect_coef(rs, dp)
+        aS = np.zeros((nz, p.nx), dtype=C128)
+        aR = np.zeros((nz, p.nx), dtype=C128)
+        for i in range(len(sv.freq)):
+            a, b = _born_unit_adj(sv, s, i, sp, dp, ref, r, data, p, mode)
+            aS += a
+            aR += b
+        for acc, g in ((gS, aS[:, sl].ravel()), (gR, aR[:, sl].ravel())):
+            if map_adjoint == "inverse":
+                v = inv[s]
+                acc += np.where(v >= 0, g[np.maximum(v, 0)], 0)
+            else:
+                f = fwd[s]
+                np.add.at(acc, f[f >= 0], g[f >= 0])
+    return gS.reshape(nz, nx), gR.reshape(nz, nx)
 
 
 def cdot(a: np.ndarray, b: np.ndarray) -> float:

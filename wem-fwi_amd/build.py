@@ -11,7 +11,12 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libwemfwi.so")
-SOURCES = ["kernels.hip", "born.hip", "host.cpp", "ops.cpp", "params.cpp", "traces.hip"]
+SOURCES = ["kernels.hip", "born.hip", "host.cpp", "ops.cpp", "params.cpp", "traces.hip", "cmap.cpp"]
+# Host-only sources compiled as plain C++: the conformal-map geometry must see
+# the C library's complex functions (std::abs of std::complex<float> is glibc's
+# hypotf there; HIP compile mode substitutes an inline formula that rounds
+# differently, and the maps truncate coordinates to grid indices).
+PLAIN_CXX = {"cmap.cpp"}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -25,8 +30,11 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".", "_") + ".o")
-        lang = ["-x", "hip"]
-        cmd = [HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src in PLAIN_CXX:
+            cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-x", "c++", "-I", os.path.join(HERE, "..", "include"),
+                   "-c", os.path.join(CSRC, src), "-o", obj]
+        else:
+            cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

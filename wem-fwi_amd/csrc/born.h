@@ -51,6 +51,8 @@ hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, 
 hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, bool acc,
                               hipStream_t s);
 hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s);
+hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, const long long* src, const float2* in,
+                             bool add, hipStream_t s);
 hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hipStream_t s);
 hipError_t launch_inject_point(float2* W, long long wstride, int nx, const int* iz, const int* ix,
                                const float2* val, const float* gw, int ng, int tap, int nrows, hipStream_t s);

@@ -167,6 +167,28 @@ __global__ void add_kernel(float2* __restrict__ out, const float2* __restrict__ 
     out[i] = cadd(out[i], in[i]);
 }
 
+// out[j] (+)= sum of in[src[c]] over c in [ptr[j], ptr[j+1]), in list order,
+// in double precision; an empty list gives zero (or leaves out[j] with add):
+// the gathers that carry a model between grids, and their sums over shots.
+__global__ void gather_sum_kernel(float2* __restrict__ out, long long n, const long long* __restrict__ ptr,
+                                  const long long* __restrict__ src, const float2* __restrict__ in, int add) {
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (long long)gridDim.x * blockDim.x) {
+    double sx = 0.0, sy = 0.0;
+    for (long long c = ptr[j]; c < ptr[j + 1]; ++c) {
+      const float2 v = in[src[c]];
+      sx += v.x;
+      sy += v.y;
+    }
+    if (add) {
+      const float2 o = out[j];
+      sx += o.x;
+      sy += o.y;
+    }
+    out[j] = make_float2((float)sx, (float)sy);
+  }
+}
+
 // out (= or +=) (float2) in
 __global__ void d2f_kernel(float2* __restrict__ out, const double2* __restrict__ in, long long n, int acc) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -405,6 +427,13 @@ hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, 
                               hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(reduce_rows_kernel, grid1d(n), dim3(256), 0, s, out, in, stride, n, nrows, acc ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, const long long* src, const float2* in,
+                             bool add, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_sum_kernel, grid1d(n), dim3(256), 0, s, out, n, ptr, src, in, add ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -178,7 +178,7 @@ Model::~Model() {
   scratch.release();
 }
 
-int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_h) {
+int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_h, const cf* slow_refl_h) {
   d = desc;
   if (d.nz < 2 || d.nx < 2 || d.nref < 1 || d.tap < 0 || d.taper < 0 || d.ntaylor < 0 ||
       d.ntaylor > 4 || !(d.dx > 0) || !(d.dz > 0))
@@ -283,16 +283,20 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
       const cd v = coef[it] * std::pow(s - (double)d.eps * s.real(), -0.5 - it);
       sc[(size_t)it * nzx + i] = make_float2((float)v.real(), (float)v.imag());
     }
-  // Reflect.h:19-26 and dReflect.h:18-27,40-47 (coefficients as written)
+  // Reflect.h:19-26 and dReflect.h:18-27,40-47 (coefficients as written);
+  // Reflect reads slow_refl when one is given
+  const cf* srf = slow_refl_h ? slow_refl_h : slow.data();
   std::vector<float2> rf(nzx, make_float2(0.f, 0.f)), dr(4 * nzx, make_float2(0.f, 0.f));
   for (int iz = 1; iz < d.nz; ++iz)
     for (int x = 0; x < d.nx; ++x) {
       const size_t i = (size_t)iz * d.nx + x, j = i - d.nx;
       const cd s1(slow[j].real(), slow[j].imag()), s2(slow[i].real(), slow[i].imag());
       const cd r1(den[j].real(), den[j].imag()), r2(den[i].real(), den[i].imag());
-      const cd a = r2 * std::sqrt(s1), b = r1 * std::sqrt(s2);
-      const cd r = (a - b) / (a + b);
+      const cd q1(srf[j].real(), srf[j].imag()), q2(srf[i].real(), srf[i].imag());
+      const cd ra = r2 * std::sqrt(q1), rb = r1 * std::sqrt(q2);
+      const cd r = (ra - rb) / (ra + rb);
       rf[i] = make_float2((float)r.real(), (float)r.imag());
+      const cd a = r2 * std::sqrt(s1), b = r1 * std::sqrt(s2);
       const cd v = 1.0 / ((a + b) * (a + b));
       const cd s1s2sq = r1 * r2 / v, r1r2sq = 2.0 * s1 * s2 / v;
       const cd c = std::sqrt(s2 / s1);
@@ -314,6 +318,8 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   t.scoef_stride = (long long)nzx;
   t.nz = d.nz;
   t.n = N;
+  t.dz = d.dz;
+  t.dk = (float)(2.0 * M_PI / ((double)d.dx * (double)(N - 1)));
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
   if (fft_ok && (rc = twiddles(N, &d_tw))) return rc;
   return WEMFWI_OK;
@@ -325,8 +331,6 @@ StepGeom Model::geom() const {
   g.tap = d.tap;
   g.ssf = d.pspi ? 0 : 1;
   g.full = (d.tap == 0 && d.nx == N) ? 1 : 0;
-  g.dz = d.dz;
-  g.dk = (float)(2.0 * M_PI / ((double)d.dx * (double)(N - 1)));
   g.ntaylor = d.ntaylor;
   g.taper = d.taper > 0 ? 1 : 0;
   return g;
@@ -498,6 +502,21 @@ int wemfwi_model_create(const wemfwi_model_desc* desc, const wemfwi_complex* slo
   wemfwi_model* h = new (std::nothrow) wemfwi_model();
   if (!h) return fail(WEMFWI_ERR_ALLOC, "model_create: out of host memory");
   int rc = h->m.build(*desc, (const cf*)slow, (const cf*)den);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return WEMFWI_OK;
+}
+
+int wemfwi_model_create_refl(const wemfwi_model_desc* desc, const wemfwi_complex* slow, const wemfwi_complex* den,
+                             const wemfwi_complex* slow_refl, wemfwi_model** out) {
+  if (!desc || !slow || !slow_refl || !out) return fail(WEMFWI_ERR_ARG, "model_create_refl: null argument");
+  *out = nullptr;
+  wemfwi_model* h = new (std::nothrow) wemfwi_model();
+  if (!h) return fail(WEMFWI_ERR_ALLOC, "model_create_refl: out of host memory");
+  int rc = h->m.build(*desc, (const cf*)slow, (const cf*)den, (const cf*)slow_refl);
   if (rc) {
     delete h;
     return rc;

@@ -80,7 +80,10 @@ struct Model {
   std::vector<StepRow> h_step;
 
   ~Model();
-  int build(const wemfwi_model_desc& desc, const std::complex<float>* slow, const std::complex<float>* den);
+  // slow_refl: slowness of the Reflect coefficients when it differs from the
+  // propagation slowness (NULL: the same)
+  int build(const wemfwi_model_desc& desc, const std::complex<float>* slow, const std::complex<float>* den,
+            const std::complex<float>* slow_refl = nullptr);
   StepGeom geom() const;
   std::vector<float> freqs;   // frequency set of the current phase table (sorted)
   int NH() const { return N / 2 + 1; }

@@ -52,7 +52,16 @@ struct Survey {
   std::vector<float> freq;
   std::vector<cf> wave;
   DevPlan rplan;  // receivers on the current grid
-  ~Survey() { rplan.release(); }
+  // per-shot device lists (conformal maps give every shot its own grid):
+  // empty unless set; then shot s has sources ssrc_*[s] and receivers srec_*[s]
+  std::vector<std::vector<int>> ssrc_iz, ssrc_ix, srec_iz, srec_ix;
+  std::vector<DevPlan> ssrc_plan, srec_plan;
+  bool per_shot() const { return !ssrc_iz.empty(); }
+  ~Survey() {
+    rplan.release();
+    for (auto& p : ssrc_plan) p.release();
+    for (auto& p : srec_plan) p.release();
+  }
 };
 
 template <typename T>
@@ -184,9 +193,15 @@ static int put(DevBuf& b, const std::vector<T>& h, hipStream_t s, const char* wh
     if (_rc) return _rc;   \
   } while (0)
 
+// How the models of a call are indexed: one per frequency (the
+// frequency-extended operators; a single model is nw copies of it) or one per
+// shot (conformal maps).
+enum class Per { Freq, Shot };
+
 // A chunk of (shot, frequency) rows with their per-row models.
 struct Rows {
   std::vector<int> shot, fi;           // shot and frequency index of each row
+  std::vector<int> slice;              // index of the row's model / perturbation slice
   std::vector<Model*> model;           // background model of each row
 };
 
@@ -296,7 +311,7 @@ struct Ctx {
     hlin.resize(n);
     for (int i = 0; i < n; ++i) {
       hlin[i].bg = bg + (size_t)i * nzx;
-      hlin[i].m = m + (adjoint ? (size_t)i * nzx : (size_t)0) + (adjoint ? 0 : m_stride * (size_t)r.fi[i]);
+      hlin[i].m = m + (adjoint ? (size_t)i * nzx : (size_t)0) + (adjoint ? 0 : m_stride * (size_t)r.slice[i]);
       hlin[i].out = out + (size_t)i * nzx;
       hlin[i].ptab = ptab(r.model[i], r.fi[i]);
       hlin[i].w = omega(r.model[i], r.fi[i]);
@@ -318,8 +333,22 @@ struct Ctx {
     return WEMFWI_OK;
   }
 
+  // consecutive rows of one shot: [i0, i1)
+  template <typename F>
+  int shot_runs(const Rows& r, F&& f) {
+    const int n = (int)r.shot.size();
+    for (int i0 = 0; i0 < n;) {
+      int i1 = i0 + 1;
+      while (i1 < n && r.shot[i1] == r.shot[i0]) ++i1;
+      RCCHK(f(i0, i1));
+      i0 = i1;
+    }
+    return WEMFWI_OK;
+  }
+
   // sources of the rows' shots, value = wavelet of the row's frequency
   int inject_sources(const Rows& r, float2* W, int ng, int offset) {
+    if (sv->per_shot()) return inject_shot_sources(r, W, ng, offset);
     const int n = (int)r.shot.size();
     hiz.resize(n);
     hix.resize(n);
@@ -342,6 +371,29 @@ struct Ctx {
     return WEMFWI_OK;
   }
 
+  // per-shot source lists: gather by cell over each shot's plan, every
+  // device of a row carrying the row's wavelet value
+  int inject_shot_sources(const Rows& r, float2* W, int ng, int offset) {
+    const int n = (int)r.shot.size();
+    hval.resize(n);
+    hvb.resize(n);
+    for (int i = 0; i < n; ++i) {
+      const cf v = sv->wave[r.fi[i]];
+      hval[i] = make_float2(v.real(), v.imag());
+      hvb[i] = i;
+    }
+    RCCHK(put(dval, hval, s, "source values"));
+    RCCHK(put(dvb, hvb, s, "source value index"));
+    return shot_runs(r, [&](int i0, int i1) {
+      const int sh = r.shot[i0];
+      RCCHK(build_plan(sv->ssrc_plan[sh], sv->ssrc_iz[sh], sv->ssrc_ix[sh], ng, offset, nx, nz));
+      HIPCHK(launch_inject_csr(W + (size_t)i0 * nzx, (long long)nzx, sv->ssrc_plan[sh].inj, dval.as<float2>() + i0,
+                               dvb.as<long long>(), 0, i1 - i0, s),
+             "inject sources");
+      return WEMFWI_OK;
+    });
+  }
+
   int vbase(const Rows& r) {
     const int n = (int)r.shot.size();
     hvb.resize(n);
@@ -352,6 +404,17 @@ struct Ctx {
 
   // W_i += R^H d (receivers of the rows' traces)
   int inject_receivers(const Rows& r, float2* W, const float2* data, int ng, int offset) {
+    if (sv->per_shot()) {
+      RCCHK(vbase(r));
+      return shot_runs(r, [&](int i0, int i1) {
+        const int sh = r.shot[i0];
+        RCCHK(build_plan(sv->srec_plan[sh], sv->srec_iz[sh], sv->srec_ix[sh], ng, offset, nx, nz));
+        HIPCHK(launch_inject_csr(W + (size_t)i0 * nzx, (long long)nzx, sv->srec_plan[sh].inj, data,
+                                 dvb.as<long long>() + i0, sv->d.nt, i1 - i0, s),
+               "inject receivers");
+        return WEMFWI_OK;
+      });
+    }
     RCCHK(build_plan(sv->rplan, sv->rec_iz, sv->rec_ix, ng, offset, nx, nz));
     RCCHK(vbase(r));
     HIPCHK(launch_inject_csr(W, (long long)nzx, sv->rplan.inj, data, dvb.as<long long>(), sv->d.nt,
@@ -362,6 +425,18 @@ struct Ctx {
 
   // data += R W_i
   int record(const Rows& r, float2* data, const float2* W, int ng, int offset) {
+    if (sv->per_shot()) {
+      RCCHK(vbase(r));
+      RCCHK(shot_runs(r, [&](int i0, int i1) {
+        const int sh = r.shot[i0];
+        RCCHK(build_plan(sv->srec_plan[sh], sv->srec_iz[sh], sv->srec_ix[sh], ng, offset, nx, nz));
+        HIPCHK(launch_record(data, dvb.as<long long>() + i0, sv->d.nt, W + (size_t)i0 * nzx, (long long)nzx,
+                             sv->srec_plan[sh].rec, i1 - i0, s),
+               "record receivers");
+        return WEMFWI_OK;
+      }));
+      return sync();
+    }
     RCCHK(build_plan(sv->rplan, sv->rec_iz, sv->rec_ix, ng, offset, nx, nz));
     RCCHK(vbase(r));
     HIPCHK(launch_record(data, dvb.as<long long>(), sv->d.nt, W, (long long)nzx, sv->rplan.rec,
@@ -388,7 +463,7 @@ struct Ctx {
   }
 };
 
-static std::vector<Rows> make_chunks(int ns, int nw, const std::vector<Model*>& per_freq_model, size_t per_row,
+static std::vector<Rows> make_chunks(int ns, int nw, const std::vector<Model*>& fm, Per per, size_t per_row,
                                      bool freq_major) {
   std::vector<std::pair<int, int>> all;
   if (freq_major) {
@@ -405,7 +480,9 @@ static std::vector<Rows> make_chunks(int ns, int nw, const std::vector<Model*>& 
     for (size_t k = b; k < std::min(all.size(), b + c); ++k) {
       r.shot.push_back(all[k].first);
       r.fi.push_back(all[k].second);
-      r.model.push_back(per_freq_model[all[k].second]);
+      const int sl = per == Per::Shot ? all[k].first : all[k].second;
+      r.slice.push_back(sl);
+      r.model.push_back(fm[sl]);
     }
     out.push_back(r);
   }
@@ -421,7 +498,7 @@ static int check_survey(Survey* sv, Model* m) {
 
 // WEM::full_fwd (wem_prop.cpp:52-74) for every (shot, frequency) row:
 // inject, Down, record, r*W, Up, record.
-static int wem_forward(const std::vector<Model*>& fm, Survey* sv, float2* data, int add, hipStream_t s) {
+static int wem_forward(const std::vector<Model*>& fm, Per per, Survey* sv, float2* data, int add, hipStream_t s) {
   RCCHK(check_survey(sv, fm[0]));
   Ctx c;
   std::vector<Model*> uniq;
@@ -430,7 +507,7 @@ static int wem_forward(const std::vector<Model*>& fm, Survey* sv, float2* data, 
   RCCHK(c.init(uniq, sv, s));
   const size_t dbytes = (size_t)sv->d.ns * sv->d.nr * sv->d.nt * sizeof(float2);
   if (!add) RCCHK(c.zero(data, dbytes));
-  auto chunks = make_chunks(sv->d.ns, sv->d.nw, fm, c.nzx * sizeof(float2), false);
+  auto chunks = make_chunks(sv->d.ns, sv->d.nw, fm, per, c.nzx * sizeof(float2), false);
   DevBuf W;
   RCCHK(W.alloc(chunks[0].shot.size() * c.nzx * sizeof(float2), "wavefield"));
   float2* w = W.as<float2>();
@@ -460,7 +537,7 @@ static int wem_forward(const std::vector<Model*>& fm, Survey* sv, float2* data, 
 //        (Born_refl::forward and the down-scattering term share one Up)
 //   U  = Up(r * D)
 //   s3 = Up(LinUp(dS; U)),                              record s3
-static int born_forward(const std::vector<Model*>& fm, Survey* sv, const float2* dS, const float2* dR,
+static int born_forward(const std::vector<Model*>& fm, Per per, Survey* sv, const float2* dS, const float2* dR,
                         size_t mstride, float2* data, int add, hipStream_t s) {
   RCCHK(check_survey(sv, fm[0]));
   std::vector<Model*> uniq;
@@ -471,8 +548,8 @@ static int born_forward(const std::vector<Model*>& fm, Survey* sv, const float2*
   const size_t dbytes = (size_t)sv->d.ns * sv->d.nr * sv->d.nt * sizeof(float2);
   if (!add) RCCHK(c.zero(data, dbytes));
   const int nw = sv->d.nw;
-  // reflectivity perturbation of every frequency's model slice
-  const size_t nslice = mstride ? (size_t)nw : 1;
+  // reflectivity perturbation of every model slice (frequency or shot)
+  const size_t nslice = mstride ? fm.size() : 1;
   DevBuf drb;
   RCCHK(drb.alloc(nslice * c.nzx * sizeof(float2), "dr"));
   for (size_t k = 0; k < nslice; ++k) {
@@ -481,7 +558,7 @@ static int born_forward(const std::vector<Model*>& fm, Survey* sv, const float2*
                                c.nx, false, s),
            "dreflect");
   }
-  auto chunks = make_chunks(sv->d.ns, nw, fm, 3 * c.nzx * sizeof(float2), false);
+  auto chunks = make_chunks(sv->d.ns, nw, fm, per, 3 * c.nzx * sizeof(float2), false);
   const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
   DevBuf B, S1, T;
   RCCHK(B.alloc(cap, "background"));
@@ -502,7 +579,7 @@ static int born_forward(const std::vector<Model*>& fm, Survey* sv, const float2*
       std::vector<MulRow> ic(n), rs1(n), rbg(n);
       for (int i = 0; i < n; ++i) {
         const size_t o = (size_t)i * c.nzx;
-        const float2* dri = drb.as<float2>() + (mstride ? (size_t)r.fi[i] * c.nzx : 0);
+        const float2* dri = drb.as<float2>() + (mstride ? (size_t)r.slice[i] * c.nzx : 0);
         ic[i] = MulRow{tt + o, bg + o, dri};
         rs1[i] = MulRow{tt + o, r.model[i]->d_refl, s1 + o};
         rbg[i] = MulRow{bg + o, r.model[i]->d_refl, bg + o};
@@ -533,7 +610,7 @@ static int born_forward(const std::vector<Model*>& fm, Survey* sv, const float2*
 // ext_rowlast: reproduce ext_born_prop.cpp's serial shot loop, in which each
 // LinDown::adjoint zeroes the last depth row (LinOneWay.cpp:64-65) so only
 // the last shot's LinUp contribution remains there.
-static int born_adjoint(const std::vector<Model*>& fm, Survey* sv, float2* gS, float2* gR, size_t mstride,
+static int born_adjoint(const std::vector<Model*>& fm, Per per, Survey* sv, float2* gS, float2* gR, size_t mstride,
                         const float2* data, int add, int exact, bool ext_rowlast, hipStream_t s) {
   RCCHK(check_survey(sv, fm[0]));
   std::vector<Model*> uniq;
@@ -542,15 +619,16 @@ static int born_adjoint(const std::vector<Model*>& fm, Survey* sv, float2* gS, f
   Ctx c;
   RCCHK(c.init(uniq, sv, s));
   const int nw = sv->d.nw, ns = sv->d.ns;
-  const size_t nslice = mstride ? (size_t)nw : 1;
+  const size_t nslice = mstride ? fm.size() : 1;
   // FP64 accumulators per model slice: dr, dS
   DevBuf accDR, accS;
   RCCHK(accDR.alloc(nslice * c.nzx * sizeof(double2), "dr accumulator"));
   RCCHK(accS.alloc(nslice * c.nzx * sizeof(double2), "dS accumulator"));
   RCCHK(c.zero(accDR.p, accDR.n));
   RCCHK(c.zero(accS.p, accS.n));
-  // frequency-major rows so that the shots of one frequency are contiguous
-  auto chunks = make_chunks(ns, nw, fm, 5 * c.nzx * sizeof(float2), true);
+  // rows of one model slice contiguous: frequency-major for per-frequency
+  // models, shot-major for per-shot models
+  auto chunks = make_chunks(ns, nw, fm, per, 5 * c.nzx * sizeof(float2), per == Per::Freq);
   const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
   DevBuf B, Rb, Qb, CS, CR;
   RCCHK(B.alloc(cap, "background"));
@@ -600,15 +678,15 @@ static int born_adjoint(const std::vector<Model*>& fm, Survey* sv, float2* gS, f
           HIPCHK(hipMemsetAsync(cs + (size_t)i * c.nzx + (size_t)(c.nz - 1) * c.nx, 0, c.nx * sizeof(float2), s),
                  "memset");
     }
-    // deterministic reduction: consecutive rows of one frequency (or all rows)
+    // deterministic reduction: consecutive rows of one model slice (or all rows)
     int i0 = 0;
     while (i0 < n) {
       int i1 = i0 + 1;
       if (mstride)
-        while (i1 < n && r.fi[i1] == r.fi[i0]) ++i1;
+        while (i1 < n && r.slice[i1] == r.slice[i0]) ++i1;
       else
         i1 = n;
-      const size_t slot = mstride ? (size_t)r.fi[i0] : 0;
+      const size_t slot = mstride ? (size_t)r.slice[i0] : 0;
       HIPCHK(launch_reduce_rows(accS.as<double2>() + slot * c.nzx, cs + (size_t)i0 * c.nzx, (long long)c.nzx,
                                 (long long)c.nzx, i1 - i0, true, s),
              "reduce");
@@ -655,7 +733,7 @@ static int bornrefl_forward(Model* m, Survey* sv, const float2* image, int nh, f
   RCCHK(c.init({m}, sv, s));
   if (!add) RCCHK(c.zero(data, (size_t)sv->d.ns * sv->d.nr * sv->d.nt * sizeof(float2)));
   std::vector<Model*> fm(sv->d.nw, m);
-  auto chunks = make_chunks(sv->d.ns, sv->d.nw, fm, 2 * c.nzx * sizeof(float2), false);
+  auto chunks = make_chunks(sv->d.ns, sv->d.nw, fm, Per::Freq, 2 * c.nzx * sizeof(float2), false);
   const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
   DevBuf B, S;
   RCCHK(B.alloc(cap, "background"));
@@ -699,7 +777,7 @@ static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const f
   RCCHK(acc.alloc(nimg * c.nzx * sizeof(double2), "image accumulator"));
   RCCHK(c.zero(acc.p, acc.n));
   std::vector<Model*> fm(nw, m);
-  auto chunks = make_chunks(sv->d.ns, nw, fm, 3 * c.nzx * sizeof(float2), true);
+  auto chunks = make_chunks(sv->d.ns, nw, fm, Per::Freq, 3 * c.nzx * sizeof(float2), true);
   const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
   DevBuf B, S, C;
   RCCHK(B.alloc(cap, "background"));
@@ -800,7 +878,7 @@ int wemfwi_survey_destroy(wemfwi_survey* s) {
 int wemfwi_wem_forward(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* data, int add, void* stream) {
   if (!model || !survey || !data) return fail(WEMFWI_ERR_ARG, "wem_forward: null argument");
   std::vector<Model*> fm(survey->s.d.nw, &model->m);
-  return wem_forward(fm, &survey->s, (float2*)data, add, (hipStream_t)stream);
+  return wem_forward(fm, Per::Freq, &survey->s, (float2*)data, add, (hipStream_t)stream);
 }
 
 int wemfwi_extwem_forward(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* data, int add,
@@ -811,14 +889,14 @@ int wemfwi_extwem_forward(wemfwi_model* const* models, wemfwi_survey* survey, we
     if (!models[i]) return fail(WEMFWI_ERR_ARG, "extwem_forward: null model");
     fm.push_back(&models[i]->m);
   }
-  return wem_forward(fm, &survey->s, (float2*)data, add, (hipStream_t)stream);
+  return wem_forward(fm, Per::Freq, &survey->s, (float2*)data, add, (hipStream_t)stream);
 }
 
 int wemfwi_born_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* dslow,
                         const wemfwi_complex* dden, wemfwi_complex* data, int add, void* stream) {
   if (!model || !survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "born_forward: null argument");
   std::vector<Model*> fm(survey->s.d.nw, &model->m);
-  return born_forward(fm, &survey->s, (const float2*)dslow, (const float2*)dden, 0, (float2*)data, add,
+  return born_forward(fm, Per::Freq, &survey->s, (const float2*)dslow, (const float2*)dden, 0, (float2*)data, add,
                       (hipStream_t)stream);
 }
 
@@ -826,8 +904,8 @@ int wemfwi_born_adjoint(wemfwi_model* model, wemfwi_survey* survey, wemfwi_compl
                         const wemfwi_complex* data, int add, int mode, void* stream) {
   if (!model || !survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "born_adjoint: null argument");
   std::vector<Model*> fm(survey->s.d.nw, &model->m);
-  return born_adjoint(fm, &survey->s, (float2*)dslow, (float2*)dden, 0, (const float2*)data, add, mode != 0, false,
-                      (hipStream_t)stream);
+  return born_adjoint(fm, Per::Freq, &survey->s, (float2*)dslow, (float2*)dden, 0, (const float2*)data, add,
+                      mode != 0, false, (hipStream_t)stream);
 }
 
 int wemfwi_extborn_forward(wemfwi_model* const* models, wemfwi_survey* survey, const wemfwi_complex* dslow,
@@ -836,7 +914,7 @@ int wemfwi_extborn_forward(wemfwi_model* const* models, wemfwi_survey* survey, c
   std::vector<Model*> fm;
   for (int i = 0; i < survey->s.d.nw; ++i) fm.push_back(&models[i]->m);
   const size_t ms = (size_t)fm[0]->d.nz * fm[0]->d.nx;
-  return born_forward(fm, &survey->s, (const float2*)dslow, (const float2*)dden, ms, (float2*)data, add,
+  return born_forward(fm, Per::Freq, &survey->s, (const float2*)dslow, (const float2*)dden, ms, (float2*)data, add,
                       (hipStream_t)stream);
 }
 
@@ -846,8 +924,84 @@ int wemfwi_extborn_adjoint(wemfwi_model* const* models, wemfwi_survey* survey, w
   std::vector<Model*> fm;
   for (int i = 0; i < survey->s.d.nw; ++i) fm.push_back(&models[i]->m);
   const size_t ms = (size_t)fm[0]->d.nz * fm[0]->d.nx;
-  return born_adjoint(fm, &survey->s, (float2*)dslow, (float2*)dden, ms, (const float2*)data, add, mode != 0,
-                      mode == 0, (hipStream_t)stream);
+  return born_adjoint(fm, Per::Freq, &survey->s, (float2*)dslow, (float2*)dden, ms, (const float2*)data, add,
+                      mode != 0, mode == 0, (hipStream_t)stream);
+}
+
+int wemfwi_survey_set_shot_devices(wemfwi_survey* survey, int shot, int nsrc, const int* src_iz, const int* src_ix,
+                                   const int* rec_iz, const int* rec_ix) {
+  if (!survey || nsrc < 1 || !src_iz || !src_ix || !rec_iz || !rec_ix)
+    return fail(WEMFWI_ERR_ARG, "survey_set_shot_devices: invalid argument");
+  Survey& s = survey->s;
+  if (shot < 0 || shot >= s.d.ns) return fail(WEMFWI_ERR_ARG, "survey_set_shot_devices: shot out of range");
+  if (!s.per_shot()) {
+    // every shot starts from its point source and the shared receivers
+    s.ssrc_iz.resize(s.d.ns);
+    s.ssrc_ix.resize(s.d.ns);
+    s.srec_iz.assign(s.d.ns, s.rec_iz);
+    s.srec_ix.assign(s.d.ns, s.rec_ix);
+    for (int i = 0; i < s.d.ns; ++i) {
+      s.ssrc_iz[i] = {s.src_iz[i]};
+      s.ssrc_ix[i] = {s.src_ix[i]};
+    }
+    s.ssrc_plan.resize(s.d.ns);
+    s.srec_plan.resize(s.d.ns);
+  }
+  s.ssrc_iz[shot].assign(src_iz, src_iz + nsrc);
+  s.ssrc_ix[shot].assign(src_ix, src_ix + nsrc);
+  s.srec_iz[shot].assign(rec_iz, rec_iz + s.d.nr);
+  s.srec_ix[shot].assign(rec_ix, rec_ix + s.d.nr);
+  s.ssrc_plan[shot].release();
+  s.srec_plan[shot].release();
+  return WEMFWI_OK;
+}
+
+static int shot_models(wemfwi_model* const* models, const Survey& s, std::vector<Model*>& fm, const char* what) {
+  if (!models) return fail(WEMFWI_ERR_ARG, std::string(what) + ": null models");
+  for (int i = 0; i < s.d.ns; ++i) {
+    if (!models[i]) return fail(WEMFWI_ERR_ARG, std::string(what) + ": null model");
+    fm.push_back(&models[i]->m);
+  }
+  return WEMFWI_OK;
+}
+
+int wemfwi_shotwem_forward(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* data, int add,
+                           void* stream) {
+  if (!survey || !data) return fail(WEMFWI_ERR_ARG, "shotwem_forward: null argument");
+  std::vector<Model*> fm;
+  RCCHK(shot_models(models, survey->s, fm, "shotwem_forward"));
+  return wem_forward(fm, Per::Shot, &survey->s, (float2*)data, add, (hipStream_t)stream);
+}
+
+int wemfwi_shotborn_forward(wemfwi_model* const* models, wemfwi_survey* survey, const wemfwi_complex* dslow,
+                            const wemfwi_complex* dden, wemfwi_complex* data, int add, void* stream) {
+  if (!survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "shotborn_forward: null argument");
+  std::vector<Model*> fm;
+  RCCHK(shot_models(models, survey->s, fm, "shotborn_forward"));
+  const size_t ms = (size_t)fm[0]->d.nz * fm[0]->d.nx;
+  return born_forward(fm, Per::Shot, &survey->s, (const float2*)dslow, (const float2*)dden, ms, (float2*)data, add,
+                      (hipStream_t)stream);
+}
+
+int wemfwi_shotborn_adjoint(wemfwi_model* const* models, wemfwi_survey* survey, wemfwi_complex* dslow,
+                            wemfwi_complex* dden, const wemfwi_complex* data, int add, int mode, void* stream) {
+  if (!survey || !dslow || !dden || !data) return fail(WEMFWI_ERR_ARG, "shotborn_adjoint: null argument");
+  std::vector<Model*> fm;
+  RCCHK(shot_models(models, survey->s, fm, "shotborn_adjoint"));
+  const size_t ms = (size_t)fm[0]->d.nz * fm[0]->d.nx;
+  return born_adjoint(fm, Per::Shot, &survey->s, (float2*)dslow, (float2*)dden, ms, (const float2*)data, add,
+                      mode != 0, false, (hipStream_t)stream);
+}
+
+int wemfwi_gather_sum(long long nout, const long long* ptr, const long long* src, const wemfwi_complex* in,
+                      wemfwi_complex* out, int add, void* stream) {
+  if (nout < 0 || (nout > 0 && (!ptr || !src || !in || !out)))
+    return fail(WEMFWI_ERR_ARG, "gather_sum: invalid argument");
+  if (nout == 0) return WEMFWI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(launch_gather_sum((float2*)out, nout, ptr, src, (const float2*)in, add != 0, s), "gather_sum");
+  HIPCHK(hipStreamSynchronize(s), "gather_sum");
+  return WEMFWI_OK;
 }
 
 int wemfwi_bornrefl_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* image,

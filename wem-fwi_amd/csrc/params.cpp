@@ -181,4 +181,20 @@ int wemfwi_device_indices(int n, float o_x, float o_z, float d_x, float d_z, flo
   return WEMFWI_OK;
 }
 
+int wemfwi_device_coords(int n, const float* x, const float* z, float ox, float oz, float dxv, float dzv, int* iz,
+                         int* ix) {
+  if (n < 0 || (n > 0 && (!x || !z || !iz || !ix))) return fail(WEMFWI_ERR_ARG, "device: invalid argument");
+  if (!(dxv > 0.f) || !(dzv > 0.f)) return fail(WEMFWI_ERR_ARG, "device: invalid model spacing");
+  for (int i = 0; i < n; ++i) {
+    const int jx = (int)((x[i] - ox) / dxv + .5f);
+    const int jz = (int)((z[i] - oz) / dzv + .5f);
+    if (jx < 0 || jz < 0)
+      return fail(WEMFWI_ERR_ARG,
+                  "device {" + std::to_string(z[i]) + "," + std::to_string(x[i]) + "} is out of bounds");
+    ix[i] = jx;
+    iz[i] = jz;
+  }
+  return WEMFWI_OK;
+}
+
 }  // extern "C"

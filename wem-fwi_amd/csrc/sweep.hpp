@@ -124,6 +124,8 @@ struct DepthTables {
   long long scoef_stride;       // nz*nx
   int nz;                       // depth samples
   int n;                        // FFT length N (lane-table row length)
+  float dz;                     // depth step of this model's grid
+  float dk;                     // wavenumber step of its scatterer grid
 };
 
 struct StepGeom {
@@ -131,8 +133,6 @@ struct StepGeom {
   int tap;      // FFT zero padding per side
   int ssf;      // 1: split-step correction on
   int full;     // 1: tap == 0 and nx == N (every lane sample is a model sample)
-  float dz;     // depth step
-  float dk;     // wavenumber step of the phase-shift / scatterer grid
   int ntaylor;  // Taylor order of the scatterer
   int taper;    // 1: a taper is applied (width > 0)
 };
@@ -553,7 +553,7 @@ __device__ __forceinline__ void scatter_row(float2 (&st)[V::E], int d, float w, 
       // (k/w)^(2 it) / N on the grid of Scatter.cpp:24-31
       const int k = t + 64 * e;
       const int kh = k < N / 2 ? k : ((N - k) < N / 2 ? N - k : 0);
-      const float kk = (float)kh * g.dk;
+      const float kk = (float)kh * dt.dk;
       const float q = kk * kk * iw2;
       float qi = q;
       for (int j = 1; j < it; ++j) qi *= q;
@@ -570,7 +570,7 @@ __device__ __forceinline__ void scatter_row(float2 (&st)[V::E], int d, float w, 
       }
     }
   }
-  const float s = w * g.dz;
+  const float s = w * dt.dz;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     // forward: (-i s) acc ; adjoint: (+i s) acc

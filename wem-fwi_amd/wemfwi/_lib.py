@@ -54,8 +54,10 @@ SIGNATURES = {
     "wemfwi_freq_to_trace": (_i, [_i, _i, _i, _vp, _i, _i, _ip, _vp, _i, _vp]),
     "wemfwi_device_indices": (_i, [_i, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                    C.c_float, C.c_float, C.c_float, _ip, _ip]),
+    "wemfwi_device_coords": (_i, [_i, _fp, _fp, C.c_float, C.c_float, C.c_float, C.c_float, _ip, _ip]),
     "wemfwi_refsampler": (_i, [_i, _i, _vp, _i, _vp, _ip]),
     "wemfwi_model_create": (_i, [C.POINTER(ModelDesc), _vp, _vp, C.POINTER(_vp)]),
+    "wemfwi_model_create_refl": (_i, [C.POINTER(ModelDesc), _vp, _vp, _vp, C.POINTER(_vp)]),
     "wemfwi_model_destroy": (_i, [_vp]),
     "wemfwi_oneway": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _i, _vp]),
     "wemfwi_onestep": (_i, [_vp, _i, _i, _fp, _ip, _vp, _vp, _i, _vp]),
@@ -69,6 +71,13 @@ SIGNATURES = {
     "wemfwi_extwem_forward": (_i, [_vp, _vp, _vp, _i, _vp]),
     "wemfwi_extborn_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_extborn_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "wemfwi_cmap_plan": (_i, [_i, _i, _i, C.c_float, C.c_float, C.c_float, C.c_float, _i, _ip, _ip, _i, _ip,
+                              _ip, C.c_float, _fp, _fp, _ip, _ip, _fp, _ip, _ip, _ip, _ip, _ip]),
+    "wemfwi_survey_set_shot_devices": (_i, [_vp, _i, _i, _ip, _ip, _ip, _ip]),
+    "wemfwi_shotwem_forward": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "wemfwi_shotborn_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_shotborn_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "wemfwi_gather_sum": (_i, [C.c_longlong, _vp, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_bornrefl_forward": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_bornrefl_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_offset_image_forward": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),

@@ -25,7 +25,7 @@ import torch
 from ._lib import check, lib
 from .core import Model, _stream_ptr
 from . import ops
-from .params import device_indices, prop_param
+from .params import CMapPlan, cmap_from_par, device_indices, prop_param
 
 
 # --------------------------------------------------------------------- vectors
@@ -237,6 +237,7 @@ class _Acquisition:
         self.freq, self.index, self.wave_f = prop_param(
             vals, wa.d, ot=wa.o, tpad=self.tpad, fmin=float(self.par.get("fmin", 1.0)),
             fmax=float(self.par.get("fmax", 0.0)), nfreq=int(self.par.get("nfreq", 0)))
+        self.ox, self.oz = ax_x.o, ax_z.o
         self.src_iz, self.src_ix = device_indices(self.par, "s", ax_x.o, ax_z.o, ax_x.d, ax_z.d)
         self.rec_iz, self.rec_ix = device_indices(self.par, "r", ax_x.o, ax_z.o, ax_x.d, ax_z.d)
         self.pad = int(self.par.get("pad", 0))
@@ -249,13 +250,69 @@ class _Acquisition:
         self.ngr = int(self.par.get("ngr", 0))
         self.ng = int(self.par.get("ng", 0))
         self._surveys = {}
+        self._cmap: Optional[CMapPlan] = None
+        self._cmap_done = False
+        self._plans = {}
+
+    # ---------------------------------------------------- conformal maps
+
+    @property
+    def cmap(self) -> Optional[CMapPlan]:
+        """Per-shot maps of PropParam::prepareConformalMaps (PropParam.h:89-131)
+        when par cmap is 1 or 2; used by WEM and BornFull (wem_prop.cpp:20-45,
+        born_prop.cpp:28-70), not by the extended operators or BornRefl, whose
+        propagation loops take the model grid as it is."""
+        if not self._cmap_done:
+            self._cmap = cmap_from_par(self.par, self.nz, self.nx, self.oz, self.ox, self.dz, self.dx,
+                                       self.src_iz, self.src_ix, self.rec_iz, self.rec_ix)
+            self._cmap_done = True
+        return self._cmap
+
+    def shot_models(self, slow, den=None) -> List[Model]:
+        """One background model per shot on its mapped grid, built as
+        wem_prop.cpp:24-40 does: map, edge-pad, Reflect from the padded map,
+        then the slowness scaled by the map's Jacobian factors, in order, twice,
+        over the first nz*nx samples of the padded array."""
+        cm = self.cmap
+        s = _host_c64(slow).ravel()
+        d = np.ones_like(s) if den is None else _host_c64(den).ravel()
+        n = self.nz * self.nx
+        out = []
+        try:
+            for k in range(cm.ns):
+                f = cm.fwd[k]
+                g = np.where(f >= 0, s[np.maximum(f, 0)], 0).astype(np.complex64).reshape(self.nz, self.nx)
+                gd = np.where(f >= 0, d[np.maximum(f, 0)], 0).astype(np.complex64).reshape(self.nz, self.nx)
+                sp = _edge_pad(g, self.pad)
+                dp = _edge_pad(gd, self.pad)
+                refl = sp.copy()
+                flat = sp.reshape(-1)
+                for _ in range(2):
+                    for j in (cm.jac[k, 0], cm.jac[k, 1]):
+                        flat.real[:n] *= j
+                        flat.imag[:n] *= j
+                out.append(Model(sp, float(cm.axes[k, 1]), float(cm.axes[k, 3]), tap=self.tap, pad=self.pad,
+                                 nref=self.nref, prop=self.prop, eps=self.eps, ntaylor=self.ntaylor, den=dp,
+                                 slow_refl=refl))
+        except Exception:
+            for m in out:
+                m.close()
+            raise
+        return out
+
+    def map_plan(self, which: str) -> ops.GatherPlan:
+        """Device gather of map_csr(which) (cached)."""
+        if which not in self._plans:
+            ptr, src = map_csr(self.cmap, self.nz, self.nx, self.pad, which)
+            self._plans[which] = ops.GatherPlan(ptr, src)
+        return self._plans[which]
 
     @property
     def nw(self) -> int:
         return len(self.freq)
 
-    def survey(self, nbins: int, refl: bool = False) -> ops.Survey:
-        key = (nbins, refl)
+    def survey(self, nbins: int, refl: bool = False, mapped: bool = False) -> ops.Survey:
+        key = (nbins, refl, mapped)
         if key not in self._surveys:
             if int(self.index.max()) >= nbins:
                 raise ValueError(f"data has {nbins} frequency bins; the survey needs bin {int(self.index.max())}")
@@ -265,6 +322,10 @@ class _Acquisition:
             else:      # wem_prop.cpp:43-44, born_prop.cpp:63-64
                 sv = ops.Survey(self.src_iz, self.src_ix, self.rec_iz, self.rec_ix, self.freq, self.index,
                                 self.wave_f[self.index], nbins, ngs=self.ngs, ngr=self.ngr, offset=self.pad)
+                if mapped:   # the mapped devices of every shot (wem_prop.cpp:21-22, 43-44)
+                    cm = self.cmap
+                    for k in range(cm.ns):
+                        sv.set_shot_devices(k, cm.src[k][0], cm.src[k][1], cm.rec_iz[k], cm.rec_ix[k])
             self._surveys[key] = sv
         return self._surveys[key]
 
@@ -313,6 +374,40 @@ class _Acquisition:
         return f
 
 
+def map_csr(cm: CMapPlan, nz: int, nx: int, pad: int, which: str):
+    """CSR lists (ptr, src) of the gathers between the model grid [nz][nx]
+    and the per-shot padded grids [ns][nz][nx + 2 pad]:
+      'fwd'   a perturbation mapped onto every shot's grid, zero-padded
+              (born_prop.cpp:50-58): out = ns*nz*(nx+2 pad) samples;
+      'inv'   the reference's inverse map of the per-shot results summed over
+              shots in shot order (born_prop.cpp:217-231): out = nz*nx;
+      'exact' the transpose of 'fwd' (the adjoint of the mapping).
+    A sample with an empty list is zero."""
+    nxp = nx + 2 * pad
+    n = nz * nx
+    i = np.arange(n)
+    pos = (i // nx) * nxp + pad + (i % nx)      # model-grid sample -> padded slice position
+    if which == "fwd":
+        idx = np.full((cm.ns, nz * nxp), -1, np.int64)
+        idx[:, pos] = cm.fwd
+        idx = idx.ravel()
+        ok = idx >= 0
+        return np.concatenate([[0], np.cumsum(ok)]), idx[ok]
+    if which == "inv":
+        v = cm.inv.astype(np.int64)                                          # [ns][n]
+        src = np.where(v >= 0, np.arange(cm.ns)[:, None] * nz * nxp + pos[np.maximum(v, 0)], -1).T
+        cnt = (src >= 0).sum(axis=1)                                         # [n], shots in order
+        return np.concatenate([[0], np.cumsum(cnt)]), src[src >= 0]
+    if which == "exact":
+        f = cm.fwd.astype(np.int64)
+        k, ii = np.nonzero(f >= 0)
+        tgt = f[k, ii]
+        order = np.lexsort((ii, k, tgt))
+        cnt = np.bincount(tgt, minlength=n)
+        return np.concatenate([[0], np.cumsum(cnt)]), (k * nz * nxp + pos[ii])[order]
+    raise ValueError(which)
+
+
 def _model_vecs(model) -> List[Vector]:
     if isinstance(model, SuperVector):
         return model.vecs
@@ -339,7 +434,22 @@ class WEM(Propagator):
 
     def forward(self, add, model, data):
         vecs = _model_vecs(model)
-        m = self.acq.model(vecs[0], vecs[1] if len(vecs) > 1 else None)
+        den = vecs[1] if len(vecs) > 1 else None
+        if self.acq.cmap is not None:   # one mapped grid per shot
+            ms = self.acq.shot_models(vecs[0], den)
+            try:
+                f = self.acq.freq_data(data)
+                sv = self.acq.survey(f.shape[-1], mapped=True)
+                if self.acq._is_time(data):
+                    ops.shotwem_forward(ms, sv, f, add=False)
+                    self.acq.to_time(f, data, add)
+                else:
+                    ops.shotwem_forward(ms, sv, f, add=bool(add))
+            finally:
+                for m in ms:
+                    m.close()
+            return
+        m = self.acq.model(vecs[0], den)
         try:
             f = self.acq.freq_data(data)
             sv = self.acq.survey(f.shape[-1])
@@ -373,15 +483,60 @@ class Born(Propagator):
 
     def set_background(self, slow):
         vecs = _model_vecs(slow)
-        if self._bg is not None:
-            self._bg.close()
-        self._bg = self.acq.model(vecs[0], vecs[1] if len(vecs) > 1 else None)
+        den = vecs[1] if len(vecs) > 1 else None
+        self._close_bg()
+        if self.acq.cmap is not None:
+            self._bg = self.acq.shot_models(vecs[0], den)
+        else:
+            self._bg = self.acq.model(vecs[0], den)
+
+    def _close_bg(self):
+        for m in (self._bg if isinstance(self._bg, list) else [self._bg]):
+            if m is not None:
+                m.close()
+        self._bg = None
+
+    def _mapped_forward(self, add, vecs, data):
+        acq = self.acq
+        ns, nz, nxp = acq.cmap.ns, acq.nz, acq.nx + 2 * acq.pad
+        plan = acq.map_plan("fwd")
+        dS = plan(vecs[0].arr.contiguous(), torch.empty((ns, nz, nxp), dtype=torch.complex64, device=vecs[0].arr.device))
+        dR = torch.zeros_like(dS)
+        if len(vecs) > 1:
+            plan(vecs[1].arr.contiguous(), dR)
+        f = acq.freq_data(data)
+        sv = acq.survey(f.shape[-1], mapped=True)
+        if acq._is_time(data):
+            ops.shotborn_forward(self._bg, sv, dS, dR, f, add=False)
+            acq.to_time(f, data, add)
+        else:
+            ops.shotborn_forward(self._bg, sv, dS, dR, f, add=bool(add))
+
+    def _mapped_adjoint(self, add, vecs, data):
+        acq = self.acq
+        f = acq.to_freq(data)
+        sv = acq.survey(f.shape[-1], mapped=True)
+        ns, nz, nxp = acq.cmap.ns, acq.nz, acq.nx + 2 * acq.pad
+        gS = torch.zeros((ns, nz, nxp), dtype=torch.complex64, device=f.device)
+        gR = torch.zeros_like(gS)
+        exact = self.adjoint_mode == "exact"
+        ops.shotborn_adjoint(self._bg, sv, gS, gR, f, add=False, exact=exact)
+        plan = acq.map_plan("exact" if exact else "inv")
+        for v, g in zip(vecs, (gS, gR)):
+            if v.arr.is_contiguous():
+                plan(g, v.arr, add=bool(add))
+            else:
+                t = plan(g, torch.empty(v.arr.shape, dtype=torch.complex64, device=g.device))
+                v.arr.add_(t) if add else v.arr.copy_(t)
 
     def forward(self, add, model, data):
         vecs = _model_vecs(model)
         if self.mode == "real":
             for v in vecs:
                 v.arr.imag.zero_()
+        if self.acq.cmap is not None:
+            self._mapped_forward(add, vecs, data)
+            return
         p = self.acq.pad
         dS = _zero_pad_dev(vecs[0].arr, p)
         dR = _zero_pad_dev(vecs[1].arr, p) if len(vecs) > 1 else torch.zeros_like(dS)
@@ -395,6 +550,12 @@ class Born(Propagator):
 
     def adjoint(self, add, model, data):
         vecs = _model_vecs(model)
+        if self.acq.cmap is not None:
+            self._mapped_adjoint(add, vecs, data)
+            if self.mode == "real":
+                for v in vecs:
+                    v.arr.imag.zero_()
+            return
         f = self.acq.to_freq(data)
         sv = self.acq.survey(f.shape[-1])
         p = self.acq.pad

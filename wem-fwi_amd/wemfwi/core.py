@@ -53,15 +53,25 @@ class Model:
 
     def __init__(self, slow, dx: float, dz: float, *, tap: int = 0, pad: int = 0,
                  nref: int = 1, prop: str = "ssf", eps: float = 0.04, ntaylor: int = 1,
-                 den=None):
+                 den=None, slow_refl=None):
+        """slow_refl: slowness of the Reflect coefficients when it differs
+        from the propagation slowness (the conformal-map grids)."""
         s = _c64_host(slow)
         nz, nx = s.shape
         d = None if den is None else _c64_host(den)
         self.desc = ModelDesc(nz=nz, nx=nx, dx=dx, dz=dz, tap=tap, taper=pad, nref=nref,
                               pspi=1 if prop == "pspi" else 0, eps=eps, ntaylor=ntaylor)
         h = C.c_void_p()
-        check(lib.wemfwi_model_create(C.byref(self.desc), s.ctypes.data,
-                                      None if d is None else d.ctypes.data, C.byref(h)))
+        if slow_refl is None:
+            check(lib.wemfwi_model_create(C.byref(self.desc), s.ctypes.data,
+                                          None if d is None else d.ctypes.data, C.byref(h)))
+        else:
+            r = _c64_host(slow_refl)
+            if r.shape != s.shape:
+                raise ValueError("slow_refl must have the shape of slow")
+            check(lib.wemfwi_model_create_refl(C.byref(self.desc), s.ctypes.data,
+                                               None if d is None else d.ctypes.data, r.ctypes.data,
+                                               C.byref(h)))
         self._h = h
         self.nz, self.nx = nz, nx
 

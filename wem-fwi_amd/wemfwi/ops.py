@@ -71,6 +71,17 @@ class Survey:
     def data_shape(self):
         return (self.ns, self.nr, self.nt)
 
+    def set_shot_devices(self, shot: int, src_iz, src_ix, rec_iz, rec_ix):
+        """Gives shot `shot` its own source points and receiver positions
+        (the mapped devices of a conformal-map grid)."""
+        siz, p_siz = _ip(src_iz)
+        six, p_six = _ip(src_ix)
+        riz, p_riz = _ip(rec_iz)
+        rix, p_rix = _ip(rec_ix)
+        if len(siz) != len(six) or len(riz) != self.nr or len(rix) != self.nr:
+            raise ValueError("set_shot_devices: device list sizes")
+        check(lib.wemfwi_survey_set_shot_devices(self._h, int(shot), len(siz), p_siz, p_six, p_riz, p_rix))
+
 
 def _ptr(t: torch.Tensor) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
@@ -163,3 +174,70 @@ def offset_image_adjoint(model: Model, survey: Survey, nh: int, image, data, add
     check(lib.wemfwi_offset_image_adjoint(model._h, survey._h, int(nh), _ptr(image), _ptr(data),
                                           int(add), _stream_ptr(stream)))
     return image
+
+
+def shotwem_forward(models: Sequence[Model], survey: Survey, data: torch.Tensor, add: bool = False,
+                    stream=None):
+    """WEM with one background model per shot (conformal-map grids)."""
+    _dev_c64(data, "data")
+    if len(models) != survey.ns:
+        raise ValueError("shotwem_forward: one model per shot")
+    check(lib.wemfwi_shotwem_forward(_models(models), survey._h, _ptr(data), int(add), _stream_ptr(stream)))
+    return data
+
+
+def shotborn_forward(models: Sequence[Model], survey: Survey, dslow, dden, data, add=False, stream=None):
+    """BornFull with one background model and perturbation slice per shot:
+    dslow, dden [ns][nz][nx]."""
+    for n, t in (("dslow", dslow), ("dden", dden), ("data", data)):
+        _dev_c64(t, n)
+    if len(models) != survey.ns:
+        raise ValueError("shotborn_forward: one model per shot")
+    check(lib.wemfwi_shotborn_forward(_models(models), survey._h, _ptr(dslow), _ptr(dden), _ptr(data),
+                                      int(add), _stream_ptr(stream)))
+    return data
+
+
+def shotborn_adjoint(models: Sequence[Model], survey: Survey, dslow, dden, data, add=False, exact=False,
+                     stream=None):
+    """Adjoint of shotborn_forward into the per-shot slices dslow, dden."""
+    for n, t in (("dslow", dslow), ("dden", dden), ("data", data)):
+        _dev_c64(t, n)
+    if len(models) != survey.ns:
+        raise ValueError("shotborn_adjoint: one model per shot")
+    check(lib.wemfwi_shotborn_adjoint(_models(models), survey._h, _ptr(dslow), _ptr(dden), _ptr(data),
+                                      int(add), int(exact), _stream_ptr(stream)))
+    return dslow, dden
+
+
+class GatherPlan:
+    """out[j] = sum of in[src[c]] for c in [ptr[j], ptr[j+1]) on the device:
+    the model <-> grid gathers of the conformal maps."""
+
+    def __init__(self, ptr: np.ndarray, src: np.ndarray, device="cuda"):
+        ptr = np.ascontiguousarray(np.asarray(ptr, dtype=np.int64))
+        src = np.ascontiguousarray(np.asarray(src, dtype=np.int64))
+        if ptr.ndim != 1 or ptr.size < 1 or ptr[0] != 0 or ptr[-1] != src.size or np.any(np.diff(ptr) < 0):
+            raise ValueError("GatherPlan: invalid pointer array")
+        self.nout = ptr.size - 1
+        self.nin_min = int(src.max()) + 1 if src.size else 0
+        self.ptr = torch.from_numpy(ptr).to(device)
+        self.src = torch.from_numpy(src if src.size else np.zeros(1, np.int64)).to(device)
+
+    @staticmethod
+    def from_index(idx: np.ndarray, offsets: Optional[np.ndarray] = None) -> "GatherPlan":
+        """Gather out[j] = in[idx[j] + offsets[j]] (idx < 0: zero)."""
+        idx = np.asarray(idx, dtype=np.int64).ravel()
+        ok = idx >= 0
+        src = idx[ok] + (0 if offsets is None else np.asarray(offsets, dtype=np.int64).ravel()[ok])
+        ptr = np.concatenate([[0], np.cumsum(ok)])
+        return GatherPlan(ptr, src)
+
+    def __call__(self, inp: torch.Tensor, out: torch.Tensor, add: bool = False, stream=None) -> torch.Tensor:
+        _dev_c64(inp, "inp")
+        _dev_c64(out, "out")
+        if out.numel() != self.nout or inp.numel() < self.nin_min:
+            raise ValueError("GatherPlan: input / output sizes do not match the plan")
+        check(lib.wemfwi_gather_sum(self.nout, _ptr(self.ptr), _ptr(self.src), _ptr(inp), _ptr(out), int(add),
+                                    _stream_ptr(stream)))
+        return out
