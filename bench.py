@@ -19,8 +19,12 @@ and the FWI gradient (WEM forward + residual + exact Born adjoint, 512 x 300,
 called once per (shot, frequency) row (a timed sample of rows, scaled), with
 the relative difference of the two gradients on the sampled rows.
 
+Also the uneven batch: 64 shots with their own apertures (256-2048) and
+bands (32-192 frequencies), nz = 400, Down forward of all rows in one batched
+call against one call per shot and one call per row (sampled).
+
 Prints one JSON line.  --cpu-sample 0 skips the CPU baseline, --fwi-sample 0
-the gradient comparison.
+the gradient comparison, --uneven-sample 0 the uneven batch.
 """
 
 from __future__ import annotations
@@ -223,6 +227,120 @@ def fwi_gradient_bench(sample_rows: int):
             "rel_err_plain_vs_batched": rel}
 
 
+# Uneven batch: 64 shots over a 4096-wide model, shot s with its own aperture
+# nx_s in {256, 512, 1024, 2048} (the window centred on the shot) and its own
+# band: nw_s in 32..192 frequencies 0.1 Hz apart from a start in [2, 5) Hz;
+# nz = 400, SSF, velocity = linear gradient + 5% smooth field (seed 5).
+UNEVEN = dict(ns=64, nz=400, width=4096, widths=(256, 512, 1024, 2048), seed=5)
+
+
+def uneven_batch_bench(sample_rows: int):
+    """Down forward of every (shot, frequency) row of the uneven survey as one
+    batch plan run (wemfwi_batch_run; also the one-call form that plans each
+    time) against one call per row on a sample of rows, and against one call
+    per shot; the sampled rows must be bitwise equal between the batch and the
+    per-row calls."""
+    import torch
+    import wemfwi
+    from wemfwi.core import BatchPlan, oneway_batch
+    from scipy.ndimage import gaussian_filter
+    c = UNEVEN
+    rng = np.random.default_rng(c["seed"])
+    nz, W0 = c["nz"], c["width"]
+    z = np.arange(nz)[:, None] * 10.0
+    v = 1500.0 + 2000.0 * z / ((nz - 1) * 10.0) + np.zeros((nz, W0))
+    pert = gaussian_filter(rng.standard_normal((nz, W0)), 8.0)
+    v = v + 0.05 * v * pert / np.max(np.abs(pert))
+    slow = (1.0 / v ** 2).astype(np.complex64)
+    dev = torch.device("cuda:0")
+    models, rows_m, rows_f, rows_shot = [], [], [], []
+    nxs, nws = [], []
+    for s in range(c["ns"]):
+        nxs.append(int(rng.choice(c["widths"])))
+        nws.append(int(rng.integers(32, 193)))
+        f0 = 2.0 + 3.0 * rng.random()
+        cx = int(rng.integers(nxs[-1] // 2, W0 - nxs[-1] // 2 + 1))
+        win = slow[:, cx - nxs[-1] // 2: cx - nxs[-1] // 2 + nxs[-1]]
+        models.append(wemfwi.Model(np.ascontiguousarray(win), 10.0, 10.0, nref=1, prop="ssf", eps=0.04))
+        for i in range(nws[-1]):
+            rows_m.append(s)
+            rows_f.append(float(np.float32(f0 + 0.1 * i)))
+            rows_shot.append(s)
+    n = len(rows_m)
+    src, out = [], []
+    spec = ricker_spectrum(rows_f)
+    for r in range(n):
+        nx = nxs[rows_m[r]]
+        a = torch.zeros((nz, nx), dtype=torch.complex64, device=dev)
+        a[0, nx // 2] = complex(spec[r])
+        src.append(a)
+        out.append(torch.empty_like(a))
+    ms = [models[m] for m in rows_m]
+    upd = sum(nxs[m] * (nz - 1) for m in rows_m)
+
+    plan = BatchPlan(ms, rows_f, src, out)
+    plan.run(False, False)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.run(False, False)
+    torch.cuda.synchronize()
+    t_b = (time.perf_counter() - t0) / reps
+    # the one-call form (plan built and dropped inside the call)
+    oneway_batch(False, False, ms, rows_f, src, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    oneway_batch(False, False, ms, rows_f, src, out)
+    torch.cuda.synchronize()
+    t_once = time.perf_counter() - t0
+    # one call per shot (all its frequencies in one launch)
+    by_shot = {}
+    for r in range(n):
+        by_shot.setdefault(rows_shot[r], []).append(r)
+    stacked = {s: (torch.stack([src[r] for r in rr]), torch.empty((len(rr), nz, nxs[s]), dtype=torch.complex64,
+                                                                  device=dev)) for s, rr in by_shot.items()}
+
+    def per_shot():
+        for s, rr in by_shot.items():
+            models[s].oneway_recursive(False, False, [rows_f[r] for r in rr], stacked[s][0], stacked[s][1])
+
+    per_shot()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per_shot()
+    torch.cuda.synchronize()
+    t_s = time.perf_counter() - t0
+    # one call per row on a sample
+    stride = max(1, n // sample_rows)
+    sample = list(range(0, n, stride))[:sample_rows]
+    one = [torch.empty_like(out[r]) for r in sample]
+
+    def per_row():
+        for k, r in enumerate(sample):
+            models[rows_m[r]].oneway_recursive(False, False, [rows_f[r]], src[r][None], one[k][None])
+
+    per_row()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per_row()
+    torch.cuda.synchronize()
+    t_r = (time.perf_counter() - t0) / len(sample) * n
+    equal = all(bool(torch.equal(one[k], out[r])) for k, r in enumerate(sample))
+    plan.close()
+    for m in models:
+        m.close()
+    return {"config": f"ssf ns=64 nz=400 nx_s in {{256,512,1024,2048}} nw_s in 32..192, {n} rows",
+            "updates": upd, "batched_ms": round(t_b * 1e3, 2),
+            "batched_gupdates_per_s": round(upd / t_b / 1e9, 2),
+            "one_call_ms": round(t_once * 1e3, 2),
+            "per_shot_ms": round(t_s * 1e3, 2),
+            "per_row_loop_ms": round(t_r * 1e3, 1),
+            "per_row_sample": f"{len(sample)} of {n} rows timed ({t_r * len(sample) / n:.2f} s), scaled",
+            "speedup_vs_row_loop": round(t_r / t_b, 1), "speedup_vs_shot_loop": round(t_s / t_b, 2),
+            "bitwise_equal_on_sample": equal}
+
+
 def measured_traffic(config: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this configuration (mean over its launches, each
@@ -255,6 +373,8 @@ def main():
                     help="rows of the CPU baseline sample (0: skip)")
     ap.add_argument("--fwi-sample", type=int, default=64,
                     help="rows of the plain per-row loop in the FWI gradient comparison (0: skip)")
+    ap.add_argument("--uneven-sample", type=int, default=32,
+                    help="rows of the per-row loop in the uneven-batch comparison (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -362,6 +482,8 @@ def main():
     line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
     if args.fwi_sample > 0:
         line["fwi_gradient"] = fwi_gradient_bench(args.fwi_sample)
+    if args.uneven_sample > 0:
+        line["uneven_batch"] = uneven_batch_bench(args.uneven_sample)
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)
     print(json.dumps(line), flush=True)

@@ -155,6 +155,30 @@ int wemfwi_oneway_recursive(wemfwi_model* model, int up, int adjoint, int nrows,
                             const float* freq_hz, const wemfwi_complex* src, wemfwi_complex* out,
                             void* stream);
 
+/* Uneven batch of recursive Down / Up sweeps: row i has its own model
+ * models[i] (aperture nx, depth count, FFT length may all differ) and
+ * frequency freq_hz[i]; out[i] = (I - L_i)^-1 src[i] over its [nz_i][nx_i]
+ * buffers (device pointers; src[i] == out[i] is the in-place recursion).
+ * Rows are grouped by kernel variant, one launch per group, the groups on
+ * separate streams; every row gets exactly the result of a wemfwi_oneway_recursive
+ * call on its own model.  No reference counterpart: the reference runs one
+ * (shot, frequency) task at a time (wem_prop.cpp:11-12). */
+int wemfwi_oneway_batch(int up, int adjoint, int nrows, wemfwi_model* const* models,
+                        const float* freq_hz, const wemfwi_complex* const* src,
+                        wemfwi_complex* const* out, void* stream);
+
+/* The same as a plan: rows, models, frequencies and buffers are fixed at
+ * creation (phase tables built, descriptors and scratch rows uploaded); each
+ * run launches the groups only.  The models and buffers must outlive the
+ * plan; a run fails with WEMFWI_ERR_ARG if a model's phase table was rebuilt
+ * since planning (e.g. by a call with other frequencies). */
+typedef struct wemfwi_batch wemfwi_batch;
+int wemfwi_batch_create(int nrows, wemfwi_model* const* models, const float* freq_hz,
+                        const wemfwi_complex* const* src, wemfwi_complex* const* out, void* stream,
+                        wemfwi_batch** plan);
+int wemfwi_batch_run(wemfwi_batch* plan, int up, int adjoint, void* stream);
+int wemfwi_batch_destroy(wemfwi_batch* plan);
+
 /* SSF / PSPI single depth step (SSF.cpp:8-60, PSPI.cpp:8-53) on nrows
  * independent rows of nx samples: row i at depth depth[i] and frequency
  * freq_hz[i].  add = 1 accumulates into out. */

@@ -487,3 +487,25 @@ def test_born_forward_homogeneous_point_scatterer_closed_form():
         want = np.array([S1[z, x] for z, x in zip(rec_iz, rec_ix)])
         g = got[0, :, sv.index[i]]
         assert np.linalg.norm(g - want) < 1e-9 * np.linalg.norm(want), (i, np.linalg.norm(g - want) / np.linalg.norm(want))
+
+
+def test_depths_without_live_reference_give_zero_rows():
+    """Slowness decreasing with depth: from depth nx-1 on, RefSampler's stale
+    buffer holds no live reference for any x (RefSampler.cpp:63-81), Selector
+    assigns nothing (Selector.cpp:7-9) and a step returns exact zeros; a Down
+    sweep from a surface source is zero below that depth."""
+    nz, nx = 140, 64
+    z = np.arange(nz)[:, None] * 10.0
+    slow = (1.0 / (1500.0 + 2000.0 * z / ((nz - 1) * 10.0) + np.zeros((nz, nx))) ** 2).astype(np.complex64)
+    ref = O.refsampler(slow, 1)
+    empty = np.nonzero((np.asarray(ref.refid) < 0).all(axis=1))[0]
+    assert empty.size > 0 and empty[0] == nx - 1 and np.array_equal(empty, np.arange(nx - 1, nz))
+    p = O.StepParams(nx=nx, dx=10.0, dz=10.0, nref=1, prop="ssf", eps=0.04)
+    w = 2 * np.pi * 3.0
+    x = np.ones(nx, complex)
+    assert np.all(O.onestep_fwd(x, nx + 3, w, slow, ref, p) == 0)
+    assert np.all(O.onestep_adj(x, nx + 3, w, slow, ref, p) == 0)
+    W = np.zeros((nz, nx), complex)
+    W[0, nx // 2] = 1.0
+    D = O.down_fwd(W, w, slow, ref, p)
+    assert np.all(D[nx:] == 0) and np.abs(D[nx - 2]).max() > 0

@@ -320,6 +320,9 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   t.n = N;
   t.dz = d.dz;
   t.dk = (float)(2.0 * M_PI / ((double)d.dx * (double)(N - 1)));
+  t.nx = d.nx;
+  t.tap = d.tap;
+  t.taper = d.taper > 0 ? 1 : 0;
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
   if (fft_ok && (rc = twiddles(N, &d_tw))) return rc;
   return WEMFWI_OK;
@@ -366,6 +369,7 @@ int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
   kt.end(s);
   if (e != hipSuccess) return hip_fail(e, "phase table kernel");
   freqs = uniq;
+  ++table_gen;
   return WEMFWI_OK;
 }
 
@@ -437,6 +441,159 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
   return WEMFWI_OK;
 }
 
+// Streams of the batched sweep: the launches of different kernel variants
+// run concurrently (default priority, created once per device).
+static int batch_streams(hipStream_t (&out)[4]) {
+  static std::map<int, std::vector<hipStream_t>> pool;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::vector<hipStream_t>& v = pool[dev];
+  while (v.size() < 4) {
+    hipStream_t st;
+    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "stream create");
+    v.push_back(st);
+  }
+  for (int i = 0; i < 4; ++i) out[i] = v[i];
+  return WEMFWI_OK;
+}
+
+// Plan of an uneven batch: Down/Up recursions out_i = (I - L_i)^-1 in_i of
+// rows that each have their own model (aperture, depth count, FFT length) and
+// frequency.  Rows are grouped by kernel variant (FFT length, full row, split
+// step); each group is one launch, the groups run on separate streams, the
+// group with the largest LDS footprint first so that its workgroups claim
+// compute units before the smaller ones fill the rest.  Descriptors, depth
+// tables and scratch rows stay on the device between runs.
+struct BatchPlan {
+  struct Group {
+    int N, off, n;
+    StepGeom g;
+    int nz;
+    const float2* tw;
+  };
+  std::vector<Model*> models;          // distinct models
+  std::vector<unsigned> gens;          // their phase-table generations at planning
+  std::vector<Group> groups;
+  Workspace desc, scratch;
+  const SweepRow* d_rows = nullptr;
+  const DepthTables* d_tabs = nullptr;
+  ~BatchPlan() {
+    desc.release();
+    scratch.release();
+  }
+  int build(int nrows, Model* const* ms, const float* freq_hz, const cf* const* in, cf* const* out,
+            hipStream_t s);
+  int run(int up, int adjoint, hipStream_t s);
+};
+
+int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf* const* in, cf* const* out,
+                     hipStream_t s) {
+  if (nrows < 1 || !ms || !freq_hz || !in || !out) return fail(WEMFWI_ERR_ARG, "batch: invalid argument");
+  std::vector<int> midx(nrows);
+  for (int i = 0; i < nrows; ++i) {
+    if (!ms[i] || !in[i] || !out[i]) return fail(WEMFWI_ERR_ARG, "batch: null model or buffer");
+    if (int rc = ms[i]->require_fft()) return rc;
+    auto it = std::find(models.begin(), models.end(), ms[i]);
+    midx[i] = (int)(it - models.begin());
+    if (it == models.end()) models.push_back(ms[i]);
+  }
+  // phase tables: every model gets the frequencies of its rows
+  std::vector<std::vector<int>> rows_of(models.size());
+  for (int i = 0; i < nrows; ++i) rows_of[midx[i]].push_back(i);
+  std::vector<float> wrow(nrows);
+  std::vector<const float2*> prow(nrows);
+  for (size_t u = 0; u < models.size(); ++u) {
+    std::vector<float> f;
+    for (int i : rows_of[u]) f.push_back(freq_hz[i]);
+    std::vector<float> w;
+    std::vector<const float2*> p;
+    if (int rc = models[u]->phase_tables((int)f.size(), f.data(), w, p, s)) return rc;
+    for (size_t k = 0; k < rows_of[u].size(); ++k) {
+      wrow[rows_of[u][k]] = w[k];
+      prow[rows_of[u][k]] = p[k];
+    }
+  }
+  for (Model* m : models) gens.push_back(m->table_gen);
+  // group rows by kernel variant, largest FFT length first
+  std::vector<long long> key(nrows);
+  for (int i = 0; i < nrows; ++i) {
+    const StepGeom g = ms[i]->geom();
+    key[i] = -((long long)ms[i]->N * 4 + g.full * 2 + g.ssf);
+  }
+  std::vector<int> order(nrows);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  size_t scr = 0;
+  std::vector<size_t> scr_off(nrows);
+  for (int k = 0; k < nrows; ++k) {
+    scr_off[k] = scr;
+    scr += (size_t)ms[order[k]]->N;
+  }
+  if (int rc = scratch.reserve(scr * sizeof(float2), "batch scratch rows")) return rc;
+  std::vector<SweepRow> h(nrows);
+  for (int k = 0; k < nrows; ++k) {
+    const int i = order[k];
+    h[k].in = (const float2*)in[i];
+    h[k].out = (float2*)out[i];
+    h[k].ptab = prow[i];
+    h[k].w = wrow[i];
+    h[k].model = midx[i];
+    h[k].recursive = 1;
+    h[k].scratch = (float2*)scratch.ptr + scr_off[k];
+  }
+  for (int k0 = 0; k0 < nrows;) {
+    int k1 = k0 + 1;
+    while (k1 < nrows && key[order[k1]] == key[order[k0]]) ++k1;
+    const Model* mg = ms[order[k0]];
+    groups.push_back(Group{mg->N, k0, k1 - k0, mg->geom(), mg->d.nz, mg->d_tw});
+    k0 = k1;
+  }
+  std::vector<DepthTables> ht;
+  for (Model* m : models) ht.push_back(m->h_tables);
+  const size_t rows_bytes = (nrows * sizeof(SweepRow) + 255) / 256 * 256;
+  if (int rc = desc.reserve(rows_bytes + ht.size() * sizeof(DepthTables), "batch descriptors")) return rc;
+  d_rows = (const SweepRow*)desc.ptr;
+  d_tabs = (const DepthTables*)((char*)desc.ptr + rows_bytes);
+  hipError_t e = hipMemcpyAsync(desc.ptr, h.data(), nrows * sizeof(SweepRow), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync((void*)d_tabs, ht.data(), ht.size() * sizeof(DepthTables), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the host staging vectors end here
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch descriptor upload");
+}
+
+int BatchPlan::run(int up, int adjoint, hipStream_t s) {
+  for (size_t u = 0; u < models.size(); ++u)
+    if (models[u]->table_gen != gens[u])
+      return fail(WEMFWI_ERR_ARG, "batch: a model's phase table was rebuilt after planning; plan again");
+  hipStream_t st[4];
+  if (int rc = batch_streams(st)) return rc;
+  hipEvent_t ready;
+  hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+  if (e != hipSuccess) return hip_fail(e, "event");
+  std::vector<hipEvent_t> done;
+  e = hipEventRecord(ready, s);
+  const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
+  for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
+    const Group& G = groups[k];
+    hipStream_t sg = st[k % 4];
+    e = hipStreamWaitEvent(sg, ready, 0);
+    if (e == hipSuccess) e = launch_oneway(G.N, d_rows + G.off, G.n, d_tabs, G.g, G.nz, step, adjoint != 0, G.tw, sg);
+    hipEvent_t ev;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) {
+      done.push_back(ev);
+      e = hipEventRecord(ev, sg);
+    }
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipEventDestroy(ready);
+  for (hipEvent_t ev : done) (void)hipEventDestroy(ev);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch run");
+}
+
 int Model::onestep(int adjoint, int nrows, const float* freq_hz, const int* depth, const cf* in, cf* out,
                    int add, hipStream_t s) {
   if (nrows < 0 || !freq_hz || !depth || !in || !out) return fail(WEMFWI_ERR_ARG, "onestep: null argument");
@@ -476,6 +633,10 @@ int Model::onestep(int adjoint, int nrows, const float* freq_hz, const int* dept
 
 using namespace wemfwi;
 
+
+struct wemfwi_batch {
+  wemfwi::BatchPlan p;
+};
 
 extern "C" {
 
@@ -541,6 +702,48 @@ int wemfwi_oneway_recursive(wemfwi_model* model, int up, int adjoint, int nrows,
                             const wemfwi_complex* src, wemfwi_complex* out, void* stream) {
   if (!model) return fail(WEMFWI_ERR_ARG, "oneway_recursive: null model");
   return model->m.oneway(up, adjoint, nrows, freq_hz, (const cf*)src, (cf*)out, 1, 1, (hipStream_t)stream);
+}
+
+int wemfwi_batch_create(int nrows, wemfwi_model* const* models, const float* freq_hz,
+                        const wemfwi_complex* const* src, wemfwi_complex* const* out, void* stream,
+                        wemfwi_batch** plan) {
+  if (!plan || nrows < 1 || !models) return fail(WEMFWI_ERR_ARG, "batch_create: invalid argument");
+  *plan = nullptr;
+  std::vector<Model*> ms(nrows);
+  for (int i = 0; i < nrows; ++i) {
+    if (!models[i]) return fail(WEMFWI_ERR_ARG, "batch_create: null model");
+    ms[i] = &models[i]->m;
+  }
+  wemfwi_batch* h = new (std::nothrow) wemfwi_batch();
+  if (!h) return fail(WEMFWI_ERR_ALLOC, "batch_create: out of host memory");
+  int rc = h->p.build(nrows, ms.data(), freq_hz, (const cf* const*)src, (cf* const*)out, (hipStream_t)stream);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *plan = h;
+  return WEMFWI_OK;
+}
+
+int wemfwi_batch_run(wemfwi_batch* plan, int up, int adjoint, void* stream) {
+  if (!plan) return fail(WEMFWI_ERR_ARG, "batch_run: null plan");
+  return plan->p.run(up, adjoint, (hipStream_t)stream);
+}
+
+int wemfwi_batch_destroy(wemfwi_batch* plan) {
+  delete plan;
+  return WEMFWI_OK;
+}
+
+int wemfwi_oneway_batch(int up, int adjoint, int nrows, wemfwi_model* const* models, const float* freq_hz,
+                        const wemfwi_complex* const* src, wemfwi_complex* const* out, void* stream) {
+  if (nrows == 0) return WEMFWI_OK;
+  wemfwi_batch* plan = nullptr;
+  int rc = wemfwi_batch_create(nrows, models, freq_hz, src, out, stream, &plan);
+  if (rc) return rc;
+  rc = wemfwi_batch_run(plan, up, adjoint, stream);
+  wemfwi_batch_destroy(plan);
+  return rc;
 }
 
 int wemfwi_model_set_freqs(wemfwi_model* model, int nfreq, const float* freq_hz, void* stream) {

@@ -86,6 +86,7 @@ struct Model {
             const std::complex<float>* slow_refl = nullptr);
   StepGeom geom() const;
   std::vector<float> freqs;   // frequency set of the current phase table (sorted)
+  unsigned table_gen = 0;     // incremented each time the phase table is rebuilt
   int NH() const { return N / 2 + 1; }
   int set_freqs(int n, const float* freq_hz, hipStream_t s);
   int require_fft() const;  // WEMFWI_ERR_UNSUPPORTED unless fft_ok

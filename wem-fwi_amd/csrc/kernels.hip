@@ -71,7 +71,9 @@ struct WavesPerBlock {
 // In-place or out-of-place Down/Up recursion (OneWay.cpp:32-109) of one
 // (shot, frequency) row per wavefront.  step = +1 runs iz = 0..nz-2 (Down
 // forward, Up adjoint), step = -1 runs iz = nz-1..1 (Down adjoint, Up forward).
-// Forward steps use depth cur, adjoint steps depth next.
+// Forward steps use depth cur, adjoint steps depth next.  The row's width,
+// padding, taper and depth count come from its model, so one launch may hold
+// rows of different apertures; the launch fixes the FFT length and variant.
 // Waves per SIMD the register allocator targets for an FFT length.
 template <int N>
 struct Occupancy {
@@ -82,7 +84,7 @@ template <class V, bool ADJ>
 __global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
     __attribute__((amdgpu_waves_per_eu(Occupancy<V::n>::value, 8)))
     oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
-                  StepGeom g, int nz, int step, const float2* __restrict__ twg) {
+                  StepGeom g, int step, const float2* __restrict__ twg) {
   constexpr int N = V::n;
   constexpr int E = V::E;
   constexpr int kWavesPerBlock = WavesPerBlock<N>::value;
@@ -96,6 +98,10 @@ __global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
   if (row >= nrows) return;
   const SweepRow R = ldu_struct(rows, row);
   const DepthTables dt = ldu_struct(models, R.model);
+  g.nx = dt.nx;
+  g.tap = dt.tap;
+  g.taper = dt.taper;
+  const int nz = dt.nz;
   float2* buf = bufs[wv];
   const int nx = g.nx;
   const bool recursive = R.recursive != 0;
@@ -202,9 +208,9 @@ static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTa
   constexpr int kWavesPerBlock = WavesPerBlock<V::n>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   if (adj)
-    hipLaunchKernelGGL((oneway_kernel<V, true>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, true>), grid, block, 0, s, rows, nrows, models, g, step, tw);
   else
-    hipLaunchKernelGGL((oneway_kernel<V, false>), grid, block, 0, s, rows, nrows, models, g, nz, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, false>), grid, block, 0, s, rows, nrows, models, g, step, tw);
   return hipGetLastError();
 }
 

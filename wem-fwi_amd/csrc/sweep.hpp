@@ -126,6 +126,9 @@ struct DepthTables {
   int n;                        // FFT length N (lane-table row length)
   float dz;                     // depth step of this model's grid
   float dk;                     // wavenumber step of its scatterer grid
+  int nx;                       // propagation width of this model
+  int tap;                      // FFT zero padding per side (N = nx + 2 tap)
+  int taper;                    // 1: a taper is applied
 };
 
 struct StepGeom {
@@ -411,8 +414,15 @@ __device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, floa
                                                  const float2* tw, int t, float2* scr) {
   constexpr int N = V::n;
   constexpr int E = V::E;
-  fft_row<N, false>(st, buf, tw, t);
   const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
+  if (s0 == s1) {
+    // no reference selects any x of this depth: Selector leaves the row zero
+    // (RefSampler's stale buffer can leave a depth without a live reference)
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
+    return;
+  }
+  fft_row<N, false>(st, buf, tw, t);
   const bool multi = s1 - s0 > 1;
   const bool all = V::FULL && ldu(dt.one_ref, d);
   const brsrc xs = make_rsrc(scr, (unsigned)N * 8u);  // the wave's scratch row, lane index t + 64 e
@@ -468,13 +478,18 @@ __device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, floa
                                                  const float2* tw, int t, float2* scr) {
   constexpr int N = V::n;
   constexpr int E = V::E;
+  const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
+  if (s0 == s1) {  // the adjoint of a zero row
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
+    return;
+  }
   if (V::SSF || g.taper) {
     float2 f[E];
     ss_taper_row<V>(f, dt, g, d, w, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], f[e]);
   }
-  const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
   const bool multi = s1 - s0 > 1;
   const bool all = V::FULL && ldu(dt.one_ref, d);
   const brsrc xs = make_rsrc(scr, (unsigned)N * 8u);  // the wave's scratch row, lane index t + 64 e

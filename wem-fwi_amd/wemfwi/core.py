@@ -148,6 +148,63 @@ class Model:
         return out
 
 
+def _batch_args(models, freq_hz, src, out):
+    n = len(models)
+    if not (len(freq_hz) == len(src) == len(out) == n) or n == 0:
+        raise ValueError("batch: one model, frequency, source and output per row")
+    for m, a, b in zip(models, src, out):
+        _dev_c64(a, "src")
+        _dev_c64(b, "out")
+        if tuple(a.shape[-2:]) != (m.nz, m.nx) or a.numel() != m.nz * m.nx or b.shape != a.shape:
+            raise ValueError("batch: buffer shape does not match its model")
+    f = np.ascontiguousarray(np.asarray(freq_hz, dtype=np.float32))
+    mh = (C.c_void_p * n)(*[m._h.value for m in models])
+    ih = (C.c_void_p * n)(*[a.data_ptr() for a in src])
+    oh = (C.c_void_p * n)(*[b.data_ptr() for b in out])
+    return n, f, mh, ih, oh
+
+
+def oneway_batch(up: bool, adjoint: bool, models: Sequence[Model], freq_hz: Sequence[float],
+                 src: Sequence[torch.Tensor], out: Sequence[torch.Tensor],
+                 stream: Optional[torch.cuda.Stream] = None):
+    """Recursive Down/Up of rows with their own models and frequencies in one
+    call: out[i] = (I - L_i)^-1 src[i], each [nz_i][nx_i] on its model's grid
+    (src[i] may be out[i])."""
+    n, f, mh, ih, oh = _batch_args(models, freq_hz, src, out)
+    check(lib.wemfwi_oneway_batch(int(up), int(adjoint), n, mh, f.ctypes.data_as(C.POINTER(C.c_float)), ih, oh,
+                                  _stream_ptr(stream)))
+    return out
+
+
+class BatchPlan:
+    """oneway_batch as a plan: rows fixed once, each run launches only.  The
+    models and tensors must stay alive (and keep their frequency sets) while
+    the plan is used."""
+
+    def __init__(self, models: Sequence[Model], freq_hz: Sequence[float], src: Sequence[torch.Tensor],
+                 out: Sequence[torch.Tensor], stream: Optional[torch.cuda.Stream] = None):
+        n, f, mh, ih, oh = _batch_args(models, freq_hz, src, out)
+        self._keep = (list(models), list(src), list(out))
+        h = C.c_void_p()
+        check(lib.wemfwi_batch_create(n, mh, f.ctypes.data_as(C.POINTER(C.c_float)), ih, oh, _stream_ptr(stream),
+                                      C.byref(h)))
+        self._h = h
+
+    def run(self, up: bool, adjoint: bool, stream: Optional[torch.cuda.Stream] = None):
+        check(lib.wemfwi_batch_run(self._h, int(up), int(adjoint), _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.wemfwi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def timing(enable: bool) -> None:
     """Enables (and clears) or disables HIP-event kernel timing in the library."""
     check(lib.wemfwi_timing(int(enable)))
