@@ -321,6 +321,35 @@ int wemfwi_offset_image_adjoint(wemfwi_model* model, wemfwi_survey* survey, int 
                                 wemfwi_complex* image, const wemfwi_complex* data, int add,
                                 void* stream);
 
+/* ---------------------------------------------------------------- extended FWI
+ * Operators of the frequency-extended objective of scripts/efwi.py:160-190
+ * (operator/src/python/Operator.py:228-292,619-775), on device models
+ * [nw][nz][nx]. */
+
+/* Matrix M[n][n] of scipy.ndimage.correlate1d (convolve = 0) or convolve1d
+ * (convolve = 1) with taps w[L], origin 0, along a line of n samples, so that
+ * out[i] = sum_j M[i][j] in[j]; boundary mode 0 reflect, 1 constant (0),
+ * 2 nearest, 3 mirror, 4 wrap (extended periodically for L > n).  Host. */
+int wemfwi_filter_matrix(int n, int L, const double* w, int mode, int convolve, double* M);
+
+/* out[i][p] (+)= c * sum_j M[i][j] in[j][p] + i * alpha * mean_j Im in[j][p]
+ * over ncol columns p, c = c_re + i c_im, M a device float matrix [nw][nw]:
+ * Derivative::forward is c = i*beta with the correlate matrix,
+ * Derivative::adjoint c = -i*beta with the convolve matrix (Operator.py:272-292).
+ * in and out are distinct device arrays [nw][ncol]. */
+int wemfwi_freq_filter(int nw, long long ncol, const float* M, float c_re, float c_im, float alpha,
+                       const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
+
+/* Soft clip of one component (comp 0: real, 1: imaginary) of n device
+ * samples with h(u) = l u - sqrt(l^2 u^2 + tau^2) (Operator.py:619-623):
+ * jacobian = 0: y = sign*h(sign*(x - x0)) + x0 (sign = +1 Softmax /
+ * HyperbolicPenalty with x0 = 0, -1 Softmin); jacobian = 1: y = x * h'(sign*(b - x0))
+ * at the background bg (Softclip, dSoftmax, dSoftmin; self-adjoint).  The
+ * other component passes through; add = 1 accumulates into out. */
+int wemfwi_soft_clip(long long n, int comp, int sign, float x0, float l, float tau, int jacobian,
+                     const wemfwi_complex* bg, const wemfwi_complex* in, wemfwi_complex* out, int add,
+                     void* stream);
+
 /* ---------------------------------------------------------------- unit operators
  * Row-batched forms of the reference's unit operators (py_wem_ops.cpp:25-206),
  * for operator tests and custom pipelines.  Wavefields are device arrays;

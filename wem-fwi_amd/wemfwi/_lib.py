@@ -92,6 +92,10 @@ SIGNATURES = {
     "wemfwi_dreflect": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_ic": (_i, [_i, C.c_longlong, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_injection": (_i, [_i, _i, _i, _ip, _ip, _i, _i, _i, _i, _vp, _vp, _i, _vp]),
+    "wemfwi_filter_matrix": (_i, [_i, _i, C.POINTER(C.c_double), _i, _i, C.POINTER(C.c_double)]),
+    "wemfwi_freq_filter": (_i, [_i, C.c_longlong, _vp, C.c_float, C.c_float, C.c_float, _vp, _vp, _i, _vp]),
+    "wemfwi_soft_clip": (_i, [C.c_longlong, _i, _i, C.c_float, C.c_float, C.c_float, _i, _vp, _vp, _vp, _i,
+                              _vp]),
     "wemfwi_timing": (_i, [_i]),
     "wemfwi_timing_get": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
 }
