@@ -350,6 +350,25 @@ int wemfwi_soft_clip(long long n, int comp, int sign, float x0, float l, float t
                      const wemfwi_complex* bg, const wemfwi_complex* in, wemfwi_complex* out, int add,
                      void* stream);
 
+/* ---------------------------------------------------------------- model interpolation
+ * Preconditioning of the FWI model (scripts/fwi.py:46-68, efwi.py:48-80). */
+
+/* Filter F[nd][nm] from a coarse axis (nm samples, spacing dm) to a fine axis
+ * (nd, dd), host: kind 0 Lanczos with a window of a coarse cells
+ * (LanczosInterpolation3D.h:29-55; the 2-D operator is taper 0), kind 1 the
+ * cubic (a, b) spline (Spline3D.h:22-52; Spline2D.h:97-127 is taper 0);
+ * taper widens the kernel to (1 + taper) dm.  Both fold the taps that fall
+ * outside the axis back onto it, as written there. */
+int wemfwi_interp_filter(int kind, int nd, float dd, int nm, float dm, float a, float b, float taper,
+                         float* F);
+
+/* out[(a n_out + i) inner + b] (+)= sum_j F[i][j] in[(a n_in + j) inner + b]
+ * for a device float matrix F[n_out][n_in] (transpose = 1: F stored
+ * [n_in][n_out], applied transposed): one axis of the separable
+ * interpolation (Interpolation3D.h:21-95).  in != out. */
+int wemfwi_axis_apply(const float* F, int n_out, int n_in, long long outer, long long inner, int transpose,
+                      const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
+
 /* ---------------------------------------------------------------- unit operators
  * Row-batched forms of the reference's unit operators (py_wem_ops.cpp:25-206),
  * for operator tests and custom pipelines.  Wavefields are device arrays;

@@ -96,6 +96,8 @@ SIGNATURES = {
     "wemfwi_freq_filter": (_i, [_i, C.c_longlong, _vp, C.c_float, C.c_float, C.c_float, _vp, _vp, _i, _vp]),
     "wemfwi_soft_clip": (_i, [C.c_longlong, _i, _i, C.c_float, C.c_float, C.c_float, _i, _vp, _vp, _vp, _i,
                               _vp]),
+    "wemfwi_interp_filter": (_i, [_i, _i, C.c_float, _i, C.c_float, C.c_float, C.c_float, C.c_float, _fp]),
+    "wemfwi_axis_apply": (_i, [_vp, _i, _i, C.c_longlong, C.c_longlong, _i, _vp, _vp, _i, _vp]),
     "wemfwi_timing": (_i, [_i]),
     "wemfwi_timing_get": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
 }

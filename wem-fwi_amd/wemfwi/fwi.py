@@ -15,6 +15,10 @@ adjoint of the linearised operator.
 Background wavefields are kept on the device for all rows of a call rather
 than checkpointed: HBM holds the Down/Up fields of the whole survey for the
 model sizes of this path, so recomputation would only cost time.
+
+fwi_problem / fit_preconditioned / shard_par assemble the driver of
+scripts/fwi.py:46-131 around it (solvers in wemfwi.solver, interpolation in
+wemfwi.interp).
 """
 
 from __future__ import annotations
@@ -86,3 +90,69 @@ class _Misfit(torch.autograd.Function):
         gS, gR = ctx.obj.gradient(slow, den, ctx.res)
         s = grad_out.to(torch.float32)
         return gS * s, gR * s, None
+
+
+# ------------------------------------------------------------------ driver
+
+def shard_par(par: Mapping, s0: int, s1: int) -> dict:
+    """The parameters of shots [s0, s1) of a regular source line
+    (n/o/d s keys of Device::createRegGeom): one process's share of the shots."""
+    p = dict(par)
+    if str(p.get("coord", "") or ""):
+        raise ValueError("shard_par: irregular (JSON) geometry is sharded by its own coordinate lists")
+    p["ns"] = s1 - s0
+    p["osx"] = float(p.get("osx", 0.0)) + s0 * float(p.get("dsx", 0.0))
+    p["osz"] = float(p.get("osz", 0.0)) + s0 * float(p.get("dsz", 0.0))
+    return p
+
+
+def fwi_problem(misfit: WEMMisfit, den: torch.Tensor, precond=None, coarse_shape=None, vmin: Optional[float] = None,
+                vmax: Optional[float] = None, mask: Optional[torch.Tensor] = None):
+    """ProblemL2NonLinear of scripts/fwi.py:97-110 on the slowness: the model
+    is the slowness itself or, with a preconditioner P (an interpolation
+    operator from a coarse grid), its coarse coefficients m = P m_pre
+    (fwi.py:46-68, 95-96); bounds 1/vmax^2 <= Re m <= 1/vmin^2 on the solver's
+    variable (fwi.py:98-104); mask multiplies the gradient (the Born input,
+    fwi.py:73-76).  The objective and gradient are summed over the processes of
+    an initialised torch.distributed group, each holding its shots."""
+    from . import solver
+
+    def fg(x):
+        m = x[0]
+        if precond is not None:
+            full = torch.zeros_like(den)
+            precond.forward(False, m, full)
+            m = full
+        res = misfit.residual(m, den)
+        J = 0.5 * float((res.arr.to(torch.complex128).abs() ** 2).sum())
+        gS, _ = misfit.gradient(m, den, res)
+        if mask is not None:
+            gS = gS * mask
+        if precond is not None:
+            gp = torch.zeros(coarse_shape, dtype=torch.complex64, device=gS.device)
+            precond.adjoint(False, gp, gS.contiguous())
+            gS = gp
+        return solver.allreduce_value_and_grad(J, [gS])
+
+    lo = None if vmax is None else 1.0 / vmax ** 2
+    hi = None if vmin is None else 1.0 / vmin ** 2
+    return solver.Problem(fg, lo, hi)
+
+
+def fit_preconditioned(precond, model: torch.Tensor, coarse_shape, niter: int) -> torch.Tensor:
+    """The coarse coefficients that best reproduce a model, by CG on
+    1/2 |P m_pre - m|^2 (the LCGsolver fit of fwi.py:58-68)."""
+    from . import solver
+
+    def fwd(v):
+        out = torch.zeros_like(model)
+        precond.forward(False, v[0], out)
+        return [out]
+
+    def adj(r):
+        out = torch.zeros(coarse_shape, dtype=torch.complex64, device=model.device)
+        precond.adjoint(False, out, r[0].contiguous())
+        return [out]
+
+    x0 = [torch.zeros(coarse_shape, dtype=torch.complex64, device=model.device)]
+    return solver.cgls(fwd, adj, x0, [model], niter)[0]
