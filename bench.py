@@ -17,7 +17,9 @@ Also reported in the same line: forward and adjoint sweep rates on their own,
 and the FWI gradient (WEM forward + residual + exact Born adjoint, 512 x 300,
 128 frequencies, 32 shots) with all rows batched against the same library
 called once per (shot, frequency) row (a timed sample of rows, scaled), with
-the relative difference of the two gradients on the sampled rows.
+the relative difference of the two gradients on the sampled rows; the batched
+gradient again with wavefield rows kept every 25 depth steps (recomputed
+segment by segment), which must equal the store-all gradient bitwise.
 
 Also the uneven batch: 64 shots with their own apertures (256-2048) and
 bands (32-192 frequencies), nz = 400, Down forward of all rows in one batched
@@ -119,6 +121,7 @@ def cpu_baseline(cfg, slow, freqs, src_ix, nrows_sample):
 # smooth field regridded to 512 x 300 plus two Gaussian anomalies of +-300 m/s
 # (radius 200 m); start model = the true model smoothed over 10 cells.
 FWI = dict(nx=512, nz=300, nw=128, fmin=2.0, fmax=15.0, ns=32, seed=4)
+CKPT = 25  # depth steps between kept wavefield rows in the checkpointed gradient
 
 
 def fwi_models():
@@ -179,6 +182,17 @@ def fwi_gradient_bench(sample_rows: int):
     batched()
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
+    # the same gradient with wavefield rows kept every CKPT depth steps
+    g_all = (gS.clone(), gR.clone())
+    sv.set_checkpoint(CKPT)
+    batched()  # warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batched()
+    torch.cuda.synchronize()
+    t_ckpt = time.perf_counter() - t0
+    ckpt_equal = bool(torch.equal(gS, g_all[0]) and torch.equal(gR, g_all[1]))
+    sv.set_checkpoint(0)
     # plain loop over a sample of rows (one survey per row, built beforehand)
     total = ns * nw
     stride = max(1, total // sample_rows)
@@ -221,6 +235,11 @@ def fwi_gradient_bench(sample_rows: int):
     return {"config": f"ssf nx={nx} nz={nz} nw={nw} {c['fmin']:g}-{c['fmax']:g} Hz ns={ns} nr={nx}",
             "batched_ms": round(t_batched * 1e3, 2),
             "batched_gupdates_per_s": round(upd / t_batched / 1e9, 2),
+            "checkpoint_every": CKPT,
+            "checkpointed_ms": round(t_ckpt * 1e3, 2),
+            "checkpointed_bitwise_equal": ckpt_equal,
+            "adjoint_row_bytes": {"store_all": 5 * nz * nx * 8,
+                                  "checkpointed": (5 * (CKPT + 2) + 2 * (-(-(nz - 1) // CKPT)) + 2) * nx * 8},
             "plain_loop_ms": round(t_plain * 1e3, 1),
             "plain_loop_sample": f"{len(rows)} of {total} rows timed ({t_plain_sample:.2f} s), scaled to all rows",
             "speedup": round(t_plain / t_batched, 2),

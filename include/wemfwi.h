@@ -212,6 +212,17 @@ typedef struct wemfwi_survey wemfwi_survey;
 int wemfwi_survey_create(const wemfwi_survey_desc* desc, wemfwi_survey** out);
 int wemfwi_survey_destroy(wemfwi_survey* survey);
 
+/* Wavefield checkpointing of the Born adjoints called with this survey
+ * (born_adjoint, extborn_adjoint, shotborn_adjoint; no counterpart in the
+ * reference, which keeps whole fields, born_prop.cpp:178-232): every >= 1
+ * keeps the background and receiver rows every `every` depth steps and
+ * recomputes the fields segment by segment, the per-row device memory
+ * dropping from 5 nz nx to (5 (every + 2) + 2 ceil((nz-1)/every) + 2) nx
+ * complex samples at the cost of one more pass of the two forward
+ * recursions.  The gradient is bitwise the same as with whole fields.
+ * 0 (the default), or every >= nz - 1, keeps whole fields. */
+int wemfwi_survey_set_checkpoint(wemfwi_survey* survey, int every);
+
 /* WEM::forward into complex data (propagator/src/WEMnl.cpp:67-82,
  * wem_prop.cpp:9-78).  model: background slowness and density on the
  * propagation grid (edge padded by "pad", wem_prop.cpp:26-33).  data

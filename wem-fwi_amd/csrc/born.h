@@ -15,6 +15,7 @@ struct LinRow {
   const float2* ptab;    // phase table of the row's frequency
   float w;
   int model;
+  int zbase;             // depth of the first row of bg / m / out (0: whole arrays)
 };
 
 // Injection of many devices as a gather per touched cell (CSR by cell).
@@ -34,8 +35,9 @@ struct RecPlan {
   const float* wgt;       // [nc]
 };
 
+// depths d of [dlo, dlo + dcount) (dlo < 0: every depth of the direction)
 hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
-                      int dir, bool adj, bool exact, const float2* tw, hipStream_t s);
+                      int dir, bool adj, bool exact, const float2* tw, hipStream_t s, int dlo = -1, int dcount = -1);
 hipError_t launch_scatter(int N, const StepRow* rows, int nrows, const DepthTables* models, StepGeom g, bool adj,
                           bool exact, const float2* tw, hipStream_t s);
 // One row of a batched product out (=|+=) op(a) * b over n samples.
@@ -54,10 +56,14 @@ hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s)
 hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, const long long* src, const float2* in,
                              bool add, hipStream_t s);
 hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hipStream_t s);
+// W holds rows zbase.. of each [nz][nx] field; sources outside [zlo, zhi]
+// (cells outside [clo, chi), with cbase = zbase*nx) are skipped
 hipError_t launch_inject_point(float2* W, long long wstride, int nx, const int* iz, const int* ix,
-                               const float2* val, const float* gw, int ng, int tap, int nrows, hipStream_t s);
+                               const float2* val, const float* gw, int ng, int tap, int nrows, hipStream_t s,
+                               int zbase = 0, int zlo = 0, int zhi = 0x7fffffff);
 hipError_t launch_inject_csr(float2* W, long long wstride, InjPlan p, const float2* val, const long long* vbase,
-                             long long vdev, int nrows, hipStream_t s);
+                             long long vdev, int nrows, hipStream_t s, long long cbase = 0, long long clo = 0,
+                             long long chi = 0x7fffffffffffffffLL);
 hipError_t launch_record(float2* val, const long long* vbase, long long vdev, const float2* W, long long wstride,
                          RecPlan p, int nrows, hipStream_t s);
 hipError_t launch_dreflect_fwd(float2* dr, const float2* coef, const float2* dS, const float2* dR, int nz, int nx,

@@ -37,7 +37,7 @@ template <int N, bool ADJ, bool EXACT>
 __global__ void __launch_bounds__(64 * kLinWaves)
     __attribute__((amdgpu_waves_per_eu(N <= 256 ? 4 : 2, 8)))
     lin_kernel(const LinRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models, StepGeom g,
-               int nz, int dir, const float2* __restrict__ twg) {
+               int dlo, int nd, int dir, const float2* __restrict__ twg) {
   using V = Var<N, false, true>;
   constexpr int E = N / 64;
   __shared__ float2 tw[plan_tw_size<N>()];
@@ -47,34 +47,33 @@ __global__ void __launch_bounds__(64 * kLinWaves)
   for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int item = blockIdx.x * kLinWaves + wv;
-  const int nd = nz - 1;
   if (item >= nrows * nd) return;
   const int row = item / nd;
-  const int j = item - row * nd;
-  const int d = dir > 0 ? j : j + 1;
+  const int d = dlo + (item - row * nd);
   const LinRow R = ldu_struct(rows, row);
   const DepthTables dt = ldu_struct(models, R.model);
   float2* buf = bufs[wv];
   const size_t nx = g.nx;
+  const int r0 = d - R.zbase;  // buffer row of depth d
   float2 st[E];
   if constexpr (!ADJ) {
     float2 b[E];
-    lrow<V>(st, R.m + d * nx, g, t);
-    lrow<V>(b, R.bg + d * nx, g, t);
+    lrow<V>(st, R.m + r0 * nx, g, t);
+    lrow<V>(b, R.bg + r0 * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmul(b[e], st[e]);
     scatter_row<V, false, false>(st, d, R.w, dt, g, buf, tw, t);
     step_forward<V>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
-    acc_row<V>(R.out + (d + dir) * nx, st, g, t);
+    acc_row<V>(R.out + (r0 + dir) * nx, st, g, t);
   } else {
-    lrow<V>(st, R.m + (d + dir) * nx, g, t);
+    lrow<V>(st, R.m + (r0 + dir) * nx, g, t);
     step_adjoint<V>(st, d, R.w, dt, g, R.ptab, buf, tw, t);
     scatter_row<V, true, EXACT>(st, d, R.w, dt, g, buf, tw, t);
     float2 b[E];
-    lrow<V>(b, R.bg + d * nx, g, t);
+    lrow<V>(b, R.bg + r0 * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], b[e]);
-    acc_row<V>(R.out + d * nx, st, g, t);
+    acc_row<V>(R.out + r0 * nx, st, g, t);
   }
 }
 
@@ -206,14 +205,17 @@ __global__ void d2f_kernel(float2* __restrict__ out, const double2* __restrict__
 // g[ig] * val[row], ig = -ng..ng, skipping cells outside [0,nx).
 __global__ void inject_point_kernel(float2* __restrict__ W, long long wstride, int nx, const int* __restrict__ iz,
                                     const int* __restrict__ ix, const float2* __restrict__ val,
-                                    const float* __restrict__ gw, int ng, int tap, int nrows) {
+                                    const float* __restrict__ gw, int ng, int tap, int nrows, int zbase, int zlo,
+                                    int zhi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int span = 2 * ng + 1;
   if (i >= nrows * span) return;
   const int row = i / span, ig = i - row * span - ng;
   const int off = ix[row] + ig + tap;
   if (off < 0 || off >= nx) return;
-  float2* p = W + row * wstride + (long long)iz[row] * nx + off;
+  const int z = iz[row];
+  if (z < zlo || z > zhi) return;  // outside the rows W holds
+  float2* p = W + row * wstride + (long long)(z - zbase) * nx + off;
   *p = cadd(*p, cscale(val[row], gw[ig + ng]));
 }
 
@@ -221,17 +223,18 @@ __global__ void inject_point_kernel(float2* __restrict__ W, long long wstride, i
 // W[row][cell] += sum_c w_c * val[vbase[row] + dev_c * vdev].
 __global__ void inject_csr_kernel(float2* __restrict__ W, long long wstride, InjPlan p,
                                   const float2* __restrict__ val, const long long* __restrict__ vbase,
-                                  long long vdev, int nrows) {
+                                  long long vdev, int nrows, long long cbase, long long clo, long long chi) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)nrows * p.ncell) return;
   const int row = (int)(i / p.ncell), cell = (int)(i - (long long)row * p.ncell);
+  if (p.cell[cell] < clo || p.cell[cell] >= chi) return;  // outside the rows W holds
   float sx = 0.f, sy = 0.f;
   for (int c = p.ptr[cell]; c < p.ptr[cell + 1]; ++c) {
     const float2 v = val[vbase[row] + (long long)p.dev[c] * vdev];
     sx += p.wgt[c] * v.x;
     sy += p.wgt[c] * v.y;
   }
-  float2* q = W + row * wstride + p.cell[cell];
+  float2* q = W + row * wstride + (p.cell[cell] - cbase);
   *q = make_float2(q->x + sx, q->y + sy);
 }
 
@@ -353,28 +356,33 @@ static dim3 grid1d(long long n, int block = 256) {
 }
 
 template <int N>
-static hipError_t launch_lin_n(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
-                               int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
-  const long long items = (long long)nrows * (nz - 1);
+static hipError_t launch_lin_n(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int dlo,
+                               int nd, int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
+  const long long items = (long long)nrows * nd;
   dim3 grid((unsigned)((items + kLinWaves - 1) / kLinWaves)), block(64 * kLinWaves);
   if (!adj)
-    hipLaunchKernelGGL((lin_kernel<N, false, false>), grid, block, 0, s, rows, nrows, models, g, nz, dir, tw);
+    hipLaunchKernelGGL((lin_kernel<N, false, false>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir, tw);
   else if (exact)
-    hipLaunchKernelGGL((lin_kernel<N, true, true>), grid, block, 0, s, rows, nrows, models, g, nz, dir, tw);
+    hipLaunchKernelGGL((lin_kernel<N, true, true>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir, tw);
   else
-    hipLaunchKernelGGL((lin_kernel<N, true, false>), grid, block, 0, s, rows, nrows, models, g, nz, dir, tw);
+    hipLaunchKernelGGL((lin_kernel<N, true, false>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir, tw);
   return hipGetLastError();
 }
 
 hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
-                      int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
-  if (nrows <= 0 || nz < 2) return hipSuccess;
+                      int dir, bool adj, bool exact, const float2* tw, hipStream_t s, int dlo, int dcount) {
+  if (dlo < 0) {  // LinDown: d = 0..nz-2 (neighbour d+1); LinUp: d = 1..nz-1 (neighbour d-1)
+    dlo = dir > 0 ? 0 : 1;
+    dcount = nz - 1;
+  }
+  if (dlo < (dir > 0 ? 0 : 1) || dlo + dcount > (dir > 0 ? nz - 1 : nz)) return hipErrorInvalidValue;
+  if (nrows <= 0 || dcount <= 0) return hipSuccess;
   switch (N) {
-    case 128: return launch_lin_n<128>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
-    case 256: return launch_lin_n<256>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
-    case 512: return launch_lin_n<512>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
-    case 1024: return launch_lin_n<1024>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
-    case 2048: return launch_lin_n<2048>(rows, nrows, models, g, nz, dir, adj, exact, tw, s);
+    case 128: return launch_lin_n<128>(rows, nrows, models, g, dlo, dcount, dir, adj, exact, tw, s);
+    case 256: return launch_lin_n<256>(rows, nrows, models, g, dlo, dcount, dir, adj, exact, tw, s);
+    case 512: return launch_lin_n<512>(rows, nrows, models, g, dlo, dcount, dir, adj, exact, tw, s);
+    case 1024: return launch_lin_n<1024>(rows, nrows, models, g, dlo, dcount, dir, adj, exact, tw, s);
+    case 2048: return launch_lin_n<2048>(rows, nrows, models, g, dlo, dcount, dir, adj, exact, tw, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -450,19 +458,22 @@ hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hip
 }
 
 hipError_t launch_inject_point(float2* W, long long wstride, int nx, const int* iz, const int* ix,
-                               const float2* val, const float* gw, int ng, int tap, int nrows, hipStream_t s) {
+                               const float2* val, const float* gw, int ng, int tap, int nrows, hipStream_t s,
+                               int zbase, int zlo, int zhi) {
   const long long n = (long long)nrows * (2 * ng + 1);
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(inject_point_kernel, grid1d(n), dim3(256), 0, s, W, wstride, nx, iz, ix, val, gw, ng, tap,
-                     nrows);
+                     nrows, zbase, zlo, zhi);
   return hipGetLastError();
 }
 
 hipError_t launch_inject_csr(float2* W, long long wstride, InjPlan p, const float2* val, const long long* vbase,
-                             long long vdev, int nrows, hipStream_t s) {
+                             long long vdev, int nrows, hipStream_t s, long long cbase, long long clo,
+                             long long chi) {
   const long long n = (long long)nrows * p.ncell;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(inject_csr_kernel, grid1d(n), dim3(256), 0, s, W, wstride, p, val, vbase, vdev, nrows);
+  hipLaunchKernelGGL(inject_csr_kernel, grid1d(n), dim3(256), 0, s, W, wstride, p, val, vbase, vdev, nrows, cbase,
+                     clo, chi);
   return hipGetLastError();
 }
 

@@ -421,6 +421,10 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
     h[i].w = wrow[i];
     h[i].model = 0;
     h[i].recursive = recursive ? 1 : 0;
+    h[i].first = -1;
+    h[i].nsteps = -1;
+    h[i].init = nullptr;
+    h[i].zbase = 0;
   }
   if ((rc = scratch.reserve((size_t)nrows * N * sizeof(float2), "scratch rows"))) return rc;
   for (int i = 0; i < nrows; ++i) h[i].scratch = (float2*)scratch.ptr + (size_t)i * N;
@@ -541,6 +545,10 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     h[k].w = wrow[i];
     h[k].model = midx[i];
     h[k].recursive = 1;
+    h[k].first = -1;
+    h[k].nsteps = -1;
+    h[k].init = nullptr;
+    h[k].zbase = 0;
     h[k].scratch = (float2*)scratch.ptr + scr_off[k];
   }
   for (int k0 = 0; k0 < nrows;) {

@@ -17,6 +17,10 @@ struct SweepRow {
   int recursive;         // 1: W[next] = P W[cur] + in[next] with W the output rows
                          // 0: out[next] = P in[cur] + in[next] (one bidiagonal step)
   float2* scratch;       // [N] row of this wave for depths with several references
+  int first;             // depth of the initial row; -1: 0 or nz-1 by the direction
+  int nsteps;            // depth steps from it; -1: nz-1 (the whole sweep)
+  const float2* init;    // initial state row [nx]; NULL: the row in[first]
+  int zbase;             // depth of the first row of in / out (0: whole [nz][nx] arrays)
 };
 
 // One independent depth step.

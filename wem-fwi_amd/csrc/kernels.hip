@@ -101,25 +101,28 @@ __global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
   g.nx = dt.nx;
   g.tap = dt.tap;
   g.taper = dt.taper;
-  const int nz = dt.nz;
   float2* buf = bufs[wv];
   const int nx = g.nx;
   const bool recursive = R.recursive != 0;
-  const int first = step > 0 ? 0 : nz - 1;
+  // a row covers depths first .. first + step * nsteps (a whole sweep, or a
+  // segment started from a checkpointed state row)
+  const int first = R.first >= 0 ? R.first : (step > 0 ? 0 : dt.nz - 1);
+  const int nsteps = R.nsteps >= 0 ? R.nsteps : dt.nz - 1;
+  const int zb = R.zbase;
   float2 st[E];
-  load_row<V>(st, R.in + (size_t)first * nx, g, t);
-  if (R.in != R.out) store_row<V>(R.out + (size_t)first * nx, st, g, t);
-  for (int k = 0; k < nz - 1; ++k) {
+  load_row<V>(st, R.init ? R.init : R.in + (size_t)(first - zb) * nx, g, t);
+  if (R.in != R.out || R.init) store_row<V>(R.out + (size_t)(first - zb) * nx, st, g, t);
+  for (int k = 0; k < nsteps; ++k) {
     const int cur = first + k * step;
     const int nxt = cur + step;
-    if (!recursive && k > 0) load_row<V>(st, R.in + (size_t)cur * nx, g, t);
+    if (!recursive && k > 0) load_row<V>(st, R.in + (size_t)(cur - zb) * nx, g, t);
     if constexpr (ADJ) step_adjoint<V, true>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
     else step_forward<V, true>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
     float2 src[E];
-    load_row<V>(src, R.in + (size_t)nxt * nx, g, t);
+    load_row<V>(src, R.in + (size_t)(nxt - zb) * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
-    store_row<V>(R.out + (size_t)nxt * nx, st, g, t);
+    store_row<V>(R.out + (size_t)(nxt - zb) * nx, st, g, t);
   }
 }
 

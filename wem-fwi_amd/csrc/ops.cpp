@@ -56,6 +56,7 @@ struct Survey {
   // empty unless set; then shot s has sources ssrc_*[s] and receivers srec_*[s]
   std::vector<std::vector<int>> ssrc_iz, ssrc_ix, srec_iz, srec_ix;
   std::vector<DevPlan> ssrc_plan, srec_plan;
+  int ckpt = 0;  // Born adjoints: depth steps between kept wavefield rows (0: keep all)
   bool per_shot() const { return !ssrc_iz.empty(); }
   ~Survey() {
     rplan.release();
@@ -293,6 +294,10 @@ struct Ctx {
       hsw[i].w = omega(r.model[i], r.fi[i]);
       hsw[i].model = model_index(r.model[i]);
       hsw[i].recursive = 1;
+      hsw[i].first = -1;
+      hsw[i].nsteps = -1;
+      hsw[i].init = nullptr;
+      hsw[i].zbase = 0;
     }
     if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
@@ -316,12 +321,74 @@ struct Ctx {
       hlin[i].ptab = ptab(r.model[i], r.fi[i]);
       hlin[i].w = omega(r.model[i], r.fi[i]);
       hlin[i].model = model_index(r.model[i]);
+      hlin[i].zbase = 0;
     }
     RCCHK(put(d_rows, hlin, s, "lin rows"));
     HIPCHK(launch_lin(N, d_rows.as<LinRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, dir, adjoint != 0,
                       exact != 0, m0->d_tw, s),
            "lin kernel");
     return sync();
+  }
+
+  // In-place sweep of a depth segment: row i of W holds depths zbase..
+  // (wstride samples apart from row to row); the sweep runs nsteps steps
+  // from depth first, starting from init + i * init_stride (NULL: the
+  // buffer's own row first), and adds the source rows already in W.
+  int sweep_seg(const Rows& r, int up, int adjoint, float2* W, size_t wstride, int zbase, int first, int nsteps,
+                const float2* init, size_t init_stride) {
+    const int n = (int)r.shot.size();
+    hsw.resize(n);
+    for (int i = 0; i < n; ++i) {
+      hsw[i].in = W + (size_t)i * wstride;
+      hsw[i].out = W + (size_t)i * wstride;
+      hsw[i].ptab = ptab(r.model[i], r.fi[i]);
+      hsw[i].w = omega(r.model[i], r.fi[i]);
+      hsw[i].model = model_index(r.model[i]);
+      hsw[i].recursive = 1;
+      hsw[i].first = first;
+      hsw[i].nsteps = nsteps;
+      hsw[i].init = init ? init + (size_t)i * init_stride : nullptr;
+      hsw[i].zbase = zbase;
+    }
+    if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
+    for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
+    RCCHK(put(d_rows, hsw, s, "sweep rows"));
+    const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
+    HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, step,
+                         adjoint != 0, m0->d_tw, s),
+           "oneway kernel");
+    return WEMFWI_OK;
+  }
+
+  // Adjoint LinDown / LinUp over depths [dlo, dlo + dcount) of segment
+  // buffers (bg, m and out share the row layout of sweep_seg).
+  int lin_seg(const Rows& r, int dir, int exact, const float2* bg, const float2* m, float2* out, size_t wstride,
+              int zbase, int dlo, int dcount) {
+    if (dcount <= 0) return WEMFWI_OK;
+    const int n = (int)r.shot.size();
+    hlin.resize(n);
+    for (int i = 0; i < n; ++i) {
+      hlin[i].bg = bg + (size_t)i * wstride;
+      hlin[i].m = m + (size_t)i * wstride;
+      hlin[i].out = out + (size_t)i * wstride;
+      hlin[i].ptab = ptab(r.model[i], r.fi[i]);
+      hlin[i].w = omega(r.model[i], r.fi[i]);
+      hlin[i].model = model_index(r.model[i]);
+      hlin[i].zbase = zbase;
+    }
+    RCCHK(put(d_rows, hlin, s, "lin rows"));
+    HIPCHK(launch_lin(N, d_rows.as<LinRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, dir, true, exact != 0,
+                      m0->d_tw, s, dlo, dcount),
+           "lin kernel");
+    return WEMFWI_OK;
+  }
+
+  // n rows of nx samples: dst + i * dpitch <- src + i * spitch
+  int copy_rows(float2* dst, size_t dpitch, const float2* src, size_t spitch, int n) {
+    HIPCHK(hipMemcpy2DAsync(dst, dpitch * sizeof(float2), src, spitch * sizeof(float2), (size_t)nx * sizeof(float2),
+                            (size_t)n, hipMemcpyDeviceToDevice, s),
+           "copy rows");
+    return WEMFWI_OK;
   }
 
   // rows: out_i (= or +=) op(A_i) * B_i over len samples, one launch
@@ -332,6 +399,15 @@ struct Ctx {
     HIPCHK(launch_mul_rows(dmul.as<MulRow>(), (int)hmul.size(), (long long)len, conjA, acc, s), "mul rows");
     return WEMFWI_OK;
   }
+
+  // Rows of the per-row fields an injection writes: every field holds the
+  // depths zbase.. with a stride of wstride samples between rows; sources
+  // outside [zlo, zhi] are skipped (the whole field by default).
+  struct Win {
+    long long wstride;
+    int zbase, zlo, zhi;
+  };
+  Win whole() const { return Win{(long long)nzx, 0, 0, nz - 1}; }
 
   // consecutive rows of one shot: [i0, i1)
   template <typename F>
@@ -347,8 +423,9 @@ struct Ctx {
   }
 
   // sources of the rows' shots, value = wavelet of the row's frequency
-  int inject_sources(const Rows& r, float2* W, int ng, int offset) {
-    if (sv->per_shot()) return inject_shot_sources(r, W, ng, offset);
+  int inject_sources(const Rows& r, float2* W, int ng, int offset) { return inject_sources(r, W, ng, offset, whole()); }
+  int inject_sources(const Rows& r, float2* W, int ng, int offset, const Win& wn) {
+    if (sv->per_shot()) return inject_shot_sources(r, W, ng, offset, wn);
     const int n = (int)r.shot.size();
     hiz.resize(n);
     hix.resize(n);
@@ -365,15 +442,15 @@ struct Ctx {
     RCCHK(put(dix, hix, s, "source ix"));
     RCCHK(put(dval, hval, s, "source values"));
     RCCHK(put(dgw, hgw, s, "source spread"));
-    HIPCHK(launch_inject_point(W, (long long)nzx, nx, diz.as<int>(), dix.as<int>(), dval.as<float2>(),
-                               dgw.as<float>(), ng, offset, n, s),
+    HIPCHK(launch_inject_point(W, wn.wstride, nx, diz.as<int>(), dix.as<int>(), dval.as<float2>(),
+                               dgw.as<float>(), ng, offset, n, s, wn.zbase, wn.zlo, wn.zhi),
            "inject sources");
     return WEMFWI_OK;
   }
 
   // per-shot source lists: gather by cell over each shot's plan, every
   // device of a row carrying the row's wavelet value
-  int inject_shot_sources(const Rows& r, float2* W, int ng, int offset) {
+  int inject_shot_sources(const Rows& r, float2* W, int ng, int offset, const Win& wn) {
     const int n = (int)r.shot.size();
     hval.resize(n);
     hvb.resize(n);
@@ -387,8 +464,9 @@ struct Ctx {
     return shot_runs(r, [&](int i0, int i1) {
       const int sh = r.shot[i0];
       RCCHK(build_plan(sv->ssrc_plan[sh], sv->ssrc_iz[sh], sv->ssrc_ix[sh], ng, offset, nx, nz));
-      HIPCHK(launch_inject_csr(W + (size_t)i0 * nzx, (long long)nzx, sv->ssrc_plan[sh].inj, dval.as<float2>() + i0,
-                               dvb.as<long long>(), 0, i1 - i0, s),
+      HIPCHK(launch_inject_csr(W + (size_t)i0 * wn.wstride, wn.wstride, sv->ssrc_plan[sh].inj,
+                               dval.as<float2>() + i0, dvb.as<long long>(), 0, i1 - i0, s,
+                               (long long)wn.zbase * nx, (long long)wn.zlo * nx, ((long long)wn.zhi + 1) * nx),
              "inject sources");
       return WEMFWI_OK;
     });
@@ -404,21 +482,25 @@ struct Ctx {
 
   // W_i += R^H d (receivers of the rows' traces)
   int inject_receivers(const Rows& r, float2* W, const float2* data, int ng, int offset) {
+    return inject_receivers(r, W, data, ng, offset, whole());
+  }
+  int inject_receivers(const Rows& r, float2* W, const float2* data, int ng, int offset, const Win& wn) {
+    const long long cb = (long long)wn.zbase * nx, clo = (long long)wn.zlo * nx, chi = ((long long)wn.zhi + 1) * nx;
     if (sv->per_shot()) {
       RCCHK(vbase(r));
       return shot_runs(r, [&](int i0, int i1) {
         const int sh = r.shot[i0];
         RCCHK(build_plan(sv->srec_plan[sh], sv->srec_iz[sh], sv->srec_ix[sh], ng, offset, nx, nz));
-        HIPCHK(launch_inject_csr(W + (size_t)i0 * nzx, (long long)nzx, sv->srec_plan[sh].inj, data,
-                                 dvb.as<long long>() + i0, sv->d.nt, i1 - i0, s),
+        HIPCHK(launch_inject_csr(W + (size_t)i0 * wn.wstride, wn.wstride, sv->srec_plan[sh].inj, data,
+                                 dvb.as<long long>() + i0, sv->d.nt, i1 - i0, s, cb, clo, chi),
                "inject receivers");
         return WEMFWI_OK;
       });
     }
     RCCHK(build_plan(sv->rplan, sv->rec_iz, sv->rec_ix, ng, offset, nx, nz));
     RCCHK(vbase(r));
-    HIPCHK(launch_inject_csr(W, (long long)nzx, sv->rplan.inj, data, dvb.as<long long>(), sv->d.nt,
-                             (int)r.shot.size(), s),
+    HIPCHK(launch_inject_csr(W, wn.wstride, sv->rplan.inj, data, dvb.as<long long>(), sv->d.nt,
+                             (int)r.shot.size(), s, cb, clo, chi),
            "inject receivers");
     return WEMFWI_OK;
   }
@@ -599,33 +681,34 @@ static int born_forward(const std::vector<Model*>& fm, Per per, Survey* sv, cons
   return c.sync();
 }
 
-// BornFull::full_adj (born_prop.cpp:178-232) per row, summed over rows in a
-// fixed order in double precision:
-//   D = Down(inj_src),  R = Up^H(R^H d)
-//   dr    += conj(D) * R                              (Born_refl::adjoint)
-//   Q     = Down^H(R^H d + conj(r) * R)               (both LinDown^H terms)
-//   dS    += LinDown^H(Q; D)
-//   U     = Up(r * D),  dS += LinUp^H(R; U)
-//   (dS, dR) += dReflect^H(dr)
-// ext_rowlast: reproduce ext_born_prop.cpp's serial shot loop, in which each
-// LinDown::adjoint zeroes the last depth row (LinOneWay.cpp:64-65) so only
-// the last shot's LinUp contribution remains there.
-static int born_adjoint(const std::vector<Model*>& fm, Per per, Survey* sv, float2* gS, float2* gR, size_t mstride,
-                        const float2* data, int add, int exact, bool ext_rowlast, hipStream_t s) {
-  RCCHK(check_survey(sv, fm[0]));
-  std::vector<Model*> uniq;
-  for (Model* m : fm)
-    if (std::find(uniq.begin(), uniq.end(), m) == uniq.end()) uniq.push_back(m);
-  Ctx c;
-  RCCHK(c.init(uniq, sv, s));
+// Reduction of a chunk's per-row contributions into the FP64 accumulators,
+// grouped by model slice (consecutive rows of one slice) or over all rows;
+// rows are added to the accumulator one at a time in row order, so the sum
+// does not depend on how the rows were cut into chunks or depth segments.
+//   depths [z0, z1) of row i at in + i * stride + (z0 - zbase) * nx
+static int reduce_chunk(Ctx& c, const Rows& r, bool sliced, double2* acc, const float2* in, size_t stride, int zbase,
+                        int z0, int z1) {
+  const int n = (int)r.shot.size();
+  int i0 = 0;
+  while (i0 < n) {
+    int i1 = i0 + 1;
+    if (sliced)
+      while (i1 < n && r.slice[i1] == r.slice[i0]) ++i1;
+    else
+      i1 = n;
+    const size_t slot = sliced ? (size_t)r.slice[i0] : 0;
+    HIPCHK(launch_reduce_rows(acc + slot * c.nzx + (size_t)z0 * c.nx, in + (size_t)i0 * stride + (size_t)(z0 - zbase) * c.nx,
+                              (long long)stride, (long long)(z1 - z0) * c.nx, i1 - i0, true, c.s),
+           "reduce");
+    i0 = i1;
+  }
+  return WEMFWI_OK;
+}
+
+static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, double2* accS, double2* accDR,
+                            bool sliced, const float2* data, int exact, bool ext_rowlast) {
+  Survey* sv = c.sv;
   const int nw = sv->d.nw, ns = sv->d.ns;
-  const size_t nslice = mstride ? fm.size() : 1;
-  // FP64 accumulators per model slice: dr, dS
-  DevBuf accDR, accS;
-  RCCHK(accDR.alloc(nslice * c.nzx * sizeof(double2), "dr accumulator"));
-  RCCHK(accS.alloc(nslice * c.nzx * sizeof(double2), "dS accumulator"));
-  RCCHK(c.zero(accDR.p, accDR.n));
-  RCCHK(c.zero(accS.p, accS.n));
   // rows of one model slice contiguous: frequency-major for per-frequency
   // models, shot-major for per-shot models
   auto chunks = make_chunks(ns, nw, fm, per, 5 * c.nzx * sizeof(float2), per == Per::Freq);
@@ -675,29 +758,20 @@ static int born_adjoint(const std::vector<Model*>& fm, Per per, Survey* sv, floa
     if (ext_rowlast) {
       for (int i = 0; i < n; ++i)
         if (r.shot[i] != ns - 1)
-          HIPCHK(hipMemsetAsync(cs + (size_t)i * c.nzx + (size_t)(c.nz - 1) * c.nx, 0, c.nx * sizeof(float2), s),
+          HIPCHK(hipMemsetAsync(cs + (size_t)i * c.nzx + (size_t)(c.nz - 1) * c.nx, 0, c.nx * sizeof(float2), c.s),
                  "memset");
     }
-    // deterministic reduction: consecutive rows of one model slice (or all rows)
-    int i0 = 0;
-    while (i0 < n) {
-      int i1 = i0 + 1;
-      if (mstride)
-        while (i1 < n && r.slice[i1] == r.slice[i0]) ++i1;
-      else
-        i1 = n;
-      const size_t slot = mstride ? (size_t)r.slice[i0] : 0;
-      HIPCHK(launch_reduce_rows(accS.as<double2>() + slot * c.nzx, cs + (size_t)i0 * c.nzx, (long long)c.nzx,
-                                (long long)c.nzx, i1 - i0, true, s),
-             "reduce");
-      HIPCHK(launch_reduce_rows(accDR.as<double2>() + slot * c.nzx, cr + (size_t)i0 * c.nzx, (long long)c.nzx,
-                                (long long)c.nzx, i1 - i0, true, s),
-             "reduce");
-      i0 = i1;
-    }
+    RCCHK(reduce_chunk(c, r, sliced, accS, cs, c.nzx, 0, 0, c.nz));
+    RCCHK(reduce_chunk(c, r, sliced, accDR, cr, c.nzx, 0, 0, c.nz));
     RCCHK(c.sync());
   }
-  // dReflect^H of the summed dr, then write the model outputs
+  return WEMFWI_OK;
+}
+
+// dReflect^H of the summed dr, then write the model outputs
+static int born_adjoint_out(Ctx& c, const std::vector<Model*>& fm, float2* gS, float2* gR, size_t mstride,
+                            size_t nslice, const double2* accS, const double2* accDR, int add) {
+  hipStream_t s = c.s;
   DevBuf fS, fR, fdr;
   RCCHK(fS.alloc(c.nzx * sizeof(float2), "tmp"));
   RCCHK(fR.alloc(c.nzx * sizeof(float2), "tmp"));
@@ -706,8 +780,8 @@ static int born_adjoint(const std::vector<Model*>& fm, Per per, Survey* sv, floa
     Model* m = fm[k];
     float2* oS = gS + k * mstride;
     float2* oR = gR + k * mstride;
-    HIPCHK(launch_d2f(fS.as<float2>(), accS.as<double2>() + k * c.nzx, (long long)c.nzx, false, s), "d2f");
-    HIPCHK(launch_d2f(fdr.as<float2>(), accDR.as<double2>() + k * c.nzx, (long long)c.nzx, false, s), "d2f");
+    HIPCHK(launch_d2f(fS.as<float2>(), accS + k * c.nzx, (long long)c.nzx, false, s), "d2f");
+    HIPCHK(launch_d2f(fdr.as<float2>(), accDR + k * c.nzx, (long long)c.nzx, false, s), "d2f");
     RCCHK(c.zero(fR.p, c.nzx * sizeof(float2)));
     HIPCHK(launch_dreflect_adj(fS.as<float2>(), fR.as<float2>(), m->d_drefl, fdr.as<float2>(), c.nz, c.nx, s),
            "dreflect adjoint");
@@ -721,6 +795,154 @@ static int born_adjoint(const std::vector<Model*>& fm, Per per, Survey* sv, floa
   }
   return c.sync();
 }
+
+// Checkpointed form of born_adjoint_all: the depth axis is cut into segments
+// [a_j, b_j], a_j = j k, b_j = min(a_j + k, nz - 1), each owning the output
+// depths [a_j, b_j) (the last one also nz - 1).  A forward pass keeps, per
+// row, D at a_j and R at a_j - 1 of every segment; the backward pass visits
+// the segments bottom-up, recomputes D and R on the segment from those rows,
+// and carries the bottom-up recursions Q = Down^H(.) and U = Up(r D) from one
+// segment to the next through their rows at a_j.  Every output sample gets
+// the same operations in the same order as with whole fields stored (the
+// sweep state outside [0, nx) is zero after a forward step and is discarded
+// by the reference restriction at the start of an adjoint step, so a state
+// reloaded from a kept row continues bit for bit), hence the same gradient.
+// Device memory per row: (5 (k + 2) + 2 nseg + 2) nx samples instead of
+// 5 nz nx.
+static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, double2* accS, double2* accDR,
+                             bool sliced, const float2* data, int exact, bool ext_rowlast, int k) {
+  Survey* sv = c.sv;
+  const int nw = sv->d.nw, ns = sv->d.ns, nz = c.nz, nx = c.nx;
+  const int nseg = (nz - 1 + k - 1) / k;
+  const int SR = std::min(k + 2, nz);       // rows of a segment buffer
+  const size_t W = (size_t)SR * nx;        // samples per row of a segment buffer
+  const size_t per_row = (5 * W + (2 * (size_t)nseg + 2) * nx) * sizeof(float2);
+  auto chunks = make_chunks(ns, nw, fm, per, per_row, per == Per::Freq);
+  const size_t nmax = chunks[0].shot.size();
+  DevBuf B, Rb, Qb, CS, CR, KD, KR, CQ, CU;
+  RCCHK(B.alloc(nmax * W * sizeof(float2), "background segment"));
+  RCCHK(Rb.alloc(nmax * W * sizeof(float2), "receiver segment"));
+  RCCHK(Qb.alloc(nmax * W * sizeof(float2), "receiver segment"));
+  RCCHK(CS.alloc(nmax * W * sizeof(float2), "dS contributions"));
+  RCCHK(CR.alloc(nmax * W * sizeof(float2), "dr contributions"));
+  RCCHK(KD.alloc(nmax * nseg * nx * sizeof(float2), "background checkpoints"));
+  RCCHK(KR.alloc(nmax * nseg * nx * sizeof(float2), "receiver checkpoints"));
+  RCCHK(CQ.alloc(nmax * nx * sizeof(float2), "carried row"));
+  RCCHK(CU.alloc(nmax * nx * sizeof(float2), "carried row"));
+  float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>(),
+         *cr = CR.as<float2>(), *kd = KD.as<float2>(), *kr = KR.as<float2>(), *cq = CQ.as<float2>(),
+         *cu = CU.as<float2>();
+  const size_t kpitch = (size_t)nseg * nx;  // checkpoint rows of one chunk row
+  for (const Rows& r : chunks) {
+    const int n = (int)r.shot.size();
+    const size_t bytes = n * W * sizeof(float2);
+    // D on [a, b] and R on [ra, rb] of segment j, from the kept rows
+    auto fields = [&](int j) -> int {
+      const int a = j * k, b = std::min(a + k, nz - 1), zb = std::max(a - 1, 0);
+      const bool last = j == nseg - 1;
+      RCCHK(c.zero(bg, bytes));
+      RCCHK(c.inject_sources(r, bg, sv->d.ngs, sv->d.offset, Ctx::Win{(long long)W, zb, j ? a + 1 : 0, b}));
+      RCCHK(c.sweep_seg(r, 0, 0, bg, W, zb, a, b - a, j ? kd + (size_t)j * nx : nullptr, kpitch));
+      const int ra = j ? a - 1 : 0, rb = last ? b : b - 1;
+      RCCHK(c.zero(R, bytes));
+      RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset, Ctx::Win{(long long)W, zb, j ? a : 0, rb}));
+      RCCHK(c.sweep_seg(r, 1, 1, R, W, zb, ra, rb - ra, j ? kr + (size_t)j * nx : nullptr, kpitch));
+      return WEMFWI_OK;
+    };
+    // forward pass: the kept rows D(a_j) and R(a_j - 1)
+    for (int j = 0; j + 1 < nseg; ++j) {
+      RCCHK(fields(j));
+      const int a = j * k, b = a + k, zb = std::max(a - 1, 0);
+      RCCHK(c.copy_rows(kd + (size_t)(j + 1) * nx, kpitch, bg + (size_t)(b - zb) * nx, W, n));
+      RCCHK(c.copy_rows(kr + (size_t)(j + 1) * nx, kpitch, R + (size_t)(b - 1 - zb) * nx, W, n));
+    }
+    // backward pass
+    for (int j = nseg - 1; j >= 0; --j) {
+      const int a = j * k, b = std::min(a + k, nz - 1), zb = std::max(a - 1, 0);
+      const bool last = j == nseg - 1;
+      const int e = last ? nz : b;  // owned depths [a, e)
+      const size_t oa = (size_t)(a - zb) * nx;
+      RCCHK(fields(j));
+      const int qe = last ? b : b - 1;  // Q source rows [a, qe]
+      RCCHK(c.zero(Q, bytes));
+      RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset, Ctx::Win{(long long)W, zb, a, qe}));
+      {
+        std::vector<MulRow> ic(n), rq(n);
+        for (int i = 0; i < n; ++i) {
+          const size_t o = (size_t)i * W + oa;
+          ic[i] = MulRow{cr + o, bg + o, R + o};
+          rq[i] = MulRow{Q + o, r.model[i]->d_refl + (size_t)a * nx, R + o};
+        }
+        RCCHK(c.mul_rows(ic, (size_t)(e - a) * nx, true, false));   // conj(D) * R
+        RCCHK(c.mul_rows(rq, (size_t)(qe - a + 1) * nx, true, true));  // Q += conj(r) R
+      }
+      RCCHK(c.sweep_seg(r, 0, 1, Q, W, zb, b, b - a, last ? nullptr : cq, nx));
+      if (j > 0) RCCHK(c.copy_rows(cq, nx, Q + oa, W, n));
+      RCCHK(c.zero(cs, bytes));
+      RCCHK(c.lin_seg(r, +1, exact, bg, Q, cs, W, zb, a, b - a));
+      {
+        std::vector<MulRow> rbg(n);
+        for (int i = 0; i < n; ++i) {
+          const size_t o = (size_t)i * W + oa;
+          rbg[i] = MulRow{bg + o, r.model[i]->d_refl + (size_t)a * nx, bg + o};
+        }
+        RCCHK(c.mul_rows(rbg, (size_t)(b - a + 1) * nx, false, false));  // r * D
+      }
+      RCCHK(c.sweep_seg(r, 1, 0, bg, W, zb, b, b - a, last ? nullptr : cu, nx));
+      if (j > 0) RCCHK(c.copy_rows(cu, nx, bg + oa, W, n));
+      const int ulo = std::max(a, 1);
+      RCCHK(c.lin_seg(r, -1, exact, bg, R, cs, W, zb, ulo, e - ulo));
+      if (ext_rowlast && last) {
+        for (int i = 0; i < n; ++i)
+          if (r.shot[i] != ns - 1)
+            HIPCHK(hipMemsetAsync(cs + (size_t)i * W + (size_t)(nz - 1 - zb) * nx, 0, nx * sizeof(float2), c.s),
+                   "memset");
+      }
+      RCCHK(reduce_chunk(c, r, sliced, accS, cs, W, zb, a, e));
+      RCCHK(reduce_chunk(c, r, sliced, accDR, cr, W, zb, a, e));
+    }
+    RCCHK(c.sync());
+  }
+  return WEMFWI_OK;
+}
+
+// BornFull::full_adj (born_prop.cpp:178-232) per row, summed over rows in a
+// fixed order in double precision:
+//   D = Down(inj_src),  R = Up^H(R^H d)
+//   dr    += conj(D) * R                              (Born_refl::adjoint)
+//   Q     = Down^H(R^H d + conj(r) * R)               (both LinDown^H terms)
+//   dS    += LinDown^H(Q; D)
+//   U     = Up(r * D),  dS += LinUp^H(R; U)
+//   (dS, dR) += dReflect^H(dr)
+// ext_rowlast: reproduce ext_born_prop.cpp's serial shot loop, in which each
+// LinDown::adjoint zeroes the last depth row (LinOneWay.cpp:64-65) so only
+// the last shot's LinUp contribution remains there.
+static int born_adjoint(const std::vector<Model*>& fm, Per per, Survey* sv, float2* gS, float2* gR, size_t mstride,
+                        const float2* data, int add, int exact, bool ext_rowlast, hipStream_t s) {
+  RCCHK(check_survey(sv, fm[0]));
+  std::vector<Model*> uniq;
+  for (Model* m : fm)
+    if (std::find(uniq.begin(), uniq.end(), m) == uniq.end()) uniq.push_back(m);
+  Ctx c;
+  RCCHK(c.init(uniq, sv, s));
+  const size_t nslice = mstride ? fm.size() : 1;
+  // FP64 accumulators per model slice: dr, dS
+  DevBuf accDR, accS;
+  RCCHK(accDR.alloc(nslice * c.nzx * sizeof(double2), "dr accumulator"));
+  RCCHK(accS.alloc(nslice * c.nzx * sizeof(double2), "dS accumulator"));
+  RCCHK(c.zero(accDR.p, accDR.n));
+  RCCHK(c.zero(accS.p, accS.n));
+  const int ck = sv->ckpt;
+  if (ck > 0 && ck < c.nz - 1) {
+    RCCHK(born_adjoint_ckpt(c, fm, per, accS.as<double2>(), accDR.as<double2>(), mstride != 0, data, exact,
+                            ext_rowlast, ck));
+  } else {
+    RCCHK(born_adjoint_all(c, fm, per, accS.as<double2>(), accDR.as<double2>(), mstride != 0, data, exact,
+                           ext_rowlast));
+  }
+  return born_adjoint_out(c, fm, gS, gR, mstride, nslice, accS.as<double2>(), accDR.as<double2>(), add);
+}
+
 
 // ------------------------------------------------------------------ BornRefl
 
@@ -872,6 +1094,12 @@ int wemfwi_survey_create(const wemfwi_survey_desc* d, wemfwi_survey** out) {
 
 int wemfwi_survey_destroy(wemfwi_survey* s) {
   delete s;
+  return WEMFWI_OK;
+}
+
+int wemfwi_survey_set_checkpoint(wemfwi_survey* survey, int every) {
+  if (!survey || every < 0) return fail(WEMFWI_ERR_ARG, "survey_set_checkpoint: invalid argument");
+  survey->s.ckpt = every;
   return WEMFWI_OK;
 }
 

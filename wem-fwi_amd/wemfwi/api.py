@@ -465,19 +465,28 @@ class WEM(Propagator):
         raise NotImplementedError("WEM is non-linear: use Born for the adjoint")
 
 
+def _with_checkpoint(sv: ops.Survey, every: int) -> ops.Survey:
+    if sv.checkpoint != every:
+        sv.set_checkpoint(every)
+    return sv
+
+
 class Born(Propagator):
     """Linearised modeling around a background (BornFull.cpp:13-117), model
     [dS, drho] on the unpadded grid.  mode='real' zeroes imaginary parts of the
     model before forward and after adjoint (WEM.py:52-67).  adjoint_mode
     'compat' keeps Scatter::adjoint's operator order (Scatter.cpp:52-66);
-    'exact' is the exact adjoint."""
+    'exact' is the exact adjoint.  checkpoint=k > 0 makes the adjoint keep
+    wavefield rows every k depth steps and recompute the rest (bitwise the
+    same result, far less device memory per row)."""
 
-    def __init__(self, model, data, wave, par, mode=None, adjoint_mode: str = "compat"):
+    def __init__(self, model, data, wave, par, mode=None, adjoint_mode: str = "compat", checkpoint: int = 0):
         super().__init__(model, data, wave, par)
         self.mode = mode
         if adjoint_mode not in ("compat", "exact"):
             raise ValueError("adjoint_mode must be 'compat' or 'exact'")
         self.adjoint_mode = adjoint_mode
+        self.checkpoint = int(checkpoint)
         self._bg = None
         self.set_background(model)
 
@@ -515,7 +524,7 @@ class Born(Propagator):
     def _mapped_adjoint(self, add, vecs, data):
         acq = self.acq
         f = acq.to_freq(data)
-        sv = acq.survey(f.shape[-1], mapped=True)
+        sv = _with_checkpoint(acq.survey(f.shape[-1], mapped=True), self.checkpoint)
         ns, nz, nxp = acq.cmap.ns, acq.nz, acq.nx + 2 * acq.pad
         gS = torch.zeros((ns, nz, nxp), dtype=torch.complex64, device=f.device)
         gR = torch.zeros_like(gS)
@@ -557,7 +566,7 @@ class Born(Propagator):
                     v.arr.imag.zero_()
             return
         f = self.acq.to_freq(data)
-        sv = self.acq.survey(f.shape[-1])
+        sv = _with_checkpoint(self.acq.survey(f.shape[-1]), self.checkpoint)
         p = self.acq.pad
         nz, nxp = self._bg.nz, self._bg.nx
         gS = torch.zeros((nz, nxp), dtype=torch.complex64, device=f.device)
@@ -632,11 +641,12 @@ class extWEM(Propagator):  # noqa: N801 - reference name
 
 class extBorn(Propagator):  # noqa: N801 - reference name
     """BornFull with one background and one perturbation slice per frequency
-    (extBornFull.cpp:68-121)."""
+    (extBornFull.cpp:68-121).  checkpoint as for Born."""
 
-    def __init__(self, model, data, wave, par, adjoint_mode: str = "compat"):
+    def __init__(self, model, data, wave, par, adjoint_mode: str = "compat", checkpoint: int = 0):
         super().__init__(model, data, wave, par)
         self.adjoint_mode = adjoint_mode
+        self.checkpoint = int(checkpoint)
         self._bg: List[Model] = []
         self.set_background(model)
 
@@ -664,7 +674,7 @@ class extBorn(Propagator):  # noqa: N801 - reference name
     def adjoint(self, add, model, data):
         vecs = _model_vecs(model)
         f = self.acq.to_freq(data)
-        sv = self.acq.survey(f.shape[-1])
+        sv = _with_checkpoint(self.acq.survey(f.shape[-1]), self.checkpoint)
         p = self.acq.pad
         nw = len(self._bg)
         nz, nxp = self._bg[0].nz, self._bg[0].nx

@@ -320,13 +320,14 @@ class ExtFWIObjective:
     HyperbolicPenalty width (geom['tau']), epsilon / alpha / beta / der_mode the
     regularisation (efwi.py:194-196,204-206); vmin / vmax add SoftMinMax on the
     slowness with tau = (0.01, 0.001) (efwi.py:168-171); mask multiplies the
-    Born input (efwi.py:145-149)."""
+    Born input (efwi.py:145-149); checkpoint as for extBorn."""
 
     def __init__(self, obs: Vector, wave: Vector, par, hyper: Hypercube, epsilon: float, alpha: float = 0.0,
                  beta: float = 1.0, der_mode: str = "reflect", tau: float = 0.001, vmin: Optional[float] = None,
                  vmax: Optional[float] = None, mask: Optional[Sequence[torch.Tensor]] = None,
-                 adjoint_mode: str = "exact"):
+                 adjoint_mode: str = "exact", checkpoint: int = 0):
         self.obs = obs
+        self.checkpoint = int(checkpoint)
         self.wave = wave
         self.par = dict(par)
         self.hyper = hyper
@@ -373,7 +374,7 @@ class ExtFWIObjective:
 
     def value_and_gradient(self, m: SuperVector):
         p, res = self.residual(m)
-        born = extBorn(p, self.obs, self.wave, self.par, adjoint_mode=self.adjoint_mode)
+        born = extBorn(p, self.obs, self.wave, self.par, adjoint_mode=self.adjoint_mode, checkpoint=self.checkpoint)
         g = m.clone().zero()
         born.adjoint(False, g, res)
         if self.mask is not None:

@@ -12,9 +12,11 @@ mode='real' zeroes the imaginary part of the gradient (WEM.py:59-67), and the
 exact adjoint of the scatterer is used by default so that the gradient is the
 adjoint of the linearised operator.
 
-Background wavefields are kept on the device for all rows of a call rather
-than checkpointed: HBM holds the Down/Up fields of the whole survey for the
-model sizes of this path, so recomputation would only cost time.
+checkpoint=k keeps the background and receiver wavefields only every k
+depth steps during the gradient and recomputes them segment by segment: the
+gradient is bitwise the same, the device memory per (shot, frequency) row
+drops from 5 nz nx samples to about (5 k + 2 nz / k) nx, so larger batches
+of rows fit one launch.
 
 fwi_problem / fit_preconditioned / shard_par assemble the driver of
 scripts/fwi.py:46-131 around it (solvers in wemfwi.solver, interpolation in
@@ -35,8 +37,9 @@ class WEMMisfit:
     [nz][nx] complex64 CUDA tensors to get a differentiable scalar."""
 
     def __init__(self, obs: Vector, wave: Vector, par: Mapping, hyper: Hypercube, mode: Optional[str] = "real",
-                 adjoint_mode: str = "exact"):
+                 adjoint_mode: str = "exact", checkpoint: int = 0):
         self.obs = obs
+        self.checkpoint = int(checkpoint)
         self.wave = wave
         self.par = dict(par)
         self.hyper = hyper
@@ -61,7 +64,8 @@ class WEMMisfit:
     def gradient(self, slow: torch.Tensor, den: torch.Tensor, res: Vector):
         m = self._model(slow, den)
         if self._born is None:
-            self._born = Born(m, self.obs, self.wave, self.par, mode=None, adjoint_mode=self.adjoint_mode)
+            self._born = Born(m, self.obs, self.wave, self.par, mode=None, adjoint_mode=self.adjoint_mode,
+                              checkpoint=self.checkpoint)
         else:
             self._born.set_background(m)
         g = m.clone().zero()
