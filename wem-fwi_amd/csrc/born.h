@@ -50,8 +50,15 @@ struct MulRow {
 hipError_t launch_mul_rows(const MulRow* rows, int nrows, long long n, bool conjA, bool acc, hipStream_t s);
 hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, const float2* B, long long sb,
                       long long n, int nrows, bool conjA, bool acc, hipStream_t s);
-hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, bool acc,
-                              hipStream_t s);
+// Rows of a reduction are summed in runs of kRunRows rows of their global
+// order (runs start at multiples of kRunRows): each run in row order, then
+// the run sums in run order.  Chunks of rows that start at multiples of
+// kRunRows therefore give the same sums however the rows are chunked.
+constexpr int kRunRows = 64;
+// out[i] (+)= sum_r in[r * stride + i] for i < n over nrows rows, the first of
+// which sits at position phase of its run
+hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, int phase,
+                              bool acc, hipStream_t s);
 hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s);
 hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, const long long* src, const float2* in,
                              bool add, hipStream_t s);

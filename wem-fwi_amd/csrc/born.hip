@@ -141,22 +141,54 @@ __global__ void mul_rows_kernel(const MulRow* __restrict__ rows, int nrows, long
 }
 
 // out[i] (+)= sum_r in[r stride + i] in double precision, rows in order
-__global__ void reduce_rows_kernel(double2* __restrict__ out, const float2* __restrict__ in, long long stride,
-                                   long long n, int nrows, int acc) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    double sx = 0.0, sy = 0.0;
-    if (acc) {
-      sx = out[i].x;
-      sy = out[i].y;
-    }
-    for (int r = 0; r < nrows; ++r) {
-      const float2 v = ldg(in, (size_t)(r * stride + i));
-      sx += v.x;
-      sy += v.y;
-    }
-    out[i] = make_double2(sx, sy);
+// out[i] (+)= sum over rows of in[r * stride + i], summed in runs: the rows
+// are cut at multiples of kRunRows of their global order (phase = position
+// of row 0 within its run), each run is summed in double in row order and
+// the run sums are added to out in run order.  One wave per run and 64
+// consecutive samples (one per lane, coalesced 512-byte row loads); the
+// waves of a workgroup take kRedWaves runs at a time and wave 0 adds their
+// sums in order from LDS.
+constexpr int kRedWaves = 8;
+__global__ void __launch_bounds__(64 * kRedWaves)
+    reduce_runs_kernel(double2* __restrict__ out, const float2* __restrict__ in, long long stride, long long n,
+                       int nrows, int phase, int acc) {
+  __shared__ double2 part[kRedWaves][64];
+  const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long i = (long long)blockIdx.x * 64 + t;
+  const bool live = i < n;
+  const int nruns = (phase + nrows + kRunRows - 1) / kRunRows;
+  double tx = 0.0, ty = 0.0;
+  if (acc && live && wv == 0) {
+    const double2 o = out[i];
+    tx = o.x;
+    ty = o.y;
   }
+  for (int g = 0; g < nruns; g += kRedWaves) {
+    const int m = g + wv;
+    if (m < nruns) {
+      const int r0 = max(0, m * kRunRows - phase), r1 = min(nrows, (m + 1) * kRunRows - phase);
+      double sx = 0.0, sy = 0.0;
+      if (live) {
+#pragma unroll 4
+        for (int r = r0; r < r1; ++r) {
+          const float2 v = ldg(in, (size_t)((long long)r * stride + i));
+          sx += v.x;
+          sy += v.y;
+        }
+      }
+      part[wv][t] = make_double2(sx, sy);
+    }
+    __syncthreads();
+    if (wv == 0) {
+      const int mend = min(kRedWaves, nruns - g);
+      for (int w = 0; w < mend; ++w) {
+        tx += part[w][t].x;
+        ty += part[w][t].y;
+      }
+    }
+    __syncthreads();
+  }
+  if (wv == 0 && live) out[i] = make_double2(tx, ty);
 }
 
 // out += in
@@ -431,10 +463,14 @@ hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, 
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, bool acc,
-                              hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_rows_kernel, grid1d(n), dim3(256), 0, s, out, in, stride, n, nrows, acc ? 1 : 0);
+hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, int phase,
+                              bool acc, hipStream_t s) {
+  if (n <= 0 || nrows <= 0) return hipSuccess;
+  if (phase < 0 || phase >= kRunRows) return hipErrorInvalidValue;
+  const long long tiles = (n + 63) / 64;
+  if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(reduce_runs_kernel, dim3((unsigned)tiles), dim3(64 * kRedWaves), 0, s, out, in, stride, n,
+                     nrows, phase, acc ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <complex>
 #include <cstring>
 #include <numeric>
@@ -201,16 +202,24 @@ enum class Per { Freq, Shot };
 
 // A chunk of (shot, frequency) rows with their per-row models.
 struct Rows {
+  size_t first = 0;                    // position of row 0 in the call's row order
   std::vector<int> shot, fi;           // shot and frequency index of each row
   std::vector<int> slice;              // index of the row's model / perturbation slice
   std::vector<Model*> model;           // background model of each row
 };
 
+// Rows per chunk: half of the free device memory, or at most
+// WEMFWI_CHUNK_ROWS rows when that variable is set (to bound memory, and to
+// exercise chunked calls on small problems).
 static size_t chunk_rows(size_t per_row_bytes, size_t nrows) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr == 0) fr = (size_t)8 << 30;
   const size_t budget = fr / 2;
   size_t c = per_row_bytes ? budget / per_row_bytes : nrows;
+  if (const char* e = std::getenv("WEMFWI_CHUNK_ROWS")) {
+    const long long cap = std::atoll(e);
+    if (cap > 0) c = std::min(c, (size_t)cap);
+  }
   if (c < 1) c = 1;
   return std::min(c, nrows);
 }
@@ -555,10 +564,13 @@ static std::vector<Rows> make_chunks(int ns, int nw, const std::vector<Model*>& 
     for (int s = 0; s < ns; ++s)
       for (int i = 0; i < nw; ++i) all.push_back({s, i});
   }
-  const size_t c = chunk_rows(per_row, all.size());
+  size_t c = chunk_rows(per_row, all.size());
+  // whole reduction runs per chunk, so that sums do not depend on the cut
+  if (c < all.size() && c >= (size_t)kRunRows) c -= c % kRunRows;
   std::vector<Rows> out;
   for (size_t b = 0; b < all.size(); b += c) {
     Rows r;
+    r.first = b;
     for (size_t k = b; k < std::min(all.size(), b + c); ++k) {
       r.shot.push_back(all[k].first);
       r.fi.push_back(all[k].second);
@@ -682,9 +694,9 @@ static int born_forward(const std::vector<Model*>& fm, Per per, Survey* sv, cons
 }
 
 // Reduction of a chunk's per-row contributions into the FP64 accumulators,
-// grouped by model slice (consecutive rows of one slice) or over all rows;
-// rows are added to the accumulator one at a time in row order, so the sum
-// does not depend on how the rows were cut into chunks or depth segments.
+// grouped by model slice (consecutive rows of one slice) or over all rows,
+// in runs of the call's row order (launch_reduce_rows), so the sum does not
+// depend on how the rows were cut into chunks or depth segments.
 //   depths [z0, z1) of row i at in + i * stride + (z0 - zbase) * nx
 static int reduce_chunk(Ctx& c, const Rows& r, bool sliced, double2* acc, const float2* in, size_t stride, int zbase,
                         int z0, int z1) {
@@ -697,8 +709,9 @@ static int reduce_chunk(Ctx& c, const Rows& r, bool sliced, double2* acc, const 
     else
       i1 = n;
     const size_t slot = sliced ? (size_t)r.slice[i0] : 0;
-    HIPCHK(launch_reduce_rows(acc + slot * c.nzx + (size_t)z0 * c.nx, in + (size_t)i0 * stride + (size_t)(z0 - zbase) * c.nx,
-                              (long long)stride, (long long)(z1 - z0) * c.nx, i1 - i0, true, c.s),
+    HIPCHK(launch_reduce_rows(acc + slot * c.nzx + (size_t)z0 * c.nx,
+                              in + (size_t)i0 * stride + (size_t)(z0 - zbase) * c.nx, (long long)stride,
+                              (long long)(z1 - z0) * c.nx, i1 - i0, (int)((r.first + i0) % kRunRows), true, c.s),
            "reduce");
     i0 = i1;
   }
@@ -1028,7 +1041,8 @@ static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const f
         int i1 = i0 + 1;
         while (i1 < n && r.fi[i1] == r.fi[i0]) ++i1;
         HIPCHK(launch_reduce_rows(acc.as<double2>() + (size_t)r.fi[i0] * c.nzx, cb + (size_t)i0 * c.nzx,
-                                  (long long)c.nzx, (long long)c.nzx, i1 - i0, true, s),
+                                  (long long)c.nzx, (long long)c.nzx, i1 - i0, (int)((r.first + i0) % kRunRows),
+                                  true, s),
                "reduce");
         i0 = i1;
       }
