@@ -197,7 +197,7 @@ def test_tables_on_any_grid_and_propagation_rejects_unbuilt_fft_length():
     from wemfwi._lib import WemfwiError
     from wemfwi.unit import Reflect, SSF
     rng = np.random.default_rng(23)
-    nz, nx = 6, 77
+    nz, nx = 6, 134  # 2 x 67: a prime factor above the largest planned radix
     s = (1 / (2000.0 + 100 * rng.random((nz, nx))) ** 2).astype(np.complex64)
     den = np.ones((nz, nx), np.complex64)
     W = _rnd((nz, nx), 24)

@@ -33,18 +33,18 @@ __device__ __forceinline__ void acc_row(float2* __restrict__ p, const float2 (&v
 // d = 1..nz-1, neighbour d-1.
 //   forward: out[d+dir] += P_d Sc_d (bg[d] * m[d])
 //   adjoint: out[d]     += conj(bg[d]) * Sc^H_d P^H_d (in[d+dir])
-template <int N, bool ADJ, bool EXACT>
+template <int N, bool ADJ, bool EXACT, bool GEN>
 __global__ void __launch_bounds__(64 * kLinWaves)
     __attribute__((amdgpu_waves_per_eu(N <= 256 ? 4 : 2, 8)))
     lin_kernel(const LinRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models, StepGeom g,
                int dlo, int nd, int dir, const float2* __restrict__ twg) {
-  using V = Var<N, false, true>;
+  using V = Var<N, false, true, GEN>;
   constexpr int E = N / 64;
-  __shared__ float2 tw[plan_tw_size<N>()];
-  __shared__ float2 bufs[kLinWaves][LdsSize<N>::value];
+  __shared__ float2 tw[TwV<V>::value];
+  __shared__ float2 bufs[kLinWaves][LdsV<V>::value];
   const int t = threadIdx.x & 63;
   const int wv = wave_id();
-  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
+  for (int i = threadIdx.x; i < TwV<V>::value; i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int item = blockIdx.x * kLinWaves + wv;
   if (item >= nrows * nd) return;
@@ -78,17 +78,17 @@ __global__ void __launch_bounds__(64 * kLinWaves)
 }
 
 // Scatter as a unit operator on independent rows: out (+)= Sc_d(in).
-template <int N, bool ADJ, bool EXACT>
+template <int N, bool ADJ, bool EXACT, bool GEN>
 __global__ void __launch_bounds__(64 * kLinWaves)
     scatter_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                    StepGeom g, const float2* __restrict__ twg) {
-  using V = Var<N, false, true>;
+  using V = Var<N, false, true, GEN>;
   constexpr int E = N / 64;
-  __shared__ float2 tw[plan_tw_size<N>()];
-  __shared__ float2 bufs[kLinWaves][LdsSize<N>::value];
+  __shared__ float2 tw[TwV<V>::value];
+  __shared__ float2 bufs[kLinWaves][LdsV<V>::value];
   const int t = threadIdx.x & 63;
   const int wv = wave_id();
-  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
+  for (int i = threadIdx.x; i < TwV<V>::value; i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kLinWaves + wv;
   if (row >= nrows) return;
@@ -387,18 +387,28 @@ static dim3 grid1d(long long n, int block = 256) {
   return dim3((unsigned)b);
 }
 
-template <int N>
-static hipError_t launch_lin_n(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int dlo,
+template <int N, bool GEN>
+static hipError_t launch_lin_v(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int dlo,
                                int nd, int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
   const long long items = (long long)nrows * nd;
   dim3 grid((unsigned)((items + kLinWaves - 1) / kLinWaves)), block(64 * kLinWaves);
   if (!adj)
-    hipLaunchKernelGGL((lin_kernel<N, false, false>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir, tw);
+    hipLaunchKernelGGL((lin_kernel<N, false, false, GEN>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir,
+                       tw);
   else if (exact)
-    hipLaunchKernelGGL((lin_kernel<N, true, true>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir, tw);
+    hipLaunchKernelGGL((lin_kernel<N, true, true, GEN>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir,
+                       tw);
   else
-    hipLaunchKernelGGL((lin_kernel<N, true, false>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir, tw);
+    hipLaunchKernelGGL((lin_kernel<N, true, false, GEN>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir,
+                       tw);
   return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_lin_n(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int dlo,
+                               int nd, int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
+  if (g.fg.n != N) return launch_lin_v<N, true>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
+  return launch_lin_v<N, false>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
 }
 
 hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
@@ -419,17 +429,24 @@ hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* m
   }
 }
 
-template <int N>
-static hipError_t launch_scatter_n(const StepRow* rows, int nrows, const DepthTables* models, StepGeom g,
+template <int N, bool GEN>
+static hipError_t launch_scatter_v(const StepRow* rows, int nrows, const DepthTables* models, StepGeom g,
                                    bool adj, bool exact, const float2* tw, hipStream_t s) {
   dim3 grid((nrows + kLinWaves - 1) / kLinWaves), block(64 * kLinWaves);
   if (!adj)
-    hipLaunchKernelGGL((scatter_kernel<N, false, false>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((scatter_kernel<N, false, false, GEN>), grid, block, 0, s, rows, nrows, models, g, tw);
   else if (exact)
-    hipLaunchKernelGGL((scatter_kernel<N, true, true>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((scatter_kernel<N, true, true, GEN>), grid, block, 0, s, rows, nrows, models, g, tw);
   else
-    hipLaunchKernelGGL((scatter_kernel<N, true, false>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((scatter_kernel<N, true, false, GEN>), grid, block, 0, s, rows, nrows, models, g, tw);
   return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_scatter_n(const StepRow* rows, int nrows, const DepthTables* models, StepGeom g,
+                                   bool adj, bool exact, const float2* tw, hipStream_t s) {
+  if (g.fg.n != N) return launch_scatter_v<N, true>(rows, nrows, models, g, adj, exact, tw, s);
+  return launch_scatter_v<N, false>(rows, nrows, models, g, adj, exact, tw, s);
 }
 
 hipError_t launch_scatter(int N, const StepRow* rows, int nrows, const DepthTables* models, StepGeom g, bool adj,

@@ -306,6 +306,149 @@ __device__ __forceinline__ void fft_row(float2 (&x)[N / 64], float2* buf, const 
   fft_passes<N, INV, 0>(x, buf, tw, t);
 }
 
+// ---------------------------------------------------------------- any length
+//
+// Rows whose FFT length n is not the power of two N of the launch (n <= N,
+// e.g. 500 or 600 samples in a 512- or 1024-lane row): the row is staged in
+// LDS and transformed by a runtime chain of Stockham passes between two LDS
+// buffers (buf and buf + LdsSize<N>), radix 4, 2, 3 and 5 as register
+// butterflies and any other prime radix as a direct DFT.  Pass p with radix
+// R and sub-transform length NS reads butterfly j's inputs at j + r n/R and
+// writes output s at (j/NS) NS R + j%NS + s NS, as in the power-of-two
+// passes.  Twiddle table: pass p holds entries (r-1) NS + m = exp(-2 pi i m
+// r / (NS R)) (m < NS, 1 <= r < R), followed for a direct-DFT radix by R
+// entries exp(-2 pi i q / R); entries are double-precision values rounded
+// once.  At most n - 1 + 3 * 61 entries (radices above 61 are not planned).
+struct FftGen {
+  int n;                   // transform length
+  int np;                  // number of passes (at most 10)
+  unsigned long long rad;  // radix of pass p in bits 6p .. 6p+5 (a word, not an
+                           // array: a runtime-indexed array would live in scratch)
+};
+__host__ __device__ constexpr int gen_radix(const FftGen& f, int p) { return (int)((f.rad >> (6 * p)) & 63u); }
+
+constexpr int kGenMaxRadix = 61;
+__host__ __device__ constexpr int gen_tw_cap(int N) { return N + 3 * kGenMaxRadix; }
+
+constexpr float kSin3 = 0.86602540378443864676f;   // sin(2 pi / 3)
+constexpr float kCos5a = 0.30901699437494742410f;  // cos(2 pi / 5)
+constexpr float kCos5b = -0.80901699437494742410f; // cos(4 pi / 5)
+constexpr float kSin5a = 0.95105651629515357212f;  // sin(2 pi / 5)
+constexpr float kSin5b = 0.58778525229247312917f;  // sin(4 pi / 5)
+
+template <bool INV>
+__device__ __forceinline__ void dft3(float2 (&v)[3]) {
+  const float2 t1 = cadd(v[1], v[2]);
+  const float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+  const float2 d = csub(v[1], v[2]);
+  const float2 t3 = mul_mi<INV>(cscale(d, kSin3));  // -i sin(2pi/3) (v1 - v2), conjugated for INV
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft5(float2 (&v)[5]) {
+  const float2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+  const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+  const float2 a = v[0];
+  const float2 p1 = make_float2(a.x + kCos5a * b1.x + kCos5b * b2.x, a.y + kCos5a * b1.y + kCos5b * b2.y);
+  const float2 p2 = make_float2(a.x + kCos5b * b1.x + kCos5a * b2.x, a.y + kCos5b * b1.y + kCos5a * b2.y);
+  // -i (s1 d1 + s2 d2) and -i (s2 d1 - s1 d2) (conjugated for INV)
+  const float2 q1 = mul_mi<INV>(make_float2(kSin5a * d1.x + kSin5b * d2.x, kSin5a * d1.y + kSin5b * d2.y));
+  const float2 q2 = mul_mi<INV>(make_float2(kSin5b * d1.x - kSin5a * d2.x, kSin5b * d1.y - kSin5a * d2.y));
+  v[0] = make_float2(a.x + b1.x + b2.x, a.y + b1.y + b2.y);
+  v[1] = cadd(p1, q1);
+  v[4] = csub(p1, q1);
+  v[2] = cadd(p2, q2);
+  v[3] = csub(p2, q2);
+}
+
+// one pass with a register-butterfly radix R (2, 3, 4, 5)
+template <int R, bool INV>
+__device__ __forceinline__ void gen_pass_r(const float2* __restrict__ A, float2* __restrict__ B,
+                                           const float2* __restrict__ tw, int n, int NS, int t) {
+  const int NB = n / R;
+  for (int j = t; j < NB; j += 64) {
+    const int q = j / NS, m = j - q * NS;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = A[lidx(j + r * NB)];
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul_tw<INV>(v[r], tw[(r - 1) * NS + m]);
+    }
+    if constexpr (R == 2) dft2<INV>(v[0], v[1]);
+    else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 3) dft3<INV>(v);
+    else dft5<INV>(v);
+    const int o = q * NS * R + m;
+#pragma unroll
+    for (int r = 0; r < R; ++r) B[lidx(o + r * NS)] = v[r];
+  }
+}
+
+// one pass with a prime radix R > 5: direct DFT, twiddled inputs re-read
+// from LDS for every output (the transform is rare, registers are not)
+template <bool INV>
+__device__ __forceinline__ void gen_pass_dft(const float2* __restrict__ A, float2* __restrict__ B,
+                                             const float2* __restrict__ tw, const float2* __restrict__ wr, int n,
+                                             int R, int NS, int t) {
+  const int NB = n / R;
+  for (int j = t; j < NB; j += 64) {
+    const int q = j / NS, m = j - q * NS;
+    const int o = q * NS * R + m;
+    for (int s = 0; s < R; ++s) {
+      float2 acc = A[lidx(j)];
+      int e = 0;  // (r s) mod R
+      for (int r = 1; r < R; ++r) {
+        e += s;
+        if (e >= R) e -= R;
+        float2 v = A[lidx(j + r * NB)];
+        if (NS > 1) v = cmul_tw<INV>(v, tw[(r - 1) * NS + m]);
+        acc = cadd(acc, cmul_tw<INV>(v, wr[e]));
+      }
+      B[lidx(o + s * NS)] = acc;
+    }
+  }
+}
+
+// Unnormalised DFT of length f.n <= N of the row held in x (strided natural
+// layout in and out; samples n.. are zero on output).
+template <int N, bool INV>
+__device__ __forceinline__ void fft_row_gen(float2 (&x)[N / 64], float2* buf, const float2* tw, int t,
+                                            const FftGen& f) {
+  constexpr int E = N / 64;
+  const int n = f.n;
+  float2* A = buf;
+  float2* B = buf + LdsSize<N>::value;
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (t + 64 * e < n) A[lidx(t + 64 * e)] = x[e];
+  wave_lds_order();
+  int NS = 1, off = 0;
+  for (int p = 0; p < f.np; ++p) {
+    const int R = gen_radix(f, p);
+    const float2* twp = tw + off;
+    switch (R) {
+      case 2: gen_pass_r<2, INV>(A, B, twp, n, NS, t); break;
+      case 3: gen_pass_r<3, INV>(A, B, twp, n, NS, t); break;
+      case 4: gen_pass_r<4, INV>(A, B, twp, n, NS, t); break;
+      case 5: gen_pass_r<5, INV>(A, B, twp, n, NS, t); break;
+      default: gen_pass_dft<INV>(A, B, twp, twp + (R - 1) * NS, n, R, NS, t); break;
+    }
+    wave_lds_order();
+    off += (R - 1) * NS + (R > 5 ? R : 0);
+    NS *= R;
+    float2* T = A;
+    A = B;
+    B = T;
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = t + 64 * e < n ? A[lidx(t + 64 * e)] : make_float2(0.f, 0.f);
+  wave_lds_order();
+}
+
 // Fills the concatenated per-pass twiddle table of FftPlan<N> (double
 // precision, rounded once).
 template <int N>

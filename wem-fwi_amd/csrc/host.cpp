@@ -144,6 +144,86 @@ static std::map<int, float2*>& twiddle_cache() {
   return c;
 }
 
+// Runtime FFT plan of a length n that is not a built power of two:
+// radix-4 passes, one radix-2 pass, then the odd primes in increasing order
+// (3 and 5 as register butterflies, larger primes up to kGenMaxRadix as
+// direct DFTs).  False when n has a larger prime factor.
+bool fft_gen_plan(int n, FftGen* f) {
+  *f = FftGen{};
+  f->n = n;
+  if (n < 2) return false;
+  int m = n, np = 0;
+  std::vector<int> rad;
+  while (m % 4 == 0) {
+    rad.push_back(4);
+    m /= 4;
+  }
+  if (m % 2 == 0) {
+    rad.push_back(2);
+    m /= 2;
+  }
+  for (int p = 3; m > 1; p += 2) {
+    while (m % p == 0) {
+      if (p > kGenMaxRadix) return false;
+      rad.push_back(p);
+      m /= p;
+    }
+  }
+  if (rad.size() > 10) return false;
+  for (int r : rad) f->rad |= (unsigned long long)r << (6 * np++);
+  f->np = np;
+  return true;
+}
+
+// Twiddle table of a runtime plan (layout of fft_row_gen), padded with
+// zeros to the gen_tw_cap(N) entries a launch stages in LDS; double
+// precision rounded once.  Cached per (n, N, device).
+static std::map<long long, float2*>& gen_twiddle_cache() {
+  static std::map<long long, float2*> c;
+  return c;
+}
+
+int gen_twiddles(const FftGen& f, int N, const float2** out) {
+  auto& c = gen_twiddle_cache();
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const long long key = ((long long)f.n * 4096 + N) * 64 + dev;
+  auto it = c.find(key);
+  if (it != c.end()) {
+    *out = it->second;
+    return WEMFWI_OK;
+  }
+  std::vector<float2> h((size_t)gen_tw_cap(N), make_float2(0.f, 0.f));
+  size_t off = 0;
+  int NS = 1;
+  for (int p = 0; p < f.np; ++p) {
+    const int R = gen_radix(f, p);
+    for (int r = 1; r < R; ++r)
+      for (int m = 0; m < NS; ++m) {
+        const double a = -2.0 * M_PI * (double)((long long)m * r) / (double)((long long)NS * R);
+        h[off + (size_t)(r - 1) * NS + m] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+    off += (size_t)(R - 1) * NS;
+    if (R > 5) {
+      for (int q = 0; q < R; ++q) {
+        const double a = -2.0 * M_PI * (double)q / (double)R;
+        h[off + q] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+      off += R;
+    }
+    NS *= R;
+  }
+  if (off > h.size()) return fail(WEMFWI_ERR_UNSUPPORTED, "FFT plan: twiddle table too large");
+  float2* d = nullptr;
+  int rc = dev_alloc(&d, h.size(), "twiddle table");
+  if (rc) return rc;
+  hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "twiddle table");
+  c[key] = d;
+  *out = d;
+  return WEMFWI_OK;
+}
+
 int twiddles(int n, const float2** out) {
   auto& c = twiddle_cache();
   int dev = 0;
@@ -185,10 +265,18 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
     return fail(WEMFWI_ERR_ARG, "model: invalid grid or parameters");
   if (d.taper == 1) return fail(WEMFWI_ERR_ARG, "model: taper width pad=1 is undefined (Taper.h divides by pad-1)");
   if (2 * d.taper > d.nx) return fail(WEMFWI_ERR_ARG, "model: taper wider than half the row");
-  N = d.nx + 2 * d.tap;
+  nk = d.nx + 2 * d.tap;
+  N = 128;
+  while (N < nk && N < 2048) N *= 2;
   // the background tables (Reflect, dReflect, scatterer coefficients,
   // RefSampler) need no FFT; the propagating entry points check fft_ok
-  fft_ok = fft_size_supported(N);
+  if (nk == N) {
+    fg = FftGen{};
+    fg.n = N;
+    fft_ok = fft_size_supported(N);
+  } else {
+    fft_ok = nk <= N && fft_gen_plan(nk, &fg);
+  }
   const size_t nzx = (size_t)d.nz * d.nx;
   slow.assign(slow_h, slow_h + nzx);
   if (den_h) den.assign(den_h, den_h + nzx);
@@ -319,12 +407,16 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   t.nz = d.nz;
   t.n = N;
   t.dz = d.dz;
-  t.dk = (float)(2.0 * M_PI / ((double)d.dx * (double)(N - 1)));
+  t.dk = (float)(2.0 * M_PI / ((double)d.dx * (double)(nk - 1)));
   t.nx = d.nx;
   t.tap = d.tap;
   t.taper = d.taper > 0 ? 1 : 0;
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
-  if (fft_ok && (rc = twiddles(N, &d_tw))) return rc;
+  if (fft_ok) {
+    if (nk == N) rc = twiddles(N, &d_tw);
+    else rc = gen_twiddles(fg, N, &d_tw);
+    if (rc) return rc;
+  }
   return WEMFWI_OK;
 }
 
@@ -334,6 +426,7 @@ StepGeom Model::geom() const {
   g.tap = d.tap;
   g.ssf = d.pspi ? 0 : 1;
   g.full = (d.tap == 0 && d.nx == N) ? 1 : 0;
+  g.fg = fg;
   g.ntaylor = d.ntaylor;
   g.taper = d.taper > 0 ? 1 : 0;
   return g;
@@ -343,8 +436,9 @@ StepGeom Model::geom() const {
 // active slot, k <= N/2) -- the k grid is symmetric, so P[N-k] = P[k].
 int Model::require_fft() const {
   if (fft_ok) return WEMFWI_OK;
-  return fail(WEMFWI_ERR_UNSUPPORTED, "model: FFT length nx+2*tap=" + std::to_string(N) +
-                                          " not supported (built: 128, 256, 512, 1024, 2048)");
+  return fail(WEMFWI_ERR_UNSUPPORTED, "model: FFT length nx+2*tap=" + std::to_string(nk) +
+                                          " not supported (lengths up to 2048 whose prime factors are at most " +
+                                          std::to_string(kGenMaxRadix) + ")");
 }
 
 int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
@@ -361,11 +455,11 @@ int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
   if ((rc = wlist.reserve(wl.size() * sizeof(double), "frequency list"))) return rc;
   hipError_t e = hipMemcpyAsync(wlist.ptr, wl.data(), wl.size() * sizeof(double), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "frequency upload");
-  const double dk = 2.0 * M_PI / ((double)d.dx * (double)(N - 1));
+  const double dk = 2.0 * M_PI / ((double)d.dx * (double)(nk - 1));
   KernelTimer kt("phase_table_kernel");
   kt.begin(s);
   e = launch_phase_table((float2*)ptab.ptr, (const double*)wlist.ptr, (int)wl.size(), d_slot_sref, nslots,
-                         N, NH(), dk, (double)d.dz, (double)d.eps, s);
+                         nk, NH(), dk, (double)d.dz, (double)d.eps, s);
   kt.end(s);
   if (e != hipSuccess) return hip_fail(e, "phase table kernel");
   freqs = uniq;
@@ -524,7 +618,7 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
   std::vector<long long> key(nrows);
   for (int i = 0; i < nrows; ++i) {
     const StepGeom g = ms[i]->geom();
-    key[i] = -((long long)ms[i]->N * 4 + g.full * 2 + g.ssf);
+    key[i] = -((((long long)ms[i]->N * 4096 + ms[i]->nk) * 2 + g.full) * 2 + g.ssf);
   }
   std::vector<int> order(nrows);
   std::iota(order.begin(), order.end(), 0);

@@ -52,8 +52,10 @@ struct Workspace {
 
 struct Model {
   wemfwi_model_desc d{};
-  int N = 0;         // FFT length nx + 2*tap
-  bool fft_ok = false;  // N is one of the built FFT lengths (propagation possible)
+  int nk = 0;        // FFT length nx + 2*tap
+  int N = 0;         // lanes of a row: the built power of two (128 ... 2048) >= nk
+  FftGen fg{};       // FFT plan of nk (runtime passes when nk != N)
+  bool fft_ok = false;  // nk fits a built row and has a plan (propagation possible)
   int nslots = 0;    // active (depth, reference) pairs
   std::vector<std::complex<float>> slow, den, sref;
   std::vector<int> refid, slot_begin, slot_ref;

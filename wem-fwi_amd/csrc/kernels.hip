@@ -15,8 +15,9 @@ namespace wemfwi {
 // Phshift_cmplx.cpp:12-24).  kz = re + i im with im <= 0; Im is formed as
 // -|b|/(2 re) (or Re as |b|/(2|im|) when a < 0) -- equal to the reference's
 // -sqrt((c-a)/2) since (c-a)(c+a) = b^2, without its cancellation.
-// Only k = 0..N/2 is stored: the grid satisfies k[N-i] = -k[i], so the
-// multiplier of index k is entry min(k, N-k).
+// Only k = 0..n/2 is used: the grid satisfies k[n-i] = -k[i], so the
+// multiplier of index k is entry min(k, n-k); n is the FFT length, NH the
+// entries per (frequency, slot) (N/2 + 1 of the row's N lanes, >= n/2 + 1).
 __global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __restrict__ wlist,
                                    int nw, const double2* __restrict__ sref, int nslots, int N,
                                    int NH, double dk, double dz, double eps) {
@@ -67,6 +68,12 @@ template <int N>
 struct WavesPerBlock {
   static constexpr int value = N >= 1024 ? 8 : 4;
 };
+// GEN rows use two LDS row buffers per wave: 4 waves of 2 x 34.8 KB at
+// N = 2048
+template <class V>
+struct WavesV {
+  static constexpr int value = (V::GEN && V::n == 2048) ? 4 : WavesPerBlock<V::n>::value;
+};
 
 // In-place or out-of-place Down/Up recursion (OneWay.cpp:32-109) of one
 // (shot, frequency) row per wavefront.  step = +1 runs iz = 0..nz-2 (Down
@@ -81,18 +88,18 @@ struct Occupancy {
 };
 
 template <class V, bool ADJ>
-__global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
-    __attribute__((amdgpu_waves_per_eu(Occupancy<V::n>::value, 8)))
+__global__ void __launch_bounds__(64 * WavesV<V>::value)
+    __attribute__((amdgpu_waves_per_eu(V::GEN ? 2 : Occupancy<V::n>::value, 8)))
     oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                   StepGeom g, int step, const float2* __restrict__ twg) {
   constexpr int N = V::n;
   constexpr int E = V::E;
-  constexpr int kWavesPerBlock = WavesPerBlock<N>::value;
-  __shared__ float2 tw[plan_tw_size<N>()];
-  __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
+  constexpr int kWavesPerBlock = WavesV<V>::value;
+  __shared__ float2 tw[TwV<V>::value];
+  __shared__ float2 bufs[kWavesPerBlock][LdsV<V>::value];
   const int t = threadIdx.x & 63;
   const int wv = wave_id();
-  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
+  for (int i = threadIdx.x; i < TwV<V>::value; i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kWavesPerBlock + wv;
   if (row >= nrows) return;
@@ -128,18 +135,18 @@ __global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
 
 // Independent single depth steps (SSF::forward/adjoint as a unit operator).
 template <class V, bool ADJ>
-__global__ void __launch_bounds__(64 * WavesPerBlock<V::n>::value)
-    __attribute__((amdgpu_waves_per_eu(Occupancy<V::n>::value, 8)))
+__global__ void __launch_bounds__(64 * WavesV<V>::value)
+    __attribute__((amdgpu_waves_per_eu(V::GEN ? 2 : Occupancy<V::n>::value, 8)))
     onestep_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                    StepGeom g, const float2* __restrict__ twg) {
   constexpr int N = V::n;
   constexpr int E = V::E;
-  constexpr int kWavesPerBlock = WavesPerBlock<N>::value;
-  __shared__ float2 tw[plan_tw_size<N>()];
-  __shared__ float2 bufs[kWavesPerBlock][LdsSize<N>::value];
+  constexpr int kWavesPerBlock = WavesV<V>::value;
+  __shared__ float2 tw[TwV<V>::value];
+  __shared__ float2 bufs[kWavesPerBlock][LdsV<V>::value];
   const int t = threadIdx.x & 63;
   const int wv = wave_id();
-  for (int i = threadIdx.x; i < plan_tw_size<N>(); i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
+  for (int i = threadIdx.x; i < TwV<V>::value; i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
   __syncthreads();
   const int row = blockIdx.x * kWavesPerBlock + wv;
   if (row >= nrows) return;
@@ -208,7 +215,7 @@ hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const d
 template <class V>
 static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
                                   int step, bool adj, const float2* tw, hipStream_t s) {
-  constexpr int kWavesPerBlock = WavesPerBlock<V::n>::value;
+  constexpr int kWavesPerBlock = WavesV<V>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   if (adj)
     hipLaunchKernelGGL((oneway_kernel<V, true>), grid, block, 0, s, rows, nrows, models, g, step, tw);
@@ -220,6 +227,10 @@ static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTa
 template <int N>
 static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
                                   int step, bool adj, const float2* tw, hipStream_t s) {
+  if (g.fg.n != N) {
+    if (g.ssf) return launch_oneway_v<Var<N, false, true, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
+    return launch_oneway_v<Var<N, false, false, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  }
   if (g.full && g.ssf) return launch_oneway_v<Var<N, true, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
   if (g.full) return launch_oneway_v<Var<N, true, false>>(rows, nrows, models, g, nz, step, adj, tw, s);
   if (g.ssf) return launch_oneway_v<Var<N, false, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
@@ -240,17 +251,24 @@ hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTabl
 }
 
 // single steps use the general variant (padding and split step allowed)
-template <int N>
-static hipError_t launch_onestep_n(const StepRow* rows, int nrows, const DepthTables* models,
+template <int N, bool GEN>
+static hipError_t launch_onestep_v(const StepRow* rows, int nrows, const DepthTables* models,
                                    StepGeom g, bool adj, const float2* tw, hipStream_t s) {
-  using V = Var<N, false, true>;
-  constexpr int kWavesPerBlock = WavesPerBlock<N>::value;
+  using V = Var<N, false, true, GEN>;
+  constexpr int kWavesPerBlock = WavesV<V>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   if (adj)
     hipLaunchKernelGGL((onestep_kernel<V, true>), grid, block, 0, s, rows, nrows, models, g, tw);
   else
     hipLaunchKernelGGL((onestep_kernel<V, false>), grid, block, 0, s, rows, nrows, models, g, tw);
   return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_onestep_n(const StepRow* rows, int nrows, const DepthTables* models,
+                                   StepGeom g, bool adj, const float2* tw, hipStream_t s) {
+  if (g.fg.n != N) return launch_onestep_v<N, true>(rows, nrows, models, g, adj, tw, s);
+  return launch_onestep_v<N, false>(rows, nrows, models, g, adj, tw, s);
 }
 
 hipError_t launch_onestep(int N, const StepRow* rows, int nrows, const DepthTables* models,

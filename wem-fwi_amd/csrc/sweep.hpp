@@ -138,17 +138,38 @@ struct StepGeom {
   int full;     // 1: tap == 0 and nx == N (every lane sample is a model sample)
   int ntaylor;  // Taylor order of the scatterer
   int taper;    // 1: a taper is applied (width > 0)
+  FftGen fg;    // FFT length and pass plan when it is not the launch's N
 };
 
-// Compile-time variants of the step: FULL (no padding samples) and SSF
-// (split-step correction).
-template <int N, bool FULL_, bool SSF_>
+// Compile-time variants of the step: FULL (no padding samples), SSF
+// (split-step correction) and GEN (FFT length g.fg.n below the N lanes of
+// the row, transformed by the runtime-planned LDS passes).
+template <int N, bool FULL_, bool SSF_, bool GEN_ = false>
 struct Var {
   static constexpr int n = N;
   static constexpr int E = N / 64;
   static constexpr bool FULL = FULL_;
   static constexpr bool SSF = SSF_;
+  static constexpr bool GEN = GEN_;
+  static_assert(!(FULL_ && GEN_), "a row shorter than N has padding samples");
 };
+
+// LDS complexes per wave and twiddle-table entries of a variant
+template <class V>
+struct LdsV {
+  static constexpr int value = V::GEN ? 2 * LdsSize<V::n>::value : LdsSize<V::n>::value;
+};
+template <class V>
+struct TwV {
+  static constexpr int value = V::GEN ? gen_tw_cap(V::n) : plan_tw_size<V::n>();
+};
+
+// the row transform of a variant
+template <class V, bool INV>
+__device__ __forceinline__ void fft_v(float2 (&x)[V::E], float2* buf, const float2* tw, int t, const StepGeom& g) {
+  if constexpr (V::GEN) fft_row_gen<V::n, INV>(x, buf, tw, t, g.fg);
+  else fft_row<V::n, INV>(x, buf, tw, t);
+}
 
 // sin and cos of x for |x| < 1e5: Cody-Waite reduction by a three-part pi/2
 // (W. J. Cody, W. Waite, "Software Manual for the Elementary Functions", 1980)
@@ -261,6 +282,21 @@ __device__ __forceinline__ float2 ptab_at(const float2* P, int t, int e) {
   return bld2(r, (unsigned)(64 - t) * 8u, 512u * (unsigned)(N / 64 - 1 - e));
 }
 
+// phase-table entry of lane sample k = t + 64 e of a variant: entry k for
+// k <= n/2, n - k above (the k grid is symmetric); lanes k >= n of a GEN row
+// read entry 0 (their spectrum samples are never used)
+template <class V>
+__device__ __forceinline__ float2 ptab_v(const float2* P, int t, int e, const StepGeom& g) {
+  if constexpr (!V::GEN) {
+    return ptab_at<V::n>(P, t, e);
+  } else {
+    const int n = g.fg.n, k = t + 64 * e;
+    const int kh = k <= n / 2 ? k : (k < n ? n - k : 0);
+    const brsrc r = make_rsrc(P, (unsigned)(n / 2 + 1) * 8u);
+    return bld2(r, (unsigned)kh * 8u, 0u);
+  }
+}
+
 // row of samples x = t + 64 e - tap (0 on padding samples)
 template <class V>
 __device__ __forceinline__ void load_samples(float2 (&v)[V::E], const float2* p, const StepGeom& g, int t) {
@@ -292,15 +328,15 @@ __device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w,
   }
   constexpr int N = V::n;
   constexpr int E = V::E;
-  fft_row<N, false>(st, buf, tw, t);
+  fft_v<V, false>(st, buf, tw, t, g);
   const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
   if (s1 - s0 == 1) {
     // one active reference: transform in place
     const int r = ldu(dt.slot_ref, s0);
     const float2* P = ptab + (size_t)s0 * (N / 2 + 1);
 #pragma unroll
-    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], ptab_at<N>(P, t, e));
-    fft_row<N, true>(st, buf, tw, t);
+    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], ptab_v<V>(P, t, e, g));
+    fft_v<V, true>(st, buf, tw, t, g);
     if (!(V::FULL && ldu(dt.one_ref, d))) {
       unsigned rid[(E + 3) / 4];
       load_refid<V>(dt, d, t, rid);
@@ -323,8 +359,8 @@ __device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w,
       const float2* P = ptab + (size_t)s * (N / 2 + 1);
       float2 Y[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) Y[e] = cmul(X[e], ptab_at<N>(P, t, e));
-      fft_row<N, true>(Y, buf, tw, t);
+      for (int e = 0; e < E; ++e) Y[e] = cmul(X[e], ptab_v<V>(P, t, e, g));
+      fft_v<V, true>(Y, buf, tw, t, g);
 #pragma unroll
       for (int e = 0; e < E; ++e)
         if (rid_at(rid, e) == r) st[e] = Y[e];
@@ -373,9 +409,9 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w,
       for (int e = 0; e < E; ++e)
         if (rid_at(rid, e) != r) st[e] = make_float2(0.f, 0.f);
     }
-    fft_row<N, false>(st, buf, tw, t);
+    fft_v<V, false>(st, buf, tw, t, g);
 #pragma unroll
-    for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], ptab_at<N>(P, t, e));
+    for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], ptab_v<V>(P, t, e, g));
   } else {
     unsigned rid[(E + 3) / 4];
     load_refid<V>(dt, d, t, rid);
@@ -389,14 +425,14 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w,
       float2 Z[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) Z[e] = (rid_at(rid, e) == r) ? st[e] : make_float2(0.f, 0.f);
-      fft_row<N, false>(Z, buf, tw, t);
+      fft_v<V, false>(Z, buf, tw, t, g);
 #pragma unroll
-      for (int e = 0; e < E; ++e) acc[e] = cadd(acc[e], cmulc(Z[e], ptab_at<N>(P, t, e)));
+      for (int e = 0; e < E; ++e) acc[e] = cadd(acc[e], cmulc(Z[e], ptab_v<V>(P, t, e, g)));
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = acc[e];
   }
-  fft_row<N, true>(st, buf, tw, t);
+  fft_v<V, true>(st, buf, tw, t, g);
   if constexpr (!V::FULL) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -422,7 +458,7 @@ __device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, floa
     for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
     return;
   }
-  fft_row<N, false>(st, buf, tw, t);
+  fft_v<V, false>(st, buf, tw, t, g);
   const bool multi = s1 - s0 > 1;
   const bool all = V::FULL && ldu(dt.one_ref, d);
   const brsrc xs = make_rsrc(scr, (unsigned)N * 8u);  // the wave's scratch row, lane index t + 64 e
@@ -441,18 +477,18 @@ __device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, floa
     // flight next to the merged output
     if (multi) {
 #pragma unroll
-      for (int e = 0; e < E / 2; ++e) Y[e] = cmul(bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e), ptab_at<N>(P, t, e));
+      for (int e = 0; e < E / 2; ++e) Y[e] = cmul(bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e), ptab_v<V>(P, t, e, g));
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int e = E / 2; e < E; ++e) Y[e] = cmul(bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e), ptab_at<N>(P, t, e));
+      for (int e = E / 2; e < E; ++e) Y[e] = cmul(bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e), ptab_v<V>(P, t, e, g));
     } else {
 #pragma unroll
-      for (int e = 0; e < E / 2; ++e) Y[e] = cmul(st[e], ptab_at<N>(P, t, e));
+      for (int e = 0; e < E / 2; ++e) Y[e] = cmul(st[e], ptab_v<V>(P, t, e, g));
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int e = E / 2; e < E; ++e) Y[e] = cmul(st[e], ptab_at<N>(P, t, e));
+      for (int e = E / 2; e < E; ++e) Y[e] = cmul(st[e], ptab_v<V>(P, t, e, g));
     }
-    fft_row<N, true>(Y, buf, tw, t);
+    fft_v<V, true>(Y, buf, tw, t, g);
     if (all) {
 #pragma unroll
       for (int e = 0; e < E; ++e) st[e] = Y[e];
@@ -509,21 +545,21 @@ __device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, floa
       const float2 v = multi ? bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e) : st[e];
       Z[e] = (all || rid_at(rid, e) == r) ? v : make_float2(0.f, 0.f);
     }
-    fft_row<N, false>(Z, buf, tw, t);
+    fft_v<V, false>(Z, buf, tw, t, g);
     const bool first = s == s0;
 #pragma unroll
     for (int e = 0; e < E / 2; ++e) {
-      const float2 z = cmulc(Z[e], ptab_at<N>(P, t, e));
+      const float2 z = cmulc(Z[e], ptab_v<V>(P, t, e, g));
       st[e] = first ? z : cadd(st[e], z);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int e = E / 2; e < E; ++e) {
-      const float2 z = cmulc(Z[e], ptab_at<N>(P, t, e));
+      const float2 z = cmulc(Z[e], ptab_v<V>(P, t, e, g));
       st[e] = first ? z : cadd(st[e], z);
     }
   }
-  fft_row<N, true>(st, buf, tw, t);
+  fft_v<V, true>(st, buf, tw, t, g);
   if constexpr (!V::FULL) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -557,24 +593,27 @@ __device__ __forceinline__ void scatter_row(float2 (&st)[V::E], int d, float w, 
     acc[e] = ADJ ? cmulc(st[e], c) : cmul(st[e], c);
   }
   const float iw2 = 1.f / (w * w);
+  const int n = V::GEN ? g.fg.n : N;  // FFT length
+  const float inv_n = 1.f / (float)n;
   for (int it = 1; it <= g.ntaylor; ++it) {
     const float2* Ci = C0 + (size_t)it * dt.scoef_stride;
     float2 Y[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) Y[e] = (ADJ && EXACT) ? cmulc(st[e], scoef_at<V>(Ci, g, t, e)) : st[e];
-    fft_row<N, false>(Y, buf, tw, t);
+    fft_v<V, false>(Y, buf, tw, t, g);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      // (k/w)^(2 it) / N on the grid of Scatter.cpp:24-31
+      // (k/w)^(2 it) / n on the grid of Scatter.cpp:24-31 (lanes k >= n of
+      // a GEN row take k = 0; their spectrum samples are never used)
       const int k = t + 64 * e;
-      const int kh = k < N / 2 ? k : ((N - k) < N / 2 ? N - k : 0);
+      const int kh = k < n / 2 ? k : ((k < n && (n - k) < n / 2) ? n - k : 0);
       const float kk = (float)kh * dt.dk;
       const float q = kk * kk * iw2;
       float qi = q;
       for (int j = 1; j < it; ++j) qi *= q;
-      Y[e] = cscale(Y[e], qi * (1.f / (float)N));
+      Y[e] = cscale(Y[e], qi * inv_n);
     }
-    fft_row<N, true>(Y, buf, tw, t);
+    fft_v<V, true>(Y, buf, tw, t, g);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       if (ADJ && EXACT) {
