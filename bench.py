@@ -25,6 +25,9 @@ Also the uneven batch: 64 shots with their own apertures (256-2048) and
 bands (32-192 frequencies), nz = 400, Down forward of all rows in one batched
 call against one call per shot and one call per row (sampled).
 
+The gradient line also carries a central-difference check on 20 random
+directions (--fd-directions 0 skips it).
+
 Prints one JSON line.  --cpu-sample 0 skips the CPU baseline, --fwi-sample 0
 the gradient comparison, --uneven-sample 0 the uneven batch.
 """
@@ -140,7 +143,43 @@ def fwi_models():
     return (1 / v ** 2).astype(np.complex64), (1 / v0 ** 2).astype(np.complex64)
 
 
-def fwi_gradient_bench(sample_rows: int):
+def fd_check(sv, obs, s0, den, gS, ndir, seed=44, rel_step=1e-3):
+    """Central differences of J along ndir smooth random slowness directions
+    (seed 44) against <grad J, delta>.  The gradient is the reference's
+    Born^H residual; the reference's Born is not the derivative of its WEM
+    (dReflect's coefficients carry an extra (a+b)^4 factor, dReflect.h:18-24,
+    and Scatter ignores the reference slownesses' dependence on S), so the two
+    only correlate; the line reports how well."""
+    import torch
+    import wemfwi
+    from wemfwi import ops
+    from scipy.ndimage import gaussian_filter
+    nz, nx = s0.shape
+    g = gS.cpu().numpy().astype(np.complex128)
+
+    def J(s):
+        m = wemfwi.Model(s.astype(np.complex64), 10.0, 10.0, den=den)
+        d = torch.zeros_like(obs)
+        ops.wem_forward(m, sv, d)
+        m.close()
+        r = (d - obs).to(torch.complex128)
+        return 0.5 * float((r.real ** 2 + r.imag ** 2).sum())
+
+    rng = np.random.default_rng(seed)
+    fd, an = [], []
+    for _ in range(ndir):
+        dl = gaussian_filter(rng.standard_normal((nz, nx)), 3.0)
+        dl = dl / np.abs(dl).max() * float(np.abs(s0).mean()) * rel_step
+        fd.append((J(s0 + dl) - J(s0 - dl)) / 2)
+        an.append(float(np.real(np.vdot(g, dl))))
+    fd, an = np.array(fd), np.array(an)
+    return {"directions": ndir, "seed": seed, "step": f"{rel_step:g} x mean slowness, smoothed 3 cells",
+            "cosine_fd_vs_gradient": round(float(np.dot(fd, an) / np.linalg.norm(fd) / np.linalg.norm(an)), 4),
+            "median_rel_err": round(float(np.median(np.abs(fd - an) / np.abs(fd))), 4),
+            "note": "Born^H r is not the derivative of WEM in the reference (dReflect, Scatter); see DESIGN"}
+
+
+def fwi_gradient_bench(sample_rows: int, fd_dirs: int = 20):
     """Wall time of one FWI gradient (WEM forward of the start model, residual,
     exact Born adjoint) with all (shot, frequency) rows batched, against the
     same library called once per (shot, frequency) row on a sample of rows."""
@@ -232,6 +271,7 @@ def fwi_gradient_bench(sample_rows: int):
                 torch.linalg.vector_norm(bS.to(torch.complex128)))
     t_plain = t_plain_sample * total / len(rows)
     upd = 12.0 * nx * (nz - 1) * total
+    fd = fd_check(sv, obs, s_start, den, gS, fd_dirs) if fd_dirs > 0 else None
     return {"config": f"ssf nx={nx} nz={nz} nw={nw} {c['fmin']:g}-{c['fmax']:g} Hz ns={ns} nr={nx}",
             "batched_ms": round(t_batched * 1e3, 2),
             "batched_gupdates_per_s": round(upd / t_batched / 1e9, 2),
@@ -243,7 +283,8 @@ def fwi_gradient_bench(sample_rows: int):
             "plain_loop_ms": round(t_plain * 1e3, 1),
             "plain_loop_sample": f"{len(rows)} of {total} rows timed ({t_plain_sample:.2f} s), scaled to all rows",
             "speedup": round(t_plain / t_batched, 2),
-            "rel_err_plain_vs_batched": rel}
+            "rel_err_plain_vs_batched": rel,
+            "fd_check": fd}
 
 
 # Uneven batch: 64 shots over a 4096-wide model, shot s with its own aperture
@@ -392,6 +433,8 @@ def main():
                     help="rows of the CPU baseline sample (0: skip)")
     ap.add_argument("--fwi-sample", type=int, default=64,
                     help="rows of the plain per-row loop in the FWI gradient comparison (0: skip)")
+    ap.add_argument("--fd-directions", type=int, default=20,
+                    help="central-difference directions of the FWI gradient check (0: skip)")
     ap.add_argument("--uneven-sample", type=int, default=32,
                     help="rows of the per-row loop in the uneven-batch comparison (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -500,7 +543,7 @@ def main():
     line["fwd_gupdates_per_s"] = round(upd_sweep / t_fwd / 1e9, 3)
     line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
     if args.fwi_sample > 0:
-        line["fwi_gradient"] = fwi_gradient_bench(args.fwi_sample)
+        line["fwi_gradient"] = fwi_gradient_bench(args.fwi_sample, args.fd_directions)
     if args.uneven_sample > 0:
         line["uneven_batch"] = uneven_batch_bench(args.uneven_sample)
     if args.cpu_sample > 0:

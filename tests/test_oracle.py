@@ -256,6 +256,39 @@ def test_dreflect_and_reflect_dot_products():
                                if np.allclose(den[3], den[2]) else r[3])
 
 
+def test_dreflect_as_written_is_the_derivative_times_interface_power():
+    """dReflect.h:18-24 sets v = (rho2 sqrt(S1) + rho1 sqrt(S2))^-2 and then
+    divides by v, so its coefficients are the derivative of Reflect's r
+    multiplied by (a + b)^4 (slowness) and sqrt(S1 S2) (a + b)^4 (density),
+    a + b = rho2 sqrt(S1) + rho1 sqrt(S2).  Pinned against a central
+    difference of reflect_coef: a restatement that 'fixed' the quirk, or
+    dropped a term, fails."""
+    nz, nx = 6, 5
+    s = (1 / RNG.uniform(1500, 3000, (nz, nx)) ** 2).astype(complex)
+    den = RNG.uniform(1, 2, (nz, nx)).astype(complex)
+    iz, ix = 3, 2
+    ab = den[iz, ix] * np.sqrt(s[iz - 1, ix]) + den[iz - 1, ix] * np.sqrt(s[iz, ix])
+    for which, (zz, scale) in {"S1": (iz - 1, ab ** 4), "S2": (iz, ab ** 4),
+                               "R1": (iz - 1, np.sqrt(s[iz - 1, ix] * s[iz, ix]) * ab ** 4),
+                               "R2": (iz, np.sqrt(s[iz - 1, ix] * s[iz, ix]) * ab ** 4)}.items():
+        dS = np.zeros((nz, nx), complex)
+        dR = np.zeros((nz, nx), complex)
+        base = s if which[0] == "S" else den
+        h = 1e-6 * abs(base[zz, ix])
+        (dS if which[0] == "S" else dR)[zz, ix] = 1.0
+        sp, sm = s.copy(), s.copy()
+        dp, dm = den.copy(), den.copy()
+        if which[0] == "S":
+            sp[zz, ix] += h
+            sm[zz, ix] -= h
+        else:
+            dp[zz, ix] += h
+            dm[zz, ix] -= h
+        deriv = (O.reflect_coef(sp, dp)[iz, ix] - O.reflect_coef(sm, dm)[iz, ix]) / (2 * h)
+        got = O.dreflect_fwd(s, den, dS, dR)[iz, ix]
+        np.testing.assert_allclose(got, deriv * scale, rtol=1e-6)
+
+
 def test_offset_ic_zero_lag_and_adjoint():
     nz, nx, nh = 5, 12, 5
     bg, sc = crand(nz, nx), crand(nz, nx)
