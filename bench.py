@@ -23,7 +23,9 @@ segment by segment), which must equal the store-all gradient bitwise.
 
 Also the uneven batch: 64 shots with their own apertures (256-2048) and
 bands (32-192 frequencies), nz = 400, Down forward of all rows in one batched
-call against one call per shot and one call per row (sampled).
+call against one call per shot and one call per row (sampled); and the
+subsurface-offset migration (nh = 11) of the same survey in one call against
+one call per (shot, frequency) row (sampled).
 
 The gradient line also carries a central-difference check on 20 random
 directions (--fd-directions 0 skips it).
@@ -50,6 +52,26 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 BYTES_PER_UPDATE = 16     # read W[iz] c64 + write W[iz+1] c64 (slowness row amortised)
+VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 (published; packed FP32 counted)
+VALU_SCALAR_TFLOPS = 78.6  # the same without packed FP32 instructions, which this build does not emit
+
+
+def sweep_flops(slow, nref: int, prop: str, nrows: int) -> float:
+    """Algorithmic FP32 flops of one Down sweep over nrows rows: per depth
+    step one forward FFT, and per active reference of the depth a phase
+    multiply and an inverse FFT; SSF adds the split-step/taper multiply; the
+    source add closes the step.  FFT = 5 N log2 N, complex multiply = 6,
+    complex add = 2 flops per sample."""
+    import wemfwi
+    _, refid = wemfwi.refsampler(slow, nref)
+    nz, nx = refid.shape
+    n = nx
+    fft = 5.0 * n * math.log2(n)
+    total = 0.0
+    for iz in range(nz - 1):
+        slots = len(set(np.unique(refid[iz]).tolist()) - {-1})
+        total += fft + slots * (6.0 * n + fft) + (6.0 * n if prop == "ssf" else 0.0) + 2.0 * n
+    return total * nrows
 
 CONFIGS = {
     # PSPI downward continuation (the headline configuration)
@@ -294,15 +316,9 @@ def fwi_gradient_bench(sample_rows: int, fd_dirs: int = 20):
 UNEVEN = dict(ns=64, nz=400, width=4096, widths=(256, 512, 1024, 2048), seed=5)
 
 
-def uneven_batch_bench(sample_rows: int):
-    """Down forward of every (shot, frequency) row of the uneven survey as one
-    batch plan run (wemfwi_batch_run; also the one-call form that plans each
-    time) against one call per row on a sample of rows, and against one call
-    per shot; the sampled rows must be bitwise equal between the batch and the
-    per-row calls."""
-    import torch
-    import wemfwi
-    from wemfwi.core import BatchPlan, oneway_batch
+def uneven_survey():
+    """The uneven survey: slowness of the 4096-wide model and, per shot, its
+    aperture (x0, nx) and band (f0 + 0.1 Hz steps, nw frequencies)."""
     from scipy.ndimage import gaussian_filter
     c = UNEVEN
     rng = np.random.default_rng(c["seed"])
@@ -312,19 +328,40 @@ def uneven_batch_bench(sample_rows: int):
     pert = gaussian_filter(rng.standard_normal((nz, W0)), 8.0)
     v = v + 0.05 * v * pert / np.max(np.abs(pert))
     slow = (1.0 / v ** 2).astype(np.complex64)
+    shots = []
+    for _ in range(c["ns"]):
+        nx = int(rng.choice(c["widths"]))
+        nw = int(rng.integers(32, 193))
+        f0 = 2.0 + 3.0 * rng.random()
+        cx = int(rng.integers(nx // 2, W0 - nx // 2 + 1))
+        freqs = [float(np.float32(f0 + 0.1 * i)) for i in range(nw)]
+        shots.append((cx - nx // 2, nx, freqs))
+    return slow, shots
+
+
+def uneven_batch_bench(sample_rows: int):
+    """Down forward of every (shot, frequency) row of the uneven survey as one
+    batch plan run (wemfwi_batch_run; also the one-call form that plans each
+    time) against one call per row on a sample of rows, and against one call
+    per shot; the sampled rows must be bitwise equal between the batch and the
+    per-row calls."""
+    import torch
+    import wemfwi
+    from wemfwi.core import BatchPlan, oneway_batch
+    c = UNEVEN
+    nz = c["nz"]
+    slow, survey = uneven_survey()
     dev = torch.device("cuda:0")
     models, rows_m, rows_f, rows_shot = [], [], [], []
     nxs, nws = [], []
-    for s in range(c["ns"]):
-        nxs.append(int(rng.choice(c["widths"])))
-        nws.append(int(rng.integers(32, 193)))
-        f0 = 2.0 + 3.0 * rng.random()
-        cx = int(rng.integers(nxs[-1] // 2, W0 - nxs[-1] // 2 + 1))
-        win = slow[:, cx - nxs[-1] // 2: cx - nxs[-1] // 2 + nxs[-1]]
+    for s, (x0, nx, freqs) in enumerate(survey):
+        nxs.append(nx)
+        nws.append(len(freqs))
+        win = slow[:, x0:x0 + nx]
         models.append(wemfwi.Model(np.ascontiguousarray(win), 10.0, 10.0, nref=1, prop="ssf", eps=0.04))
-        for i in range(nws[-1]):
+        for f in freqs:
             rows_m.append(s)
-            rows_f.append(float(np.float32(f0 + 0.1 * i)))
+            rows_f.append(f)
             rows_shot.append(s)
     n = len(rows_m)
     src, out = [], []
@@ -399,6 +436,91 @@ def uneven_batch_bench(sample_rows: int):
             "per_row_sample": f"{len(sample)} of {n} rows timed ({t_r * len(sample) / n:.2f} s), scaled",
             "speedup_vs_row_loop": round(t_r / t_b, 1), "speedup_vs_shot_loop": round(t_s / t_b, 2),
             "bitwise_equal_on_sample": equal}
+
+
+UNEVEN_NH = 11  # subsurface offsets of the uneven migration
+
+
+def uneven_image_bench(sample_rows: int):
+    """Subsurface-offset migration (nh = 11) of the uneven survey: every
+    (shot, frequency) row of the 64 shots in one call
+    (wemfwi_uneven_offset_image_adjoint: batched Down and Up^H sweeps, FP64
+    image) against one call per (shot, frequency) row on a sample of rows
+    (scaled to all rows); the sampled rows' image from the per-row calls is
+    compared with one batched call over the same rows."""
+    import torch
+    import wemfwi
+    from wemfwi.ops import ImageShot, uneven_offset_image_adjoint
+    c = UNEVEN
+    nz, W0, nh = c["nz"], c["width"], UNEVEN_NH
+    slow, survey = uneven_survey()
+    rng = np.random.default_rng(55)
+    dev = torch.device("cuda:0")
+    shots, parts = [], []
+    for (x0, nx, freqs) in survey:
+        m = wemfwi.Model(np.ascontiguousarray(slow[:, x0:x0 + nx]), 10.0, 10.0, nref=1, prop="ssf", eps=0.04)
+        rix = np.arange(0, nx, 2)
+        riz = np.zeros_like(rix)
+        nw = len(freqs)
+        data = torch.from_numpy((rng.standard_normal((len(rix), nw)) + 1j * rng.standard_normal((len(rix), nw)))
+                                .astype(np.complex64)).to(dev)
+        wav = ricker_spectrum(freqs)
+        shots.append(ImageShot(m, x0, freqs, wav, 0, nx // 2, riz, rix, data))
+        parts.append((m, x0, nx, freqs, wav, riz, rix, data))
+    rows = [(s, i) for s in range(len(survey)) for i in range(len(survey[s][2]))]
+    upd = 2.0 * sum(survey[s][1] * (nz - 1) for s, _ in rows)
+    img = torch.zeros((nh, nz, W0), dtype=torch.complex64, device=dev)
+    uneven_offset_image_adjoint(shots, nh, img)  # warmup (plans, phase tables)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    uneven_offset_image_adjoint(shots, nh, img)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    # one call per (shot, frequency) row, on a sample
+    stride = max(1, len(rows) // sample_rows)
+    sample = rows[::stride][:sample_rows]
+    one = []
+    for s, i in sample:
+        m, x0, nx, freqs, wav, riz, rix, data = parts[s]
+        one.append(ImageShot(m, x0, [freqs[i]], [wav[i]], 0, nx // 2, riz, rix, data[:, i:i + 1].contiguous()))
+    loop_img = torch.zeros_like(img)
+
+    def loop():
+        loop_img.zero_()
+        for sh in one:
+            uneven_offset_image_adjoint([sh], nh, loop_img, add=True)
+
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loop()
+    torch.cuda.synchronize()
+    t_loop_sample = time.perf_counter() - t0
+    # the same rows batched in one call
+    by_shot = {}
+    for s, i in sample:
+        by_shot.setdefault(s, []).append(i)
+    grouped = []
+    for s, ii in by_shot.items():
+        m, x0, nx, freqs, wav, riz, rix, data = parts[s]
+        grouped.append(ImageShot(m, x0, [freqs[i] for i in ii], [wav[i] for i in ii], 0, nx // 2, riz, rix,
+                                 data[:, ii].contiguous()))
+    bimg = torch.zeros_like(img)
+    uneven_offset_image_adjoint(grouped, nh, bimg)
+    torch.cuda.synchronize()
+    rel = float(torch.linalg.vector_norm((loop_img - bimg).to(torch.complex128)) /
+                torch.linalg.vector_norm(bimg.to(torch.complex128)))
+    t_loop = t_loop_sample * len(rows) / len(sample)
+    for m, *_ in parts:
+        m.close()
+    return {"config": f"ssf ns={len(survey)} nz={nz} nx_s in {{256,512,1024,2048}} nw_s in 32..192, "
+                      f"{len(rows)} rows, nh={nh}, image {nh}x{nz}x{W0} (FP64 reduction)",
+            "updates": upd, "batched_ms": round(t_b * 1e3, 2),
+            "batched_gupdates_per_s": round(upd / t_b / 1e9, 2),
+            "per_row_loop_ms": round(t_loop * 1e3, 1),
+            "per_row_sample": f"{len(sample)} of {len(rows)} rows timed ({t_loop_sample:.2f} s), scaled",
+            "speedup_vs_row_loop": round(t_loop / t_b, 1),
+            "rel_err_row_loop_vs_batched_on_sample": rel}
 
 
 def measured_traffic(config: str, kernel: str):
@@ -540,12 +662,22 @@ def main():
                      "kernel": "oneway_kernel", "kernel_ms": round(avg, 3),
                      "phase_table_ms": round(t_ms / max(t_n, 1), 3)},
     }
+    # the same kernel against the FP32 VALU roofline (PSPI with 4 references
+    # sits near the ridge: both fractions are reported)
+    flops = sweep_flops(slow, cfg["nref"], cfg["prop"], nrows)
+    tfl = flops / (avg / 1e3) / 1e12
+    line["roofline"]["valu"] = {"achieved": round(tfl, 2), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": round(tfl / VALU_PEAK_TFLOPS, 4),
+                                "frac_of_scalar_fp32": round(tfl / VALU_SCALAR_TFLOPS, 4),
+                                "flops_per_launch": flops,
+                                "flop_per_byte": round(flops / (BYTES_PER_UPDATE * upd_sweep), 2)}
     line["fwd_gupdates_per_s"] = round(upd_sweep / t_fwd / 1e9, 3)
     line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
     if args.fwi_sample > 0:
         line["fwi_gradient"] = fwi_gradient_bench(args.fwi_sample, args.fd_directions)
     if args.uneven_sample > 0:
         line["uneven_batch"] = uneven_batch_bench(args.uneven_sample)
+        line["uneven_offset_image"] = uneven_image_bench(args.uneven_sample)
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)
     print(json.dumps(line), flush=True)

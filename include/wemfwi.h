@@ -332,6 +332,33 @@ int wemfwi_offset_image_adjoint(wemfwi_model* model, wemfwi_survey* survey, int 
                                 wemfwi_complex* image, const wemfwi_complex* data, int add,
                                 void* stream);
 
+/* Subsurface-offset migration of shots with uneven apertures and bands (no
+ * single counterpart in the reference, which migrates one (shot, frequency)
+ * task at a time on one grid, ext_born_refl.cpp:60-110 with the image of
+ * wemfwi_offset_image_adjoint): shot s propagates on its own background
+ * `model` (nz x nx_s, its aperture), placed at image column x0; its rows are
+ * its nw frequencies.  Per row: D = Down(source spread ngs at (src_iz,
+ * src_ix)), R = Up^H(receivers at (rec_iz, rec_ix) <- data[r][i]), and
+ * image[h][z][x0 + x] += conj(D(z, x-h)) R(z, x+h) over h = -(nh/2)..nh/2,
+ * nh odd <= 41.  All rows of the call are swept in batched launches; the
+ * image (device [nh][nz][nx_image]) is summed in FP64, shot by shot in
+ * order, rows in order.  Frequencies, wavelet and receiver positions are
+ * host arrays; data is a device array [nr][nw] per shot. */
+typedef struct wemfwi_image_shot {
+  wemfwi_model* model;
+  int x0;
+  int nw;
+  const float* freq_hz;
+  const wemfwi_complex* wavelet;
+  int src_iz, src_ix;
+  int nr;
+  const int* rec_iz;
+  const int* rec_ix;
+  const wemfwi_complex* data;
+} wemfwi_image_shot;
+int wemfwi_uneven_offset_image_adjoint(const wemfwi_image_shot* shots, int ns, int ngs, int nh, int nz,
+                                       int nx_image, wemfwi_complex* image, int add, void* stream);
+
 /* ---------------------------------------------------------------- extended FWI
  * Operators of the frequency-extended objective of scripts/efwi.py:160-190
  * (operator/src/python/Operator.py:228-292,619-775), on device models

@@ -564,27 +564,6 @@ static int batch_streams(hipStream_t (&out)[4]) {
 // group with the largest LDS footprint first so that its workgroups claim
 // compute units before the smaller ones fill the rest.  Descriptors, depth
 // tables and scratch rows stay on the device between runs.
-struct BatchPlan {
-  struct Group {
-    int N, off, n;
-    StepGeom g;
-    int nz;
-    const float2* tw;
-  };
-  std::vector<Model*> models;          // distinct models
-  std::vector<unsigned> gens;          // their phase-table generations at planning
-  std::vector<Group> groups;
-  Workspace desc, scratch;
-  const SweepRow* d_rows = nullptr;
-  const DepthTables* d_tabs = nullptr;
-  ~BatchPlan() {
-    desc.release();
-    scratch.release();
-  }
-  int build(int nrows, Model* const* ms, const float* freq_hz, const cf* const* in, cf* const* out,
-            hipStream_t s);
-  int run(int up, int adjoint, hipStream_t s);
-};
 
 int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf* const* in, cf* const* out,
                      hipStream_t s) {

@@ -100,6 +100,34 @@ struct Model {
               std::complex<float>* out, int add, hipStream_t s);
 };
 
+// Sweeps of rows with their own models (widths, bands) planned once: rows
+// grouped by kernel variant (largest FFT length first), descriptors and
+// scratch rows uploaded; run() launches the groups on side streams.
+struct BatchPlan {
+  struct Group {
+    int N, off, n;
+    StepGeom g;
+    int nz;
+    const float2* tw;
+  };
+  std::vector<Model*> models;          // distinct models
+  std::vector<unsigned> gens;          // their phase-table generations at planning
+  std::vector<Group> groups;
+  Workspace desc, scratch;
+  const SweepRow* d_rows = nullptr;
+  const DepthTables* d_tabs = nullptr;
+  ~BatchPlan() {
+    desc.release();
+    scratch.release();
+  }
+  int build(int nrows, Model* const* ms, const float* freq_hz, const std::complex<float>* const* in,
+            std::complex<float>* const* out, hipStream_t s);
+  int run(int up, int adjoint, hipStream_t s);
+};
+
+// cos^2 source/receiver spread weights of half-width ng (Injection.h:53-65)
+std::vector<float> cos2_window(int ng);
+
 }  // namespace wemfwi
 
 struct wemfwi_model {

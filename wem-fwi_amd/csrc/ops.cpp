@@ -24,7 +24,7 @@ namespace wemfwi {
 
 // ------------------------------------------------------------------ survey
 
-static std::vector<float> cos2_window(int ng) {
+std::vector<float> cos2_window(int ng) {
   // Injection.h:53-65 (cos^2 spread; float arithmetic as the reference)
   std::vector<float> g(2 * ng + 1, 1.f);
   if (ng == 0) return g;

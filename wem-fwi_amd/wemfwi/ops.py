@@ -189,6 +189,60 @@ def offset_image_adjoint(model: Model, survey: Survey, nh: int, image, data, add
     return image
 
 
+class ImageShotDesc(C.Structure):
+    _fields_ = [
+        ("model", C.c_void_p), ("x0", C.c_int), ("nw", C.c_int), ("freq_hz", C.POINTER(C.c_float)),
+        ("wavelet", C.c_void_p), ("src_iz", C.c_int), ("src_ix", C.c_int), ("nr", C.c_int),
+        ("rec_iz", C.POINTER(C.c_int)), ("rec_ix", C.POINTER(C.c_int)), ("data", C.c_void_p),
+    ]
+
+
+class ImageShot:
+    """One shot of an uneven migration: its background `model` on its
+    aperture (nz x nx_s), placed at image column x0; its frequencies and
+    source spectrum; its source point and receivers on the aperture grid; its
+    data, a CUDA complex64 tensor [nr][nw]."""
+
+    def __init__(self, model: Model, x0: int, freq_hz, wavelet, src_iz: int, src_ix: int, rec_iz, rec_ix,
+                 data: torch.Tensor):
+        self.model = model
+        self.x0 = int(x0)
+        self.freq = np.ascontiguousarray(np.asarray(freq_hz, dtype=np.float32))
+        self.wavelet = np.ascontiguousarray(np.asarray(wavelet, dtype=np.complex64))
+        self.src = (int(src_iz), int(src_ix))
+        self.rec_iz, _ = _ip(rec_iz)
+        self.rec_ix, _ = _ip(rec_ix)
+        if len(self.wavelet) != len(self.freq) or len(self.rec_iz) != len(self.rec_ix):
+            raise ValueError("ImageShot: frequency / wavelet or receiver list sizes differ")
+        _dev_c64(data, "data")
+        if tuple(data.shape) != (len(self.rec_iz), len(self.freq)):
+            raise ValueError("ImageShot: data must be [nr][nw]")
+        self.data = data
+
+    def desc(self) -> ImageShotDesc:
+        return ImageShotDesc(model=self.model._h, x0=self.x0, nw=len(self.freq),
+                             freq_hz=self.freq.ctypes.data_as(C.POINTER(C.c_float)),
+                             wavelet=self.wavelet.ctypes.data, src_iz=self.src[0], src_ix=self.src[1],
+                             nr=len(self.rec_iz), rec_iz=self.rec_iz.ctypes.data_as(C.POINTER(C.c_int)),
+                             rec_ix=self.rec_ix.ctypes.data_as(C.POINTER(C.c_int)), data=_ptr(self.data))
+
+
+def uneven_offset_image_adjoint(shots: Sequence[ImageShot], nh: int, image: torch.Tensor, ngs: int = 0,
+                                add: bool = False, stream=None) -> torch.Tensor:
+    """Subsurface-offset migration of shots with their own apertures and
+    frequency bands into image [nh][nz][nx_image] (CUDA complex64): all
+    (shot, frequency) rows swept in batched launches, the image summed in
+    FP64 shot by shot."""
+    _dev_c64(image, "image")
+    if image.dim() != 3 or image.shape[0] != nh:
+        raise ValueError("image must be [nh][nz][nx_image]")
+    arr = (ImageShotDesc * max(1, len(shots)))(*[sh.desc() for sh in shots])
+    check(lib.wemfwi_uneven_offset_image_adjoint(C.cast(arr, C.c_void_p), len(shots), int(ngs), int(nh), int(image.shape[1]),
+                                                 int(image.shape[2]), _ptr(image), int(add),
+                                                 _stream_ptr(stream)))
+    return image
+
+
 def shotwem_forward(models: Sequence[Model], survey: Survey, data: torch.Tensor, add: bool = False,
                     stream=None):
     """WEM with one background model per shot (conformal-map grids)."""
