@@ -73,33 +73,51 @@ __global__ void rec_rows_kernel(const RecRow* __restrict__ rows, int nrows) {
 
 // img[h][z][x0 + x] += sum over the nrows rows of one shot (in order) of
 // conj(D[z][x-h]) R[z][x+h], h = j - nh/2, terms with x-h or x+h outside
-// [0, nx) absent; one thread per (z, x) holds the nh lags in FP64.
+// [0, nx) absent.  A workgroup owns one depth z and kTX consecutive x (one
+// per thread, nh lags in FP64 registers); per row the D and R segments with
+// their nh/2 halos are staged in LDS once, so every sample is read from HBM
+// once per row instead of once per lag.
+constexpr int kTX = 256;
+constexpr int kHalo = 20;  // nh <= 41
 template <int NHM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTX)
     offset_image_rows_kernel(double2* __restrict__ img, long long hstride, int nxi, int x0,
                              const float2* __restrict__ D, const float2* __restrict__ R, long long rstride,
                              int nrows, int nz, int nx, int nh) {
-  const long long n = (long long)nz * nx;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int x = (int)(q % nx);
-    const int z = (int)(q / nx);
-    double ax[NHM], ay[NHM];
+  __shared__ float2 sd[kTX + 2 * kHalo], sr[kTX + 2 * kHalo];
+  const int z = blockIdx.y;
+  const int xb = blockIdx.x * kTX;
+  const int t = threadIdx.x;
+  const int x = xb + t;
+  const int hh = nh / 2;
+  const int seg = kTX + 2 * hh;
+  double ax[NHM], ay[NHM];
 #pragma unroll
-    for (int j = 0; j < NHM; ++j) ax[j] = ay[j] = 0.0;
-    for (int r = 0; r < nrows; ++r) {
-      const float2* d = D + r * rstride + (long long)z * nx;
-      const float2* s = R + r * rstride + (long long)z * nx;
+  for (int j = 0; j < NHM; ++j) ax[j] = ay[j] = 0.0;
+  for (int r = 0; r < nrows; ++r) {
+    const float2* d = D + r * rstride + (long long)z * nx;
+    const float2* s = R + r * rstride + (long long)z * nx;
+    for (int i = t; i < seg; i += kTX) {
+      const int xx = xb - hh + i;
+      const bool in = xx >= 0 && xx < nx;
+      sd[i] = in ? ldg(d, (size_t)xx) : make_float2(0.f, 0.f);
+      sr[i] = in ? ldg(s, (size_t)xx) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (x < nx) {
 #pragma unroll
       for (int j = 0; j < NHM; ++j) {
-        const int h = j - nh / 2;
+        const int h = j - hh;
         if (j < nh && x - h >= 0 && x - h < nx && x + h >= 0 && x + h < nx) {
-          const float2 v = cmulc(ldg(s, (size_t)(x + h)), ldg(d, (size_t)(x - h)));
+          const float2 v = cmulc(sr[t + hh + h], sd[t + hh - h]);
           ax[j] += v.x;
           ay[j] += v.y;
         }
       }
     }
+    __syncthreads();
+  }
+  if (x < nx) {
 #pragma unroll
     for (int j = 0; j < NHM; ++j) {
       if (j >= nh) break;
@@ -113,11 +131,9 @@ __global__ void __launch_bounds__(256)
 template <int NHM>
 hipError_t launch_oir(double2* img, long long hstride, int nxi, int x0, const float2* D, const float2* R,
                       long long rstride, int nrows, int nz, int nx, int nh, hipStream_t s) {
-  const long long n = (long long)nz * nx;
-  long long b = (n + 255) / 256;
-  if (b > 65536) b = 65536;
-  hipLaunchKernelGGL((offset_image_rows_kernel<NHM>), dim3((unsigned)b), dim3(256), 0, s, img, hstride, nxi, x0, D,
-                     R, rstride, nrows, nz, nx, nh);
+  dim3 grid((unsigned)((nx + kTX - 1) / kTX), (unsigned)nz);
+  hipLaunchKernelGGL((offset_image_rows_kernel<NHM>), grid, dim3(kTX), 0, s, img, hstride, nxi, x0, D, R, rstride,
+                     nrows, nz, nx, nh);
   return hipGetLastError();
 }
 
@@ -125,12 +141,12 @@ hipError_t launch_offset_image_rows(double2* img, long long hstride, int nxi, in
                                    const float2* R, long long rstride, int nrows, int nz, int nx, int nh,
                                    hipStream_t s) {
   if (nrows <= 0 || nz <= 0 || nx <= 0) return hipSuccess;
+  if (nh > 2 * kHalo + 1 || nz > 65535) return hipErrorInvalidValue;
   if (nh <= 1) return launch_oir<1>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
   if (nh <= 3) return launch_oir<3>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
   if (nh <= 11) return launch_oir<11>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
   if (nh <= 21) return launch_oir<21>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
-  if (nh <= 41) return launch_oir<41>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
-  return hipErrorInvalidValue;
+  return launch_oir<41>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
 }
 
 template <typename T>
