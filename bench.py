@@ -671,6 +671,8 @@ def main():
                                 "frac_of_scalar_fp32": round(tfl / VALU_SCALAR_TFLOPS, 4),
                                 "flops_per_launch": flops,
                                 "flop_per_byte": round(flops / (BYTES_PER_UPDATE * upd_sweep), 2)}
+    # the sweep autotuner's radix-plan choice for this workload's launches
+    line["autotune"] = [e for e in wemfwi.autotune_report() if e["N"] == nx]
     line["fwd_gupdates_per_s"] = round(upd_sweep / t_fwd / 1e9, 3)
     line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
     if args.fwi_sample > 0:

@@ -1,0 +1,96 @@
+"""The sweep autotuner: every radix plan it may choose (FftPlan<N, K>, forced
+with WEMFWI_FFT_PLAN) matches the FP64 oracle on the Down/Up recursions
+(relative L2 <= 1e-5) for full and padded rows, SSF and PSPI, forward and
+adjoint; the tuner records one decision per (length, variant) with a timing
+for every plan and uses the fastest."""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import wem_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+CASES = [
+    # nx, tap, pad(taper), nref, prop     -> N, plans
+    (1024, 0, 0, 1, "ssf"),                # 1024, full row
+    (1000, 12, 8, 2, "pspi"),              # 1024, padded
+    (2048, 0, 0, 2, "pspi"),               # 2048, full row
+    (2040, 4, 0, 1, "ssf"),                # 2048, padded
+]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _slow(nz, nx, seed):
+    rng = np.random.default_rng(seed)
+    z = np.linspace(0, 1, nz)[:, None]
+    v = (1500 + 2000 * z) * (1 + 0.05 * rng.standard_normal((nz, nx)))
+    return (1 / v ** 2).astype(np.complex64)
+
+
+@pytest.fixture
+def forced_plan():
+    old = os.environ.get("WEMFWI_FFT_PLAN")
+    yield
+    if old is None:
+        os.environ.pop("WEMFWI_FFT_PLAN", None)
+    else:
+        os.environ["WEMFWI_FFT_PLAN"] = old
+
+
+@pytest.mark.parametrize("nx,tap,pad,nref,prop", CASES)
+def test_every_plan_matches_oracle(nx, tap, pad, nref, prop, forced_plan):
+    import torch
+    from wemfwi import Model
+    nz = 12
+    s = _slow(nz, nx, 1)
+    m = Model(s, 10.0, 10.0, tap=tap, pad=pad, nref=nref, prop=prop, eps=0.04)
+    p = O.StepParams(nx=nx, dx=10.0, dz=10.0, tap=tap, pad=pad, nref=nref, eps=0.04, prop=prop)
+    ref = O.refsampler(s, nref)
+    rng = np.random.default_rng(2)
+    W = np.zeros((2, nz, nx), dtype=np.complex64)
+    W[:, 0, nx // 3: nx // 3 + 7] = rng.standard_normal((2, 7)) + 1j * rng.standard_normal((2, 7))
+    W[:, nz - 1, nx // 2: nx // 2 + 7] = rng.standard_normal((2, 7)) + 1j * rng.standard_normal((2, 7))
+    freqs = np.array([4.0, 23.0], dtype=np.float32)
+    nplans = 4 if nx + 2 * tap == 1024 else 3
+    for k in range(nplans):
+        os.environ["WEMFWI_FFT_PLAN"] = str(k)
+        for up, adjoint in ((False, False), (False, True), (True, False), (True, True)):
+            print(f"plan {k} nk={nx + 2 * tap} up={up} adjoint={adjoint}", flush=True)
+            t = torch.from_numpy(W).cuda()
+            m.oneway(up, adjoint, freqs, t)
+            fn = {(False, False): O.down_fwd, (False, True): O.down_adj,
+                  (True, False): O.up_fwd, (True, True): O.up_adj}[(up, adjoint)]
+            got = t.cpu().numpy()
+            for i in range(2):
+                want = fn(W[i].astype(complex), 2 * np.pi * float(freqs[i]), s, ref, p)
+                assert rel(got[i], want) < TOL, (k, up, adjoint, i, rel(got[i], want))
+
+
+def test_tuner_records_one_decision_per_variant(forced_plan):
+    import torch
+    import wemfwi
+    from wemfwi import Model
+    os.environ.pop("WEMFWI_FFT_PLAN", None)
+    nz, nx = 20, 2044   # padded 2048 row, PSPI adjoint: a key no other test of this file tunes
+    s = _slow(nz, nx, 3)
+    m = Model(s, 10.0, 10.0, tap=2, nref=2, prop="pspi", eps=0.04)
+    W = torch.zeros((64, nz, nx), dtype=torch.complex64, device="cuda")
+    W[:, 0, 100] = 1.0
+    freqs = np.linspace(3, 20, 64).astype(np.float32)
+    m.oneway(False, True, freqs, W)
+    m.oneway(False, True, freqs, W)
+    rep = [e for e in wemfwi.autotune_report() if e["N"] == 2048 and e["full"] == 0 and e["ssf"] == 0
+           and e["adjoint"] == 1]
+    assert len(rep) == 1
+    e = rep[0]
+    assert len(e["ms"]) == 3 and all(v > 0 for v in e["ms"])
+    assert e["plan"] == int(np.argmin(e["ms"]))
+    assert np.isfinite(float(W.abs().max()))

@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -27,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
 def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".", "_") + ".o")
         if src in PLAIN_CXX:
@@ -35,10 +36,19 @@ def build(verbose: bool = False) -> str:
                    "-c", os.path.join(CSRC, src), "-o", obj]
         else:
             cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmds.append(cmd)
+        objs.append(obj)
+    # sources compile independently: run them side by side (MAX_JOBS caps it)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1), 16))
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(run, c) for c in cmds]:
+            f.result()
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

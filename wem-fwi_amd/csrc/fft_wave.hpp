@@ -151,51 +151,82 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
 
 // ---------------------------------------------------------------- plans
 
-// Radix plans (every radix divides E = N/64; at most 4 passes).
-template <int N>
+// Radix plans (every radix divides E = N/64).  K numbers the alternative
+// plans of a length that the launch autotuner times (K = 0: the default).
+template <int N, int K = 0>
 struct FftPlan;
 template <>
-struct FftPlan<128> {
+struct FftPlan<128, 0> {
   static constexpr int P = 7;
   static constexpr int R[P] = {2, 2, 2, 2, 2, 2, 2};
 };
 template <>
-struct FftPlan<256> {
+struct FftPlan<256, 0> {
   static constexpr int P = 4;
   static constexpr int R[P] = {4, 4, 4, 4};
 };
 template <>
-struct FftPlan<512> {
+struct FftPlan<512, 0> {
   static constexpr int P = 3;
   static constexpr int R[P] = {8, 8, 8};
 };
 template <>
-struct FftPlan<1024> {
+struct FftPlan<1024, 0> {
   static constexpr int P = 3;
   static constexpr int R[P] = {16, 4, 16};
 };
 template <>
-struct FftPlan<2048> {
+struct FftPlan<1024, 1> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {16, 16, 4};
+};
+template <>
+struct FftPlan<1024, 2> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {4, 16, 16};
+};
+template <>
+struct FftPlan<1024, 3> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {8, 8, 16};
+};
+template <>
+struct FftPlan<2048, 0> {
   static constexpr int P = 3;
   static constexpr int R[P] = {16, 8, 16};
 };
-
+template <>
+struct FftPlan<2048, 1> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {8, 16, 16};
+};
+template <>
+struct FftPlan<2048, 2> {
+  static constexpr int P = 3;
+  static constexpr int R[P] = {16, 16, 8};
+};
+// number of plans of a length
 template <int N>
+struct FftPlans {
+  static constexpr int value = N == 1024 ? 4 : (N == 2048 ? 3 : 1);
+};
+
+template <int N, int K = 0>
 __host__ __device__ constexpr int plan_ns(int p) {
   int ns = 1;
-  for (int q = 0; q < p; ++q) ns *= FftPlan<N>::R[q];
+  for (int q = 0; q < p; ++q) ns *= FftPlan<N, K>::R[q];
   return ns;
 }
 // offset of pass p's twiddles in the concatenated table
-template <int N>
+template <int N, int K = 0>
 __host__ __device__ constexpr int plan_tw_off(int p) {
   int off = 0;
-  for (int q = 0; q < p; ++q) off += (FftPlan<N>::R[q] - 1) * plan_ns<N>(q);
+  for (int q = 0; q < p; ++q) off += (FftPlan<N, K>::R[q] - 1) * plan_ns<N, K>(q);
   return off;
 }
-template <int N>
+template <int N, int K = 0>
 __host__ __device__ constexpr int plan_tw_size() {
-  return plan_tw_off<N>(FftPlan<N>::P);
+  return plan_tw_off<N, K>(FftPlan<N, K>::P);
 }
 
 // ---------------------------------------------------------------- passes
@@ -209,12 +240,12 @@ __device__ __forceinline__ void wave_lds_order() {
 
 // Pass p of the plan.  FROM_LDS: inputs come from LDS (else from x in the
 // strided natural layout); TO_LDS: outputs go to LDS (else to x).
-template <int N, int p, bool INV, bool FROM_LDS, bool TO_LDS>
+template <int N, int K, int p, bool INV, bool FROM_LDS, bool TO_LDS>
 __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __restrict__ buf,
                                               const float2* __restrict__ tw, int t) {
   constexpr int E = N / 64;
-  constexpr int R = FftPlan<N>::R[p];
-  constexpr int NS = plan_ns<N>(p);
+  constexpr int R = FftPlan<N, K>::R[p];
+  constexpr int NS = plan_ns<N, K>(p);
   constexpr int NB = N / R;
   constexpr int BPL = E / R;
   static_assert(BPL * R == E, "radix must divide the per-lane count");
@@ -229,7 +260,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
     wave_lds_order();
   }
   // twiddle base: entry (r-1)*NS + (j & (NS-1)), j = t + 64 b
-  const float2* twb = tw + plan_tw_off<N>(p) + (NS >= 64 ? t : (t & (NS - 1)));
+  const float2* twb = tw + plan_tw_off<N, K>(p) + (NS >= 64 ? t : (t & (NS - 1)));
   // output base of butterfly j = t + 64 b
   int obase;
   if constexpr (NS == 1) obase = R * t + ((R * t) >> 4);
@@ -290,20 +321,20 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
   if constexpr (TO_LDS) wave_lds_order();
 }
 
-template <int N, bool INV, int p>
+template <int N, int K, bool INV, int p>
 __device__ __forceinline__ void fft_passes(float2 (&x)[N / 64], float2* buf, const float2* tw, int t) {
-  constexpr int P = FftPlan<N>::P;
+  constexpr int P = FftPlan<N, K>::P;
   if constexpr (p < P) {
-    stockham_pass<N, p, INV, (p > 0), (p < P - 1)>(x, buf, tw, t);
-    fft_passes<N, INV, p + 1>(x, buf, tw, t);
+    stockham_pass<N, K, p, INV, (p > 0), (p < P - 1)>(x, buf, tw, t);
+    fft_passes<N, K, INV, p + 1>(x, buf, tw, t);
   }
 }
 
 // Unnormalised DFT (forward: exp(-2 pi i jk/N); inverse: exp(+2 pi i jk/N))
 // of the row held in x (strided natural layout in and out).
-template <int N, bool INV>
+template <int N, bool INV, int K = 0>
 __device__ __forceinline__ void fft_row(float2 (&x)[N / 64], float2* buf, const float2* tw, int t) {
-  fft_passes<N, INV, 0>(x, buf, tw, t);
+  fft_passes<N, K, INV, 0>(x, buf, tw, t);
 }
 
 // ---------------------------------------------------------------- any length
@@ -449,17 +480,17 @@ __device__ __forceinline__ void fft_row_gen(float2 (&x)[N / 64], float2* buf, co
   wave_lds_order();
 }
 
-// Fills the concatenated per-pass twiddle table of FftPlan<N> (double
+// Fills the concatenated per-pass twiddle table of FftPlan<N, K> (double
 // precision, rounded once).
-template <int N>
+template <int N, int K = 0>
 __global__ void twiddle_table_kernel(float2* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= plan_tw_size<N>()) return;
+  if (i >= plan_tw_size<N, K>()) return;
   int p = 0;
-  while (p + 1 < FftPlan<N>::P && plan_tw_off<N>(p + 1) <= i) ++p;
-  const int R = FftPlan<N>::R[p];
-  const int NS = plan_ns<N>(p);
-  const int k = i - plan_tw_off<N>(p);
+  while (p + 1 < FftPlan<N, K>::P && plan_tw_off<N, K>(p + 1) <= i) ++p;
+  const int R = FftPlan<N, K>::R[p];
+  const int NS = plan_ns<N, K>(p);
+  const int k = i - plan_tw_off<N, K>(p);
   const int r = k / NS + 1;
   const int jm = k % NS;
   const double a = -2.0 * 3.14159265358979323846 * (double)(jm * r) / (double)(NS * R);

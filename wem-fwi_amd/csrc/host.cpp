@@ -224,20 +224,23 @@ int gen_twiddles(const FftGen& f, int N, const float2** out) {
   return WEMFWI_OK;
 }
 
-int twiddles(int n, const float2** out) {
+int twiddles(int n, const float2** out) { return twiddles(n, 0, out); }
+
+int twiddles(int n, int plan, const float2** out) {
   auto& c = twiddle_cache();
   int dev = 0;
   (void)hipGetDevice(&dev);
-  const int key = n * 64 + dev;
+  const int key = (n * 8 + plan) * 64 + dev;
   auto it = c.find(key);
   if (it != c.end()) {
     *out = it->second;
     return WEMFWI_OK;
   }
   float2* d = nullptr;
-  int rc = dev_alloc(&d, (size_t)twiddle_table_size(n), "twiddle table");
+  if (twiddle_table_size(n, plan) <= 0) return fail(WEMFWI_ERR_ARG, "twiddles: no such FFT plan");
+  int rc = dev_alloc(&d, (size_t)twiddle_table_size(n, plan), "twiddle table");
   if (rc) return rc;
-  hipError_t e = launch_twiddles(n, d, nullptr);
+  hipError_t e = launch_twiddles(n, plan, d, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_fail(e, "twiddle table");
   c[key] = d;
@@ -839,6 +842,16 @@ int wemfwi_timing(int enable) {
   g_timing = enable != 0;
   if (enable) timing_table().clear();
   return WEMFWI_OK;
+}
+
+int wemfwi_autotune_report(char* buf, int cap) {
+  const std::string r = autotune_report();
+  if (buf && cap > 0) {
+    const size_t n = std::min((size_t)cap - 1, r.size());
+    std::memcpy(buf, r.data(), n);
+    buf[n] = 0;
+  }
+  return (int)r.size();
 }
 
 int wemfwi_timing_get(const char* kernel, double* total_ms, long* count) {

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "sweep.hpp"
 
 namespace wemfwi {
@@ -36,8 +38,14 @@ struct StepRow {
 };
 
 bool fft_size_supported(int n);
-int twiddle_table_size(int N);
-hipError_t launch_twiddles(int N, float2* out, hipStream_t s);
+// FFT radix plans of a built length (the sweep autotuner chooses among them)
+int fft_plans(int N);
+int twiddle_table_size(int N, int plan);
+hipError_t launch_twiddles(int N, int plan, float2* out, hipStream_t s);
+// twiddle table of plan `plan` of length N on the current device (cached)
+int twiddles(int n, int plan, const float2** out);
+// JSON list of the sweep autotuner's decisions
+std::string autotune_report();
 
 hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const double2* sref,
                               int nslots, int N, int NH, double dk, double dz, double eps,

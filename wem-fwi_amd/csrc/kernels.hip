@@ -1,8 +1,14 @@
 // HIP kernels of the one-way WEM hot path for gfx950, and their launchers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+#include <vector>
 
+#include "../../include/wemfwi.h"
 #include "kernels.h"
 #include "sweep.hpp"
 
@@ -167,31 +173,54 @@ __global__ void __launch_bounds__(64 * WavesV<V>::value)
 
 // ------------------------------------------------------------------ launchers
 
-template <int N>
-static hipError_t launch_twiddles_n(float2* out, hipStream_t s) {
-  const int n = plan_tw_size<N>();
-  hipLaunchKernelGGL((twiddle_table_kernel<N>), dim3((n + 255) / 256), dim3(256), 0, s, out);
+template <int N, int K = 0>
+static int tw_size_k(int k) {
+  if constexpr (K + 1 < FftPlans<N>::value) {
+    if (k != K) return tw_size_k<N, K + 1>(k);
+  }
+  return k == K ? plan_tw_size<N, K>() : 0;
+}
+
+template <int N, int K = 0>
+static hipError_t launch_twiddles_k(int k, float2* out, hipStream_t s) {
+  if constexpr (K + 1 < FftPlans<N>::value) {
+    if (k != K) return launch_twiddles_k<N, K + 1>(k, out, s);
+  }
+  if (k != K) return hipErrorInvalidValue;
+  const int n = plan_tw_size<N, K>();
+  hipLaunchKernelGGL((twiddle_table_kernel<N, K>), dim3((n + 255) / 256), dim3(256), 0, s, out);
   return hipGetLastError();
 }
 
-int twiddle_table_size(int N) {
+int fft_plans(int N) {
   switch (N) {
-    case 128: return plan_tw_size<128>();
-    case 256: return plan_tw_size<256>();
-    case 512: return plan_tw_size<512>();
-    case 1024: return plan_tw_size<1024>();
-    case 2048: return plan_tw_size<2048>();
+    case 128: return FftPlans<128>::value;
+    case 256: return FftPlans<256>::value;
+    case 512: return FftPlans<512>::value;
+    case 1024: return FftPlans<1024>::value;
+    case 2048: return FftPlans<2048>::value;
     default: return 0;
   }
 }
 
-hipError_t launch_twiddles(int N, float2* out, hipStream_t s) {
+int twiddle_table_size(int N, int k) {
   switch (N) {
-    case 128: return launch_twiddles_n<128>(out, s);
-    case 256: return launch_twiddles_n<256>(out, s);
-    case 512: return launch_twiddles_n<512>(out, s);
-    case 1024: return launch_twiddles_n<1024>(out, s);
-    case 2048: return launch_twiddles_n<2048>(out, s);
+    case 128: return tw_size_k<128>(k);
+    case 256: return tw_size_k<256>(k);
+    case 512: return tw_size_k<512>(k);
+    case 1024: return tw_size_k<1024>(k);
+    case 2048: return tw_size_k<2048>(k);
+    default: return 0;
+  }
+}
+
+hipError_t launch_twiddles(int N, int k, float2* out, hipStream_t s) {
+  switch (N) {
+    case 128: return launch_twiddles_k<128>(k, out, s);
+    case 256: return launch_twiddles_k<256>(k, out, s);
+    case 512: return launch_twiddles_k<512>(k, out, s);
+    case 1024: return launch_twiddles_k<1024>(k, out, s);
+    case 2048: return launch_twiddles_k<2048>(k, out, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -224,6 +253,147 @@ static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTa
   return hipGetLastError();
 }
 
+// the four register-FFT variants of radix plan K
+template <int N, int K>
+static hipError_t launch_oneway_k(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                                  int step, bool adj, const float2* tw, hipStream_t s) {
+  if (g.full && g.ssf)
+    return launch_oneway_v<Var<N, true, true, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  if (g.full) return launch_oneway_v<Var<N, true, false, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  if (g.ssf) return launch_oneway_v<Var<N, false, true, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  return launch_oneway_v<Var<N, false, false, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
+}
+
+template <int N, int K = 0>
+static hipError_t launch_oneway_plan(int k, const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g,
+                                     int nz, int step, bool adj, const float2* tw, hipStream_t s) {
+  if constexpr (K + 1 < FftPlans<N>::value) {
+    if (k != K) return launch_oneway_plan<N, K + 1>(k, rows, nrows, models, g, nz, step, adj, tw, s);
+  }
+  if (k != K) return hipErrorInvalidValue;
+  return launch_oneway_k<N, K>(rows, nrows, models, g, nz, step, adj, tw, s);
+}
+
+// ------------------------------------------------------------------ autotuner
+//
+// FFT lengths with several radix plans (FftPlans<N> > 1) pick theirs by
+// measurement: the first sweep launch of each (N, full row, split step,
+// adjoint, device) copies the descriptor of its first row, builds up to 4096
+// rows (at most 1 GiB of scratch) with that row's model and frequency, runs
+// each plan over the first kTuneSteps depth steps (one warm-up launch, then
+// the fastest of three timed with HIP events) and keeps the fastest plan for
+// every later launch.  WEMFWI_FFT_PLAN=k forces plan k, WEMFWI_AUTOTUNE=0 keeps plan 0;
+// a launch inside stream capture uses the recorded choice or plan 0.
+constexpr int kTuneSteps = 32;
+
+struct TuneEntry {
+  int N, full, ssf, adj, dev, plan, rows, steps;
+  std::vector<float> ms;  // mean launch time of each plan
+};
+
+static std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+static std::vector<TuneEntry>& tune_table() {
+  static std::vector<TuneEntry> t;
+  return t;
+}
+
+static int env_int(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  return (e && *e) ? std::atoi(e) : dflt;
+}
+
+template <int N>
+static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz, int step,
+                      bool adj, hipStream_t s) {
+  const int np = FftPlans<N>::value;
+  const int forced = env_int("WEMFWI_FFT_PLAN", -1);
+  if (forced >= 0) return forced < np ? forced : 0;
+  if (np < 2 || env_int("WEMFWI_AUTOTUNE", 1) == 0) return 0;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  for (const TuneEntry& e : tune_table())
+    if (e.N == N && e.full == g.full && e.ssf == g.ssf && e.adj == (int)adj && e.dev == dev) return e.plan;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+  // a sample of rows like the first one, on scratch buffers
+  if (hipStreamSynchronize(s) != hipSuccess) return 0;
+  SweepRow r0;
+  DepthTables dt;
+  if (hipMemcpy(&r0, rows, sizeof(r0), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+  if (hipMemcpy(&dt, models + r0.model, sizeof(dt), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+  const int T = std::min(dt.nz - 1, kTuneSteps);
+  const size_t cap_rows = ((size_t)1 << 30) / ((size_t)(T + 1) * std::max(dt.nx, 1) * sizeof(float2) + 1);
+  const int R = (int)std::min<size_t>((size_t)std::min(nrows, 4096), std::max<size_t>(cap_rows, 1));
+  TuneEntry ent{N, g.full, g.ssf, (int)adj, dev, 0, R, T, {}};
+  if (T < 1 || R < 1) {
+    tune_table().push_back(ent);
+    return 0;
+  }
+  const size_t rowlen = (size_t)(T + 1) * dt.nx;
+  float2 *buf = nullptr, *scr = nullptr;
+  SweepRow* drows = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ok = hipMalloc((void**)&buf, (size_t)R * rowlen * sizeof(float2)) == hipSuccess &&
+            hipMalloc((void**)&scr, (size_t)R * N * sizeof(float2)) == hipSuccess &&
+            hipMalloc((void**)&drows, (size_t)R * sizeof(SweepRow)) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  if (ok) ok = hipMemsetAsync(buf, 0, (size_t)R * rowlen * sizeof(float2), s) == hipSuccess;
+  if (ok) {
+    std::vector<SweepRow> h(R, r0);
+    for (int i = 0; i < R; ++i) {
+      h[i].in = buf + (size_t)i * rowlen;
+      h[i].out = buf + (size_t)i * rowlen;
+      h[i].scratch = scr + (size_t)i * N;
+      h[i].recursive = 1;
+      h[i].first = step > 0 ? 0 : T;
+      h[i].nsteps = T;
+      h[i].init = nullptr;
+      h[i].zbase = 0;
+    }
+    ok = hipMemcpy(drows, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  float best = 0.f;
+  for (int k = 0; ok && k < np; ++k) {
+    const float2* tw = nullptr;
+    if (twiddles(N, k, &tw) != WEMFWI_OK) {
+      ok = false;
+      break;
+    }
+    ok = launch_oneway_plan<N>(k, drows, R, models, g, nz, step, adj, tw, s) == hipSuccess;
+    float ms = 0.f;
+    for (int it = 0; ok && it < 3; ++it) {
+      ok = hipEventRecord(e0, s) == hipSuccess &&
+           launch_oneway_plan<N>(k, drows, R, models, g, nz, step, adj, tw, s) == hipSuccess &&
+           hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+      float m1 = 0.f;
+      ok = ok && hipEventElapsedTime(&m1, e0, e1) == hipSuccess;
+      if (ok && (it == 0 || m1 < ms)) ms = m1;
+    }
+    if (!ok) break;
+    ent.ms.push_back(ms);
+    if (k == 0 || ms < best) {
+      best = ms;
+      ent.plan = k;
+    }
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    ent.plan = 0;
+    ent.ms.clear();
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(drows);
+  (void)hipFree(scr);
+  (void)hipFree(buf);
+  tune_table().push_back(ent);
+  return ent.plan;
+}
+
 template <int N>
 static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
                                   int step, bool adj, const float2* tw, hipStream_t s) {
@@ -231,10 +401,31 @@ static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTa
     if (g.ssf) return launch_oneway_v<Var<N, false, true, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
     return launch_oneway_v<Var<N, false, false, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
   }
-  if (g.full && g.ssf) return launch_oneway_v<Var<N, true, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
-  if (g.full) return launch_oneway_v<Var<N, true, false>>(rows, nrows, models, g, nz, step, adj, tw, s);
-  if (g.ssf) return launch_oneway_v<Var<N, false, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
-  return launch_oneway_v<Var<N, false, false>>(rows, nrows, models, g, nz, step, adj, tw, s);
+  const int k = tuned_plan<N>(rows, nrows, models, g, nz, step, adj, s);
+  if (k != 0 && twiddles(N, k, &tw) != WEMFWI_OK) return hipErrorInvalidValue;
+  return launch_oneway_plan<N>(k, rows, nrows, models, g, nz, step, adj, tw, s);
+}
+
+// JSON list of the autotuner's decisions so far
+std::string autotune_report() {
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  std::string out = "[";
+  for (size_t i = 0; i < tune_table().size(); ++i) {
+    const TuneEntry& e = tune_table()[i];
+    char head[256];
+    std::snprintf(head, sizeof(head),
+                  "%s{\"kernel\": \"oneway_kernel\", \"N\": %d, \"full\": %d, \"ssf\": %d, \"adjoint\": %d, "
+                  "\"device\": %d, \"rows\": %d, \"steps\": %d, \"plan\": %d, \"ms\": [",
+                  i ? ", " : "", e.N, e.full, e.ssf, e.adj, e.dev, e.rows, e.steps, e.plan);
+    out += head;
+    for (size_t k = 0; k < e.ms.size(); ++k) {
+      char v[32];
+      std::snprintf(v, sizeof(v), "%s%.4f", k ? ", " : "", e.ms[k]);
+      out += v;
+    }
+    out += "]}";
+  }
+  return out + "]";
 }
 
 hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,

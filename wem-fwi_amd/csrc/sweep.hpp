@@ -142,15 +142,17 @@ struct StepGeom {
 };
 
 // Compile-time variants of the step: FULL (no padding samples), SSF
-// (split-step correction) and GEN (FFT length g.fg.n below the N lanes of
-// the row, transformed by the runtime-planned LDS passes).
-template <int N, bool FULL_, bool SSF_, bool GEN_ = false>
+// (split-step correction), GEN (FFT length g.fg.n below the N lanes of the
+// row, transformed by the runtime-planned LDS passes) and K (radix plan of
+// the register FFT, FftPlan<N, K>).
+template <int N, bool FULL_, bool SSF_, bool GEN_ = false, int K_ = 0>
 struct Var {
   static constexpr int n = N;
   static constexpr int E = N / 64;
   static constexpr bool FULL = FULL_;
   static constexpr bool SSF = SSF_;
   static constexpr bool GEN = GEN_;
+  static constexpr int K = K_;
   static_assert(!(FULL_ && GEN_), "a row shorter than N has padding samples");
 };
 
@@ -161,14 +163,14 @@ struct LdsV {
 };
 template <class V>
 struct TwV {
-  static constexpr int value = V::GEN ? gen_tw_cap(V::n) : plan_tw_size<V::n>();
+  static constexpr int value = V::GEN ? gen_tw_cap(V::n) : plan_tw_size<V::n, V::K>();
 };
 
 // the row transform of a variant
 template <class V, bool INV>
 __device__ __forceinline__ void fft_v(float2 (&x)[V::E], float2* buf, const float2* tw, int t, const StepGeom& g) {
   if constexpr (V::GEN) fft_row_gen<V::n, INV>(x, buf, tw, t, g.fg);
-  else fft_row<V::n, INV>(x, buf, tw, t);
+  else fft_row<V::n, INV, V::K>(x, buf, tw, t);
 }
 
 // sin and cos of x for |x| < 1e5: Cody-Waite reduction by a three-part pi/2

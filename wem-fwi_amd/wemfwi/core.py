@@ -210,6 +210,17 @@ def timing(enable: bool) -> None:
     check(lib.wemfwi_timing(int(enable)))
 
 
+def autotune_report() -> list:
+    """The sweep autotuner's decisions so far: per (FFT length, full row,
+    split step, adjoint, device) the radix plan chosen and the measured mean
+    launch time of every plan (ms)."""
+    import json
+    n = lib.wemfwi_autotune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib.wemfwi_autotune_report(buf, n + 1)
+    return json.loads(buf.value.decode())
+
+
 def kernel_time(name: str):
     """(total milliseconds, launch count) of a library kernel since timing(True)."""
     ms = C.c_double()
