@@ -454,7 +454,7 @@ int wemfwi_timing(int enable);
 int wemfwi_timing_get(const char* kernel, double* total_ms, long* count);
 
 /* Decisions of the sweep autotuner so far (no counterpart in the reference,
- * whose FFTW plans are chosen by FFTW_ESTIMATE, FFT1_c.cpp:20-26): a JSON
+ * whose FFTW plans are chosen by FFTW_ESTIMATE, FFT1_c.cpp:34-38): a JSON
  * list with, per (FFT length, full row, split step, adjoint, device), the
  * plan chosen and the measured mean launch time of every radix plan.
  * Writes at most cap bytes (NUL-terminated) and returns the full length. */
