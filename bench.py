@@ -10,8 +10,11 @@ frequency x shot rows go to the GPU in one launch per sweep.
 One step = the phase tables of the 256 frequencies (Phshift::setFreq for all
 of them) + Down forward over all 4096 rows (sources -> wavefield) + Down
 adjoint over all rows (in place), i.e. 2 sweeps of 1024 x 499 x 4096 wavefield
-updates.  The metric is wavefield updates per second (x.z.w, summed over the
-shots), the reference's unit of work.
+updates.  The timed steps replay that sequence as one captured hipGraph of
+batch-plan launches; eager steps before it warm up (the sweep autotuner
+decides there) and give the per-launch kernel times of the roofline.  The
+metric is wavefield updates per second (x.z.w, summed over the shots), the
+reference's unit of work.
 
 Also reported in the same line: forward and adjoint sweep rates on their own,
 and the FWI gradient (WEM forward + residual + exact Born adjoint, 512 x 300,
@@ -603,6 +606,8 @@ def main():
         model.oneway_recursive(False, False, f_rows, S, W)
         model.oneway(False, True, f_rows, W)
 
+    # eager steps: warm-up (the sweep autotuner decides here) and the
+    # per-launch kernel timing of the roofline
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -614,6 +619,24 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wemfwi.timing(False)
+    eager_ms = (t1 - t0) * 1e3 / args.steps
+    # the timed steps: the same work (phase tables of the 256 frequencies in
+    # place, Down forward S -> W, Down adjoint in place) as one captured
+    # hipGraph of batch-plan launches, replayed
+    from wemfwi.core import BatchGraph, BatchPlan
+    rows_s = [S[i] for i in range(nrows)]
+    rows_w = [W[i] for i in range(nrows)]
+    p_fwd = BatchPlan([model] * nrows, f_rows, rows_s, rows_w)
+    p_adj = BatchPlan([model] * nrows, f_rows, rows_w, rows_w)
+    graph = BatchGraph([(p_fwd, False, False, True), (p_adj, False, True, False)])
+    for _ in range(args.warmup):
+        graph.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
     upd_sweep = nx * (nz - 1) * nrows
     value = 2 * upd_sweep / (ms_step / 1e3) / 1e9
@@ -653,7 +676,9 @@ def main():
         "config": {"workload": f"{cfg['prop']} downward continuation nref={cfg['nref']} "
                                f"nx={nx} nz={nz} nw={nw} {cfg['fmin']:g}-{cfg['fmax']:g} Hz "
                                f"ns={ns}",
-                   "rows": nrows, "updates_per_step": 2 * upd_sweep},
+                   "rows": nrows, "updates_per_step": 2 * upd_sweep,
+                   "launch": "hipGraph of phase tables + batched Down forward + Down adjoint",
+                   "eager_ms_per_step": round(eager_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),

@@ -179,6 +179,26 @@ int wemfwi_batch_create(int nrows, wemfwi_model* const* models, const float* fre
 int wemfwi_batch_run(wemfwi_batch* plan, int up, int adjoint, void* stream);
 int wemfwi_batch_destroy(wemfwi_batch* plan);
 
+/* A sequence of batch sweeps captured once into a hipGraph and replayed
+ * (no counterpart in the reference, which runs one TBB task per (shot,
+ * frequency), wem_prop.cpp:11-12).  Op i runs plan i's rows as Down/Up
+ * (up), forward/adjoint; tables = 1 first recomputes the phase tables of
+ * the plan's models in place (Phshift::setFreq for the plan's frequencies,
+ * Phshift_cmplx.cpp:47-113).  Creation runs the sequence once eagerly (the
+ * sweep autotuner decides then), captures it on a private stream and
+ * instantiates it; the graph holds kernel launches only.  A launch replays
+ * it on `stream` and waits; it fails with WEMFWI_ERR_ARG if a plan's model
+ * had its phase table rebuilt since planning.  Plans must outlive the
+ * graph. */
+typedef struct wemfwi_batch_op {
+  wemfwi_batch* plan;
+  int up, adjoint, tables;
+} wemfwi_batch_op;
+typedef struct wemfwi_graph wemfwi_graph;
+int wemfwi_batch_graph_create(const wemfwi_batch_op* ops, int nops, void* stream, wemfwi_graph** graph);
+int wemfwi_graph_launch(wemfwi_graph* graph, void* stream);
+int wemfwi_graph_destroy(wemfwi_graph* graph);
+
 /* SSF / PSPI single depth step (SSF.cpp:8-60, PSPI.cpp:8-53) on nrows
  * independent rows of nx samples: row i at depth depth[i] and frequency
  * freq_hz[i].  add = 1 accumulates into out. */

@@ -444,6 +444,15 @@ int Model::require_fft() const {
                                           std::to_string(kGenMaxRadix) + ")");
 }
 
+int Model::enqueue_tables(hipStream_t s) {
+  if (int rc = require_fft()) return rc;
+  if (freqs.empty()) return fail(WEMFWI_ERR_ARG, "model: no frequency set to recompute");
+  const double dk = 2.0 * M_PI / ((double)d.dx * (double)(nk - 1));
+  hipError_t e = launch_phase_table((float2*)ptab.ptr, (const double*)wlist.ptr, (int)freqs.size(), d_slot_sref,
+                                    nslots, nk, NH(), dk, (double)d.dz, (double)d.eps, s);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "phase table kernel");
+}
+
 int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
   if (int rc = require_fft()) return rc;
   std::vector<float> uniq(freq_hz, freq_hz + n);
@@ -647,34 +656,48 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
   return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch descriptor upload");
 }
 
-int BatchPlan::run(int up, int adjoint, hipStream_t s) {
+BatchPlan::~BatchPlan() {
+  for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+  desc.release();
+  scratch.release();
+}
+
+int BatchPlan::check() const {
   for (size_t u = 0; u < models.size(); ++u)
     if (models[u]->table_gen != gens[u])
       return fail(WEMFWI_ERR_ARG, "batch: a model's phase table was rebuilt after planning; plan again");
+  return WEMFWI_OK;
+}
+
+int BatchPlan::enqueue(int up, int adjoint, hipStream_t s, bool tables) {
+  if (int rc = check()) return rc;
+  if (tables)
+    for (Model* m : models)
+      if (int rc = m->enqueue_tables(s)) return rc;
   hipStream_t st[4];
   if (int rc = batch_streams(st)) return rc;
-  hipEvent_t ready;
-  hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
-  if (e != hipSuccess) return hip_fail(e, "event");
-  std::vector<hipEvent_t> done;
-  e = hipEventRecord(ready, s);
+  while (events.size() < groups.size() + 1) {
+    hipEvent_t ev;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "event");
+    events.push_back(ev);
+  }
+  hipError_t e = hipEventRecord(events[0], s);
   const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
   for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
     const Group& G = groups[k];
     hipStream_t sg = st[k % 4];
-    e = hipStreamWaitEvent(sg, ready, 0);
+    e = hipStreamWaitEvent(sg, events[0], 0);
     if (e == hipSuccess) e = launch_oneway(G.N, d_rows + G.off, G.n, d_tabs, G.g, G.nz, step, adjoint != 0, G.tw, sg);
-    hipEvent_t ev;
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-    if (e == hipSuccess) {
-      done.push_back(ev);
-      e = hipEventRecord(ev, sg);
-    }
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+    if (e == hipSuccess) e = hipEventRecord(events[k + 1], sg);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, events[k + 1], 0);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipEventDestroy(ready);
-  for (hipEvent_t ev : done) (void)hipEventDestroy(ev);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch run");
+}
+
+int BatchPlan::run(int up, int adjoint, hipStream_t s) {
+  if (int rc = enqueue(up, adjoint, s, false)) return rc;
+  hipError_t e = hipStreamSynchronize(s);
   return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch run");
 }
 
@@ -816,6 +839,70 @@ int wemfwi_batch_run(wemfwi_batch* plan, int up, int adjoint, void* stream) {
 
 int wemfwi_batch_destroy(wemfwi_batch* plan) {
   delete plan;
+  return WEMFWI_OK;
+}
+
+struct wemfwi_graph {
+  std::vector<wemfwi_batch*> plans;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  ~wemfwi_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+int wemfwi_batch_graph_create(const wemfwi_batch_op* ops, int nops, void* stream, wemfwi_graph** out) {
+  if (!ops || nops < 1 || !out) return fail(WEMFWI_ERR_ARG, "batch_graph_create: invalid argument");
+  *out = nullptr;
+  for (int i = 0; i < nops; ++i)
+    if (!ops[i].plan) return fail(WEMFWI_ERR_ARG, "batch_graph_create: null plan");
+  hipStream_t s = (hipStream_t)stream;
+  // one eager pass: the sweep autotuner decides before capture
+  for (int i = 0; i < nops; ++i) {
+    if (int rc = ops[i].plan->p.enqueue(ops[i].up, ops[i].adjoint, s, ops[i].tables != 0)) return rc;
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "batch graph warm-up");
+  wemfwi_graph* g = new (std::nothrow) wemfwi_graph();
+  if (!g) return fail(WEMFWI_ERR_ALLOC, "batch_graph_create: out of host memory");
+  e = hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal);
+  int rc = e == hipSuccess ? WEMFWI_OK : hip_fail(e, "begin capture");
+  for (int i = 0; i < nops && rc == WEMFWI_OK; ++i) {
+    rc = ops[i].plan->p.enqueue(ops[i].up, ops[i].adjoint, g->cap, ops[i].tables != 0);
+    g->plans.push_back(ops[i].plan);
+  }
+  if (e == hipSuccess) {
+    hipError_t e2 = hipStreamEndCapture(g->cap, &g->graph);
+    if (rc == WEMFWI_OK && e2 != hipSuccess) rc = hip_fail(e2, "end capture");
+  }
+  if (rc == WEMFWI_OK) {
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = hip_fail(e, "graph instantiate");
+  }
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return WEMFWI_OK;
+}
+
+int wemfwi_graph_launch(wemfwi_graph* g, void* stream) {
+  if (!g || !g->exec) return fail(WEMFWI_ERR_ARG, "graph_launch: null graph");
+  for (wemfwi_batch* p : g->plans)
+    if (int rc = p->p.check()) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipGraphLaunch(g->exec, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "graph launch");
+}
+
+int wemfwi_graph_destroy(wemfwi_graph* g) {
+  delete g;
   return WEMFWI_OK;
 }
 

@@ -91,6 +91,9 @@ struct Model {
   unsigned table_gen = 0;     // incremented each time the phase table is rebuilt
   int NH() const { return N / 2 + 1; }
   int set_freqs(int n, const float* freq_hz, hipStream_t s);
+  // recomputes the phase table of the current frequency set in place (one
+  // kernel launch, no host copies: capturable)
+  int enqueue_tables(hipStream_t s);
   int require_fft() const;  // WEMFWI_ERR_UNSUPPORTED unless fft_ok
   int phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow, std::vector<const float2*>& prow,
                    hipStream_t s);
@@ -116,12 +119,16 @@ struct BatchPlan {
   Workspace desc, scratch;
   const SweepRow* d_rows = nullptr;
   const DepthTables* d_tabs = nullptr;
-  ~BatchPlan() {
-    desc.release();
-    scratch.release();
-  }
+  std::vector<hipEvent_t> events;      // fork / join events of enqueue (one more than groups)
+  ~BatchPlan();
   int build(int nrows, Model* const* ms, const float* freq_hz, const std::complex<float>* const* in,
             std::complex<float>* const* out, hipStream_t s);
+  // WEMFWI_ERR_ARG if a model's phase table was rebuilt since planning
+  int check() const;
+  // launches the groups behind the work already on s and makes s wait for
+  // them (no host synchronisation, so it can be captured into a graph);
+  // tables = true first recomputes the models' phase tables in place
+  int enqueue(int up, int adjoint, hipStream_t s, bool tables);
   int run(int up, int adjoint, hipStream_t s);
 };
 

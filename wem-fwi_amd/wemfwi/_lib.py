@@ -66,6 +66,9 @@ SIGNATURES = {
     "wemfwi_batch_create": (_i, [_i, _vp, _fp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "wemfwi_batch_run": (_i, [_vp, _i, _i, _vp]),
     "wemfwi_batch_destroy": (_i, [_vp]),
+    "wemfwi_batch_graph_create": (_i, [_vp, _i, _vp, C.POINTER(_vp)]),
+    "wemfwi_graph_launch": (_i, [_vp, _vp]),
+    "wemfwi_graph_destroy": (_i, [_vp]),
     "wemfwi_model_set_freqs": (_i, [_vp, _i, _fp, _vp]),
     "wemfwi_survey_create": (_i, [_vp, C.POINTER(_vp)]),
     "wemfwi_survey_destroy": (_i, [_vp]),

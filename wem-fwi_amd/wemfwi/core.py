@@ -205,6 +205,40 @@ class BatchPlan:
             pass
 
 
+class _BatchOp(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("up", C.c_int), ("adjoint", C.c_int), ("tables", C.c_int)]
+
+
+class BatchGraph:
+    """A sequence of batch-plan sweeps captured once into a hipGraph and
+    replayed by launch(): ops are (plan, up, adjoint, tables) with tables =
+    True recomputing the plan models' phase tables first.  Creation runs the
+    sequence once (the sweep autotuner decides then).  The plans must stay
+    alive while the graph is used."""
+
+    def __init__(self, ops, stream: Optional[torch.cuda.Stream] = None):
+        ops = list(ops)
+        self._keep = [op[0] for op in ops]
+        arr = (_BatchOp * len(ops))(*[_BatchOp(p._h, int(up), int(adj), int(tab)) for p, up, adj, tab in ops])
+        h = C.c_void_p()
+        check(lib.wemfwi_batch_graph_create(C.cast(arr, C.c_void_p), len(ops), _stream_ptr(stream), C.byref(h)))
+        self._h = h
+
+    def launch(self, stream: Optional[torch.cuda.Stream] = None):
+        check(lib.wemfwi_graph_launch(self._h, _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.wemfwi_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def timing(enable: bool) -> None:
     """Enables (and clears) or disables HIP-event kernel timing in the library."""
     check(lib.wemfwi_timing(int(enable)))
