@@ -138,10 +138,43 @@ def cpu_baseline(cfg, slow, freqs, src_ix, nrows_sample):
     cpu_wem.oneway(W, fr, slow, ref, p, up=False, adjoint=True, nthreads=cores)
     dt = time.perf_counter() - t0
     upd = 2.0 * nx * (nz - 1) * len(rows)
+    # one thread on a smaller sample (the per-core rate)
+    n1 = max(1, len(rows) // 64)
+    W1 = W[:n1].copy()
+    t0 = time.perf_counter()
+    cpu_wem.oneway(W1, fr[:n1], slow, ref, p, up=False, adjoint=False, nthreads=1)
+    cpu_wem.oneway(W1, fr[:n1], slow, ref, p, up=False, adjoint=True, nthreads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": upd / dt / 1e9, "unit": "Gupdates/s", "cores": cores, "kind": "port",
             "sample": f"{len(rows)} of {nw * len(src_ix)} (shot, frequency) rows, Down forward + "
                       f"adjoint, FP32 C restatement (oracle/cpu_wem.c, OpenMP, radix-2 FFT), "
-                      f"{dt:.2f} s"}
+                      f"{dt:.2f} s",
+            "single_thread_value": 2.0 * nx * (nz - 1) * n1 / dt1 / 1e9,
+            "single_thread_sample": f"{n1} rows, {dt1:.2f} s",
+            "host": host_cpu()}
+
+
+def host_cpu() -> str:
+    """CPU model and topology of the host (lscpu when present)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    topo = ""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keep = ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)")
+        topo = ", ".join(l.split(":", 1)[0].strip() + " " + l.split(":", 1)[1].strip()
+                         for l in out.splitlines() if l.split(":", 1)[0].strip() in keep)
+    except Exception:
+        pass
+    return f"{model}; {topo}" if topo else model
 
 
 # FWI gradient configuration: 512 x 300 grid, 128 frequencies 2-15 Hz, 32 shots,
