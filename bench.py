@@ -336,8 +336,8 @@ def fwi_gradient_bench(sample_rows: int, fd_dirs: int = 20):
             "checkpoint_every": CKPT,
             "checkpointed_ms": round(t_ckpt * 1e3, 2),
             "checkpointed_bitwise_equal": ckpt_equal,
-            "adjoint_row_bytes": {"store_all": 5 * nz * nx * 8,
-                                  "checkpointed": (5 * (CKPT + 2) + 2 * (-(-(nz - 1) // CKPT)) + 2) * nx * 8},
+            "adjoint_row_bytes": {"store_all": 4 * nz * nx * 8,
+                                  "checkpointed": (4 * (CKPT + 2) + 2 * (-(-(nz - 1) // CKPT)) + 2) * nx * 8},
             "plain_loop_ms": round(t_plain * 1e3, 1),
             "plain_loop_sample": f"{len(rows)} of {total} rows timed ({t_plain_sample:.2f} s), scaled to all rows",
             "speedup": round(t_plain / t_batched, 2),

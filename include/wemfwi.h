@@ -237,7 +237,7 @@ int wemfwi_survey_destroy(wemfwi_survey* survey);
  * reference, which keeps whole fields, born_prop.cpp:178-232): every >= 1
  * keeps the background and receiver rows every `every` depth steps and
  * recomputes the fields segment by segment, the per-row device memory
- * dropping from 5 nz nx to (5 (every + 2) + 2 ceil((nz-1)/every) + 2) nx
+ * dropping from 4 nz nx to (4 (every + 2) + 2 ceil((nz-1)/every) + 2) nx
  * complex samples at the cost of one more pass of the two forward
  * recursions.  The gradient is bitwise the same as with whole fields.
  * 0 (the default), or every >= nz - 1, keeps whole fields. */

@@ -9,13 +9,15 @@ namespace wemfwi {
 
 // One (shot, frequency) row of a LinDown / LinUp application.
 struct LinRow {
-  const float2* bg;      // [nz][nx] background wavefield
+  float2* bg;            // [nz][nx] background wavefield
   const float2* m;       // forward: perturbation [nz][nx]; adjoint: wavefield [nz][nx]
   float2* out;           // forward: wavefield (+=); adjoint: model contribution (+=)
   const float2* ptab;    // phase table of the row's frequency
   float w;
   int model;
   int zbase;             // depth of the first row of bg / m / out (0: whole arrays)
+  const float2* post;    // adjoint only, or NULL: after use, bg[d] <- post[d] * bg[d] (post: [nz][nx]
+                         // by absolute depth, e.g. the model's reflection coefficient)
 };
 
 // Injection of many devices as a gather per touched cell (CSR by cell).
@@ -57,8 +59,9 @@ hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, 
 constexpr int kRunRows = 64;
 // out[i] (+)= sum_r in[r * stride + i] for i < n over nrows rows, the first of
 // which sits at position phase of its run
+// in2 non-NULL: the summand is conj(in) * in2
 hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, int phase,
-                              bool acc, hipStream_t s);
+                              bool acc, hipStream_t s, const float2* in2 = nullptr);
 hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s);
 hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, const long long* src, const float2* in,
                              bool add, hipStream_t s);

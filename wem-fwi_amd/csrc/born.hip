@@ -74,6 +74,13 @@ __global__ void __launch_bounds__(64 * kLinWaves)
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], b[e]);
     acc_row<V>(R.out + r0 * nx, st, g, t);
+    if (R.post) {
+      // bg[d] <- post[d] * bg[d]: this item is the only reader of bg row d
+      float2 q[E];
+      lrow<V>(q, R.post + (size_t)d * nx, g, t);
+#pragma unroll
+      for (int e = 0; e < E; ++e) row_store<V>(R.bg + r0 * nx, cmul(q[e], b[e]), g, t, e);
+    }
   }
 }
 
@@ -147,11 +154,13 @@ __global__ void mul_rows_kernel(const MulRow* __restrict__ rows, int nrows, long
 // the run sums are added to out in run order.  One wave per run and 64
 // consecutive samples (one per lane, coalesced 512-byte row loads); the
 // waves of a workgroup take kRedWaves runs at a time and wave 0 adds their
-// sums in order from LDS.
+// sums in order from LDS.  PROD: the summand is conj(in) * in2, formed in
+// FP32 as a product row would be (the product is never stored).
 constexpr int kRedWaves = 8;
+template <bool PROD>
 __global__ void __launch_bounds__(64 * kRedWaves)
-    reduce_runs_kernel(double2* __restrict__ out, const float2* __restrict__ in, long long stride, long long n,
-                       int nrows, int phase, int acc) {
+    reduce_runs_kernel(double2* __restrict__ out, const float2* __restrict__ in, const float2* __restrict__ in2,
+                       long long stride, long long n, int nrows, int phase, int acc) {
   __shared__ double2 part[kRedWaves][64];
   const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long i = (long long)blockIdx.x * 64 + t;
@@ -171,7 +180,8 @@ __global__ void __launch_bounds__(64 * kRedWaves)
       if (live) {
 #pragma unroll 4
         for (int r = r0; r < r1; ++r) {
-          const float2 v = ldg(in, (size_t)((long long)r * stride + i));
+          float2 v = ldg(in, (size_t)((long long)r * stride + i));
+          if constexpr (PROD) v = cmulc(ldg(in2, (size_t)((long long)r * stride + i)), v);
           sx += v.x;
           sy += v.y;
         }
@@ -481,13 +491,17 @@ hipError_t launch_mul(float2* out, long long so, const float2* A, long long sa, 
 }
 
 hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, int phase,
-                              bool acc, hipStream_t s) {
+                              bool acc, hipStream_t s, const float2* in2) {
   if (n <= 0 || nrows <= 0) return hipSuccess;
   if (phase < 0 || phase >= kRunRows) return hipErrorInvalidValue;
   const long long tiles = (n + 63) / 64;
   if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(reduce_runs_kernel, dim3((unsigned)tiles), dim3(64 * kRedWaves), 0, s, out, in, stride, n,
-                     nrows, phase, acc ? 1 : 0);
+  if (in2)
+    hipLaunchKernelGGL(reduce_runs_kernel<true>, dim3((unsigned)tiles), dim3(64 * kRedWaves), 0, s, out, in, in2,
+                       stride, n, nrows, phase, acc ? 1 : 0);
+  else
+    hipLaunchKernelGGL(reduce_runs_kernel<false>, dim3((unsigned)tiles), dim3(64 * kRedWaves), 0, s, out, in,
+                       in, stride, n, nrows, phase, acc ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -318,9 +318,11 @@ struct Ctx {
     return sync();
   }
 
-  // LinDown (dir=+1) / LinUp (dir=-1)
-  int lin(const Rows& r, int dir, int adjoint, int exact, const float2* bg, const float2* m, size_t m_stride,
-          float2* out) {
+  // LinDown (dir=+1) / LinUp (dir=-1); post_refl (adjoint): each bg row,
+  // once used, is replaced by r * bg (r the row model's reflection
+  // coefficient)
+  int lin(const Rows& r, int dir, int adjoint, int exact, float2* bg, const float2* m, size_t m_stride,
+          float2* out, bool post_refl = false) {
     const int n = (int)r.shot.size();
     hlin.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -331,6 +333,7 @@ struct Ctx {
       hlin[i].w = omega(r.model[i], r.fi[i]);
       hlin[i].model = model_index(r.model[i]);
       hlin[i].zbase = 0;
+      hlin[i].post = post_refl ? r.model[i]->d_refl : nullptr;
     }
     RCCHK(put(d_rows, hlin, s, "lin rows"));
     HIPCHK(launch_lin(N, d_rows.as<LinRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, dir, adjoint != 0,
@@ -371,8 +374,8 @@ struct Ctx {
 
   // Adjoint LinDown / LinUp over depths [dlo, dlo + dcount) of segment
   // buffers (bg, m and out share the row layout of sweep_seg).
-  int lin_seg(const Rows& r, int dir, int exact, const float2* bg, const float2* m, float2* out, size_t wstride,
-              int zbase, int dlo, int dcount) {
+  int lin_seg(const Rows& r, int dir, int exact, float2* bg, const float2* m, float2* out, size_t wstride,
+              int zbase, int dlo, int dcount, bool post_refl = false) {
     if (dcount <= 0) return WEMFWI_OK;
     const int n = (int)r.shot.size();
     hlin.resize(n);
@@ -384,6 +387,7 @@ struct Ctx {
       hlin[i].w = omega(r.model[i], r.fi[i]);
       hlin[i].model = model_index(r.model[i]);
       hlin[i].zbase = zbase;
+      hlin[i].post = post_refl ? r.model[i]->d_refl : nullptr;
     }
     RCCHK(put(d_rows, hlin, s, "lin rows"));
     HIPCHK(launch_lin(N, d_rows.as<LinRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, dir, true, exact != 0,
@@ -699,7 +703,7 @@ static int born_forward(const std::vector<Model*>& fm, Per per, Survey* sv, cons
 // depend on how the rows were cut into chunks or depth segments.
 //   depths [z0, z1) of row i at in + i * stride + (z0 - zbase) * nx
 static int reduce_chunk(Ctx& c, const Rows& r, bool sliced, double2* acc, const float2* in, size_t stride, int zbase,
-                        int z0, int z1) {
+                        int z0, int z1, const float2* in2 = nullptr) {
   const int n = (int)r.shot.size();
   int i0 = 0;
   while (i0 < n) {
@@ -709,9 +713,10 @@ static int reduce_chunk(Ctx& c, const Rows& r, bool sliced, double2* acc, const 
     else
       i1 = n;
     const size_t slot = sliced ? (size_t)r.slice[i0] : 0;
-    HIPCHK(launch_reduce_rows(acc + slot * c.nzx + (size_t)z0 * c.nx,
-                              in + (size_t)i0 * stride + (size_t)(z0 - zbase) * c.nx, (long long)stride,
-                              (long long)(z1 - z0) * c.nx, i1 - i0, (int)((r.first + i0) % kRunRows), true, c.s),
+    const size_t off = (size_t)i0 * stride + (size_t)(z0 - zbase) * c.nx;
+    HIPCHK(launch_reduce_rows(acc + slot * c.nzx + (size_t)z0 * c.nx, in + off, (long long)stride,
+                              (long long)(z1 - z0) * c.nx, i1 - i0, (int)((r.first + i0) % kRunRows), true, c.s,
+                              in2 ? in2 + off : nullptr),
            "reduce");
     i0 = i1;
   }
@@ -724,16 +729,15 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
   const int nw = sv->d.nw, ns = sv->d.ns;
   // rows of one model slice contiguous: frequency-major for per-frequency
   // models, shot-major for per-shot models
-  auto chunks = make_chunks(ns, nw, fm, per, 5 * c.nzx * sizeof(float2), per == Per::Freq);
+  auto chunks = make_chunks(ns, nw, fm, per, 4 * c.nzx * sizeof(float2), per == Per::Freq);
   const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
-  DevBuf B, Rb, Qb, CS, CR;
+  DevBuf B, Rb, Qb, CS;
   RCCHK(B.alloc(cap, "background"));
   RCCHK(Rb.alloc(cap, "receiver wavefield"));
   RCCHK(Qb.alloc(cap, "receiver wavefield"));
   RCCHK(CS.alloc(cap, "dS contributions"));
-  RCCHK(CR.alloc(cap, "dr contributions"));
-  float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>(),
-         *cr = CR.as<float2>();
+  float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>();
+  const size_t last = (size_t)(c.nz - 1) * c.nx;
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
     const size_t bytes = n * c.nzx * sizeof(float2);
@@ -743,28 +747,30 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
     RCCHK(c.zero(R, bytes));
     RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset));
     RCCHK(c.sweep(r, 1, 1, R, R));
+    // dr += conj(D) * R (Born_refl::adjoint), summed straight from D and R
+    // before LinDown^H rescales D
+    RCCHK(reduce_chunk(c, r, sliced, accDR, bg, c.nzx, 0, 0, c.nz, R));
     RCCHK(c.zero(Q, bytes));
     RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset));
     {
-      std::vector<MulRow> ic(n), rq(n);
+      std::vector<MulRow> rq(n);
       for (int i = 0; i < n; ++i) {
         const size_t o = (size_t)i * c.nzx;
-        ic[i] = MulRow{cr + o, bg + o, R + o};
         rq[i] = MulRow{Q + o, r.model[i]->d_refl, R + o};
       }
-      RCCHK(c.mul_rows(ic, c.nzx, true, false));  // conj(D) * R   (Born_refl::adjoint)
       RCCHK(c.mul_rows(rq, c.nzx, true, true));   // Q += conj(r) R
     }
     RCCHK(c.sweep(r, 0, 1, Q, Q));
     RCCHK(c.zero(cs, bytes));
-    RCCHK(c.lin(r, +1, 1, exact, bg, Q, 0, cs));
+    // LinDown^H; each D row d < nz-1 it reads becomes r * D (the Up source)
+    RCCHK(c.lin(r, +1, 1, exact, bg, Q, 0, cs, true));
     {
       std::vector<MulRow> rbg(n);
       for (int i = 0; i < n; ++i) {
-        const size_t o = (size_t)i * c.nzx;
-        rbg[i] = MulRow{bg + o, r.model[i]->d_refl, bg + o};
+        const size_t o = (size_t)i * c.nzx + last;
+        rbg[i] = MulRow{bg + o, r.model[i]->d_refl + last, bg + o};
       }
-      RCCHK(c.mul_rows(rbg, c.nzx, false, false));  // r * D
+      RCCHK(c.mul_rows(rbg, c.nx, false, false));  // r * D at the last depth
     }
     RCCHK(c.sweep(r, 1, 0, bg, bg));
     RCCHK(c.lin(r, -1, 1, exact, bg, R, 0, cs));
@@ -775,7 +781,6 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
                  "memset");
     }
     RCCHK(reduce_chunk(c, r, sliced, accS, cs, c.nzx, 0, 0, c.nz));
-    RCCHK(reduce_chunk(c, r, sliced, accDR, cr, c.nzx, 0, 0, c.nz));
     RCCHK(c.sync());
   }
   return WEMFWI_OK;
@@ -820,8 +825,8 @@ static int born_adjoint_out(Ctx& c, const std::vector<Model*>& fm, float2* gS, f
 // sweep state outside [0, nx) is zero after a forward step and is discarded
 // by the reference restriction at the start of an adjoint step, so a state
 // reloaded from a kept row continues bit for bit), hence the same gradient.
-// Device memory per row: (5 (k + 2) + 2 nseg + 2) nx samples instead of
-// 5 nz nx.
+// Device memory per row: (4 (k + 2) + 2 nseg + 2) nx samples instead of
+// 4 nz nx.
 static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, double2* accS, double2* accDR,
                              bool sliced, const float2* data, int exact, bool ext_rowlast, int k) {
   Survey* sv = c.sv;
@@ -829,22 +834,20 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
   const int nseg = (nz - 1 + k - 1) / k;
   const int SR = std::min(k + 2, nz);       // rows of a segment buffer
   const size_t W = (size_t)SR * nx;        // samples per row of a segment buffer
-  const size_t per_row = (5 * W + (2 * (size_t)nseg + 2) * nx) * sizeof(float2);
+  const size_t per_row = (4 * W + (2 * (size_t)nseg + 2) * nx) * sizeof(float2);
   auto chunks = make_chunks(ns, nw, fm, per, per_row, per == Per::Freq);
   const size_t nmax = chunks[0].shot.size();
-  DevBuf B, Rb, Qb, CS, CR, KD, KR, CQ, CU;
+  DevBuf B, Rb, Qb, CS, KD, KR, CQ, CU;
   RCCHK(B.alloc(nmax * W * sizeof(float2), "background segment"));
   RCCHK(Rb.alloc(nmax * W * sizeof(float2), "receiver segment"));
   RCCHK(Qb.alloc(nmax * W * sizeof(float2), "receiver segment"));
   RCCHK(CS.alloc(nmax * W * sizeof(float2), "dS contributions"));
-  RCCHK(CR.alloc(nmax * W * sizeof(float2), "dr contributions"));
   RCCHK(KD.alloc(nmax * nseg * nx * sizeof(float2), "background checkpoints"));
   RCCHK(KR.alloc(nmax * nseg * nx * sizeof(float2), "receiver checkpoints"));
   RCCHK(CQ.alloc(nmax * nx * sizeof(float2), "carried row"));
   RCCHK(CU.alloc(nmax * nx * sizeof(float2), "carried row"));
   float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>(),
-         *cr = CR.as<float2>(), *kd = KD.as<float2>(), *kr = KR.as<float2>(), *cq = CQ.as<float2>(),
-         *cu = CU.as<float2>();
+         *kd = KD.as<float2>(), *kr = KR.as<float2>(), *cq = CQ.as<float2>(), *cu = CU.as<float2>();
   const size_t kpitch = (size_t)nseg * nx;  // checkpoint rows of one chunk row
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
@@ -876,30 +879,31 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
       const int e = last ? nz : b;  // owned depths [a, e)
       const size_t oa = (size_t)(a - zb) * nx;
       RCCHK(fields(j));
+      // dr += conj(D) * R over the owned depths, before LinDown^H rescales D
+      RCCHK(reduce_chunk(c, r, sliced, accDR, bg, W, zb, a, e, R));
       const int qe = last ? b : b - 1;  // Q source rows [a, qe]
       RCCHK(c.zero(Q, bytes));
       RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset, Ctx::Win{(long long)W, zb, a, qe}));
       {
-        std::vector<MulRow> ic(n), rq(n);
+        std::vector<MulRow> rq(n);
         for (int i = 0; i < n; ++i) {
           const size_t o = (size_t)i * W + oa;
-          ic[i] = MulRow{cr + o, bg + o, R + o};
           rq[i] = MulRow{Q + o, r.model[i]->d_refl + (size_t)a * nx, R + o};
         }
-        RCCHK(c.mul_rows(ic, (size_t)(e - a) * nx, true, false));   // conj(D) * R
         RCCHK(c.mul_rows(rq, (size_t)(qe - a + 1) * nx, true, true));  // Q += conj(r) R
       }
       RCCHK(c.sweep_seg(r, 0, 1, Q, W, zb, b, b - a, last ? nullptr : cq, nx));
       if (j > 0) RCCHK(c.copy_rows(cq, nx, Q + oa, W, n));
       RCCHK(c.zero(cs, bytes));
-      RCCHK(c.lin_seg(r, +1, exact, bg, Q, cs, W, zb, a, b - a));
-      {
+      // LinDown^H over [a, b); the D rows it reads become r * D (Up sources)
+      RCCHK(c.lin_seg(r, +1, exact, bg, Q, cs, W, zb, a, b - a, true));
+      if (last) {  // the Up recursion starts from r * D at the last depth
         std::vector<MulRow> rbg(n);
         for (int i = 0; i < n; ++i) {
-          const size_t o = (size_t)i * W + oa;
-          rbg[i] = MulRow{bg + o, r.model[i]->d_refl + (size_t)a * nx, bg + o};
+          const size_t o = (size_t)i * W + (size_t)(b - zb) * nx;
+          rbg[i] = MulRow{bg + o, r.model[i]->d_refl + (size_t)b * nx, bg + o};
         }
-        RCCHK(c.mul_rows(rbg, (size_t)(b - a + 1) * nx, false, false));  // r * D
+        RCCHK(c.mul_rows(rbg, (size_t)nx, false, false));
       }
       RCCHK(c.sweep_seg(r, 1, 0, bg, W, zb, b, b - a, last ? nullptr : cu, nx));
       if (j > 0) RCCHK(c.copy_rows(cu, nx, bg + oa, W, n));
@@ -912,7 +916,6 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
                    "memset");
       }
       RCCHK(reduce_chunk(c, r, sliced, accS, cs, W, zb, a, e));
-      RCCHK(reduce_chunk(c, r, sliced, accDR, cr, W, zb, a, e));
     }
     RCCHK(c.sync());
   }
@@ -1329,7 +1332,7 @@ static int unit_lin(Model& m, int up, int adjoint, int mode, int nrows, const fl
   }
   std::vector<LinRow> h(nrows);
   for (int i = 0; i < nrows; ++i)
-    h[i] = LinRow{bg + i * nzx, in + i * nzx, out + i * nzx, p[i], w[i], 0};
+    h[i] = LinRow{const_cast<float2*>(bg) + i * nzx, in + i * nzx, out + i * nzx, p[i], w[i], 0, 0, nullptr};
   DevBuf rows;
   RCCHK(put(rows, h, s, "lin rows"));
   HIPCHK(launch_lin(m.N, rows.as<LinRow>(), nrows, m.d_tables, m.geom(), m.d.nz, up ? -1 : +1, adjoint != 0,

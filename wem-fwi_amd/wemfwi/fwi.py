@@ -15,7 +15,7 @@ adjoint of the linearised operator.
 checkpoint=k keeps the background and receiver wavefields only every k
 depth steps during the gradient and recomputes them segment by segment: the
 gradient is bitwise the same, the device memory per (shot, frequency) row
-drops from 5 nz nx samples to about (5 k + 2 nz / k) nx, so larger batches
+drops from 4 nz nx samples to about (4 k + 2 nz / k) nx, so larger batches
 of rows fit one launch.
 
 fwi_problem / fit_preconditioned / shard_par assemble the driver of

@@ -76,8 +76,8 @@ class Survey:
         """Born adjoints with this survey keep the background and receiver
         wavefield rows every `every` depth steps and recompute the rest
         segment by segment (0: keep whole fields).  The result is bitwise
-        the same; the per-row device memory drops from 5 nz nx samples to
-        about (5 every + 2 nz / every) nx."""
+        the same; the per-row device memory drops from 4 nz nx samples to
+        about (4 every + 2 nz / every) nx."""
         every = int(every)
         if every < 0:
             raise ValueError("checkpoint interval must be >= 0")
