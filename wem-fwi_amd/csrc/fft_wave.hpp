@@ -342,7 +342,7 @@ __device__ __forceinline__ void fft_row(float2 (&x)[N / 64], float2* buf, const 
 // Rows whose FFT length n is not the power of two N of the launch (n <= N,
 // e.g. 500 or 600 samples in a 512- or 1024-lane row): the row is staged in
 // LDS and transformed by a runtime chain of Stockham passes between two LDS
-// buffers (buf and buf + LdsSize<N>), radix 4, 2, 3 and 5 as register
+// buffers (buf and buf + LdsSize<N>), radix 16, 8, 4, 2, 3 and 5 as register
 // butterflies and any other prime radix as a direct DFT.  Pass p with radix
 // R and sub-transform length NS reads butterfly j's inputs at j + r n/R and
 // writes output s at (j/NS) NS R + j%NS + s NS, as in the power-of-two
@@ -359,6 +359,9 @@ struct FftGen {
 __host__ __device__ constexpr int gen_radix(const FftGen& f, int p) { return (int)((f.rad >> (6 * p)) & 63u); }
 
 constexpr int kGenMaxRadix = 61;
+// radices transformed as a direct DFT (their own R-entry table follows the
+// pass twiddles); 2, 3, 4, 5, 8 and 16 are register butterflies
+__host__ __device__ constexpr bool gen_direct(int R) { return R > 5 && R != 8 && R != 16; }
 __host__ __device__ constexpr int gen_tw_cap(int N) { return N + 3 * kGenMaxRadix; }
 
 constexpr float kSin3 = 0.86602540378443864676f;   // sin(2 pi / 3)
@@ -395,29 +398,58 @@ __device__ __forceinline__ void dft5(float2 (&v)[5]) {
   v[3] = csub(p2, q2);
 }
 
-// one pass with a register-butterfly radix R (2, 3, 4, 5)
-template <int R, bool INV>
+// q = j / NS for j < 2^21 from a per-pass multiplier mg = ceil(2^32 / NS)
+// (exact: j (mg NS - 2^32) < 2^32)
+__device__ __forceinline__ int div_ns(int j, unsigned long long mg) {
+  return (int)(((unsigned long long)(unsigned)j * mg) >> 32);
+}
+
+// butterfly j of a pass with a register-butterfly radix R (2, 3, 4, 5, 8,
+// 16); MG: j / NS by the multiplier mg (else by integer division)
+template <int R, bool INV, bool MG>
+__device__ __forceinline__ void gen_bfly(const float2* __restrict__ A, float2* __restrict__ B,
+                                         const float2* __restrict__ tw, int NB, int NS, unsigned long long mg,
+                                         int j) {
+  const int q = MG ? div_ns(j, mg) : j / NS, m = j - q * NS;
+  float2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = A[lidx(j + r * NB)];
+  if (NS > 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul_tw<INV>(v[r], tw[(r - 1) * NS + m]);
+  }
+  if constexpr (R == 3) dft3<INV>(v);
+  else if constexpr (R == 5) dft5<INV>(v);
+  else dft<R, INV>(v);
+  const int o = q * NS * R + m;
+#pragma unroll
+  for (int r = 0; r < R; ++r) B[lidx(o + r * NS)] = v[r];
+}
+
+// one pass with a register-butterfly radix; PAIR: two butterflies per lane
+// and iteration, so that the LDS loads of the second overlap the first, and
+// the quotient j / NS by multiplication (measured: faster in rows of 1024
+// and 2048 lanes, slower in rows of 512)
+template <int R, bool INV, bool PAIR>
 __device__ __forceinline__ void gen_pass_r(const float2* __restrict__ A, float2* __restrict__ B,
                                            const float2* __restrict__ tw, int n, int NS, int t) {
   const int NB = n / R;
-  for (int j = t; j < NB; j += 64) {
-    const int q = j / NS, m = j - q * NS;
-    float2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = A[lidx(j + r * NB)];
-    if (NS > 1) {
-#pragma unroll
-      for (int r = 1; r < R; ++r) v[r] = cmul_tw<INV>(v[r], tw[(r - 1) * NS + m]);
+  if constexpr (PAIR) {
+    const unsigned long long mg = ((1ull << 32) + (unsigned long long)NS - 1) / (unsigned long long)NS;
+    for (int j = t; j < NB; j += 128) {
+      gen_bfly<R, INV, true>(A, B, tw, NB, NS, mg, j);
+      if (j + 64 < NB) gen_bfly<R, INV, true>(A, B, tw, NB, NS, mg, j + 64);
     }
-    if constexpr (R == 2) dft2<INV>(v[0], v[1]);
-    else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
-    else if constexpr (R == 3) dft3<INV>(v);
-    else dft5<INV>(v);
-    const int o = q * NS * R + m;
-#pragma unroll
-    for (int r = 0; r < R; ++r) B[lidx(o + r * NS)] = v[r];
+  } else {
+    for (int j = t; j < NB; j += 64) gen_bfly<R, INV, false>(A, B, tw, NB, NS, 0ull, j);
   }
 }
+
+// largest register radix of runtime plans in rows of N lanes: rows of 1024
+// and 2048 lanes run at 1-2 waves per SIMD for their LDS anyway, so radix 8
+// and 16 butterflies (and two per iteration) cost them no occupancy; shorter
+// rows keep to radix 4 and their register budget
+__host__ __device__ constexpr int gen_max_radix(int N) { return N >= 1024 ? 16 : 4; }
 
 // one pass with a prime radix R > 5: direct DFT, twiddled inputs re-read
 // from LDS for every output (the transform is rare, registers are not)
@@ -461,15 +493,22 @@ __device__ __forceinline__ void fft_row_gen(float2 (&x)[N / 64], float2* buf, co
   for (int p = 0; p < f.np; ++p) {
     const int R = gen_radix(f, p);
     const float2* twp = tw + off;
+    constexpr bool kBig = gen_max_radix(N) == 16;
     switch (R) {
-      case 2: gen_pass_r<2, INV>(A, B, twp, n, NS, t); break;
-      case 3: gen_pass_r<3, INV>(A, B, twp, n, NS, t); break;
-      case 4: gen_pass_r<4, INV>(A, B, twp, n, NS, t); break;
-      case 5: gen_pass_r<5, INV>(A, B, twp, n, NS, t); break;
+      case 2: gen_pass_r<2, INV, kBig>(A, B, twp, n, NS, t); break;
+      case 3: gen_pass_r<3, INV, kBig>(A, B, twp, n, NS, t); break;
+      case 4: gen_pass_r<4, INV, kBig>(A, B, twp, n, NS, t); break;
+      case 5: gen_pass_r<5, INV, kBig>(A, B, twp, n, NS, t); break;
+      case 8:
+        if constexpr (kBig) gen_pass_r<8, INV, true>(A, B, twp, n, NS, t);
+        break;
+      case 16:
+        if constexpr (kBig) gen_pass_r<16, INV, true>(A, B, twp, n, NS, t);
+        break;
       default: gen_pass_dft<INV>(A, B, twp, twp + (R - 1) * NS, n, R, NS, t); break;
     }
     wave_lds_order();
-    off += (R - 1) * NS + (R > 5 ? R : 0);
+    off += (R - 1) * NS + (gen_direct(R) ? R : 0);
     NS *= R;
     float2* T = A;
     A = B;

@@ -144,24 +144,27 @@ static std::map<int, float2*>& twiddle_cache() {
   return c;
 }
 
-// Runtime FFT plan of a length n that is not a built power of two:
-// radix-4 passes, one radix-2 pass, then the odd primes in increasing order
-// (3 and 5 as register butterflies, larger primes up to kGenMaxRadix as
-// direct DFTs).  False when n has a larger prime factor.
-bool fft_gen_plan(int n, FftGen* f) {
+// Runtime FFT plan of a length n that is not a built power of two, in rows
+// whose largest register radix is maxr (16 or 4): the power of two as passes
+// of the largest radix, then one smaller power-of-two pass, then the odd
+// primes in increasing order (3 and 5 as register butterflies, larger
+// primes up to kGenMaxRadix as direct DFTs).  False when n has a larger
+// prime factor.
+bool fft_gen_plan(int n, int maxr, FftGen* f) {
   *f = FftGen{};
   f->n = n;
   if (n < 2) return false;
   int m = n, np = 0;
   std::vector<int> rad;
-  while (m % 4 == 0) {
-    rad.push_back(4);
-    m /= 4;
+  while (m % maxr == 0) {
+    rad.push_back(maxr);
+    m /= maxr;
   }
-  if (m % 2 == 0) {
-    rad.push_back(2);
-    m /= 2;
-  }
+  for (int r : {8, 4, 2})
+    if (r < maxr && m % r == 0) {
+      rad.push_back(r);
+      m /= r;
+    }
   for (int p = 3; m > 1; p += 2) {
     while (m % p == 0) {
       if (p > kGenMaxRadix) return false;
@@ -204,7 +207,7 @@ int gen_twiddles(const FftGen& f, int N, const float2** out) {
         h[off + (size_t)(r - 1) * NS + m] = make_float2((float)std::cos(a), (float)std::sin(a));
       }
     off += (size_t)(R - 1) * NS;
-    if (R > 5) {
+    if (gen_direct(R)) {
       for (int q = 0; q < R; ++q) {
         const double a = -2.0 * M_PI * (double)q / (double)R;
         h[off + q] = make_float2((float)std::cos(a), (float)std::sin(a));
@@ -278,7 +281,7 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
     fg.n = N;
     fft_ok = fft_size_supported(N);
   } else {
-    fft_ok = nk <= N && fft_gen_plan(nk, &fg);
+    fft_ok = nk <= N && fft_gen_plan(nk, gen_max_radix(N), &fg);
   }
   const size_t nzx = (size_t)d.nz * d.nx;
   slow.assign(slow_h, slow_h + nzx);
