@@ -454,6 +454,27 @@ int wemfwi_reflect(wemfwi_model* model, int adjoint, int nrows, const wemfwi_com
 int wemfwi_dreflect(wemfwi_model* model, int adjoint, wemfwi_complex* dslow, wemfwi_complex* dden,
                     wemfwi_complex* dr, int add, void* stream);
 
+/* Phshift::forward/adjoint (Phshift_cmplx.cpp:47-113, phase_shift.ispc:2-32,
+ * py_wem_ops.cpp:95-113) on nrows independent wavenumber rows of nk samples:
+ * out[k] (+)= in[k] exp(-i kz dz) (adjoint: the conjugate factor), kz the
+ * damped vertical wavenumber of the reference slowness sref[i] at frequency
+ * freq_hz[i] (host arrays) on the k grid of Phshift_cmplx.cpp:12-24
+ * (dk = 2 pi / (dx (nk - 1))). */
+int wemfwi_phshift(int nk, float dx, float dz, float eps, int adjoint, int nrows, const float* freq_hz,
+                   const wemfwi_complex* sref, const wemfwi_complex* in, wemfwi_complex* out, int add,
+                   void* stream);
+
+/* SplitStep::forward/adjoint (SplitStep.cpp:5-29, py_wem_ops.cpp:25-46) on
+ * nrows rows of the model's width: at the nloc positions loc (setLocation),
+ * row i at depth depth[i] (setDepth), frequency freq_hz[i] (setFreq) and
+ * reference slowness sref[i] (setSlow): forward out[x] = in[x] exp(-i w dz
+ * (S - sref)/sqrt(sref)), adjoint out[x] = in[x] exp(+i w dz conj((S -
+ * sref)/sqrt(sref))), assigned at loc; add = 0 zeroes the rows of out first
+ * (host arrays freq_hz, depth, sref, loc). */
+int wemfwi_splitstep(wemfwi_model* model, int adjoint, int nrows, const float* freq_hz, const int* depth,
+                     const wemfwi_complex* sref, int nloc, const int* loc, const wemfwi_complex* in,
+                     wemfwi_complex* out, int add, void* stream);
+
 /* IC (ic.cpp:5-45) on n samples: forward d = bg * m; adjoint m = conj(bg) * d. */
 int wemfwi_ic(int adjoint, long long n, const wemfwi_complex* bg, wemfwi_complex* m, wemfwi_complex* d,
               int add, void* stream);

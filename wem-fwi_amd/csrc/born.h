@@ -63,6 +63,12 @@ constexpr int kRunRows = 64;
 hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, int phase,
                               bool acc, hipStream_t s, const float2* in2 = nullptr);
 hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s);
+// Phshift rows (w, sref per row; device arrays)
+hipError_t launch_phshift(float2* out, const float2* in, int nk, double dk, double dz, double eps, const double* w,
+                          const double2* sref, int nrows, bool adjoint, bool add, hipStream_t s);
+// out[r][loc[j]] = in[r][loc[j]] * mult[r][j]
+hipError_t launch_gather_mul(float2* out, const float2* in, const float2* mult, const int* loc, int nloc, int nx,
+                             int nrows, hipStream_t s);
 hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, const long long* src, const float2* in,
                              bool add, hipStream_t s);
 hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hipStream_t s);

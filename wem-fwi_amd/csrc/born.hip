@@ -201,6 +201,55 @@ __global__ void __launch_bounds__(64 * kRedWaves)
   if (wv == 0 && live) out[i] = make_double2(tx, ty);
 }
 
+// Phshift::forward/adjoint (phase_shift.ispc:2-32) on independent k rows:
+// out[k] (+)= in[k] P(k) (forward) or in[k] conj(P(k)) (adjoint) with
+// P = exp(-i kz dz), kz = re + i im of phase_shift.ispc:8-12 (im <= 0), on
+// the k grid of Phshift_cmplx.cpp:12-24: k[i] = i dk for i < nk/2,
+// k[nk-i] = -k[i] for 1 <= i < nk/2, other entries 0.  Evaluated in FP64 as
+// the phase tables are (see phase_table_kernel).
+__global__ void phshift_rows_kernel(float2* __restrict__ out, const float2* __restrict__ in, int nk, double dk,
+                                    double dz, double eps, const double* __restrict__ w,
+                                    const double2* __restrict__ sref, int nrows, int adjoint, int add) {
+  const long long n = (long long)nrows * nk;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int row = (int)(i / nk), ik = (int)(i - (long long)row * nk);
+    const int h = nk / 2;
+    double kk = 0.0;
+    if (ik < h) kk = ik * dk;
+    else if (nk - ik >= 1 && nk - ik < h) kk = -(nk - ik) * dk;
+    const double ww = w[row];
+    const double2 sr = sref[row];
+    const double a = ww * ww * sr.x - kk * kk;
+    const double b = ww * ww * (sr.y - eps * sr.x);
+    const double c = hypot(a, b);
+    const double big = sqrt(0.5 * (c + fabs(a)));
+    const double small = big > 0.0 ? fabs(b) / (2.0 * big) : 0.0;
+    const double re = a >= 0.0 ? big : small;
+    const double im = a >= 0.0 ? -small : -big;
+    const double amp = exp(im * dz);
+    double sn, cs;
+    sincos(re * dz, &sn, &cs);
+    const float2 p = make_float2((float)(amp * cs), (float)(adjoint ? amp * sn : -amp * sn));
+    const float2 v = cmul(in[i], p);
+    out[i] = add ? cadd(out[i], v) : v;
+  }
+}
+
+// out[r][loc[j]] = in[r][loc[j]] * mult[r][j] for nloc positions per row
+// (rows of width nx); other samples of out untouched
+__global__ void gather_mul_kernel(float2* __restrict__ out, const float2* __restrict__ in,
+                                  const float2* __restrict__ mult, const int* __restrict__ loc, int nloc, int nx,
+                                  int nrows) {
+  const long long n = (long long)nrows * nloc;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(i / nloc), j = (int)(i - (long long)r * nloc);
+    const long long x = (long long)r * nx + loc[j];
+    out[x] = cmul(in[x], mult[i]);
+  }
+}
+
 // out += in
 __global__ void add_kernel(float2* __restrict__ out, const float2* __restrict__ in, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -509,6 +558,23 @@ hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, con
                              bool add, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(gather_sum_kernel, grid1d(n), dim3(256), 0, s, out, n, ptr, src, in, add ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_phshift(float2* out, const float2* in, int nk, double dk, double dz, double eps, const double* w,
+                          const double2* sref, int nrows, bool adjoint, bool add, hipStream_t s) {
+  const long long n = (long long)nrows * nk;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(phshift_rows_kernel, grid1d(n), dim3(256), 0, s, out, in, nk, dk, dz, eps, w, sref, nrows,
+                     adjoint ? 1 : 0, add ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_mul(float2* out, const float2* in, const float2* mult, const int* loc, int nloc, int nx,
+                             int nrows, hipStream_t s) {
+  const long long n = (long long)nrows * nloc;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_mul_kernel, grid1d(n), dim3(256), 0, s, out, in, mult, loc, nloc, nx, nrows);
   return hipGetLastError();
 }
 

@@ -1432,6 +1432,75 @@ int wemfwi_dreflect(wemfwi_model* model, int adjoint, wemfwi_complex* dslow, wem
   return WEMFWI_OK;
 }
 
+int wemfwi_phshift(int nk, float dx, float dz, float eps, int adjoint, int nrows, const float* freq_hz,
+                   const wemfwi_complex* sref, const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream) {
+  if (nk < 2 || !(dx > 0.f) || nrows < 0 || (nrows > 0 && (!freq_hz || !sref || !in || !out)))
+    return fail(WEMFWI_ERR_ARG, "phshift: invalid argument");
+  if (nrows == 0) return WEMFWI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  // Phshift.h:24 w = 2 pi freq; Phshift_cmplx.cpp:13 dk = 2 pi / (dx (nk - 1))
+  std::vector<double> w(nrows);
+  std::vector<double2> sr(nrows);
+  for (int i = 0; i < nrows; ++i) {
+    w[i] = 2.0 * M_PI * (double)freq_hz[i];
+    sr[i] = make_double2(sref[i].re, sref[i].im);
+  }
+  DevBuf dw, dsr;
+  RCCHK(put(dw, w, s, "phshift frequencies"));
+  RCCHK(put(dsr, sr, s, "phshift slownesses"));
+  const double dk = 2.0 * M_PI / ((double)dx * (double)(nk - 1));
+  HIPCHK(launch_phshift((float2*)out, (const float2*)in, nk, dk, (double)dz, (double)eps, dw.as<double>(),
+                        dsr.as<double2>(), nrows, adjoint != 0, add != 0, s),
+         "phshift");
+  HIPCHK(hipStreamSynchronize(s), "phshift");
+  return WEMFWI_OK;
+}
+
+int wemfwi_splitstep(wemfwi_model* model, int adjoint, int nrows, const float* freq_hz, const int* depth,
+                     const wemfwi_complex* sref, int nloc, const int* loc, const wemfwi_complex* in,
+                     wemfwi_complex* out, int add, void* stream) {
+  if (!model || nrows < 0 || nloc < 0 || (nrows > 0 && (!freq_hz || !depth || !sref || !in || !out)) ||
+      (nloc > 0 && !loc))
+    return fail(WEMFWI_ERR_ARG, "splitstep: invalid argument");
+  Model& m = model->m;
+  const int nx = m.d.nx, nz = m.d.nz;
+  for (int i = 0; i < nrows; ++i)
+    if (depth[i] < 0 || depth[i] >= nz) return fail(WEMFWI_ERR_ARG, "splitstep: depth out of range");
+  for (int j = 0; j < nloc; ++j)
+    if (loc[j] < 0 || loc[j] >= nx) return fail(WEMFWI_ERR_ARG, "splitstep: location out of range");
+  hipStream_t s = (hipStream_t)stream;
+  if (!add && nrows > 0)
+    HIPCHK(hipMemsetAsync(out, 0, (size_t)nrows * nx * sizeof(float2), s), "memset");
+  if (nrows == 0 || nloc == 0) {
+    HIPCHK(hipStreamSynchronize(s), "splitstep");
+    return WEMFWI_OK;
+  }
+  // SplitStep.cpp:12-13 / 25-26, multipliers in FP64 on the host from the
+  // model's slowness: forward exp(-i w dz (S - sref)/sqrt(sref)), adjoint
+  // exp(+i w dz conj((S - sref)/sqrt(sref)))
+  using cd = std::complex<double>;
+  std::vector<float2> mult((size_t)nrows * nloc);
+  for (int i = 0; i < nrows; ++i) {
+    const double w = 2.0 * M_PI * (double)freq_hz[i];
+    const cd r(sref[i].re, sref[i].im);
+    for (int j = 0; j < nloc; ++j) {
+      const cf sv = m.slow[(size_t)depth[i] * nx + loc[j]];
+      const cd q = (cd(sv.real(), sv.imag()) - r) / std::sqrt(r);
+      const cd e = adjoint ? std::exp(cd(0.0, 1.0) * w * (double)m.d.dz * std::conj(q))
+                           : std::exp(cd(0.0, -1.0) * w * (double)m.d.dz * q);
+      mult[(size_t)i * nloc + j] = make_float2((float)e.real(), (float)e.imag());
+    }
+  }
+  std::vector<int> hl(loc, loc + nloc);
+  DevBuf dm, dl;
+  RCCHK(put(dm, mult, s, "splitstep multipliers"));
+  RCCHK(put(dl, hl, s, "splitstep locations"));
+  HIPCHK(launch_gather_mul((float2*)out, (const float2*)in, dm.as<float2>(), dl.as<int>(), nloc, nx, nrows, s),
+         "splitstep");
+  HIPCHK(hipStreamSynchronize(s), "splitstep");
+  return WEMFWI_OK;
+}
+
 int wemfwi_ic(int adjoint, long long n, const wemfwi_complex* bg, wemfwi_complex* m, wemfwi_complex* d, int add,
               void* stream) {
   if (n < 0 || !bg || !m || !d) return fail(WEMFWI_ERR_ARG, "ic: invalid argument");

@@ -73,6 +73,8 @@ SIGNATURES = {
     "wemfwi_survey_create": (_i, [_vp, C.POINTER(_vp)]),
     "wemfwi_survey_destroy": (_i, [_vp]),
     "wemfwi_survey_set_checkpoint": (_i, [_vp, _i]),
+    "wemfwi_phshift": (_i, [_i, C.c_float, C.c_float, C.c_float, _i, _i, _fp, _vp, _vp, _vp, _i, _vp]),
+    "wemfwi_splitstep": (_i, [_vp, _i, _i, _fp, _ip, _vp, _i, _ip, _vp, _vp, _i, _vp]),
     "wemfwi_autotune_report": (_i, [C.c_char_p, _i]),
     "wemfwi_uneven_offset_image_adjoint": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _i, _vp]),
     "wemfwi_wem_forward": (_i, [_vp, _vp, _vp, _i, _vp]),

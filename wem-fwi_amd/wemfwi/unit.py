@@ -1,7 +1,8 @@
 """The reference's unit operators (propagator/src/python/py_wem_ops.cpp:25-206)
-on the MI355X library: RefSampler, SSF / PSPI (one depth step), Down / Up,
-Scatter, LinDown / LinUp, Reflect, dReflect, IC and Injection, with the same
-constructor arguments, setFreq / setDepth / setStep / setShot state and
+on the MI355X library: RefSampler, Phshift, SplitStep, SSF / PSPI (one depth
+step), Down / Up, Scatter, LinDown / LinUp, Reflect, dReflect, IC and
+Injection, with the same constructor arguments, setFreq / setDepth / setSlow /
+setLocation / setStep / setShot state and
 forward(model, data, add) / adjoint(model, data, add) argument order as the
 C++ classes.
 
@@ -50,8 +51,12 @@ class RefSampler:
         self.nref = int(nref)
         self.sref, self.refid = _refsampler(_host_c64(slow), self.nref)
 
-    def getRefSlow(self, iz: int):  # noqa: N802 - reference name
-        return list(self.sref[iz])
+    def getRefSlow(self, iz: int, iref: Optional[int] = None):  # noqa: N802 - reference name
+        """Reference slowness iref of depth iz (RefSampler.h:19); without
+        iref, the list of all nref of them."""
+        if iref is None:
+            return [complex(v) for v in self.sref[iz]]
+        return complex(self.sref[iz][int(iref)])
 
     def getRefLoc(self, iz: int, iref: int):  # noqa: N802
         return list(np.nonzero(self.refid[iz] == iref)[0])
@@ -66,6 +71,93 @@ class _Stateful:
 
     def setDepth(self, iz: int):  # noqa: N802
         self.depth = int(iz)
+
+
+class Phshift:
+    """The phase shift of one reference slowness in the wavenumber domain
+    (Phshift.h:15-72, Phshift_cmplx.cpp:8-113, phase_shift.ispc:2-32) on rows
+    [..., nk], nk and the spacing from the first axis of hyper:
+    forward data (+)= model exp(-i kz dz), adjoint model (+)= data times the
+    conjugate factor, kz the damped vertical wavenumber of the slowness set by
+    setSlow at the frequency (Hz) set by setFreq.  nref and
+    setRef select the reference for the reflection helper of the C++ class,
+    which the Python bindings do not expose; they are kept as state."""
+
+    def __init__(self, dz: float, hyper, nref: int, eps: float = 0.04):
+        ax = hyper.getAxis(1)
+        self.nk, self.dx = int(ax.n), float(ax.d)
+        if self.nk < 2:
+            raise ValueError("Phshift: the wavenumber axis needs at least 2 samples")
+        self.dz, self.nref, self.eps = float(dz), int(nref), float(eps)
+        self.freq, self.slow, self.iref = 0.0, complex(0.0), 0
+
+    def setFreq(self, f: float):  # noqa: N802
+        self.freq = float(f)
+
+    def setSlow(self, s: complex):  # noqa: N802
+        self.slow = complex(s)
+
+    def setRef(self, iref: int):  # noqa: N802
+        self.iref = int(iref)
+
+    def _apply(self, adjoint, model, data, add):
+        src, dst = (model, data) if not adjoint else (data, model)
+        n = _rows(src, (self.nk,))
+        if tuple(dst.shape) != tuple(src.shape):
+            raise ValueError("Phshift: model and data shapes differ")
+        _dev_c64(dst, "wavefield")
+        f = np.full(n, self.freq, np.float32)
+        sr = np.full(n, self.slow, np.complex64)
+        check(lib.wemfwi_phshift(self.nk, self.dx, self.dz, self.eps, int(adjoint), n,
+                                 f.ctypes.data_as(C.POINTER(C.c_float)), C.c_void_p(sr.ctypes.data), _ptr(src),
+                                 _ptr(dst), int(bool(add)), _stream_ptr(None)))
+
+    def forward(self, model, data, add=False):
+        self._apply(False, model, data, add)
+
+    def adjoint(self, model, data, add=False):
+        self._apply(True, model, data, add)
+
+
+class SplitStep(_Stateful):
+    """The split-step correction of one reference slowness (SplitStep.h:8-32,
+    SplitStep.cpp:5-29) on rows [..., nx] of the slowness grid: at the
+    positions set by setLocation, forward data[x] = model[x] exp(-i w dz
+    (S[iz][x] - sref)/sqrt(sref)), adjoint model[x] = data[x] exp(+i w dz
+    conj(...)); the other samples are zero (add = False) or left as they
+    were (add = True), as in the reference."""
+
+    def __init__(self, slow: Vector):
+        ax_x, ax_z = slow.getHyper().getAxis(1), slow.getHyper().getAxis(2)
+        self.m = Model(_host_c64(slow), ax_x.d, ax_z.d)
+        self.loc = np.zeros(0, np.int32)
+        self.slow = complex(1.0)
+
+    def setLocation(self, loc: Sequence[int]):  # noqa: N802
+        self.loc = np.ascontiguousarray(np.asarray(loc, dtype=np.int32).reshape(-1))
+
+    def setSlow(self, s: complex):  # noqa: N802
+        self.slow = complex(s)
+
+    def _apply(self, adjoint, model, data, add):
+        src, dst = (model, data) if not adjoint else (data, model)
+        n = _rows(src, (self.m.nx,))
+        if tuple(dst.shape) != tuple(src.shape):
+            raise ValueError("SplitStep: model and data shapes differ")
+        _dev_c64(dst, "wavefield")
+        f = np.full(n, self.freq, np.float32)
+        d = np.full(n, self.depth, np.int32)
+        sr = np.full(n, self.slow, np.complex64)
+        check(lib.wemfwi_splitstep(self.m._h, int(adjoint), n, f.ctypes.data_as(C.POINTER(C.c_float)),
+                                   d.ctypes.data_as(C.POINTER(C.c_int)), C.c_void_p(sr.ctypes.data), len(self.loc),
+                                   self.loc.ctypes.data_as(C.POINTER(C.c_int)), _ptr(src), _ptr(dst),
+                                   int(bool(add)), _stream_ptr(None)))
+
+    def forward(self, model, data, add=False):
+        self._apply(False, model, data, add)
+
+    def adjoint(self, model, data, add=False):
+        self._apply(True, model, data, add)
 
 
 class SSF(_Stateful):
