@@ -114,6 +114,12 @@ SIGNATURES = {
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libwemfwi.so not built at {LIB_PATH} (run wem-fwi_amd/build.py)")
+    # torch ships its own libamdhip64.so.7 and libhsa-runtime64.so.1 (same
+    # sonames as /opt/rocm's).  Loaded first, they satisfy this library's
+    # dependencies and the process has one HIP runtime; loading this library
+    # first would bring in /opt/rocm's copies beside torch's, and the second
+    # HSA runtime reports no device.
+    import torch  # noqa: F401
     try:
         lib = C.CDLL(LIB_PATH)
     except OSError as e:  # pragma: no cover - environment specific

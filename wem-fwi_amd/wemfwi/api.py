@@ -416,7 +416,40 @@ def _model_vecs(model) -> List[Vector]:
     return [model]
 
 
-class Propagator(Operator):
+def _par_dict(par) -> dict:
+    """A paramObj-like argument as a dict (a Mapping, or an object holding
+    one in .pars as genericIO.pythonParams does)."""
+    return dict(getattr(par, "pars", par))
+
+
+class _Picklable:
+    """Pickling of an operator (WEM.py:32-35, 45-47, 77-80): the native
+    objects (the acquisition's surveys and device plans, the background
+    models) are dropped and rebuilt on unpickling from the wave, the
+    parameters and a host copy of the background, so an unpickled operator
+    computes what the pickled one did."""
+
+    _native = ("acq", "_bg")
+
+    def __getstate__(self):
+        state = self.__dict__.copy()
+        for k in self._native:
+            state.pop(k, None)
+        return state
+
+    def __setstate__(self, state):
+        self.__dict__.update(state)
+        self.acq = _Acquisition(self.wave, self._hyper, self.par)
+        self._bg = None
+        if getattr(self, "_bg_host", None) is not None:
+            self._load_bg(self._bg_host)
+
+    def get_nfreq(self) -> int:
+        """Number of propagated frequencies (PropParam.h:42, pyWEM.cpp:36-39)."""
+        return self.acq.nw
+
+
+class Propagator(_Picklable, Operator):
     """Base of the shot x frequency operators (WEM.py:8-41)."""
 
     def __init__(self, model, data, wave, par):
@@ -424,8 +457,19 @@ class Propagator(Operator):
             raise TypeError("Model vector should be a superVector")
         super().__init__(model, data)
         self.wave = wave
-        self.par = dict(par)
-        self.acq = _Acquisition(wave, _model_vecs(model)[0].getHyper(), self.par)
+        self.par = _par_dict(par)
+        self._hyper = _model_vecs(model)[0].getHyper()
+        self.acq = _Acquisition(wave, self._hyper, self.par)
+
+    @classmethod
+    def from_subspace(cls, subdomain, subrange, wave, par, *args, **kw):
+        """The operator of a subset of the shots (WEM.py:24-30): par's ns and
+        osx are taken from the third (shot) axis of subrange."""
+        pars = _par_dict(par)
+        ax = subrange.getHyper().getAxis(3)
+        pars["ns"] = ax.n
+        pars["osx"] = ax.o
+        return cls(subdomain, subrange, wave, pars, *args, **kw)
 
 
 class WEM(Propagator):
@@ -492,12 +536,16 @@ class Born(Propagator):
 
     def set_background(self, slow):
         vecs = _model_vecs(slow)
-        den = vecs[1] if len(vecs) > 1 else None
+        self._load_bg([_host_c64(v) for v in vecs[:2]])
+
+    def _load_bg(self, host):
+        den = host[1] if len(host) > 1 else None
         self._close_bg()
         if self.acq.cmap is not None:
-            self._bg = self.acq.shot_models(vecs[0], den)
+            self._bg = self.acq.shot_models(host[0], den)
         else:
-            self._bg = self.acq.model(vecs[0], den)
+            self._bg = self.acq.model(host[0], den)
+        self._bg_host = host
 
     def _close_bg(self):
         for m in (self._bg if isinstance(self._bg, list) else [self._bg]):
@@ -583,24 +631,48 @@ class Born(Propagator):
                 v.arr.imag.zero_()
 
 
-class BornRefl(Operator):
+class BornRefl(_Picklable, Operator):
     """Frequency-extended reflectivity modeling / migration
     (BornRefl.h:9-58, ext_born_refl.cpp:8-110): model is an image
     [nw][nz][nx] on the unpadded grid, data float traces (the reference's
-    interface) or complex frequency data."""
+    interface) or complex frequency data.  add_prec / add_pad register
+    operators that set_background applies to the slowness first
+    (WEM.py:101-115)."""
 
     def __init__(self, model, slowness, data, wave, par):
         super().__init__(model, data)
         self.wave = wave
-        self.par = dict(par)
-        self.acq = _Acquisition(wave, slowness.getHyper(), self.par)
+        self.par = _par_dict(par)
+        self._hyper = slowness.getHyper()
+        self.acq = _Acquisition(wave, self._hyper, self.par)
         self._bg = None
+        self.prec = None
+        self.pad = None
         self.set_background(slowness)
 
+    def add_prec(self, op):
+        self.prec = op
+        self.big_model = op.range.clone()
+
+    def add_pad(self, op):
+        self.pad = op
+        self.pad_model = op.range.clone()
+
     def set_background(self, slow):
+        if self.prec is not None:
+            self.prec.forward(False, slow, self.big_model)
+            slow = self.big_model
+        if self.pad is not None:
+            self.pad.forward(False, slow, self.pad_model)
+            slow = self.pad_model
+        self._load_bg([_host_c64(slow)])
+
+    def _load_bg(self, host):
         if self._bg is not None:
             self._bg.close()
-        self._bg = self.acq.model(slow, None, padded=False)
+            self._bg = None
+        self._bg = self.acq.model(host[0], None, padded=False)
+        self._bg_host = host
 
     def forward(self, add, model, data):
         f = self.acq.freq_data(data)
@@ -652,11 +724,15 @@ class extBorn(Propagator):  # noqa: N801 - reference name
 
     def set_background(self, slow):
         vecs = _model_vecs(slow)
-        S = _host_c64(vecs[0])
-        R = _host_c64(vecs[1]) if len(vecs) > 1 else None
-        for m in self._bg:
+        self._load_bg([_host_c64(v) for v in vecs[:2]])
+
+    def _load_bg(self, host):
+        S = host[0]
+        R = host[1] if len(host) > 1 else None
+        for m in self._bg or []:
             m.close()
         self._bg = [self.acq.model(S[i], None if R is None else R[i]) for i in range(S.shape[0])]
+        self._bg_host = host
 
     def forward(self, add, model, data):
         vecs = _model_vecs(model)
