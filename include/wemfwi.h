@@ -379,6 +379,21 @@ typedef struct wemfwi_image_shot {
 int wemfwi_uneven_offset_image_adjoint(const wemfwi_image_shot* shots, int ns, int ngs, int nh, int nz,
                                        int nx_image, wemfwi_complex* image, int add, void* stream);
 
+/* Extended least-squares objective of the same shots and its gradient:
+ * J = 1/2 sum_s ||L_s I - d_s||^2 and grad = sum_s L_s^H (L_s I - d_s), where
+ * I = model is a device image [nh][nz][nx_image], d_s = shots[s].data the
+ * observed data [nr][nw], and L_s I models each (shot, frequency) row as
+ * BornRefl::fwd_propagate (ext_born_refl.cpp:8-56: D = Down(source), Up,
+ * receivers) with the offset scatter U(z, y) = sum_h I(h, z, x0 + y - h)
+ * D(z, y - 2h) (the adjoint of the offset imaging condition) in place of IC.
+ * L_s^H is wemfwi_uneven_offset_image_adjoint of the residual; D is computed
+ * once per row for both.  grad (+)= the gradient (device, distinct from
+ * model); *misfit (host) = J, summed in FP64 in a fixed order.  Arguments
+ * and errors as wemfwi_uneven_offset_image_adjoint. */
+int wemfwi_uneven_offset_gradient(const wemfwi_image_shot* shots, int ns, int ngs, int nh, int nz, int nx_image,
+                                  const wemfwi_complex* model, wemfwi_complex* grad, double* misfit, int add,
+                                  void* stream);
+
 /* ---------------------------------------------------------------- extended FWI
  * Operators of the frequency-extended objective of scripts/efwi.py:160-190
  * (operator/src/python/Operator.py:228-292,619-775), on device models

@@ -40,6 +40,10 @@ reference's propagator (paths are relative to the reference repository):
   extborn_forward / _adjoint   propagator/src/ext_born_prop.cpp:8-194
   offset_ic_fwd / _adj         no reference counterpart (subsurface-offset
                                imaging condition, extends ic.cpp)
+  offset_forward,              BornRefl's fwd/adj propagation
+  offset_objective             (ext_born_refl.cpp:8-110) with the offset
+                               imaging condition in place of ic.cpp: the
+                               extended least-squares objective and gradient
   mapped_models,               propagator/src/wem_prop.cpp:20-45 and
   wem_forward_mapped,          born_prop.cpp:28-70,136-174,217-231 (conformal
   born_*_mapped                maps; tables from oracle/cmap_oracle.py)
@@ -932,6 +936,42 @@ def offset_ic_fwd(bg: np.ndarray, image: np.ndarray) -> np.ndarray:
         xs = x[ok]
         out[:, xs + h] += bg[:, xs - h] * image[j][:, xs]
     return out
+
+
+def offset_forward(sv: Survey, slow, image) -> np.ndarray:
+    """Data of the subsurface-offset extended image [nh][nz][nx] (the
+    adjoint of the offset migration): per (shot, frequency) D = Down(source)
+    as BornRefl::fwd_propagate (ext_born_refl.cpp:8-56), the scattered field
+    offset_ic_fwd(D, image) in place of IC, Up, and the receivers."""
+    sp = np.asarray(slow, dtype=C128)
+    p = sv.step_params(padded=False)
+    ref = refsampler(sp, sv.nref)
+    data = np.zeros((sv.ns, sv.nr, sv.nt_f), dtype=C128)
+    for i in range(len(sv.freq)):
+        w = sv.omega(i)
+        bi = sv.index[i]
+        for s in range(sv.ns):
+            bg = inj_wavelet_fwd(sv.wave_f[bi], [sv.src_iz[s]], [sv.src_ix[s]], sv.ng, 0, sv.nz, sv.nx)
+            bg = down_fwd(bg, w, sp, ref, p)
+            sc = up_fwd(offset_ic_fwd(bg, image), w, sp, ref, p)
+            data[s, :, bi] += inj_rec_adj(sc, sv.rec_iz, sv.rec_ix, 0, 0)
+    return data
+
+
+def offset_objective(sv: Survey, slow, image, dobs):
+    """J = 1/2 ||L I - dobs||^2 over the survey's frequency bins and its
+    gradient L^H (L I - dobs) [nh][nz][nx], L = offset_forward; the adjoint
+    is the offset migration of bornrefl_wavefields with offset_ic_adj."""
+    nh = image.shape[0]
+    res = offset_forward(sv, slow, image) - np.asarray(dobs, dtype=C128)
+    mask = np.zeros(sv.nt_f, bool)
+    mask[np.asarray(sv.index)] = True
+    res[:, :, ~mask] = 0.0
+    J = 0.5 * float(np.sum(np.abs(res) ** 2))
+    grad = np.zeros((nh, sv.nz, sv.nx), dtype=C128)
+    for _, _, bg, sc in bornrefl_wavefields(sv, slow, res):
+        grad += offset_ic_adj(bg, sc, nh)
+    return J, grad
 
 
 def extwem_forward(sv: Survey, slow3, den3) -> np.ndarray:

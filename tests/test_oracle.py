@@ -340,6 +340,42 @@ def test_born_dot_product_exact_mode():
     assert abs(lhs - rhs) <= 1e-9 * abs(lhs)
 
 
+def test_offset_forward_and_objective():
+    """The offset-extended modelling L: with one lag it is BornRefl's forward
+    with the image repeated over frequency (ext_born_refl.cpp:8-56); L and
+    the offset migration are an exact adjoint pair; the objective's gradient
+    is the derivative of J (central differences exact for a quadratic), and
+    <I, grad J(I; 0)> = 2 J(I; 0)."""
+    sv, slow, den = _survey(pad=0, tap=2)
+    rng = np.random.default_rng(8)
+    nw = len(sv.freq)
+    img1 = crand(1, sv.nz, sv.nx, rng=rng)
+    np.testing.assert_allclose(O.offset_forward(sv, slow, img1),
+                               O.bornrefl_forward(sv, slow, np.repeat(img1, nw, axis=0)), rtol=1e-12, atol=1e-300)
+    nh = 5
+    img = crand(nh, sv.nz, sv.nx, rng=rng)
+    d = np.zeros((sv.ns, sv.nr, sv.nt_f), complex)
+    d[:, :, sv.index] = crand(sv.ns, sv.nr, nw, rng=rng)
+    adj = np.zeros_like(img)
+    for _, _, bg, sc in O.bornrefl_wavefields(sv, slow, d):
+        adj += O.offset_ic_adj(bg, sc, nh)
+    lhs = O.cdot(O.offset_forward(sv, slow, img), d)
+    rhs = O.cdot(img, adj)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    # the objective
+    J0, g0 = O.offset_objective(sv, slow, img, np.zeros_like(d))
+    assert abs(O.cdot(img, g0) - 2 * J0) <= 1e-12 * J0
+    J, g = O.offset_objective(sv, slow, img, d)
+    V = crand(nh, sv.nz, sv.nx, rng=rng)
+    e = 1e-3
+    Jp, _ = O.offset_objective(sv, slow, img + e * V, d)
+    Jm, _ = O.offset_objective(sv, slow, img - e * V, d)
+    assert abs((Jp - Jm) / (2 * e) - O.cdot(V, g)) <= 1e-8 * abs(O.cdot(V, g))
+    Jz, gz = O.offset_objective(sv, slow, np.zeros_like(img), d)
+    np.testing.assert_allclose(gz, -adj, rtol=1e-12, atol=1e-300)
+    assert abs(Jz - 0.5 * float(np.sum(np.abs(d) ** 2))) <= 1e-12 * Jz
+
+
 def test_bornrefl_dot_product():
     sv, slow, den = _survey(pad=2, tap=2)
     rng = np.random.default_rng(6)

@@ -7,6 +7,12 @@
 // of frequencies.  All (shot, frequency) rows of a group of shots are swept by
 // two batched launches (rows of different widths share a launch per FFT
 // length); the image is reduced in FP64, shot after shot and row after row.
+//
+// The same driver gives the extended least-squares objective of such shots,
+// J(I) = 1/2 sum_s ||L_s I - d_s||^2, and its gradient sum_s L_s^H (L_s I - d_s):
+// L_s models each row as BornRefl's forward (ext_born_refl.cpp:8-56) with the
+// offset scatter U(z, y) = sum_h I(h, z, y - h) D(z, y - 2h) in place of IC, so
+// a group runs D = Down(sources) once and shares it between L and L^H.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -149,6 +155,136 @@ hipError_t launch_offset_image_rows(double2* img, long long hstride, int nxi, in
   return launch_oir<41>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
 }
 
+// U_r[z][y] = sum_h I[h][z][x0 + y - h] D_r[z][y - 2h] for the nrows rows of
+// one shot, terms with y - h or y - 2h outside [0, nx) absent: the adjoint of
+// offset_image_rows_kernel with respect to R.  A workgroup owns one depth and
+// kTX consecutive y; the image tile (nh lags, halo nh/2) is staged in LDS once
+// for all rows of the shot, each row's D segment (halo 2 (nh/2)) once per
+// row.  Every (z, y < nx) of every row is written.
+template <int NHM>
+__global__ void __launch_bounds__(kTX)
+    offset_scatter_rows_kernel(float2* __restrict__ U, const float2* __restrict__ img, long long hstride, int nxi,
+                               int x0, const float2* __restrict__ D, long long rstride, int nrows, int nz, int nx,
+                               int nh) {
+  constexpr int HM = NHM / 2;
+  __shared__ float2 si[NHM * (kTX + 2 * HM)];
+  __shared__ float2 sd[kTX + 4 * HM];
+  const int hh = nh / 2;
+  const int wi = kTX + 2 * hh;
+  const int wd = kTX + 4 * hh;
+  const int z = blockIdx.y;
+  const int xb = blockIdx.x * kTX;
+  const int t = threadIdx.x;
+  const int y = xb + t;
+  for (int i = t; i < nh * wi; i += kTX) {
+    const int j = i / wi, c = i - j * wi;
+    const int xx = xb - hh + c;
+    si[i] = (xx >= 0 && xx < nx) ? img[j * hstride + (long long)z * nxi + x0 + xx] : make_float2(0.f, 0.f);
+  }
+  for (int r = 0; r < nrows; ++r) {
+    const float2* d = D + r * rstride + (long long)z * nx;
+    __syncthreads();
+    for (int i = t; i < wd; i += kTX) {
+      const int xx = xb - 2 * hh + i;
+      sd[i] = (xx >= 0 && xx < nx) ? ldg(d, (size_t)xx) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (y < nx) {
+      float ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int j = 0; j < NHM; ++j) {
+        const int h = j - hh;
+        if (j < nh && y - h >= 0 && y - h < nx && y - 2 * h >= 0 && y - 2 * h < nx) {
+          const float2 a = si[j * wi + t - h + hh];
+          const float2 b = sd[t - 2 * h + 2 * hh];
+          ax = fmaf(a.x, b.x, fmaf(-a.y, b.y, ax));
+          ay = fmaf(a.x, b.y, fmaf(a.y, b.x, ay));
+        }
+      }
+      U[r * rstride + (long long)z * nx + y] = make_float2(ax, ay);
+    }
+  }
+}
+
+template <int NHM>
+hipError_t launch_osr(float2* U, const float2* img, long long hstride, int nxi, int x0, const float2* D,
+                      long long rstride, int nrows, int nz, int nx, int nh, hipStream_t s) {
+  dim3 grid((unsigned)((nx + kTX - 1) / kTX), (unsigned)nz);
+  hipLaunchKernelGGL((offset_scatter_rows_kernel<NHM>), grid, dim3(kTX), 0, s, U, img, hstride, nxi, x0, D, rstride,
+                     nrows, nz, nx, nh);
+  return hipGetLastError();
+}
+
+hipError_t launch_offset_scatter_rows(float2* U, const float2* img, long long hstride, int nxi, int x0,
+                                      const float2* D, long long rstride, int nrows, int nz, int nx, int nh,
+                                      hipStream_t s) {
+  if (nrows <= 0 || nz <= 0 || nx <= 0) return hipSuccess;
+  if (nh > 2 * kHalo + 1 || nz > 65535) return hipErrorInvalidValue;
+  if (nh <= 1) return launch_osr<1>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
+  if (nh <= 3) return launch_osr<3>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
+  if (nh <= 11) return launch_osr<11>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
+  if (nh <= 21) return launch_osr<21>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
+  return launch_osr<41>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
+}
+
+// Residual of one row at its receivers: res[r * stride + col] =
+// W[iz_r][ix_r] - dobs[r * stride + col] (Injection::adjoint without spread,
+// Injection.cpp:56-67, minus the observed sample).  Grid: (receiver blocks,
+// rows, strided past 65535).
+struct ResRow {
+  const float2* W;
+  int nx, nr;
+  const int* iz;
+  const int* ix;
+  const float2* dobs;
+  float2* res;
+  int stride, col;
+};
+
+__global__ void residual_rows_kernel(const ResRow* __restrict__ rows, int nrows) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const ResRow R = rows[row];
+    if (r >= R.nr) continue;
+    const size_t k = (size_t)r * R.stride + R.col;
+    R.res[k] = csub(R.W[(size_t)R.iz[r] * R.nx + R.ix[r]], R.dobs[k]);
+  }
+}
+
+// Sum of |a|^2 in FP64 in a fixed order: kSumBlocks blocks reduce fixed
+// strided slices into part[], one block adds part[] in order to *out.
+constexpr int kSumBlocks = 256;
+constexpr int kSumThreads = 256;
+
+__global__ void __launch_bounds__(kSumThreads)
+    sumsq_partial_kernel(const float2* __restrict__ a, long long n, double* __restrict__ part) {
+  __shared__ double red[kSumThreads];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * kSumThreads + threadIdx.x; i < n;
+       i += (long long)kSumBlocks * kSumThreads) {
+    const float2 v = a[i];
+    acc += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kSumThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kSumThreads) sumsq_final_kernel(const double* __restrict__ part, double* out) {
+  __shared__ double red[kSumThreads];
+  red[threadIdx.x] = (int)threadIdx.x < kSumBlocks ? part[threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int w = kSumThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out += red[0];
+}
+
 template <typename T>
 int upload_vec(Workspace& w, const std::vector<T>& h, hipStream_t s, const char* what) {
   if (h.empty()) return WEMFWI_OK;
@@ -170,9 +306,13 @@ int upload_vec(Workspace& w, const std::vector<T>& h, hipStream_t s, const char*
   } while (0)
 
 // One group of shots [s0, s1): inject, sweep both wavefields of all rows,
-// reduce the image shot by shot.
+// reduce the image shot by shot.  With a model image (fwi), the receiver
+// wavefield is the residual's: the shots' data are the observed data, the
+// rows are first modelled from the image (scatter, Up, record), and
+// 1/2 sum |residual|^2 is added to *jacc (device).
 int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vector<float>& gw, int ngs, int nh,
-                int nxi, double2* acc, hipStream_t s) {
+                int nxi, double2* acc, hipStream_t s, const float2* fwi = nullptr, double* jacc = nullptr,
+                double* jpart = nullptr) {
   std::vector<Model*> rm;
   std::vector<float> rf;
   std::vector<size_t> roff;
@@ -219,6 +359,35 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
       hr[k] = RecRow{R + roff[k], nx, S.nr, riz, riz + S.nr, (const float2*)S.data, S.nw, i};
     }
   }
+  // fwi: the residual of every row, shot by shot as the data ([nr][nw])
+  ScopedWs resw, resrw;
+  float2* res = nullptr;
+  int nr_max = 0;
+  size_t nres = 0;
+  if (fwi) {
+    std::vector<size_t> res_off(s1 - s0);
+    for (int sh = s0; sh < s1; ++sh) {
+      res_off[sh - s0] = nres;
+      nres += (size_t)shots[sh].nr * shots[sh].nw;
+      nr_max = std::max(nr_max, shots[sh].nr);
+    }
+    if (nres > 0) {
+      UCHK(resw.reserve(nres * sizeof(float2), "residuals"));
+      res = (float2*)resw.ptr;
+    }
+    std::vector<ResRow> hq(nrows);
+    k = 0;
+    for (int sh = s0; sh < s1; ++sh) {
+      const wemfwi_image_shot& S = shots[sh];
+      const int* riz = ddev + dev_off[sh - s0];
+      for (int i = 0; i < S.nw; ++i, ++k) {
+        float2* rs = res + res_off[sh - s0];
+        hq[k] = ResRow{R + roff[k], S.model->m.d.nx, S.nr, riz, riz + S.nr, (const float2*)S.data, rs, S.nw, i};
+        hr[k].data = rs;   // the receiver wavefield injects the residual
+      }
+    }
+    UCHK(upload_vec(resrw, hq, s, "residual rows"));
+  }
   UCHK(upload_vec(srcw, hs, s, "source rows"));
   UCHK(upload_vec(recw, hr, s, "receiver rows"));
   UCHK(upload_vec(gww, gw, s, "spread weights"));
@@ -226,8 +395,10 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
   hipLaunchKernelGGL(src_rows_kernel, dim3((nrows * span + 255) / 256), dim3(256), 0, s, (const SrcRow*)srcw.ptr,
                      nrows, (const float*)gww.ptr, ngs);
   UHIP(hipGetLastError(), "inject sources");
-  hipLaunchKernelGGL(rec_rows_kernel, dim3((nrows + 63) / 64), dim3(64), 0, s, (const RecRow*)recw.ptr, nrows);
-  UHIP(hipGetLastError(), "inject receivers");
+  if (!fwi) {
+    hipLaunchKernelGGL(rec_rows_kernel, dim3((nrows + 63) / 64), dim3(64), 0, s, (const RecRow*)recw.ptr, nrows);
+    UHIP(hipGetLastError(), "inject receivers");
+  }
   UHIP(hipStreamSynchronize(s), "inject");
   // D = Down(sources), R = Up^H(receivers), in place, every row in one plan
   for (int i = 0; i < nrows; ++i) pin[i] = pout[i] = (cf*)(D + roff[i]);
@@ -240,6 +411,33 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
   {
     BatchPlan p;
     UCHK(p.build(nrows, rm.data(), rf.data(), pin.data(), pout.data(), s));
+    if (fwi) {
+      // L I: offset scatter of the image on D into every row, Up, record
+      k = 0;
+      const long long hs_img = (long long)shots[s0].model->m.d.nz * nxi;
+      for (int sh = s0; sh < s1; ++sh) {
+        const Model& m = shots[sh].model->m;
+        UHIP(launch_offset_scatter_rows(R + roff[k], fwi, hs_img, nxi, shots[sh].x0, D + roff[k],
+                                        (long long)m.d.nz * m.d.nx, shots[sh].nw, m.d.nz, m.d.nx, nh, s),
+             "offset scatter");
+        k += shots[sh].nw;
+      }
+      UCHK(p.run(1, 0, s));
+      if (nr_max > 0) {
+        hipLaunchKernelGGL(residual_rows_kernel, dim3((unsigned)((nr_max + 255) / 256), (unsigned)std::min(nrows, 65535)),
+                           dim3(256), 0, s, (const ResRow*)resrw.ptr, nrows);
+        UHIP(hipGetLastError(), "residual");
+        hipLaunchKernelGGL(sumsq_partial_kernel, dim3(kSumBlocks), dim3(kSumThreads), 0, s, (const float2*)res,
+                           (long long)nres, jpart);
+        UHIP(hipGetLastError(), "misfit");
+        hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(kSumThreads), 0, s, (const double*)jpart, jacc);
+        UHIP(hipGetLastError(), "misfit");
+      }
+      // R^H residual, then Up^H as in the migration
+      UHIP(hipMemsetAsync(R, 0, total * sizeof(float2), s), "memset");
+      hipLaunchKernelGGL(rec_rows_kernel, dim3((nrows + 63) / 64), dim3(64), 0, s, (const RecRow*)recw.ptr, nrows);
+      UHIP(hipGetLastError(), "inject residuals");
+    }
     UCHK(p.run(1, 1, s));
   }
   // image, shot after shot
@@ -261,11 +459,16 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
 
 using namespace wemfwi;
 
-extern "C" int wemfwi_uneven_offset_image_adjoint(const wemfwi_image_shot* shots, int ns, int ngs, int nh, int nz,
-                                                  int nx_image, wemfwi_complex* image, int add, void* stream) {
+namespace {
+
+// Validation, grouping and the FP64 image of both entry points; fwi (device
+// image [nh][nz][nx_image]) selects the objective and gradient, whose misfit
+// goes to *misfit (host).
+int uneven_run(const char* name, const wemfwi_image_shot* shots, int ns, int ngs, int nh, int nz, int nx_image,
+               const float2* fwi, wemfwi_complex* image, double* misfit, int add, hipStream_t s) {
+  const std::string who = name;
   if (!shots || ns < 0 || ngs < 0 || nh < 1 || nh > 41 || nh % 2 == 0 || nz < 2 || nx_image < 1 || !image)
-    return fail(WEMFWI_ERR_ARG, "uneven_offset_image_adjoint: invalid argument (nh odd, 1..41)");
-  hipStream_t s = (hipStream_t)stream;
+    return fail(WEMFWI_ERR_ARG, who + ": invalid argument (nh odd, 1..41)");
   size_t budget = 0;
   {
     size_t fr = 0, tot = 0;
@@ -275,30 +478,32 @@ extern "C" int wemfwi_uneven_offset_image_adjoint(const wemfwi_image_shot* shots
   // validate every shot before touching the image
   for (int sh = 0; sh < ns; ++sh) {
     const wemfwi_image_shot& S = shots[sh];
+    const std::string at = who + ": shot " + std::to_string(sh);
     if (!S.model || S.nw < 0 || (S.nw > 0 && (!S.freq_hz || !S.wavelet || !S.data)) || S.nr < 0 ||
         (S.nr > 0 && (!S.rec_iz || !S.rec_ix)))
-      return fail(WEMFWI_ERR_ARG, "uneven_offset_image_adjoint: shot " + std::to_string(sh) + ": null argument");
+      return fail(WEMFWI_ERR_ARG, at + ": null argument");
     const Model& m = S.model->m;
-    if (m.d.nz != nz)
-      return fail(WEMFWI_ERR_ARG, "uneven_offset_image_adjoint: shot " + std::to_string(sh) + ": model nz differs");
-    if (S.x0 < 0 || S.x0 + m.d.nx > nx_image)
-      return fail(WEMFWI_ERR_ARG, "uneven_offset_image_adjoint: shot " + std::to_string(sh) +
-                                      ": aperture outside the image");
-    if (S.src_iz < 0 || S.src_iz >= nz)
-      return fail(WEMFWI_ERR_ARG, "uneven_offset_image_adjoint: shot " + std::to_string(sh) + ": source depth");
+    if (m.d.nz != nz) return fail(WEMFWI_ERR_ARG, at + ": model nz differs");
+    if (S.x0 < 0 || S.x0 + m.d.nx > nx_image) return fail(WEMFWI_ERR_ARG, at + ": aperture outside the image");
+    if (S.src_iz < 0 || S.src_iz >= nz) return fail(WEMFWI_ERR_ARG, at + ": source depth");
     for (int r = 0; r < S.nr; ++r)
       if (S.rec_iz[r] < 0 || S.rec_iz[r] >= nz || S.rec_ix[r] < 0 || S.rec_ix[r] >= m.d.nx)
-        return fail(WEMFWI_ERR_ARG, "uneven_offset_image_adjoint: shot " + std::to_string(sh) +
-                                        ": receiver outside the aperture");
+        return fail(WEMFWI_ERR_ARG, at + ": receiver outside the aperture");
     if (S.nw > 0) {
       if (int rc = m.require_fft()) return rc;
     }
   }
   const size_t npix = (size_t)nh * nz * nx_image;
-  ScopedWs accw;
+  ScopedWs accw, jw;
   UCHK(accw.reserve(npix * sizeof(double2), "image accumulator"));
   double2* acc = (double2*)accw.ptr;
   UHIP(hipMemsetAsync(acc, 0, npix * sizeof(double2), s), "memset");
+  double* jacc = nullptr;
+  if (fwi) {
+    UCHK(jw.reserve((1 + kSumBlocks) * sizeof(double), "misfit"));
+    jacc = (double*)jw.ptr;
+    UHIP(hipMemsetAsync(jacc, 0, sizeof(double), s), "memset");
+  }
   const std::vector<float> gw = cos2_window(ngs);
   // groups of consecutive shots whose two wavefields fit half the free memory
   int s0 = 0;
@@ -312,10 +517,33 @@ extern "C" int wemfwi_uneven_offset_image_adjoint(const wemfwi_image_shot* shots
       bytes += b;
       ++s1;
     }
-    UCHK(image_group(shots, s0, s1, gw, ngs, nh, nx_image, acc, s));
+    UCHK(image_group(shots, s0, s1, gw, ngs, nh, nx_image, acc, s, fwi, jacc, jacc ? jacc + 1 : nullptr));
     s0 = s1;
   }
   UHIP(launch_d2f((float2*)image, acc, (long long)npix, add != 0, s), "image");
+  if (fwi) {
+    double j = 0.0;
+    UHIP(hipMemcpyAsync(&j, jacc, sizeof(double), hipMemcpyDeviceToHost, s), "misfit");
+    UHIP(hipStreamSynchronize(s), "misfit");
+    *misfit = 0.5 * j;
+  }
   UHIP(hipStreamSynchronize(s), "image");
   return WEMFWI_OK;
+}
+
+}  // namespace
+
+extern "C" int wemfwi_uneven_offset_image_adjoint(const wemfwi_image_shot* shots, int ns, int ngs, int nh, int nz,
+                                                  int nx_image, wemfwi_complex* image, int add, void* stream) {
+  return uneven_run("uneven_offset_image_adjoint", shots, ns, ngs, nh, nz, nx_image, nullptr, image, nullptr, add,
+                    (hipStream_t)stream);
+}
+
+extern "C" int wemfwi_uneven_offset_gradient(const wemfwi_image_shot* shots, int ns, int ngs, int nh, int nz,
+                                             int nx_image, const wemfwi_complex* model, wemfwi_complex* grad,
+                                             double* misfit, int add, void* stream) {
+  if (!model || !misfit) return fail(WEMFWI_ERR_ARG, "uneven_offset_gradient: null model or misfit");
+  if (model == grad) return fail(WEMFWI_ERR_ARG, "uneven_offset_gradient: model and gradient must differ");
+  return uneven_run("uneven_offset_gradient", shots, ns, ngs, nh, nz, nx_image, (const float2*)model, grad, misfit,
+                    add, (hipStream_t)stream);
 }

@@ -243,6 +243,26 @@ def uneven_offset_image_adjoint(shots: Sequence[ImageShot], nh: int, image: torc
     return image
 
 
+def uneven_offset_gradient(shots: Sequence[ImageShot], nh: int, model: torch.Tensor, grad: torch.Tensor,
+                           ngs: int = 0, add: bool = False, stream=None) -> float:
+    """Extended least-squares objective J = 1/2 sum_s ||L_s I - d_s||^2 of
+    shots with their own apertures and bands (each ImageShot's data are its
+    observed data) and its gradient sum_s L_s^H (L_s I - d_s) into grad;
+    model I and grad are [nh][nz][nx_image] (CUDA complex64).  L_s is the
+    adjoint of the shot's offset migration (uneven_offset_image_adjoint).
+    Returns J (FP64 sum)."""
+    _dev_c64(model, "model")
+    _dev_c64(grad, "grad")
+    if model.dim() != 3 or model.shape[0] != nh or tuple(grad.shape) != tuple(model.shape):
+        raise ValueError("model and grad must both be [nh][nz][nx_image]")
+    arr = (ImageShotDesc * max(1, len(shots)))(*[sh.desc() for sh in shots])
+    j = C.c_double(0.0)
+    check(lib.wemfwi_uneven_offset_gradient(C.cast(arr, C.c_void_p), len(shots), int(ngs), int(nh),
+                                            int(model.shape[1]), int(model.shape[2]), _ptr(model), _ptr(grad),
+                                            C.byref(j), int(add), _stream_ptr(stream)))
+    return float(j.value)
+
+
 def shotwem_forward(models: Sequence[Model], survey: Survey, data: torch.Tensor, add: bool = False,
                     stream=None):
     """WEM with one background model per shot (conformal-map grids)."""
