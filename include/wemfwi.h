@@ -57,6 +57,20 @@ int wemfwi_version(void);
 /* Number of visible devices (0 without a GPU). */
 int wemfwi_device_count(void);
 
+/* Device memory cache of the library's per-call buffers (no counterpart in
+ * the reference, whose operators allocate host scratch per TBB task,
+ * wem_prop.cpp:14-17).  Buffers freed at the end of a call are kept per
+ * device for the next call of a similar size instead of being returned to
+ * the driver, which clears freed VRAM before handing it out again (hundreds
+ * of ms for the tens of GB of a batched call).  At most WEMFWI_CACHE_LIMIT_MB
+ * (environment; default half of the device memory) is kept; a failing
+ * allocation empties the cache and retries.  wemfwi_cache_bytes: bytes held
+ * for the current device; wemfwi_cache_release: returns every cached block
+ * of every device to the driver (for callers that allocate elsewhere, e.g.
+ * torch, after a large call). */
+long long wemfwi_cache_bytes(void);
+int wemfwi_cache_release(void);
+
 /* Frequency grid and wavelet spectrum of PropParam::calcPropParam
  * (propagator/src/PropParam.h:45-85), host only, float32 arithmetic as the
  * reference: the float wavelet wave[nt] (time origin ot, step dt) is

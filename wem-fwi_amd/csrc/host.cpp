@@ -6,8 +6,10 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -121,8 +123,107 @@ int refsampler_host(int nz, int nx, const cf* slow, int nref, cf* sref, int* ref
 
 // ------------------------------------------------------------------ device helpers
 
+namespace {
+
+struct BlockCache {
+  std::mutex mu;
+  std::map<int, std::multimap<size_t, void*>> free_blocks;  // device -> size -> block
+  std::map<int, size_t> held;                               // device -> cached bytes
+  std::map<int, size_t> limit;                              // device -> cap on cached bytes
+  std::map<void*, std::pair<int, size_t>> live;             // block -> (device, size)
+};
+
+BlockCache& block_cache() {
+  static BlockCache* c = new BlockCache;  // never destroyed: blocks go back with the process
+  return *c;
+}
+
+size_t cache_limit(BlockCache& c, int dev) {
+  auto it = c.limit.find(dev);
+  if (it != c.limit.end()) return it->second;
+  size_t lim = 0;
+  if (const char* e = std::getenv("WEMFWI_CACHE_LIMIT_MB")) {
+    lim = (size_t)std::max(0LL, std::atoll(e)) << 20;
+  } else {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) lim = tot / 2;
+  }
+  c.limit[dev] = lim;
+  return lim;
+}
+
+// frees the cached blocks of device dev (the caller holds the lock and has
+// made dev current)
+void flush_device(BlockCache& c, int dev) {
+  auto& m = c.free_blocks[dev];
+  for (auto& b : m) (void)hipFree(b.second);
+  m.clear();
+  c.held[dev] = 0;
+}
+
+}  // namespace
+
+int cached_malloc(void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (bytes == 0) return WEMFWI_OK;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto& m = c.free_blocks[dev];
+  auto it = m.lower_bound(bytes);
+  if (it != m.end() && it->first <= bytes + bytes / 4 + ((size_t)2 << 20)) {
+    *p = it->second;
+    c.held[dev] -= it->first;
+    c.live[*p] = {dev, it->first};
+    m.erase(it);
+    return WEMFWI_OK;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && !m.empty()) {
+    (void)hipGetLastError();
+    flush_device(c, dev);
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + " (" + std::to_string(bytes) +
+                                      " bytes): " + hipGetErrorString(e));
+  }
+  c.live[*p] = {dev, bytes};
+  return WEMFWI_OK;
+}
+
+void cached_free(void* p) {
+  if (!p) return;
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.live.find(p);
+  if (it == c.live.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  const int dev = it->second.first;
+  const size_t size = it->second.second;
+  c.live.erase(it);
+  int cur = dev;
+  (void)hipGetDevice(&cur);
+  if (cur != dev) (void)hipSetDevice(dev);
+  // the block may still be read or written by queued work (an error path
+  // returns before its stream is drained): wait as hipFree does
+  (void)hipDeviceSynchronize();
+  if (c.held[dev] + size <= cache_limit(c, dev)) {
+    c.free_blocks[dev].emplace(size, p);
+    c.held[dev] += size;
+  } else {
+    (void)hipFree(p);
+  }
+  if (cur != dev) (void)hipSetDevice(cur);
+}
+
 void Workspace::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) cached_free(ptr);
   ptr = nullptr;
   cap = 0;
 }
@@ -130,11 +231,7 @@ void Workspace::release() {
 int Workspace::reserve(size_t bytes, const char* what) {
   if (bytes <= cap) return WEMFWI_OK;
   release();
-  hipError_t e = hipMalloc(&ptr, bytes);
-  if (e != hipSuccess) {
-    ptr = nullptr;
-    return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
-  }
+  if (int rc = cached_malloc(&ptr, bytes, what)) return rc;
   cap = bytes;
   return WEMFWI_OK;
 }
@@ -758,6 +855,28 @@ int wemfwi_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
+}
+
+long long wemfwi_cache_bytes(void) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  return (long long)c.held[dev];
+}
+
+int wemfwi_cache_release(void) {
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess) return WEMFWI_OK;
+  for (auto& kv : c.free_blocks) {
+    if (kv.second.empty()) continue;
+    (void)hipSetDevice(kv.first);
+    flush_device(c, kv.first);
+  }
+  (void)hipSetDevice(cur);
+  return WEMFWI_OK;
 }
 
 int wemfwi_refsampler(int nz, int nx, const wemfwi_complex* slow, int nref, wemfwi_complex* sref,

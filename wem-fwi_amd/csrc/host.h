@@ -18,6 +18,16 @@ int refsampler_host(int nz, int nx, const std::complex<float>* slow, int nref, s
                     int* refid);
 int twiddles(int n, const float2** out);
 
+// Per-call device buffers (wavefields, accumulators, descriptors) come from a
+// per-device cache of freed blocks: cached_free synchronises the device, as
+// hipFree does, and keeps the block for a later cached_malloc of a size it
+// fits (at most 1/4 + 2 MiB larger), so repeated calls of the same shape do
+// not return tens of GB to the driver and wait for it to clear them again.
+// The cache holds at most WEMFWI_CACHE_LIMIT_MB (default half of the device
+// memory); a failing hipMalloc empties it and retries.
+int cached_malloc(void** p, size_t bytes, const char* what);
+void cached_free(void* p);
+
 // Device allocation of n elements (WEMFWI_ERR_ALLOC on failure).
 template <typename T>
 int dev_alloc(T** p, size_t n, const char* what) {

@@ -147,19 +147,14 @@ static int build_plan(DevPlan& P, const std::vector<int>& iz, const std::vector<
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
+  ~DevBuf() { cached_free(p); }
   int alloc(size_t bytes, const char* what) {
-    if (p) (void)hipFree(p);
+    cached_free(p);
     p = nullptr;
     n = bytes;
-    if (bytes == 0) return WEMFWI_OK;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return fail(WEMFWI_ERR_ALLOC, std::string("hipMalloc failed for ") + what + " (" + std::to_string(bytes) +
-                                        " bytes)");
+    if (int rc = cached_malloc(&p, bytes, what)) {
+      n = 0;
+      return rc;
     }
     return WEMFWI_OK;
   }

@@ -48,6 +48,8 @@ SIGNATURES = {
     "wemfwi_last_error": (C.c_char_p, []),
     "wemfwi_version": (_i, []),
     "wemfwi_device_count": (_i, []),
+    "wemfwi_cache_bytes": (C.c_longlong, []),
+    "wemfwi_cache_release": (_i, []),
     "wemfwi_prop_param": (_i, [_i, C.c_float, C.c_float, _fp, _i, C.c_float, C.c_float, _i, _i, _ip,
                                _fp, _ip, _vp]),
     "wemfwi_trace_to_freq": (_i, [_i, _i, _i, _vp, _i, _ip, _i, _vp, _i, _vp]),

@@ -239,6 +239,18 @@ class BatchGraph:
             pass
 
 
+def cache_bytes() -> int:
+    """Bytes of freed per-call device buffers the library keeps for reuse on
+    the current device (wemfwi_cache_bytes)."""
+    return int(lib.wemfwi_cache_bytes())
+
+
+def empty_cache() -> None:
+    """Return the library's cached device buffers to the driver
+    (wemfwi_cache_release), e.g. before torch needs the memory."""
+    check(lib.wemfwi_cache_release())
+
+
 def timing(enable: bool) -> None:
     """Enables (and clears) or disables HIP-event kernel timing in the library."""
     check(lib.wemfwi_timing(int(enable)))
