@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -77,154 +78,207 @@ __global__ void rec_rows_kernel(const RecRow* __restrict__ rows, int nrows) {
   }
 }
 
-// img[h][z][x0 + x] += sum over the nrows rows of one shot (in order) of
-// conj(D[z][x-h]) R[z][x+h], h = j - nh/2, terms with x-h or x+h outside
-// [0, nx) absent.  A workgroup owns one depth z and kTX consecutive x (one
-// per thread, nh lags in FP64 registers); per row the D and R segments with
-// their nh/2 halos are staged in LDS once, so every sample is read from HBM
-// once per row instead of once per lag.
+// One shot of a group on the device: aperture columns [x0, x0 + nx) of the
+// image, nw rows of nz x nx samples from row offset roff of the group's D
+// and R buffers.
+struct UShot {
+  int x0, nx, nw;
+  long long roff;
+};
+
+// img[h][z][x] += for each shot covering image column x (in shot order), the
+// sum over its rows (in row order) of conj(D[z][a-h]) R[z][a+h], a = x - x0
+// the aperture column, h = j - nh/2, terms with a-h or a+h outside [0, nx)
+// absent.  A workgroup owns one depth z and kTX consecutive image columns
+// (one per thread, the shot's nh lags in FP64 registers, added to the FP64
+// image after each shot, so the sums run in the order of one launch per
+// shot).  kRows rows at a time have their D and R segments (nh/2 halos)
+// loaded together and staged in LDS, so every sample is read from HBM once
+// per row with kRows rows of loads in flight.  One launch covers every shot
+// of the group; block b takes tile order[b / nz] at depth b % nz, the tiles
+// ordered by decreasing work (rows covering them) so the longest ones start
+// first.
 constexpr int kTX = 256;
 constexpr int kHalo = 20;  // nh <= 41
+constexpr int kRows = 4;
+constexpr int kSeg = kTX + 2 * kHalo;
 template <int NHM>
 __global__ void __launch_bounds__(kTX)
-    offset_image_rows_kernel(double2* __restrict__ img, long long hstride, int nxi, int x0,
-                             const float2* __restrict__ D, const float2* __restrict__ R, long long rstride,
-                             int nrows, int nz, int nx, int nh) {
-  __shared__ float2 sd[kTX + 2 * kHalo], sr[kTX + 2 * kHalo];
-  const int z = blockIdx.y;
-  const int xb = blockIdx.x * kTX;
+    offset_image_shots_kernel(double2* __restrict__ img, long long hstride, int nxi, const UShot* __restrict__ shots,
+                              int nshots, const int* __restrict__ order, const float2* __restrict__ D,
+                              const float2* __restrict__ R, int nz, int nh) {
+  __shared__ float2 sd[kRows * kSeg], sr[kRows * kSeg];
+  const int z = blockIdx.x % nz;
+  const int X = order[blockIdx.x / nz] * kTX;
   const int t = threadIdx.x;
-  const int x = xb + t;
   const int hh = nh / 2;
   const int seg = kTX + 2 * hh;
-  double ax[NHM], ay[NHM];
+  for (int sh = 0; sh < nshots; ++sh) {
+    const UShot S = shots[sh];
+    if (S.nw == 0 || S.x0 >= X + kTX || S.x0 + S.nx <= X) continue;  // the same for the whole workgroup
+    const int a0 = X - S.x0;  // aperture column of thread 0
+    const int x = a0 + t;
+    const int nx = S.nx;
+    const long long rstride = (long long)nz * nx;
+    double ax[NHM], ay[NHM];
 #pragma unroll
-  for (int j = 0; j < NHM; ++j) ax[j] = ay[j] = 0.0;
-  for (int r = 0; r < nrows; ++r) {
-    const float2* d = D + r * rstride + (long long)z * nx;
-    const float2* s = R + r * rstride + (long long)z * nx;
-    for (int i = t; i < seg; i += kTX) {
-      const int xx = xb - hh + i;
-      const bool in = xx >= 0 && xx < nx;
-      sd[i] = in ? ldg(d, (size_t)xx) : make_float2(0.f, 0.f);
-      sr[i] = in ? ldg(s, (size_t)xx) : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    if (x < nx) {
+    for (int j = 0; j < NHM; ++j) ax[j] = ay[j] = 0.0;
+    for (int r0 = 0; r0 < S.nw; r0 += kRows) {
+      const int kr = min(kRows, S.nw - r0);
+      const float2* d = D + S.roff + r0 * rstride + (long long)z * nx;
+      const float2* q = R + S.roff + r0 * rstride + (long long)z * nx;
+      for (int i = t; i < seg; i += kTX) {
+        const int xx = a0 - hh + i;
+        const bool in = xx >= 0 && xx < nx;
+        float2 vd[kRows], vr[kRows];
 #pragma unroll
-      for (int j = 0; j < NHM; ++j) {
-        const int h = j - hh;
-        if (j < nh && x - h >= 0 && x - h < nx && x + h >= 0 && x + h < nx) {
-          const float2 v = cmulc(sr[t + hh + h], sd[t + hh - h]);
-          ax[j] += v.x;
-          ay[j] += v.y;
+        for (int k = 0; k < kRows; ++k) {
+          const bool ok = in && k < kr;
+          vd[k] = ok ? ldg(d + k * rstride, (size_t)xx) : make_float2(0.f, 0.f);
+          vr[k] = ok ? ldg(q + k * rstride, (size_t)xx) : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+          sd[k * kSeg + i] = vd[k];
+          sr[k * kSeg + i] = vr[k];
         }
       }
-    }
-    __syncthreads();
-  }
-  if (x < nx) {
+      __syncthreads();
+      if (x >= 0 && x < nx) {
+        for (int k = 0; k < kr; ++k) {
 #pragma unroll
-    for (int j = 0; j < NHM; ++j) {
-      if (j >= nh) break;
-      double2* o = img + j * hstride + (long long)z * nxi + x0 + x;
-      const double2 a = *o;
-      *o = make_double2(a.x + ax[j], a.y + ay[j]);
+          for (int j = 0; j < NHM; ++j) {
+            const int h = j - hh;
+            if (j < nh && x - h >= 0 && x - h < nx && x + h >= 0 && x + h < nx) {
+              const float2 v = cmulc(sr[k * kSeg + t + hh + h], sd[k * kSeg + t + hh - h]);
+              ax[j] += v.x;
+              ay[j] += v.y;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (x >= 0 && x < nx) {
+#pragma unroll
+      for (int j = 0; j < NHM; ++j) {
+        if (j >= nh) break;
+        double2* o = img + j * hstride + (long long)z * nxi + X + t;
+        const double2 v = *o;
+        *o = make_double2(v.x + ax[j], v.y + ay[j]);
+      }
     }
   }
 }
 
 template <int NHM>
-hipError_t launch_oir(double2* img, long long hstride, int nxi, int x0, const float2* D, const float2* R,
-                      long long rstride, int nrows, int nz, int nx, int nh, hipStream_t s) {
-  dim3 grid((unsigned)((nx + kTX - 1) / kTX), (unsigned)nz);
-  hipLaunchKernelGGL((offset_image_rows_kernel<NHM>), grid, dim3(kTX), 0, s, img, hstride, nxi, x0, D, R, rstride,
-                     nrows, nz, nx, nh);
+hipError_t launch_ois(double2* img, long long hstride, int nxi, const UShot* shots, int nshots, const int* order,
+                      int ntiles, const float2* D, const float2* R, int nz, int nh, hipStream_t s) {
+  hipLaunchKernelGGL((offset_image_shots_kernel<NHM>), dim3((unsigned)(ntiles * nz)), dim3(kTX), 0, s, img, hstride,
+                     nxi, shots, nshots, order, D, R, nz, nh);
   return hipGetLastError();
 }
 
-hipError_t launch_offset_image_rows(double2* img, long long hstride, int nxi, int x0, const float2* D,
-                                   const float2* R, long long rstride, int nrows, int nz, int nx, int nh,
-                                   hipStream_t s) {
-  if (nrows <= 0 || nz <= 0 || nx <= 0) return hipSuccess;
-  if (nh > 2 * kHalo + 1 || nz > 65535) return hipErrorInvalidValue;
-  if (nh <= 1) return launch_oir<1>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
-  if (nh <= 3) return launch_oir<3>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
-  if (nh <= 11) return launch_oir<11>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
-  if (nh <= 21) return launch_oir<21>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
-  return launch_oir<41>(img, hstride, nxi, x0, D, R, rstride, nrows, nz, nx, nh, s);
+hipError_t launch_offset_image_shots(double2* img, long long hstride, int nxi, const UShot* shots, int nshots,
+                                     const int* order, int ntiles, const float2* D, const float2* R, int nz, int nh,
+                                     hipStream_t s) {
+  if (nshots <= 0 || nz <= 0 || ntiles <= 0) return hipSuccess;
+  if (nh > 2 * kHalo + 1 || (long long)ntiles * nz > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (nh <= 1) return launch_ois<1>(img, hstride, nxi, shots, nshots, order, ntiles, D, R, nz, nh, s);
+  if (nh <= 3) return launch_ois<3>(img, hstride, nxi, shots, nshots, order, ntiles, D, R, nz, nh, s);
+  if (nh <= 11) return launch_ois<11>(img, hstride, nxi, shots, nshots, order, ntiles, D, R, nz, nh, s);
+  if (nh <= 21) return launch_ois<21>(img, hstride, nxi, shots, nshots, order, ntiles, D, R, nz, nh, s);
+  return launch_ois<41>(img, hstride, nxi, shots, nshots, order, ntiles, D, R, nz, nh, s);
 }
 
-// U_r[z][y] = sum_h I[h][z][x0 + y - h] D_r[z][y - 2h] for the nrows rows of
-// one shot, terms with y - h or y - 2h outside [0, nx) absent: the adjoint of
-// offset_image_rows_kernel with respect to R.  A workgroup owns one depth and
-// kTX consecutive y; the image tile (nh lags, halo nh/2) is staged in LDS once
-// for all rows of the shot, each row's D segment (halo 2 (nh/2)) once per
-// row.  Every (z, y < nx) of every row is written.
+// U_r[z][y] = sum_h I[h][z][x0 + y - h] D_r[z][y - 2h] for every row r of
+// every shot, y an aperture column, terms with y - h or y - 2h outside
+// [0, nx) absent: the adjoint of offset_image_shots_kernel with respect to
+// R.  A workgroup owns one depth and kTX consecutive image columns; the
+// image tile of all nh lags (halo nh/2) is staged in LDS once for every shot
+// that covers the tile, kRows rows' D segments (halo 2 (nh/2)) at a time.
+// Every (z, y < nx) of every row is written (each aperture column lies in
+// exactly one tile).  One launch covers every shot of the group, tiles in
+// the order of offset_image_shots_kernel.
 template <int NHM>
 __global__ void __launch_bounds__(kTX)
-    offset_scatter_rows_kernel(float2* __restrict__ U, const float2* __restrict__ img, long long hstride, int nxi,
-                               int x0, const float2* __restrict__ D, long long rstride, int nrows, int nz, int nx,
-                               int nh) {
+    offset_scatter_shots_kernel(float2* __restrict__ U, const float2* __restrict__ img, long long hstride, int nxi,
+                                const UShot* __restrict__ shots, int nshots, const int* __restrict__ order,
+                                const float2* __restrict__ D, int nz, int nh) {
   constexpr int HM = NHM / 2;
+  constexpr int WD = kTX + 4 * HM;
   __shared__ float2 si[NHM * (kTX + 2 * HM)];
-  __shared__ float2 sd[kTX + 4 * HM];
+  __shared__ float2 sd[kRows * WD];
   const int hh = nh / 2;
   const int wi = kTX + 2 * hh;
   const int wd = kTX + 4 * hh;
-  const int z = blockIdx.y;
-  const int xb = blockIdx.x * kTX;
+  const int z = blockIdx.x % nz;
+  const int X = order[blockIdx.x / nz] * kTX;
   const int t = threadIdx.x;
-  const int y = xb + t;
   for (int i = t; i < nh * wi; i += kTX) {
     const int j = i / wi, c = i - j * wi;
-    const int xx = xb - hh + c;
-    si[i] = (xx >= 0 && xx < nx) ? img[j * hstride + (long long)z * nxi + x0 + xx] : make_float2(0.f, 0.f);
+    const int xc = X - hh + c;
+    si[i] = (xc >= 0 && xc < nxi) ? img[j * hstride + (long long)z * nxi + xc] : make_float2(0.f, 0.f);
   }
-  for (int r = 0; r < nrows; ++r) {
-    const float2* d = D + r * rstride + (long long)z * nx;
-    __syncthreads();
-    for (int i = t; i < wd; i += kTX) {
-      const int xx = xb - 2 * hh + i;
-      sd[i] = (xx >= 0 && xx < nx) ? ldg(d, (size_t)xx) : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    if (y < nx) {
-      float ax = 0.f, ay = 0.f;
+  for (int sh = 0; sh < nshots; ++sh) {
+    const UShot S = shots[sh];
+    if (S.nw == 0 || S.x0 >= X + kTX || S.x0 + S.nx <= X) continue;  // the same for the whole workgroup
+    const int a0 = X - S.x0;
+    const int y = a0 + t;
+    const int nx = S.nx;
+    const long long rstride = (long long)nz * nx;
+    for (int r0 = 0; r0 < S.nw; r0 += kRows) {
+      const int kr = min(kRows, S.nw - r0);
+      const float2* d = D + S.roff + r0 * rstride + (long long)z * nx;
+      __syncthreads();
+      for (int i = t; i < wd; i += kTX) {
+        const int xx = a0 - 2 * hh + i;
+        const bool in = xx >= 0 && xx < nx;
+        float2 v[kRows];
 #pragma unroll
-      for (int j = 0; j < NHM; ++j) {
-        const int h = j - hh;
-        if (j < nh && y - h >= 0 && y - h < nx && y - 2 * h >= 0 && y - 2 * h < nx) {
-          const float2 a = si[j * wi + t - h + hh];
-          const float2 b = sd[t - 2 * h + 2 * hh];
-          ax = fmaf(a.x, b.x, fmaf(-a.y, b.y, ax));
-          ay = fmaf(a.x, b.y, fmaf(a.y, b.x, ay));
+        for (int k = 0; k < kRows; ++k) v[k] = (in && k < kr) ? ldg(d + k * rstride, (size_t)xx) : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) sd[k * WD + i] = v[k];
+      }
+      __syncthreads();
+      if (y >= 0 && y < nx) {
+        for (int k = 0; k < kr; ++k) {
+          float ax = 0.f, ay = 0.f;
+#pragma unroll
+          for (int j = 0; j < NHM; ++j) {
+            const int h = j - hh;
+            if (j < nh && y - h >= 0 && y - h < nx && y - 2 * h >= 0 && y - 2 * h < nx) {
+              const float2 p = si[j * wi + t - h + hh];
+              const float2 b = sd[k * WD + t - 2 * h + 2 * hh];
+              ax = fmaf(p.x, b.x, fmaf(-p.y, b.y, ax));
+              ay = fmaf(p.x, b.y, fmaf(p.y, b.x, ay));
+            }
+          }
+          U[S.roff + (r0 + k) * rstride + (long long)z * nx + y] = make_float2(ax, ay);
         }
       }
-      U[r * rstride + (long long)z * nx + y] = make_float2(ax, ay);
     }
   }
 }
 
 template <int NHM>
-hipError_t launch_osr(float2* U, const float2* img, long long hstride, int nxi, int x0, const float2* D,
-                      long long rstride, int nrows, int nz, int nx, int nh, hipStream_t s) {
-  dim3 grid((unsigned)((nx + kTX - 1) / kTX), (unsigned)nz);
-  hipLaunchKernelGGL((offset_scatter_rows_kernel<NHM>), grid, dim3(kTX), 0, s, U, img, hstride, nxi, x0, D, rstride,
-                     nrows, nz, nx, nh);
+hipError_t launch_oss(float2* U, const float2* img, long long hstride, int nxi, const UShot* shots, int nshots,
+                      const int* order, int ntiles, const float2* D, int nz, int nh, hipStream_t s) {
+  hipLaunchKernelGGL((offset_scatter_shots_kernel<NHM>), dim3((unsigned)(ntiles * nz)), dim3(kTX), 0, s, U, img,
+                     hstride, nxi, shots, nshots, order, D, nz, nh);
   return hipGetLastError();
 }
 
-hipError_t launch_offset_scatter_rows(float2* U, const float2* img, long long hstride, int nxi, int x0,
-                                      const float2* D, long long rstride, int nrows, int nz, int nx, int nh,
-                                      hipStream_t s) {
-  if (nrows <= 0 || nz <= 0 || nx <= 0) return hipSuccess;
-  if (nh > 2 * kHalo + 1 || nz > 65535) return hipErrorInvalidValue;
-  if (nh <= 1) return launch_osr<1>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
-  if (nh <= 3) return launch_osr<3>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
-  if (nh <= 11) return launch_osr<11>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
-  if (nh <= 21) return launch_osr<21>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
-  return launch_osr<41>(U, img, hstride, nxi, x0, D, rstride, nrows, nz, nx, nh, s);
+hipError_t launch_offset_scatter_shots(float2* U, const float2* img, long long hstride, int nxi, const UShot* shots,
+                                       int nshots, const int* order, int ntiles, const float2* D, int nz, int nh,
+                                       hipStream_t s) {
+  if (nshots <= 0 || nz <= 0 || ntiles <= 0) return hipSuccess;
+  if (nh > 2 * kHalo + 1 || (long long)ntiles * nz > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (nh <= 1) return launch_oss<1>(U, img, hstride, nxi, shots, nshots, order, ntiles, D, nz, nh, s);
+  if (nh <= 3) return launch_oss<3>(U, img, hstride, nxi, shots, nshots, order, ntiles, D, nz, nh, s);
+  if (nh <= 11) return launch_oss<11>(U, img, hstride, nxi, shots, nshots, order, ntiles, D, nz, nh, s);
+  if (nh <= 21) return launch_oss<21>(U, img, hstride, nxi, shots, nshots, order, ntiles, D, nz, nh, s);
+  return launch_oss<41>(U, img, hstride, nxi, shots, nshots, order, ntiles, D, nz, nh, s);
 }
 
 // Residual of one row at its receivers: res[r * stride + col] =
@@ -328,6 +382,7 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
   }
   const int nrows = (int)rm.size();
   if (nrows == 0) return WEMFWI_OK;
+  const int nz = shots[s0].model->m.d.nz;
   ScopedWs Dw, Rw, srcw, recw, gww, devw;
   UCHK(Dw.reserve(total * sizeof(float2), "source wavefields"));
   UCHK(Rw.reserve(total * sizeof(float2), "receiver wavefields"));
@@ -345,6 +400,35 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
   }
   UCHK(upload_vec(devw, hdev, s, "receiver positions"));
   const int* ddev = (const int*)devw.ptr;
+  // the group's shots for the scatter and imaging launches
+  ScopedWs ushw;
+  {
+    std::vector<UShot> hu;
+    size_t k0 = 0;
+    for (int sh = s0; sh < s1; ++sh) {
+      const Model& m = shots[sh].model->m;
+      hu.push_back(UShot{shots[sh].x0, m.d.nx, shots[sh].nw, (long long)(shots[sh].nw > 0 ? roff[k0] : 0)});
+      k0 += shots[sh].nw;
+    }
+    UCHK(upload_vec(ushw, hu, s, "shot table"));
+  }
+  // image tiles by decreasing work (rows of the shots covering them), ties
+  // by position
+  const int ntiles = (nxi + kTX - 1) / kTX;
+  ScopedWs ordw;
+  {
+    std::vector<long long> work(ntiles, 0);
+    for (int sh = s0; sh < s1; ++sh) {
+      if (shots[sh].nw == 0) continue;
+      const int a = shots[sh].x0 / kTX;
+      const int b = (shots[sh].x0 + shots[sh].model->m.d.nx - 1) / kTX;
+      for (int tl = a; tl <= b; ++tl) work[tl] += shots[sh].nw;
+    }
+    std::vector<int> ord(ntiles);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return work[p] > work[q]; });
+    UCHK(upload_vec(ordw, ord, s, "tile order"));
+  }
   std::vector<SrcRow> hs(nrows);
   std::vector<RecRow> hr(nrows);
   std::vector<const cf*> pin(nrows);
@@ -413,15 +497,9 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
     UCHK(p.build(nrows, rm.data(), rf.data(), pin.data(), pout.data(), s));
     if (fwi) {
       // L I: offset scatter of the image on D into every row, Up, record
-      k = 0;
-      const long long hs_img = (long long)shots[s0].model->m.d.nz * nxi;
-      for (int sh = s0; sh < s1; ++sh) {
-        const Model& m = shots[sh].model->m;
-        UHIP(launch_offset_scatter_rows(R + roff[k], fwi, hs_img, nxi, shots[sh].x0, D + roff[k],
-                                        (long long)m.d.nz * m.d.nx, shots[sh].nw, m.d.nz, m.d.nx, nh, s),
-             "offset scatter");
-        k += shots[sh].nw;
-      }
+      UHIP(launch_offset_scatter_shots(R, fwi, (long long)nz * nxi, nxi, (const UShot*)ushw.ptr, s1 - s0,
+                                       (const int*)ordw.ptr, ntiles, D, nz, nh, s),
+           "offset scatter");
       UCHK(p.run(1, 0, s));
       if (nr_max > 0) {
         hipLaunchKernelGGL(residual_rows_kernel, dim3((unsigned)((nr_max + 255) / 256), (unsigned)std::min(nrows, 65535)),
@@ -440,16 +518,10 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
     }
     UCHK(p.run(1, 1, s));
   }
-  // image, shot after shot
-  k = 0;
-  const long long hstride = (long long)shots[s0].model->m.d.nz * nxi;
-  for (int sh = s0; sh < s1; ++sh) {
-    const Model& m = shots[sh].model->m;
-    UHIP(launch_offset_image_rows(acc, hstride, nxi, shots[sh].x0, D + roff[k], R + roff[k],
-                                  (long long)m.d.nz * m.d.nx, shots[sh].nw, m.d.nz, m.d.nx, nh, s),
-         "offset image");
-    k += shots[sh].nw;
-  }
+  // image, every shot in one launch, sums in shot order
+  UHIP(launch_offset_image_shots(acc, (long long)nz * nxi, nxi, (const UShot*)ushw.ptr, s1 - s0,
+                                 (const int*)ordw.ptr, ntiles, D, R, nz, nh, s),
+       "offset image");
   UHIP(hipStreamSynchronize(s), "offset image");
   return WEMFWI_OK;
 }
