@@ -12,7 +12,11 @@ the exact dot test <Lm, d> = <m, L*d> on random complex m, d (seed 33; the
 difference within 1e-5 of |Lm| |d|), and
 the nh = 21 subsurface-offset image, whose h = 0 slice is the zero-lag image
 summed over frequency and whose other lags obey the image's adjoint pairing
-with offset_image_forward."""
+with offset_image_forward.
+
+The uneven survey (64 shots, apertures 256-2048, 32-192 frequencies, nz = 400,
+nh = 11) at full size through size-independent properties of the extended
+least-squares gradient."""
 
 import numpy as np
 import pytest
@@ -137,3 +141,71 @@ def test_born_pair_full_grid_oracle_dot_and_offset_image():
     lhs = O.cdot(dI.cpu().numpy().astype(complex), d.cpu().numpy().astype(complex))
     rhs = O.cdot(Ih.astype(complex), img.cpu().numpy().astype(complex))
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def _uneven_full():
+    """The uneven survey at full size: 64 shots over a 4096-wide, 400-deep
+    model (1500 -> 3500 m/s plus a smoothed 5 % field, seed 5), apertures
+    drawn from {256, 512, 1024, 2048}, 32-192 frequencies from 2-5 Hz in
+    0.1 Hz steps, receivers on every second aperture column, random data
+    (seed 55): 7308 (shot, frequency) rows."""
+    import torch
+    from wemfwi import Model
+    from wemfwi.ops import ImageShot
+    rng = np.random.default_rng(5)
+    nz, W0 = 400, 4096
+    z = np.arange(nz)[:, None] * 10.0
+    v = 1500.0 + 2000.0 * z / ((nz - 1) * 10.0) + np.zeros((nz, W0))
+    pert = gaussian_filter(rng.standard_normal((nz, W0)), 8.0)
+    slow = (1.0 / (v + 0.05 * v * pert / np.max(np.abs(pert))) ** 2).astype(np.complex64)
+    geo = []
+    for _ in range(64):
+        nx = int(rng.choice((256, 512, 1024, 2048)))
+        nw = int(rng.integers(32, 193))
+        f0 = 2.0 + 3.0 * rng.random()
+        cx = int(rng.integers(nx // 2, W0 - nx // 2 + 1))
+        geo.append((cx - nx // 2, nx, [float(np.float32(f0 + 0.1 * i)) for i in range(nw)]))
+    drng = np.random.default_rng(55)
+    shots = []
+    for x0, nx, freqs in geo:
+        m = Model(np.ascontiguousarray(slow[:, x0:x0 + nx]), 10.0, 10.0, nref=1, prop="ssf", eps=0.04)
+        rix = np.arange(0, nx, 2)
+        nw = len(freqs)
+        d = (drng.standard_normal((len(rix), nw)) + 1j * drng.standard_normal((len(rix), nw))).astype(np.complex64)
+        shots.append(ImageShot(m, x0, freqs, _ricker(freqs), 0, nx // 2, np.zeros_like(rix), rix,
+                               torch.from_numpy(d).cuda()))
+    return shots, nz, W0
+
+
+def test_uneven_extended_gradient_full_size_properties():
+    """At the full uneven size (nh = 11) the oracle is out of reach; the
+    gradient is checked through properties that hold at any size: at I = 0
+    it is minus the migration of the observed data, bitwise, with J = 1/2
+    ||d||^2; with zero observed data <I, L^H L I> = 2 J (1e-5 relative, the
+    FP32 rounding of L and L^H); repeated calls give the same J and
+    gradient bitwise."""
+    import torch
+    from wemfwi.ops import uneven_offset_gradient, uneven_offset_image_adjoint
+    shots, nz, W0 = _uneven_full()
+    nh = 11
+    print("uneven full size: migration and gradient at I = 0", flush=True)
+    mig = torch.zeros((nh, nz, W0), dtype=torch.complex64, device="cuda")
+    uneven_offset_image_adjoint(shots, nh, mig, ngs=0)
+    g0 = torch.zeros_like(mig)
+    J0 = uneven_offset_gradient(shots, nh, torch.zeros_like(mig), g0, ngs=0)
+    assert torch.equal(g0, -mig)
+    want = 0.5 * sum(float(torch.sum(sh.data.to(torch.complex128).abs() ** 2)) for sh in shots)
+    assert abs(J0 - want) <= 1e-12 * want
+    print("uneven full size: <I, L^H L I> = 2 J", flush=True)
+    I = mig / torch.linalg.vector_norm(mig)
+    for sh in shots:
+        sh.data.zero_()
+    g = torch.zeros_like(mig)
+    J = uneven_offset_gradient(shots, nh, I, g, ngs=0)
+    ip = float((I.to(torch.complex128) * g.to(torch.complex128).conj()).real.sum())
+    print(f"<I, g> = {ip:.10e}, 2 J = {2 * J:.10e}", flush=True)
+    assert abs(ip - 2 * J) <= 1e-5 * 2 * J
+    g2 = torch.zeros_like(mig)
+    assert uneven_offset_gradient(shots, nh, I, g2, ngs=0) == J and torch.equal(g, g2)
+    for sh in shots:
+        sh.model.close()
