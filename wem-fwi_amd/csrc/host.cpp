@@ -631,6 +631,9 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
     h[i].nsteps = -1;
     h[i].init = nullptr;
     h[i].zbase = 0;
+    h[i].smul = nullptr;
+    h[i].out2 = nullptr;
+    h[i].omul = nullptr;
   }
   if ((rc = scratch.reserve((size_t)nrows * N * sizeof(float2), "scratch rows"))) return rc;
   for (int i = 0; i < nrows; ++i) h[i].scratch = (float2*)scratch.ptr + (size_t)i * N;
@@ -734,6 +737,9 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     h[k].nsteps = -1;
     h[k].init = nullptr;
     h[k].zbase = 0;
+    h[k].smul = nullptr;
+    h[k].out2 = nullptr;
+    h[k].omul = nullptr;
     h[k].scratch = (float2*)scratch.ptr + scr_off[k];
   }
   for (int k0 = 0; k0 < nrows;) {

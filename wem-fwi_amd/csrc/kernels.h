@@ -23,6 +23,10 @@ struct SweepRow {
   int nsteps;            // depth steps from it; -1: nz-1 (the whole sweep)
   const float2* init;    // initial state row [nx]; NULL: the row in[first]
   int zbase;             // depth of the first row of in / out (0: whole [nz][nx] arrays)
+  const float2* smul;    // [nz][nx] (absolute depths) multiplier of every source row
+                         // (also the initial row unless init is set); NULL: none
+  float2* out2;          // second output rows (layout of out): conj(omul) * result; NULL: none
+  const float2* omul;    // [nz][nx] (absolute depths) multiplier of out2
 };
 
 // One independent depth step.

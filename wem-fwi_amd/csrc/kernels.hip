@@ -124,7 +124,17 @@ __global__ void __launch_bounds__(64 * WavesV<V>::value)
   const int zb = R.zbase;
   float2 st[E];
   load_row<V>(st, R.init ? R.init : R.in + (size_t)(first - zb) * nx, g, t);
-  if (R.in != R.out || R.init) store_row<V>(R.out + (size_t)(first - zb) * nx, st, g, t);
+  if (R.smul && !R.init) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], row_sample<V>(R.smul + (size_t)first * nx, g, t, e));
+  }
+  if (R.in != R.out || R.init || R.smul) store_row<V>(R.out + (size_t)(first - zb) * nx, st, g, t);
+  if (R.out2) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      row_store<V>(R.out2 + (size_t)(first - zb) * nx, cmulc(st[e], row_sample<V>(R.omul + (size_t)first * nx, g, t, e)),
+                   g, t, e);
+  }
   for (int k = 0; k < nsteps; ++k) {
     const int cur = first + k * step;
     const int nxt = cur + step;
@@ -133,9 +143,19 @@ __global__ void __launch_bounds__(64 * WavesV<V>::value)
     else step_forward<V, true>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
     float2 src[E];
     load_row<V>(src, R.in + (size_t)(nxt - zb) * nx, g, t);
+    if (R.smul) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) src[e] = cmul(src[e], row_sample<V>(R.smul + (size_t)nxt * nx, g, t, e));
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
     store_row<V>(R.out + (size_t)(nxt - zb) * nx, st, g, t);
+    if (R.out2) {
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        row_store<V>(R.out2 + (size_t)(nxt - zb) * nx, cmulc(st[e], row_sample<V>(R.omul + (size_t)nxt * nx, g, t, e)),
+                     g, t, e);
+    }
   }
 }
 
@@ -353,6 +373,9 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
       h[i].nsteps = T;
       h[i].init = nullptr;
       h[i].zbase = 0;
+      h[i].smul = nullptr;
+      h[i].out2 = nullptr;
+      h[i].omul = nullptr;
     }
     ok = hipMemcpy(drows, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice) == hipSuccess;
   }

@@ -287,8 +287,12 @@ struct Ctx {
     return (float)m->h_wlist[k];
   }
 
-  // in-place / recursive sweep of W (rows x nzx) with sources src
-  int sweep(const Rows& r, int up, int adjoint, const float2* src, float2* W) {
+  // in-place / recursive sweep of W (rows x nzx) with sources src;
+  // src_refl: every source row (and the initial row) is multiplied by the row
+  // model's reflection coefficients r first; q2: rows (layout of W) that
+  // also receive conj(r) * the result
+  int sweep(const Rows& r, int up, int adjoint, const float2* src, float2* W, bool src_refl = false,
+            float2* q2 = nullptr) {
     const int n = (int)r.shot.size();
     hsw.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -302,6 +306,9 @@ struct Ctx {
       hsw[i].nsteps = -1;
       hsw[i].init = nullptr;
       hsw[i].zbase = 0;
+      hsw[i].smul = src_refl ? r.model[i]->d_refl : nullptr;
+      hsw[i].out2 = q2 ? q2 + (size_t)i * nzx : nullptr;
+      hsw[i].omul = q2 ? r.model[i]->d_refl : nullptr;
     }
     if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
@@ -342,7 +349,7 @@ struct Ctx {
   // from depth first, starting from init + i * init_stride (NULL: the
   // buffer's own row first), and adds the source rows already in W.
   int sweep_seg(const Rows& r, int up, int adjoint, float2* W, size_t wstride, int zbase, int first, int nsteps,
-                const float2* init, size_t init_stride) {
+                const float2* init, size_t init_stride, float2* q2 = nullptr) {
     const int n = (int)r.shot.size();
     hsw.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -356,6 +363,9 @@ struct Ctx {
       hsw[i].nsteps = nsteps;
       hsw[i].init = init ? init + (size_t)i * init_stride : nullptr;
       hsw[i].zbase = zbase;
+      hsw[i].smul = nullptr;
+      hsw[i].out2 = q2 ? q2 + (size_t)i * wstride : nullptr;
+      hsw[i].omul = q2 ? r.model[i]->d_refl : nullptr;
     }
     if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
@@ -535,14 +545,6 @@ struct Ctx {
     return sync();
   }
 
-  // W_i *= (conj) a_i, a_i = per-row shared array (e.g. the row model's r)
-  int scale_rows(const Rows& r, float2* W, const std::vector<const float2*>& a, bool conjA) {
-    const int n = (int)r.shot.size();
-    std::vector<MulRow> m(n);
-    for (int i = 0; i < n; ++i) m[i] = MulRow{W + (size_t)i * nzx, a[i], W + (size_t)i * nzx};
-    return mul_rows(m, nzx, conjA, false);
-  }
-
   int zero(void* p, size_t bytes) {
     HIPCHK(hipMemsetAsync(p, 0, bytes, s), "memset");
     return WEMFWI_OK;
@@ -610,10 +612,7 @@ static int wem_forward(const std::vector<Model*>& fm, Per per, Survey* sv, float
     RCCHK(c.inject_sources(r, w, sv->d.ngs, sv->d.offset));
     RCCHK(c.sweep(r, 0, 0, w, w));
     RCCHK(c.record(r, data, w, sv->d.ngr, sv->d.offset));
-    std::vector<const float2*> refl(n);
-    for (int i = 0; i < n; ++i) refl[i] = r.model[i]->d_refl;
-    RCCHK(c.scale_rows(r, w, refl, false));
-    RCCHK(c.sweep(r, 1, 0, w, w));
+    RCCHK(c.sweep(r, 1, 0, w, w, true));   // Up with sources r * W
     RCCHK(c.record(r, data, w, sv->d.ngr, sv->d.offset));
   }
   return c.sync();
@@ -741,20 +740,11 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
     RCCHK(c.sweep(r, 0, 0, bg, bg));
     RCCHK(c.zero(R, bytes));
     RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset));
-    RCCHK(c.sweep(r, 1, 1, R, R));
+    RCCHK(c.sweep(r, 1, 1, R, R, false, Q));   // R, and Q = conj(r) R on every row
     // dr += conj(D) * R (Born_refl::adjoint), summed straight from D and R
     // before LinDown^H rescales D
     RCCHK(reduce_chunk(c, r, sliced, accDR, bg, c.nzx, 0, 0, c.nz, R));
-    RCCHK(c.zero(Q, bytes));
-    RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset));
-    {
-      std::vector<MulRow> rq(n);
-      for (int i = 0; i < n; ++i) {
-        const size_t o = (size_t)i * c.nzx;
-        rq[i] = MulRow{Q + o, r.model[i]->d_refl, R + o};
-      }
-      RCCHK(c.mul_rows(rq, c.nzx, true, true));   // Q += conj(r) R
-    }
+    RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset));   // Q = conj(r) R + R^H d
     RCCHK(c.sweep(r, 0, 1, Q, Q));
     RCCHK(c.zero(cs, bytes));
     // LinDown^H; each D row d < nz-1 it reads becomes r * D (the Up source)
@@ -848,7 +838,8 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
     const int n = (int)r.shot.size();
     const size_t bytes = n * W * sizeof(float2);
     // D on [a, b] and R on [ra, rb] of segment j, from the kept rows
-    auto fields = [&](int j) -> int {
+    // (backward pass: with Q = conj(r) R on the rows of R's segment)
+    auto fields = [&](int j, float2* q2) -> int {
       const int a = j * k, b = std::min(a + k, nz - 1), zb = std::max(a - 1, 0);
       const bool last = j == nseg - 1;
       RCCHK(c.zero(bg, bytes));
@@ -857,12 +848,12 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
       const int ra = j ? a - 1 : 0, rb = last ? b : b - 1;
       RCCHK(c.zero(R, bytes));
       RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset, Ctx::Win{(long long)W, zb, j ? a : 0, rb}));
-      RCCHK(c.sweep_seg(r, 1, 1, R, W, zb, ra, rb - ra, j ? kr + (size_t)j * nx : nullptr, kpitch));
+      RCCHK(c.sweep_seg(r, 1, 1, R, W, zb, ra, rb - ra, j ? kr + (size_t)j * nx : nullptr, kpitch, q2));
       return WEMFWI_OK;
     };
     // forward pass: the kept rows D(a_j) and R(a_j - 1)
     for (int j = 0; j + 1 < nseg; ++j) {
-      RCCHK(fields(j));
+      RCCHK(fields(j, nullptr));
       const int a = j * k, b = a + k, zb = std::max(a - 1, 0);
       RCCHK(c.copy_rows(kd + (size_t)(j + 1) * nx, kpitch, bg + (size_t)(b - zb) * nx, W, n));
       RCCHK(c.copy_rows(kr + (size_t)(j + 1) * nx, kpitch, R + (size_t)(b - 1 - zb) * nx, W, n));
@@ -873,20 +864,11 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
       const bool last = j == nseg - 1;
       const int e = last ? nz : b;  // owned depths [a, e)
       const size_t oa = (size_t)(a - zb) * nx;
-      RCCHK(fields(j));
+      RCCHK(fields(j, Q));   // Q = conj(r) R on R's rows [ra, rb], which hold [a, qe]
       // dr += conj(D) * R over the owned depths, before LinDown^H rescales D
       RCCHK(reduce_chunk(c, r, sliced, accDR, bg, W, zb, a, e, R));
       const int qe = last ? b : b - 1;  // Q source rows [a, qe]
-      RCCHK(c.zero(Q, bytes));
       RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset, Ctx::Win{(long long)W, zb, a, qe}));
-      {
-        std::vector<MulRow> rq(n);
-        for (int i = 0; i < n; ++i) {
-          const size_t o = (size_t)i * W + oa;
-          rq[i] = MulRow{Q + o, r.model[i]->d_refl + (size_t)a * nx, R + o};
-        }
-        RCCHK(c.mul_rows(rq, (size_t)(qe - a + 1) * nx, true, true));  // Q += conj(r) R
-      }
       RCCHK(c.sweep_seg(r, 0, 1, Q, W, zb, b, b - a, last ? nullptr : cq, nx));
       if (j > 0) RCCHK(c.copy_rows(cq, nx, Q + oa, W, n));
       RCCHK(c.zero(cs, bytes));
