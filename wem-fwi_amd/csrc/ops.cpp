@@ -288,11 +288,11 @@ struct Ctx {
   }
 
   // in-place / recursive sweep of W (rows x nzx) with sources src;
-  // src_refl: every source row (and the initial row) is multiplied by the row
-  // model's reflection coefficients r first; q2: rows (layout of W) that
-  // also receive conj(r) * the result
-  int sweep(const Rows& r, int up, int adjoint, const float2* src, float2* W, bool src_refl = false,
-            float2* q2 = nullptr) {
+  // smul: per row, an [nz][nx] table multiplying every source row (and the
+  // initial row) as it is read; q2: rows (layout of W) that also receive
+  // conj(r) * the result, r the row model's reflection coefficients
+  int sweep(const Rows& r, int up, int adjoint, const float2* src, float2* W,
+            const std::vector<const float2*>* smul = nullptr, float2* q2 = nullptr) {
     const int n = (int)r.shot.size();
     hsw.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -306,7 +306,7 @@ struct Ctx {
       hsw[i].nsteps = -1;
       hsw[i].init = nullptr;
       hsw[i].zbase = 0;
-      hsw[i].smul = src_refl ? r.model[i]->d_refl : nullptr;
+      hsw[i].smul = smul ? (*smul)[i] : nullptr;
       hsw[i].out2 = q2 ? q2 + (size_t)i * nzx : nullptr;
       hsw[i].omul = q2 ? r.model[i]->d_refl : nullptr;
     }
@@ -545,6 +545,13 @@ struct Ctx {
     return sync();
   }
 
+  // the row models' reflection coefficient tables
+  static std::vector<const float2*> refl_rows(const Rows& r) {
+    std::vector<const float2*> v(r.shot.size());
+    for (size_t i = 0; i < v.size(); ++i) v[i] = r.model[i]->d_refl;
+    return v;
+  }
+
   int zero(void* p, size_t bytes) {
     HIPCHK(hipMemsetAsync(p, 0, bytes, s), "memset");
     return WEMFWI_OK;
@@ -612,7 +619,10 @@ static int wem_forward(const std::vector<Model*>& fm, Per per, Survey* sv, float
     RCCHK(c.inject_sources(r, w, sv->d.ngs, sv->d.offset));
     RCCHK(c.sweep(r, 0, 0, w, w));
     RCCHK(c.record(r, data, w, sv->d.ngr, sv->d.offset));
-    RCCHK(c.sweep(r, 1, 0, w, w, true));   // Up with sources r * W
+    {
+      const auto rr = Ctx::refl_rows(r);
+      RCCHK(c.sweep(r, 1, 0, w, w, &rr));   // Up with sources r * W
+    }
     RCCHK(c.record(r, data, w, sv->d.ngr, sv->d.offset));
   }
   return c.sync();
@@ -668,21 +678,22 @@ static int born_forward(const std::vector<Model*>& fm, Per per, Survey* sv, cons
     RCCHK(c.sweep(r, 0, 0, s1, s1));
     RCCHK(c.record(r, data, s1, sv->d.ngr, sv->d.offset));
     {
-      std::vector<MulRow> ic(n), rs1(n), rbg(n);
+      std::vector<MulRow> ic(n), rs1(n);
       for (int i = 0; i < n; ++i) {
         const size_t o = (size_t)i * c.nzx;
         const float2* dri = drb.as<float2>() + (mstride ? (size_t)r.slice[i] * c.nzx : 0);
         ic[i] = MulRow{tt + o, bg + o, dri};
         rs1[i] = MulRow{tt + o, r.model[i]->d_refl, s1 + o};
-        rbg[i] = MulRow{bg + o, r.model[i]->d_refl, bg + o};
       }
       RCCHK(c.mul_rows(ic, c.nzx, false, false));   // D * dr      (Born_refl::forward)
       RCCHK(c.mul_rows(rs1, c.nzx, false, true));   // + r * s1    (down-scattering)
-      RCCHK(c.mul_rows(rbg, c.nzx, false, false));  // r * D       (background up-going source)
     }
     RCCHK(c.sweep(r, 1, 0, tt, tt));
     RCCHK(c.record(r, data, tt, sv->d.ngr, sv->d.offset));
-    RCCHK(c.sweep(r, 1, 0, bg, bg));
+    {
+      const auto rr = Ctx::refl_rows(r);
+      RCCHK(c.sweep(r, 1, 0, bg, bg, &rr));         // U = Up(r * D), the product taken as it is read
+    }
     RCCHK(c.zero(s1, bytes));
     RCCHK(c.lin(r, -1, 0, 0, bg, dS, mstride, s1));
     RCCHK(c.sweep(r, 1, 0, s1, s1));
@@ -740,7 +751,7 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
     RCCHK(c.sweep(r, 0, 0, bg, bg));
     RCCHK(c.zero(R, bytes));
     RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset));
-    RCCHK(c.sweep(r, 1, 1, R, R, false, Q));   // R, and Q = conj(r) R on every row
+    RCCHK(c.sweep(r, 1, 1, R, R, nullptr, Q));   // R, and Q = conj(r) R on every row
     // dr += conj(D) * R (Born_refl::adjoint), summed straight from D and R
     // before LinDown^H rescales D
     RCCHK(reduce_chunk(c, r, sliced, accDR, bg, c.nzx, 0, 0, c.nz, R));
@@ -961,21 +972,18 @@ static int bornrefl_forward(Model* m, Survey* sv, const float2* image, int nh, f
     RCCHK(c.inject_sources(r, bg, sv->d.ngs, 0));
     RCCHK(c.sweep(r, 0, 0, bg, bg));
     if (nh <= 0) {
-      std::vector<MulRow> ic(n);
-      for (int i = 0; i < n; ++i) {
-        const size_t o = (size_t)i * c.nzx;
-        ic[i] = MulRow{sc + o, bg + o, image + (size_t)r.fi[i] * c.nzx};
-      }
-      RCCHK(c.mul_rows(ic, c.nzx, false, false));
-    }
-    if (nh > 0) {
+      // sc = Up(D * image[w]): the IC product taken as the Up sweep reads D
+      std::vector<const float2*> im(n);
+      for (int i = 0; i < n; ++i) im[i] = image + (size_t)r.fi[i] * c.nzx;
+      RCCHK(c.sweep(r, 1, 0, bg, sc, &im));
+    } else {
       for (int i = 0; i < n; ++i) {
         const size_t o = (size_t)i * c.nzx;
         HIPCHK(hipMemsetAsync(sc + o, 0, c.nzx * sizeof(float2), s), "memset");
         HIPCHK(launch_offset_ic_fwd(sc + o, bg + o, image, nh, c.nz, c.nx, s), "offset ic");
       }
+      RCCHK(c.sweep(r, 1, 0, sc, sc));
     }
-    RCCHK(c.sweep(r, 1, 0, sc, sc));
     RCCHK(c.record(r, data, sc, 0, 0));
   }
   return c.sync();
@@ -992,13 +1000,12 @@ static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const f
   RCCHK(acc.alloc(nimg * c.nzx * sizeof(double2), "image accumulator"));
   RCCHK(c.zero(acc.p, acc.n));
   std::vector<Model*> fm(nw, m);
-  auto chunks = make_chunks(sv->d.ns, nw, fm, Per::Freq, 3 * c.nzx * sizeof(float2), true);
+  auto chunks = make_chunks(sv->d.ns, nw, fm, Per::Freq, 2 * c.nzx * sizeof(float2), true);
   const size_t cap = chunks[0].shot.size() * c.nzx * sizeof(float2);
-  DevBuf B, S, C;
+  DevBuf B, S;
   RCCHK(B.alloc(cap, "background"));
   RCCHK(S.alloc(cap, "receiver wavefield"));
-  RCCHK(C.alloc(cap, "image contributions"));
-  float2 *bg = B.as<float2>(), *sc = S.as<float2>(), *cb = C.as<float2>();
+  float2 *bg = B.as<float2>(), *sc = S.as<float2>();
   DevBuf pb, ps;
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
@@ -1010,19 +1017,14 @@ static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const f
     RCCHK(c.inject_receivers(r, sc, data, 0, 0));
     RCCHK(c.sweep(r, 1, 1, sc, sc));
     if (nh <= 0) {
-      std::vector<MulRow> ic(n);
-      for (int i = 0; i < n; ++i) {
-        const size_t o = (size_t)i * c.nzx;
-        ic[i] = MulRow{cb + o, bg + o, sc + o};
-      }
-      RCCHK(c.mul_rows(ic, c.nzx, true, false));
+      // image[w] += conj(D) R, the product formed in the reduction
       int i0 = 0;
       while (i0 < n) {
         int i1 = i0 + 1;
         while (i1 < n && r.fi[i1] == r.fi[i0]) ++i1;
-        HIPCHK(launch_reduce_rows(acc.as<double2>() + (size_t)r.fi[i0] * c.nzx, cb + (size_t)i0 * c.nzx,
+        HIPCHK(launch_reduce_rows(acc.as<double2>() + (size_t)r.fi[i0] * c.nzx, bg + (size_t)i0 * c.nzx,
                                   (long long)c.nzx, (long long)c.nzx, i1 - i0, (int)((r.first + i0) % kRunRows),
-                                  true, s),
+                                  true, s, sc + (size_t)i0 * c.nzx),
                "reduce");
         i0 = i1;
       }
