@@ -18,6 +18,7 @@ struct LinRow {
   int zbase;             // depth of the first row of bg / m / out (0: whole arrays)
   const float2* post;    // adjoint only, or NULL: after use, bg[d] <- post[d] * bg[d] (post: [nz][nx]
                          // by absolute depth, e.g. the model's reflection coefficient)
+  int assign;            // adjoint only: 1 stores the contribution instead of adding it
 };
 
 // Injection of many devices as a gather per touched cell (CSR by cell).

@@ -73,7 +73,12 @@ __global__ void __launch_bounds__(64 * kLinWaves)
     lrow<V>(b, R.bg + r0 * nx, g, t);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmulc(st[e], b[e]);
-    acc_row<V>(R.out + r0 * nx, st, g, t);
+    if (R.assign) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) row_store<V>(R.out + r0 * nx, st[e], g, t, e);
+    } else {
+      acc_row<V>(R.out + r0 * nx, st, g, t);
+    }
     if (R.post) {
       // bg[d] <- post[d] * bg[d]: this item is the only reader of bg row d
       float2 q[E];

@@ -322,9 +322,9 @@ struct Ctx {
 
   // LinDown (dir=+1) / LinUp (dir=-1); post_refl (adjoint): each bg row,
   // once used, is replaced by r * bg (r the row model's reflection
-  // coefficient)
+  // coefficient); assign (adjoint): out rows are stored, not added to
   int lin(const Rows& r, int dir, int adjoint, int exact, float2* bg, const float2* m, size_t m_stride,
-          float2* out, bool post_refl = false) {
+          float2* out, bool post_refl = false, bool assign = false) {
     const int n = (int)r.shot.size();
     hlin.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -336,6 +336,7 @@ struct Ctx {
       hlin[i].model = model_index(r.model[i]);
       hlin[i].zbase = 0;
       hlin[i].post = post_refl ? r.model[i]->d_refl : nullptr;
+      hlin[i].assign = assign ? 1 : 0;
     }
     RCCHK(put(d_rows, hlin, s, "lin rows"));
     HIPCHK(launch_lin(N, d_rows.as<LinRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, dir, adjoint != 0,
@@ -380,7 +381,7 @@ struct Ctx {
   // Adjoint LinDown / LinUp over depths [dlo, dlo + dcount) of segment
   // buffers (bg, m and out share the row layout of sweep_seg).
   int lin_seg(const Rows& r, int dir, int exact, float2* bg, const float2* m, float2* out, size_t wstride,
-              int zbase, int dlo, int dcount, bool post_refl = false) {
+              int zbase, int dlo, int dcount, bool post_refl = false, bool assign = false) {
     if (dcount <= 0) return WEMFWI_OK;
     const int n = (int)r.shot.size();
     hlin.resize(n);
@@ -393,6 +394,7 @@ struct Ctx {
       hlin[i].model = model_index(r.model[i]);
       hlin[i].zbase = zbase;
       hlin[i].post = post_refl ? r.model[i]->d_refl : nullptr;
+      hlin[i].assign = assign ? 1 : 0;
     }
     RCCHK(put(d_rows, hlin, s, "lin rows"));
     HIPCHK(launch_lin(N, d_rows.as<LinRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, dir, true, exact != 0,
@@ -757,9 +759,10 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
     RCCHK(reduce_chunk(c, r, sliced, accDR, bg, c.nzx, 0, 0, c.nz, R));
     RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset));   // Q = conj(r) R + R^H d
     RCCHK(c.sweep(r, 0, 1, Q, Q));
-    RCCHK(c.zero(cs, bytes));
+    // LinDown^H stores rows 0..nz-2 of cs (no fill); row nz-1 starts at 0
+    HIPCHK(hipMemset2DAsync(cs + last, c.nzx * sizeof(float2), 0, c.nx * sizeof(float2), n, c.s), "memset");
     // LinDown^H; each D row d < nz-1 it reads becomes r * D (the Up source)
-    RCCHK(c.lin(r, +1, 1, exact, bg, Q, 0, cs, true));
+    RCCHK(c.lin(r, +1, 1, exact, bg, Q, 0, cs, true, true));
     {
       std::vector<MulRow> rbg(n);
       for (int i = 0; i < n; ++i) {
@@ -882,9 +885,13 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
       RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset, Ctx::Win{(long long)W, zb, a, qe}));
       RCCHK(c.sweep_seg(r, 0, 1, Q, W, zb, b, b - a, last ? nullptr : cq, nx));
       if (j > 0) RCCHK(c.copy_rows(cq, nx, Q + oa, W, n));
-      RCCHK(c.zero(cs, bytes));
+      // LinDown^H stores the rows [a, b) of cs (no fill); the last segment's
+      // row nz - 1 = b starts at 0
+      if (last)
+        HIPCHK(hipMemset2DAsync(cs + (size_t)(b - zb) * nx, W * sizeof(float2), 0, nx * sizeof(float2), n, c.s),
+               "memset");
       // LinDown^H over [a, b); the D rows it reads become r * D (Up sources)
-      RCCHK(c.lin_seg(r, +1, exact, bg, Q, cs, W, zb, a, b - a, true));
+      RCCHK(c.lin_seg(r, +1, exact, bg, Q, cs, W, zb, a, b - a, true, true));
       if (last) {  // the Up recursion starts from r * D at the last depth
         std::vector<MulRow> rbg(n);
         for (int i = 0; i < n; ++i) {
@@ -1311,7 +1318,7 @@ static int unit_lin(Model& m, int up, int adjoint, int mode, int nrows, const fl
   }
   std::vector<LinRow> h(nrows);
   for (int i = 0; i < nrows; ++i)
-    h[i] = LinRow{const_cast<float2*>(bg) + i * nzx, in + i * nzx, out + i * nzx, p[i], w[i], 0, 0, nullptr};
+    h[i] = LinRow{const_cast<float2*>(bg) + i * nzx, in + i * nzx, out + i * nzx, p[i], w[i], 0, 0, nullptr, 0};
   DevBuf rows;
   RCCHK(put(rows, h, s, "lin rows"));
   HIPCHK(launch_lin(m.N, rows.as<LinRow>(), nrows, m.d_tables, m.geom(), m.d.nz, up ? -1 : +1, adjoint != 0,
