@@ -634,6 +634,7 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
     h[i].smul = nullptr;
     h[i].out2 = nullptr;
     h[i].omul = nullptr;
+    h[i].src_skip = 0;
   }
   if ((rc = scratch.reserve((size_t)nrows * N * sizeof(float2), "scratch rows"))) return rc;
   for (int i = 0; i < nrows; ++i) h[i].scratch = (float2*)scratch.ptr + (size_t)i * N;
@@ -740,6 +741,7 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     h[k].smul = nullptr;
     h[k].out2 = nullptr;
     h[k].omul = nullptr;
+    h[k].src_skip = 0;
     h[k].scratch = (float2*)scratch.ptr + scr_off[k];
   }
   for (int k0 = 0; k0 < nrows;) {

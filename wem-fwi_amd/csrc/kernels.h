@@ -27,6 +27,8 @@ struct SweepRow {
                          // (also the initial row unless init is set); NULL: none
   float2* out2;          // second output rows (layout of out): conj(omul) * result; NULL: none
   const float2* omul;    // [nz][nx] (absolute depths) multiplier of out2
+  int src_skip;          // 1: only the source rows of depths src_lo..src_hi are read (the
+  int src_lo, src_hi;    // others are zero and need not be initialised); 0: every row
 };
 
 // One independent depth step.

@@ -123,12 +123,19 @@ __global__ void __launch_bounds__(64 * WavesV<V>::value)
   const int nsteps = R.nsteps >= 0 ? R.nsteps : dt.nz - 1;
   const int zb = R.zbase;
   float2 st[E];
-  load_row<V>(st, R.init ? R.init : R.in + (size_t)(first - zb) * nx, g, t);
+  if (R.init || !R.src_skip || (first >= R.src_lo && first <= R.src_hi)) {
+    load_row<V>(st, R.init ? R.init : R.in + (size_t)(first - zb) * nx, g, t);
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
+  }
   if (R.smul && !R.init) {
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmul(st[e], row_sample<V>(R.smul + (size_t)first * nx, g, t, e));
   }
-  if (R.in != R.out || R.init || R.smul) store_row<V>(R.out + (size_t)(first - zb) * nx, st, g, t);
+  // the first row is stored unless it already holds the state (in place,
+  // read as it is); with src_skip it may not have been initialised
+  if (R.in != R.out || R.init || R.smul || R.src_skip) store_row<V>(R.out + (size_t)(first - zb) * nx, st, g, t);
   if (R.out2) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -141,14 +148,16 @@ __global__ void __launch_bounds__(64 * WavesV<V>::value)
     if (!recursive && k > 0) load_row<V>(st, R.in + (size_t)(cur - zb) * nx, g, t);
     if constexpr (ADJ) step_adjoint<V, true>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
     else step_forward<V, true>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
-    float2 src[E];
-    load_row<V>(src, R.in + (size_t)(nxt - zb) * nx, g, t);
-    if (R.smul) {
+    if (!R.src_skip || (nxt >= R.src_lo && nxt <= R.src_hi)) {
+      float2 src[E];
+      load_row<V>(src, R.in + (size_t)(nxt - zb) * nx, g, t);
+      if (R.smul) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) src[e] = cmul(src[e], row_sample<V>(R.smul + (size_t)nxt * nx, g, t, e));
+        for (int e = 0; e < E; ++e) src[e] = cmul(src[e], row_sample<V>(R.smul + (size_t)nxt * nx, g, t, e));
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
     }
-#pragma unroll
-    for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
     store_row<V>(R.out + (size_t)(nxt - zb) * nx, st, g, t);
     if (R.out2) {
 #pragma unroll
@@ -376,6 +385,7 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
       h[i].smul = nullptr;
       h[i].out2 = nullptr;
       h[i].omul = nullptr;
+      h[i].src_skip = 0;
     }
     ok = hipMemcpy(drows, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice) == hipSuccess;
   }

@@ -59,6 +59,20 @@ struct Survey {
   std::vector<DevPlan> ssrc_plan, srec_plan;
   int ckpt = 0;  // Born adjoints: depth steps between kept wavefield rows (0: keep all)
   bool per_shot() const { return !ssrc_iz.empty(); }
+  // depths holding sources (receivers = false) or receivers of any shot:
+  // [lo, hi], lo > hi when there are none
+  void depth_range(bool receivers, int& lo, int& hi) const {
+    lo = 1 << 30;
+    hi = -1;
+    auto take = [&](const std::vector<int>& v) {
+      for (int z : v) {
+        lo = std::min(lo, z);
+        hi = std::max(hi, z);
+      }
+    };
+    take(receivers ? rec_iz : src_iz);
+    for (const auto& v : receivers ? srec_iz : ssrc_iz) take(v);
+  }
   ~Survey() {
     rplan.release();
     for (auto& p : ssrc_plan) p.release();
@@ -291,8 +305,10 @@ struct Ctx {
   // smul: per row, an [nz][nx] table multiplying every source row (and the
   // initial row) as it is read; q2: rows (layout of W) that also receive
   // conj(r) * the result, r the row model's reflection coefficients
+  // src_rows: only the source rows of depths [lo, hi] are read (nullptr: all)
   int sweep(const Rows& r, int up, int adjoint, const float2* src, float2* W,
-            const std::vector<const float2*>* smul = nullptr, float2* q2 = nullptr) {
+            const std::vector<const float2*>* smul = nullptr, float2* q2 = nullptr,
+            const std::pair<int, int>* src_rows = nullptr) {
     const int n = (int)r.shot.size();
     hsw.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -309,6 +325,9 @@ struct Ctx {
       hsw[i].smul = smul ? (*smul)[i] : nullptr;
       hsw[i].out2 = q2 ? q2 + (size_t)i * nzx : nullptr;
       hsw[i].omul = q2 ? r.model[i]->d_refl : nullptr;
+      hsw[i].src_skip = src_rows ? 1 : 0;
+      hsw[i].src_lo = src_rows ? src_rows->first : 0;
+      hsw[i].src_hi = src_rows ? src_rows->second : -1;
     }
     if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
@@ -367,6 +386,7 @@ struct Ctx {
       hsw[i].smul = nullptr;
       hsw[i].out2 = q2 ? q2 + (size_t)i * wstride : nullptr;
       hsw[i].omul = q2 ? r.model[i]->d_refl : nullptr;
+      hsw[i].src_skip = 0;
     }
     if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
@@ -554,6 +574,15 @@ struct Ctx {
     return v;
   }
 
+  // zeroes depths [lo, hi] of n rows of nzx samples (nothing when lo > hi)
+  int zero_depths(float2* W, int n, int lo, int hi) {
+    if (lo > hi || n <= 0) return WEMFWI_OK;
+    HIPCHK(hipMemset2DAsync(W + (size_t)lo * nx, nzx * sizeof(float2), 0, (size_t)(hi - lo + 1) * nx * sizeof(float2),
+                            (size_t)n, s),
+           "memset");
+    return WEMFWI_OK;
+  }
+
   int zero(void* p, size_t bytes) {
     HIPCHK(hipMemsetAsync(p, 0, bytes, s), "memset");
     return WEMFWI_OK;
@@ -615,11 +644,14 @@ static int wem_forward(const std::vector<Model*>& fm, Per per, Survey* sv, float
   DevBuf W;
   RCCHK(W.alloc(chunks[0].shot.size() * c.nzx * sizeof(float2), "wavefield"));
   float2* w = W.as<float2>();
+  std::pair<int, int> srows;
+  sv->depth_range(false, srows.first, srows.second);
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
-    RCCHK(c.zero(w, n * c.nzx * sizeof(float2)));
+    // only the source depths are read as sources: only they are cleared
+    RCCHK(c.zero_depths(w, n, srows.first, srows.second));
     RCCHK(c.inject_sources(r, w, sv->d.ngs, sv->d.offset));
-    RCCHK(c.sweep(r, 0, 0, w, w));
+    RCCHK(c.sweep(r, 0, 0, w, w, nullptr, nullptr, &srows));
     RCCHK(c.record(r, data, w, sv->d.ngr, sv->d.offset));
     {
       const auto rr = Ctx::refl_rows(r);
@@ -745,15 +777,19 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
   RCCHK(CS.alloc(cap, "dS contributions"));
   float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>();
   const size_t last = (size_t)(c.nz - 1) * c.nx;
+  std::pair<int, int> srows, rrows;
+  sv->depth_range(false, srows.first, srows.second);
+  sv->depth_range(true, rrows.first, rrows.second);
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
-    const size_t bytes = n * c.nzx * sizeof(float2);
-    RCCHK(c.zero(bg, bytes));
+    // the sweeps read only the source / receiver depths as sources: only
+    // those rows are cleared before the injection
+    RCCHK(c.zero_depths(bg, n, srows.first, srows.second));
     RCCHK(c.inject_sources(r, bg, sv->d.ngs, sv->d.offset));
-    RCCHK(c.sweep(r, 0, 0, bg, bg));
-    RCCHK(c.zero(R, bytes));
+    RCCHK(c.sweep(r, 0, 0, bg, bg, nullptr, nullptr, &srows));
+    RCCHK(c.zero_depths(R, n, rrows.first, rrows.second));
     RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset));
-    RCCHK(c.sweep(r, 1, 1, R, R, nullptr, Q));   // R, and Q = conj(r) R on every row
+    RCCHK(c.sweep(r, 1, 1, R, R, nullptr, Q, &rrows));   // R, and Q = conj(r) R on every row
     // dr += conj(D) * R (Born_refl::adjoint), summed straight from D and R
     // before LinDown^H rescales D
     RCCHK(reduce_chunk(c, r, sliced, accDR, bg, c.nzx, 0, 0, c.nz, R));
