@@ -682,7 +682,7 @@ static int batch_streams(hipStream_t (&out)[4]) {
 // tables and scratch rows stay on the device between runs.
 
 int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf* const* in, cf* const* out,
-                     hipStream_t s) {
+                     hipStream_t s, const int* src_lo, const int* src_hi) {
   if (nrows < 1 || !ms || !freq_hz || !in || !out) return fail(WEMFWI_ERR_ARG, "batch: invalid argument");
   std::vector<int> midx(nrows);
   for (int i = 0; i < nrows; ++i) {
@@ -741,7 +741,9 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     h[k].smul = nullptr;
     h[k].out2 = nullptr;
     h[k].omul = nullptr;
-    h[k].src_skip = 0;
+    h[k].src_skip = (src_lo && src_hi) ? 1 : 0;
+    h[k].src_lo = (src_lo && src_hi) ? src_lo[i] : 0;
+    h[k].src_hi = (src_lo && src_hi) ? src_hi[i] : -1;
     h[k].scratch = (float2*)scratch.ptr + scr_off[k];
   }
   for (int k0 = 0; k0 < nrows;) {

@@ -131,8 +131,11 @@ struct BatchPlan {
   const DepthTables* d_tabs = nullptr;
   std::vector<hipEvent_t> events;      // fork / join events of enqueue (one more than groups)
   ~BatchPlan();
+  // src_lo / src_hi (optional, per row): only the source rows of those
+  // depths are read (the sweep kernel's src_skip)
   int build(int nrows, Model* const* ms, const float* freq_hz, const std::complex<float>* const* in,
-            std::complex<float>* const* out, hipStream_t s);
+            std::complex<float>* const* out, hipStream_t s, const int* src_lo = nullptr,
+            const int* src_hi = nullptr);
   // WEMFWI_ERR_ARG if a model's phase table was rebuilt since planning
   int check() const;
   // launches the groups behind the work already on s and makes s wait for

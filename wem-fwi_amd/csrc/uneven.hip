@@ -388,8 +388,40 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
   UCHK(Rw.reserve(total * sizeof(float2), "receiver wavefields"));
   float2* D = (float2*)Dw.ptr;
   float2* R = (float2*)Rw.ptr;
-  UHIP(hipMemsetAsync(D, 0, total * sizeof(float2), s), "memset");
-  UHIP(hipMemsetAsync(R, 0, total * sizeof(float2), s), "memset");
+  // The sweeps read only the source depth (D) and the receiver depths (R)
+  // as sources, so only those rows of each wavefield are cleared.
+  std::vector<int> dlo(nrows), dhi(nrows), rlo(nrows), rhi(nrows);
+  auto clear_rows = [&](float2* W, const std::vector<int>& lo, const std::vector<int>& hi) -> int {
+    size_t k0 = 0;
+    for (int sh = s0; sh < s1; ++sh) {
+      const wemfwi_image_shot& S = shots[sh];
+      const int nx = S.model->m.d.nx;
+      if (S.nw > 0 && lo[k0] <= hi[k0])
+        UHIP(hipMemset2DAsync(W + roff[k0] + (size_t)lo[k0] * nx, (size_t)nz * nx * sizeof(float2), 0,
+                              (size_t)(hi[k0] - lo[k0] + 1) * nx * sizeof(float2), (size_t)S.nw, s),
+             "memset");
+      k0 += S.nw;
+    }
+    return WEMFWI_OK;
+  };
+  {
+    size_t k0 = 0;
+    for (int sh = s0; sh < s1; ++sh) {
+      const wemfwi_image_shot& S = shots[sh];
+      int lo = 1 << 30, hi = -1;
+      for (int r = 0; r < S.nr; ++r) {
+        lo = std::min(lo, S.rec_iz[r]);
+        hi = std::max(hi, S.rec_iz[r]);
+      }
+      for (int i = 0; i < S.nw; ++i, ++k0) {
+        dlo[k0] = dhi[k0] = S.src_iz;
+        rlo[k0] = lo;
+        rhi[k0] = hi;
+      }
+    }
+  }
+  UCHK(clear_rows(D, dlo, dhi));
+  if (!fwi) UCHK(clear_rows(R, rlo, rhi));   // with a model image, the scatter writes every row of R
   // receiver coordinates of the group, one block per shot
   std::vector<int> hdev;
   std::vector<size_t> dev_off(s1 - s0);
@@ -488,19 +520,21 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
   for (int i = 0; i < nrows; ++i) pin[i] = pout[i] = (cf*)(D + roff[i]);
   {
     BatchPlan p;
-    UCHK(p.build(nrows, rm.data(), rf.data(), pin.data(), pout.data(), s));
+    UCHK(p.build(nrows, rm.data(), rf.data(), pin.data(), pout.data(), s, dlo.data(), dhi.data()));
     UCHK(p.run(0, 0, s));
   }
   for (int i = 0; i < nrows; ++i) pin[i] = pout[i] = (cf*)(R + roff[i]);
   {
-    BatchPlan p;
-    UCHK(p.build(nrows, rm.data(), rf.data(), pin.data(), pout.data(), s));
+    BatchPlan p;   // Up^H of the receiver (or residual) rows
+    UCHK(p.build(nrows, rm.data(), rf.data(), pin.data(), pout.data(), s, rlo.data(), rhi.data()));
     if (fwi) {
+      BatchPlan pu;   // Up of the scattered field: every row is a source
+      UCHK(pu.build(nrows, rm.data(), rf.data(), pin.data(), pout.data(), s));
       // L I: offset scatter of the image on D into every row, Up, record
       UHIP(launch_offset_scatter_shots(R, fwi, (long long)nz * nxi, nxi, (const UShot*)ushw.ptr, s1 - s0,
                                        (const int*)ordw.ptr, ntiles, D, nz, nh, s),
            "offset scatter");
-      UCHK(p.run(1, 0, s));
+      UCHK(pu.run(1, 0, s));
       if (nr_max > 0) {
         hipLaunchKernelGGL(residual_rows_kernel, dim3((unsigned)((nr_max + 255) / 256), (unsigned)std::min(nrows, 65535)),
                            dim3(256), 0, s, (const ResRow*)resrw.ptr, nrows);
@@ -512,7 +546,7 @@ int image_group(const wemfwi_image_shot* shots, int s0, int s1, const std::vecto
         UHIP(hipGetLastError(), "misfit");
       }
       // R^H residual, then Up^H as in the migration
-      UHIP(hipMemsetAsync(R, 0, total * sizeof(float2), s), "memset");
+      UCHK(clear_rows(R, rlo, rhi));
       hipLaunchKernelGGL(rec_rows_kernel, dim3((nrows + 63) / 64), dim3(64), 0, s, (const RecRow*)recw.ptr, nrows);
       UHIP(hipGetLastError(), "inject residuals");
     }
