@@ -368,8 +368,10 @@ struct Ctx {
   // (wstride samples apart from row to row); the sweep runs nsteps steps
   // from depth first, starting from init + i * init_stride (NULL: the
   // buffer's own row first), and adds the source rows already in W.
+  // src_rows: only the source rows of depths [lo, hi] are read (nullptr: all)
   int sweep_seg(const Rows& r, int up, int adjoint, float2* W, size_t wstride, int zbase, int first, int nsteps,
-                const float2* init, size_t init_stride, float2* q2 = nullptr) {
+                const float2* init, size_t init_stride, float2* q2 = nullptr,
+                const std::pair<int, int>* src_rows = nullptr) {
     const int n = (int)r.shot.size();
     hsw.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -386,7 +388,9 @@ struct Ctx {
       hsw[i].smul = nullptr;
       hsw[i].out2 = q2 ? q2 + (size_t)i * wstride : nullptr;
       hsw[i].omul = q2 ? r.model[i]->d_refl : nullptr;
-      hsw[i].src_skip = 0;
+      hsw[i].src_skip = src_rows ? 1 : 0;
+      hsw[i].src_lo = src_rows ? src_rows->first : 0;
+      hsw[i].src_hi = src_rows ? src_rows->second : -1;
     }
     if (scr.n < (size_t)n * N * sizeof(float2)) RCCHK(scr.alloc((size_t)n * N * sizeof(float2), "scratch rows"));
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
@@ -574,11 +578,13 @@ struct Ctx {
     return v;
   }
 
-  // zeroes depths [lo, hi] of n rows of nzx samples (nothing when lo > hi)
-  int zero_depths(float2* W, int n, int lo, int hi) {
+  // zeroes depths [lo, hi] of n rows of nzx samples (nothing when lo > hi);
+  // with a window (wstride samples per row, first depth zbase) its rows
+  int zero_depths(float2* W, int n, int lo, int hi, size_t wstride = 0, int zbase = 0) {
     if (lo > hi || n <= 0) return WEMFWI_OK;
-    HIPCHK(hipMemset2DAsync(W + (size_t)lo * nx, nzx * sizeof(float2), 0, (size_t)(hi - lo + 1) * nx * sizeof(float2),
-                            (size_t)n, s),
+    const size_t pitch = wstride ? wstride : nzx;
+    HIPCHK(hipMemset2DAsync(W + (size_t)(lo - zbase) * nx, pitch * sizeof(float2), 0,
+                            (size_t)(hi - lo + 1) * nx * sizeof(float2), (size_t)n, s),
            "memset");
     return WEMFWI_OK;
   }
@@ -884,21 +890,27 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
   float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>(),
          *kd = KD.as<float2>(), *kr = KR.as<float2>(), *cq = CQ.as<float2>(), *cu = CU.as<float2>();
   const size_t kpitch = (size_t)nseg * nx;  // checkpoint rows of one chunk row
+  std::pair<int, int> srows, rrows;
+  sv->depth_range(false, srows.first, srows.second);
+  sv->depth_range(true, rrows.first, rrows.second);
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
-    const size_t bytes = n * W * sizeof(float2);
     // D on [a, b] and R on [ra, rb] of segment j, from the kept rows
     // (backward pass: with Q = conj(r) R on the rows of R's segment)
+    // Only the device depths inside a segment's injection window are read
+    // as sources, and only those window rows are cleared.
     auto fields = [&](int j, float2* q2) -> int {
       const int a = j * k, b = std::min(a + k, nz - 1), zb = std::max(a - 1, 0);
       const bool last = j == nseg - 1;
-      RCCHK(c.zero(bg, bytes));
+      const std::pair<int, int> sw{std::max(srows.first, j ? a + 1 : 0), std::min(srows.second, b)};
+      RCCHK(c.zero_depths(bg, n, sw.first, sw.second, W, zb));
       RCCHK(c.inject_sources(r, bg, sv->d.ngs, sv->d.offset, Ctx::Win{(long long)W, zb, j ? a + 1 : 0, b}));
-      RCCHK(c.sweep_seg(r, 0, 0, bg, W, zb, a, b - a, j ? kd + (size_t)j * nx : nullptr, kpitch));
+      RCCHK(c.sweep_seg(r, 0, 0, bg, W, zb, a, b - a, j ? kd + (size_t)j * nx : nullptr, kpitch, nullptr, &sw));
       const int ra = j ? a - 1 : 0, rb = last ? b : b - 1;
-      RCCHK(c.zero(R, bytes));
+      const std::pair<int, int> rw{std::max(rrows.first, j ? a : 0), std::min(rrows.second, rb)};
+      RCCHK(c.zero_depths(R, n, rw.first, rw.second, W, zb));
       RCCHK(c.inject_receivers(r, R, data, sv->d.ngr, sv->d.offset, Ctx::Win{(long long)W, zb, j ? a : 0, rb}));
-      RCCHK(c.sweep_seg(r, 1, 1, R, W, zb, ra, rb - ra, j ? kr + (size_t)j * nx : nullptr, kpitch, q2));
+      RCCHK(c.sweep_seg(r, 1, 1, R, W, zb, ra, rb - ra, j ? kr + (size_t)j * nx : nullptr, kpitch, q2, &rw));
       return WEMFWI_OK;
     };
     // forward pass: the kept rows D(a_j) and R(a_j - 1)
