@@ -707,12 +707,14 @@ static int born_forward(const std::vector<Model*>& fm, Per per, Survey* sv, cons
   RCCHK(S1.alloc(cap, "scattered"));
   RCCHK(T.alloc(cap, "scattered"));
   float2 *bg = B.as<float2>(), *s1 = S1.as<float2>(), *tt = T.as<float2>();
+  std::pair<int, int> srows;
+  sv->depth_range(false, srows.first, srows.second);
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
     const size_t bytes = n * c.nzx * sizeof(float2);
-    RCCHK(c.zero(bg, bytes));
+    RCCHK(c.zero_depths(bg, n, srows.first, srows.second));   // only the source depths are read as sources
     RCCHK(c.inject_sources(r, bg, sv->d.ngs, sv->d.offset));
-    RCCHK(c.sweep(r, 0, 0, bg, bg));
+    RCCHK(c.sweep(r, 0, 0, bg, bg, nullptr, nullptr, &srows));
     RCCHK(c.zero(s1, bytes));
     RCCHK(c.lin(r, +1, 0, 0, bg, dS, mstride, s1));
     RCCHK(c.sweep(r, 0, 0, s1, s1));
@@ -1020,12 +1022,13 @@ static int bornrefl_forward(Model* m, Survey* sv, const float2* image, int nh, f
   RCCHK(B.alloc(cap, "background"));
   RCCHK(S.alloc(cap, "scattered"));
   float2 *bg = B.as<float2>(), *sc = S.as<float2>();
+  std::pair<int, int> srows;
+  sv->depth_range(false, srows.first, srows.second);
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
-    const size_t bytes = n * c.nzx * sizeof(float2);
-    RCCHK(c.zero(bg, bytes));
+    RCCHK(c.zero_depths(bg, n, srows.first, srows.second));   // only the source depths are read as sources
     RCCHK(c.inject_sources(r, bg, sv->d.ngs, 0));
-    RCCHK(c.sweep(r, 0, 0, bg, bg));
+    RCCHK(c.sweep(r, 0, 0, bg, bg, nullptr, nullptr, &srows));
     if (nh <= 0) {
       // sc = Up(D * image[w]): the IC product taken as the Up sweep reads D
       std::vector<const float2*> im(n);
@@ -1062,15 +1065,18 @@ static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const f
   RCCHK(S.alloc(cap, "receiver wavefield"));
   float2 *bg = B.as<float2>(), *sc = S.as<float2>();
   DevBuf pb, ps;
+  std::pair<int, int> srows, rrows;
+  sv->depth_range(false, srows.first, srows.second);
+  sv->depth_range(true, rrows.first, rrows.second);
   for (const Rows& r : chunks) {
     const int n = (int)r.shot.size();
-    const size_t bytes = n * c.nzx * sizeof(float2);
-    RCCHK(c.zero(bg, bytes));
+    // only the source / receiver depths are read as sources and cleared
+    RCCHK(c.zero_depths(bg, n, srows.first, srows.second));
     RCCHK(c.inject_sources(r, bg, sv->d.ngs, 0));
-    RCCHK(c.sweep(r, 0, 0, bg, bg));
-    RCCHK(c.zero(sc, bytes));
+    RCCHK(c.sweep(r, 0, 0, bg, bg, nullptr, nullptr, &srows));
+    RCCHK(c.zero_depths(sc, n, rrows.first, rrows.second));
     RCCHK(c.inject_receivers(r, sc, data, 0, 0));
-    RCCHK(c.sweep(r, 1, 1, sc, sc));
+    RCCHK(c.sweep(r, 1, 1, sc, sc, nullptr, nullptr, &rrows));
     if (nh <= 0) {
       // image[w] += conj(D) R, the product formed in the reduction
       int i0 = 0;
