@@ -234,7 +234,9 @@ def fd_check(sv, obs, s0, den, gS, ndir, seed=44, rel_step=1e-3):
     return {"directions": ndir, "seed": seed, "step": f"{rel_step:g} x mean slowness, smoothed 3 cells",
             "cosine_fd_vs_gradient": round(float(np.dot(fd, an) / np.linalg.norm(fd) / np.linalg.norm(an)), 4),
             "median_rel_err": round(float(np.median(np.abs(fd - an) / np.abs(fd))), 4),
-            "note": "Born^H r is not the derivative of WEM in the reference (dReflect, Scatter); see DESIGN"}
+            "note": "Born^H r is not the derivative of WEM in the reference: dReflect as written is the "
+                    "derivative of Reflect times (a+b)^4, and Scatter does not follow the reference "
+                    "slownesses' dependence on S"}
 
 
 def fwi_gradient_bench(sample_rows: int, fd_dirs: int = 20):
