@@ -36,7 +36,7 @@ std::vector<float> cos2_window(int ng) {
 }
 
 struct DevPlan {
-  int nx = -1, ng = -1, offset = -1;
+  int nx = -1, nz = -1, ng = -1, offset = -1;
   InjPlan inj{};
   RecPlan rec{};
   std::vector<void*> allocs;
@@ -95,7 +95,7 @@ static int upload(std::vector<void*>& allocs, T** p, const std::vector<T>& h, co
 // Receiver injection / recording plans of a device list on an nx-wide grid.
 static int build_plan(DevPlan& P, const std::vector<int>& iz, const std::vector<int>& ix, int ng, int offset,
                       int nx, int nz) {
-  if (P.nx == nx && P.ng == ng && P.offset == offset) return WEMFWI_OK;
+  if (P.nx == nx && P.nz == nz && P.ng == ng && P.offset == offset) return WEMFWI_OK;
   P.release();
   const std::vector<float> g = cos2_window(ng);
   // recording: contributions by device, in the order of Injection.cpp:69-80
@@ -151,6 +151,7 @@ static int build_plan(DevPlan& P, const std::vector<int>& iz, const std::vector<
   P.inj = InjPlan{(int)keys.size(), d_cells, d_iptr, d_idev, d_iw};
   P.rec = RecPlan{(int)ix.size(), d_rptr, d_rcell, d_rw};
   P.nx = nx;
+  P.nz = nz;
   P.ng = ng;
   P.offset = offset;
   return WEMFWI_OK;
@@ -628,8 +629,21 @@ static std::vector<Rows> make_chunks(int ns, int nw, const std::vector<Model*>& 
   return out;
 }
 
+// Every source and receiver depth of the survey (shared and per-shot lists)
+// must lie inside the model before anything is queued: the depth-only clears
+// and the injection windows address wavefield rows by these indices.
 static int check_survey(Survey* sv, Model* m) {
   if (!sv || !m) return fail(WEMFWI_ERR_ARG, "null survey or model");
+  const int nz = m->d.nz;
+  auto inside = [nz](const std::vector<int>& v) {
+    for (int z : v)
+      if (z < 0 || z >= nz) return false;
+    return true;
+  };
+  bool ok = inside(sv->src_iz) && inside(sv->rec_iz);
+  for (const auto& v : sv->ssrc_iz) ok = ok && inside(v);
+  for (const auto& v : sv->srec_iz) ok = ok && inside(v);
+  if (!ok) return fail(WEMFWI_ERR_ARG, "device depth index outside the model");
   return WEMFWI_OK;
 }
 
