@@ -98,16 +98,36 @@ def test_refsampler_rejects_nonpositive_reference():
 
 
 def test_refsampler_partition_after_buffer_fills():
-    # once the buffer holds no zeros the partition follows the sorted buffer
-    # midpoints; check against a direct evaluation of the while loops
-    nz, nx, nref = 40, 8, 2
+    """Once the nx-long reference buffer holds live or stale values, each
+    depth's partition follows the midpoints of the sorted buffer.  Checked
+    against a vectorised restatement without the while loops: the
+    comparison (s - R[i]) < (R[i+1] - s) is monotone in s, so the loops of
+    RefSampler.cpp:49-81 put a sample in class c = the number of leading
+    i = 0, 1, ... for which it fails, and class c holds the c-th smallest
+    live reference.  The buffer history (live values written to its first
+    nref entries, then the whole buffer sorted, RefSampler.cpp:18,63-65) is
+    replayed alongside."""
+    nz, nx, nref = 40, 8, 3
     rng = np.random.default_rng(7)
     s = (1 / rng.uniform(1500, 4500, (nz, nx)) ** 2).astype(np.complex64)
     t = O.refsampler(s, nref)
-    # every depth is a partition into at most nref classes, x unassigned only
-    # when nref == 1
-    assert np.all(t.refid >= 0)
-    assert np.all(t.refid < nref)
+    f32 = np.float32
+    buf = np.zeros(nx, dtype=f32)
+    filled = False
+    for iz in range(nz):
+        live = t.sref[iz].real.astype(f32)
+        buf[:nref] = live
+        buf.sort()
+        filled = filled or bool(np.all(buf > 0))
+        sx = s[iz].real.astype(f32)
+        fails = np.stack([(sx - buf[i]).astype(f32) >= (buf[i + 1] - sx).astype(f32)
+                          for i in range(nref - 1)])
+        lead = np.cumprod(fails, axis=0).sum(axis=0)        # leading failures per sample
+        want = np.argsort(live, kind="stable")[lead]
+        np.testing.assert_array_equal(t.refid[iz], want, err_msg=f"depth {iz}")
+    assert filled        # the case the test is about was reached
+    # the partition is not the trivial one: several classes in use
+    assert len(np.unique(t.refid[nz - 1])) > 1
 
 
 # ---------------------------------------------------------------- one step
@@ -251,9 +271,16 @@ def test_dreflect_and_reflect_dot_products():
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
     r = O.reflect_coef(s, den)
     assert np.all(r[0] == 0)
-    # one interface with equal density: r = (sqrt(S1)-sqrt(S2))/(sqrt(S1)+sqrt(S2))
-    np.testing.assert_allclose(r[3], (np.sqrt(s[2]) - np.sqrt(s[3])) / (np.sqrt(s[2]) + np.sqrt(s[3]))
-                               if np.allclose(den[3], den[2]) else r[3])
+    # constant density: r = (v2 - v1)/(v2 + v1) at every interface, from
+    # velocities (the normal-incidence coefficient of acoustic impedance)
+    v = RNG.uniform(1500, 3000, (nz, nx))
+    r = O.reflect_coef((1 / v ** 2).astype(complex), np.ones((nz, nx), complex))
+    np.testing.assert_allclose(r[1:], (v[1:] - v[:-1]) / (v[1:] + v[:-1]), rtol=1e-12)
+    # constant slowness: r = (rho2 - rho1)/(rho2 + rho1)
+    r = O.reflect_coef(np.full((nz, nx), 1 / 2000.0 ** 2, complex), den)
+    np.testing.assert_allclose(r[1:], (den[1:] - den[:-1]) / (den[1:] + den[:-1]), rtol=1e-12)
+    # a uniform medium reflects nothing
+    assert np.all(O.reflect_coef(np.full((nz, nx), 1e-7, complex), np.ones((nz, nx), complex)) == 0)
 
 
 def test_dreflect_as_written_is_the_derivative_times_interface_power():
@@ -578,3 +605,182 @@ def test_depths_without_live_reference_give_zero_rows():
     W[0, nx // 2] = 1.0
     D = O.down_fwd(W, w, slow, ref, p)
     assert np.all(D[nx:] == 0) and np.abs(D[nx - 2]).max() > 0
+
+
+# ---------------------------------------------------------------------------
+# pins of the scatterer's Taylor terms, the LinDown / LinUp depth indexing
+# and the split-step exponent, from the mathematics of the linearised step
+# ---------------------------------------------------------------------------
+
+def _kgrid_ref(n, dx):
+    """The reference k grid written out (Phshift_cmplx.cpp:12-24): dk =
+    2 pi/(dx (n-1)), k[n/2] = 0 (and k[n/2+1] for odd n)."""
+    dk = 2 * np.pi / (dx * (n - 1))
+    j = np.arange(n)
+    k = np.where(j < n // 2, j * dk, -(n - j) * dk).astype(float)
+    k[n // 2] = 0.0
+    if n % 2:
+        k[n // 2 + 1] = 0.0
+    return k
+
+
+def _binom_half(j):
+    """Coefficient of x^j in (1 - x)^(-1/2): binom(2j, j) / 4^j."""
+    return math.comb(2 * j, j) / 4.0 ** j
+
+
+def _scatter_series(k, w, dz, sp, ntaylor):
+    """k-domain multiplier of the laterally constant scatterer: the first
+    ntaylor + 1 terms of -i w dz (S' - k^2/w^2)^(-1/2) expanded in
+    x = k^2/(w^2 S')."""
+    return -1j * w * dz * sum(_binom_half(j) * (k * k / (w * w)) ** j * sp ** (-0.5 - j)
+                              for j in range(ntaylor + 1))
+
+
+@pytest.mark.parametrize("ntaylor", [0, 1, 2, 3, 4])
+def test_scatter_multiplier_is_the_binomial_series_of_the_inverse_root(ntaylor):
+    """In laterally constant S the scatterer is k-diagonal (Scatter.cpp:35-49
+    with Scatter.h:36 coefficients).  Its multiplier must be the
+    (ntaylor+1)-term binomial series of -i w dz (S' - k^2/w^2)^(-1/2),
+    S' = S - eps Re S, whose coefficients binom(2j, j)/4^j come from
+    (1 - x)^(-1/2) here, not from the oracle's table; and it must lie within
+    the series remainder sum_{j > ntaylor} c_j x^j S'^(-1/2) of the closed
+    form, x = k^2/(w^2 S') <= 0.5 on this grid.  A wrong coefficient of any
+    order fails the first check; the second ties the series to the closed
+    form it approximates."""
+    nx, dx, dz, eps = 32, 25.0, 10.0, 0.04
+    s = np.full((3, nx), 1.0 / 2000.0 ** 2, dtype=np.complex64)
+    p = O.StepParams(nx=nx, dx=dx, dz=dz, tap=0, pad=0, nref=1, eps=eps, prop="ssf", ntaylor=ntaylor)
+    w = 2 * np.pi * 60.0
+    S = float(s[1, 0].real)
+    sp = S - eps * S
+    k = _kgrid_ref(nx, dx)
+    x = k * k / (w * w * sp)
+    assert x.max() < 0.5
+    rng = np.random.default_rng(3)
+    row = rng.standard_normal(nx) + 1j * rng.standard_normal(nx)
+    mult = np.fft.fft(O.scatter_fwd(row, 1, w, s, p)) / np.fft.fft(row)
+    np.testing.assert_allclose(mult, _scatter_series(k, w, dz, sp, ntaylor), rtol=1e-12)
+    closed = -1j * w * dz * (sp - k * k / (w * w)) ** -0.5
+    rem = sum(_binom_half(j) * x ** j for j in range(ntaylor + 1, ntaylor + 80)) * sp ** -0.5 * w * dz
+    assert np.all(np.abs(mult - closed) <= rem * (1 + 1e-9) + 1e-12 * np.abs(closed))
+
+
+def _layered(nx, nz, v):
+    return np.repeat((1.0 / np.asarray(v, float) ** 2)[:, None], nx, axis=1).astype(np.complex64)
+
+
+@pytest.mark.parametrize("up", [False, True])
+def test_lin_oneway_closed_form_in_a_layered_medium(up):
+    """A perturbation on the single depth z0 of a layered medium (nref = 1,
+    laterally constant S, so every step is k-diagonal and the split step is
+    1): LinDown puts P_z0 Sc_z0 (bg[z0] dS[z0]) on row z0 + 1 and nothing
+    elsewhere; LinUp puts the same on row z0 - 1 (LinOneWay.cpp:33-58,
+    89-115).  P and Sc are built here from the k grid with the slowness of
+    depth z0; every depth has its own slowness, so an operator taken at
+    z0 +- 1, or a result written to another row, fails.  The adjoints
+    (LinOneWay.cpp:60-87, 117-142) return conj(bg[z0]) Sc^H P^H d on row z0
+    (both adjoint modes coincide in laterally constant S)."""
+    nx, nz, dx, dz, eps, z0 = 16, 7, 12.0, 8.0, 0.04, 3
+    s = _layered(nx, nz, [1500, 1700, 2100, 2600, 2900, 3300, 3600])
+    p = O.StepParams(nx=nx, dx=dx, dz=dz, tap=0, pad=0, nref=1, eps=eps, prop="ssf", ntaylor=1)
+    ref = O.refsampler(s, 1)
+    rng = np.random.default_rng(17)
+    bg = crand(nz, nx, rng=rng)
+    m = np.zeros((nz, nx), complex)
+    m[z0] = crand(nx, rng=rng)
+    w = 2 * np.pi * 9.0
+    S0 = float(s[z0, 0].real)
+    k = _kgrid_ref(nx, dx)
+    P = _homog_step(nx, dx, dz, w, S0, eps)
+    F = np.exp(-2j * np.pi * np.outer(np.arange(nx), np.arange(nx)) / nx) / np.sqrt(nx)
+    Sc = F.conj().T @ np.diag(_scatter_series(k, w, dz, S0 * (1 - eps), 1)) @ F
+    row = z0 - 1 if up else z0 + 1
+    fwd = (O.linup_fwd if up else O.lindown_fwd)(m, bg, w, s, ref, p)
+    want = np.zeros((nz, nx), complex)
+    want[row] = P @ Sc @ (bg[z0] * m[z0])
+    assert np.linalg.norm(fwd - want) <= 1e-12 * np.linalg.norm(want)
+    d = np.zeros((nz, nx), complex)
+    d[row] = crand(nx, rng=rng)
+    wantH = np.zeros((nz, nx), complex)
+    wantH[z0] = np.conj(bg[z0]) * (Sc.conj().T @ P.conj().T @ d[row])
+    for mode in ("exact", "compat"):
+        adj = (O.linup_adj if up else O.lindown_adj)(d, bg, w, s, ref, p, mode=mode)
+        assert np.linalg.norm(adj - wantH) <= 1e-12 * np.linalg.norm(wantH), mode
+
+
+def test_split_step_closed_form_and_its_derivative_is_the_scatterer():
+    """A constant row through one SSF step (nref = 1, eps = 0, no taper or
+    FFT pad) has only the k = 0 component: out(x) = exp(-i w sqrt(Sref) dz)
+    * exp(-i w dz (S(x) - Sref)/sqrt(Sref)) (the vertical phase shift times
+    SplitStep.cpp:13).  The scatterer is the linearisation of the step: the
+    derivative of out(x) with respect to S(x), at S(x) = Sref, divided by
+    out(x), equals the k = 0 scatterer of that sample, -i w dz S^(-1/2)
+    (Scatter.cpp:82, ntaylor = 0; its scale is pinned against the phase
+    shift by the linearisation test below).  The textbook split step
+    (exponent halved) fails both checks.  Slownesses are powers of two so
+    that the float32 model holds S +- h exactly."""
+    nx, nz, dx, dz, iz = 32, 4, 10.0, 10.0, 2
+    sref = 2.0 ** -22                                # v = 2048 m/s
+    x = np.arange(nx)
+    s = np.full((nz, nx), sref, dtype=np.complex64)
+    s[iz] = (sref * (1 + 2.0 ** -5 * np.round(8 * np.sin(2 * np.pi * x / nx)))).astype(np.complex64)
+    x1 = 5
+    s[iz, nx // 2] = sref                            # RefSampler samples x = nx/2 (nref = 1)
+    s[iz, x1] = sref
+    p = O.StepParams(nx=nx, dx=dx, dz=dz, tap=0, pad=0, nref=1, eps=0.0, prop="ssf", ntaylor=0)
+    ref = O.refsampler(s, 1)
+    assert float(ref.sref[iz, 0].real) == sref and np.all(ref.refid[iz] == 0)
+    w = 2 * np.pi * 17.0
+    out = O.onestep_fwd(np.ones(nx, complex), iz, w, s, ref, p)
+    S = s[iz].real.astype(float)
+    want = np.exp(-1j * w * np.sqrt(sref) * dz) * np.exp(-1j * w * dz * (S - sref) / np.sqrt(sref))
+    np.testing.assert_allclose(out, want, rtol=1e-12)
+    h = 2.0 ** -34
+    sp, sm = s.copy(), s.copy()
+    sp[iz, x1] += h
+    sm[iz, x1] -= h
+    assert float(sp[iz, x1].real) - float(sm[iz, x1].real) == 2 * h
+    op = O.onestep_fwd(np.ones(nx, complex), iz, w, sp, O.refsampler(sp, 1), p)
+    om = O.onestep_fwd(np.ones(nx, complex), iz, w, sm, O.refsampler(sm, 1), p)
+    dlog = (op[x1] - om[x1]) / (2 * h) / out[x1]
+    e = np.zeros(nx, complex)
+    e[x1] = 1.0
+    sc0 = O.scatter_fwd(e, iz, w, s, p)[x1]
+    assert abs(dlog - sc0) <= 1e-6 * abs(sc0), (dlog, sc0)
+
+
+def test_born_transmission_is_twice_the_derivative_of_wem():
+    """The reference's linearisation test (propagator/dat/linTest.py:52-76)
+    in a setting where its Born operator is a derivative: PSPI, nref = 1,
+    eps = 0, homogeneous S0 and a perturbation dS constant in x on depth z0.
+    The perturbed WEM then phase-shifts depth z0 with the exact reference
+    slowness S0 + dS, so its derivative is -i dz w^2/(2 kz) dS per k; Born's
+    scatterer (ntaylor = 4) is -i w dz (S0 - k^2/w^2)^(-1/2) = -i dz w^2/kz,
+    twice that, to the Taylor remainder (x = k^2/(w^2 S0) <= 0.13 here).
+    Receivers on the bottom row see only the transmitted field: the
+    reflection terms vanish there (r = 0 in the background; dReflect and
+    the perturbed Reflect leave the last row zero, dReflect.h:37,65), so
+    Born(dS) = WEM(S0 + dS) - WEM(S0 - dS) up to O(dS^3).  A wrong scale
+    of the scatterer or of the phase shift, or a Born term on the wrong
+    depth, fails."""
+    nx, nz, z0 = 32, 8, 3
+    s0, ds = 2.0 ** -22, 2.0 ** -32
+    rng = np.random.default_rng(29)
+    nt = 16
+    sv = O.Survey(nx=nx, nz=nz, dx=60.0, dz=10.0, freq=np.array([50.0, 65.0], np.float32),
+                  index=np.array([3, 4]), wave_f=crand(nt, rng=rng), src_iz=[0], src_ix=[nx // 2],
+                  rec_iz=[nz - 1] * nx, rec_ix=list(range(nx)), nt_f=nt, tap=0, pad=0, nref=1, eps=0.0,
+                  prop="pspi", ntaylor=4, ngs=2, ngr=0, ng=0)
+    slow = np.full((nz, nx), s0, dtype=np.complex64)
+    den = np.ones((nz, nx), dtype=np.complex64)
+    dS = np.zeros((nz, nx), dtype=np.complex64)
+    dS[z0] = ds
+    born = O.born_forward(sv, slow, den, dS, np.zeros_like(dS))[:, :, sv.index]
+    sp, sm = slow.copy(), slow.copy()
+    sp[z0] += ds
+    sm[z0] -= ds
+    assert np.all(sp[z0].real - sm[z0].real == 2 * ds)
+    fd = (O.wem_forward(sv, sp, den) - O.wem_forward(sv, sm, den))[:, :, sv.index]
+    err = np.linalg.norm(born - fd) / np.linalg.norm(fd)
+    assert err < 2e-6, err
