@@ -39,11 +39,97 @@ __device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
   else return cmul(a, w);
 }
 
+// a * w (forward) or a * conj(w) (inverse) with w = hi + lo carried as a
+// float pair per part (one rounding at the end of each chain)
+template <bool INV>
+__device__ __forceinline__ float2 cmul_tw_pair(float2 a, float2 hi, float2 lo) {
+  const float wi = INV ? -hi.y : hi.y, li = INV ? -lo.y : lo.y;
+  return make_float2(fmaf(a.x, hi.x, fmaf(-a.y, wi, fmaf(a.x, lo.x, -a.y * li))),
+                     fmaf(a.x, wi, fmaf(a.y, hi.x, fmaf(a.x, li, a.y * lo.x))));
+}
+
 // forward: multiply by -i ; inverse: multiply by +i
 template <bool INV>
 __device__ __forceinline__ float2 mul_mi(float2 a) {
   if constexpr (INV) return make_float2(-a.y, a.x);
   else return make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- rounding
+//
+// Tables and constants applied again at every depth step: the wavefield
+// passes through each FFT twiddle and butterfly constant hundreds of times,
+// so an error of the constant's modulus compounds coherently instead of
+// averaging out.  Rounding cos and sin separately to the nearest float
+// leaves a modulus error of up to 2^-24 whose mean over a plan's table is not
+// zero (all three constants of the radix-16 butterfly, cos(pi/8), sin(pi/8)
+// and sqrt(1/2), round down: about -5e-8 per depth step in rows of 1024).
+
+// the float below (dir < 0) or above (dir > 0) the finite nonzero f
+__host__ __device__ inline float float_step(float f, int dir) {
+  unsigned u = __builtin_bit_cast(unsigned, f);
+  const bool up = (dir > 0) == (f > 0.f);  // away from zero
+  u = up ? u + 1u : u - 1u;
+  return __builtin_bit_cast(float, u);
+}
+
+// the two floats that bracket x (equal when x is a float)
+__host__ __device__ inline void float_bracket(double x, float& lo, float& hi) {
+  const float f = (float)x;
+  if ((double)f == x || f == 0.f) {
+    lo = hi = f;
+  } else if ((double)f > x) {
+    hi = f;
+    lo = float_step(f, -1);
+  } else {
+    lo = f;
+    hi = float_step(f, +1);
+  }
+}
+
+// The complex value z = re + i im rounded to the one of the (up to) four
+// bracketing float pairs whose ratio to z is nearest 1 in |log modulus| +
+// |phase|, to first order |Re d| + |Im d| with d = zf / z - 1; ties go to
+// the per-part nearest.  Modulus and phase errors then stay below 2^-25
+// relative and of either sign, so the per-step gain of a chain of such
+// factors averages out over a table.
+__host__ __device__ inline float2 round_complex(double re, double im) {
+  float rl, rh, il, ih;
+  float_bracket(re, rl, rh);
+  float_bracket(im, il, ih);
+  float2 best = make_float2((float)re, (float)im);
+  // (zf - z) conj(z) = |z|^2 d: the common factor |z|^2 does not change the
+  // order of the candidates
+  auto cost = [&](float a, float b) {
+    const double dr = (double)a - re, di = (double)b - im;
+    const double pr = dr * re + di * im, pi = di * re - dr * im;
+    return fabs(pr) + fabs(pi);
+  };
+  double bc = cost(best.x, best.y);
+  const float rs[2] = {rl, rh}, is[2] = {il, ih};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const double c = cost(rs[a], is[b]);
+      if (c < bc) {
+        bc = c;
+        best = make_float2(rs[a], is[b]);
+      }
+    }
+  return best;
+}
+
+// A real constant carried as an unevaluated float pair hi + lo (hi the
+// nearest float, lo the nearest float of the remainder): x * (hi + lo) =
+// fma(x, hi, x * lo) rounds once and carries the constant to ~2^-48.
+struct Konst {
+  float hi, lo;
+};
+__host__ __device__ constexpr Konst konst(double c) { return Konst{(float)c, (float)(c - (double)(float)c)}; }
+__device__ __forceinline__ float kmul(float x, Konst k) { return fmaf(x, k.hi, x * k.lo); }
+// x * a + y * b with both constants as pairs, one rounding at the end of
+// each chain
+__device__ __forceinline__ float kdot(float x, Konst a, float y, Konst b) {
+  return fmaf(x, a.hi, fmaf(y, b.hi, fmaf(x, a.lo, y * b.lo)));
 }
 
 // padded LDS index (one pad slot every 16 complex)
@@ -73,11 +159,18 @@ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2&
   a3 = csub(t1, t3);
 }
 
-constexpr float kSqrtHalf = 0.70710678118654752440f;
-constexpr float kCos8 = 0.92387953251128675613f;   // cos(pi/8)
-constexpr float kSin8 = 0.38268343236508977173f;   // sin(pi/8)
+// cos(pi m / 8) and sin(pi m / 8), m = 0..15
+constexpr double kCos16[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                               0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                               -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                               0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+constexpr double kSin16[16] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                               1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                               0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                               -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
 
-// multiply by W16^m = exp(-2*pi*i*m/16) (forward) or its conjugate (inverse)
+// multiply by W16^m = exp(-2*pi*i*m/16) (forward) or its conjugate
+// (inverse); the constants are float pairs (Konst)
 template <bool INV, int M>
 __device__ __forceinline__ float2 tw16(float2 a) {
   constexpr int m = M & 15;
@@ -85,15 +178,17 @@ __device__ __forceinline__ float2 tw16(float2 a) {
   else if constexpr (m == 4) return mul_mi<INV>(a);
   else if constexpr (m == 8) return make_float2(-a.x, -a.y);
   else if constexpr (m == 12) return mul_mi<!INV>(a);
-  else {
-    constexpr float c[16] = {1.f, kCos8, kSqrtHalf, kSin8, 0.f, -kSin8, -kSqrtHalf, -kCos8,
-                             -1.f, -kCos8, -kSqrtHalf, -kSin8, 0.f, kSin8, kSqrtHalf, kCos8};
-    constexpr float s[16] = {0.f, kSin8, kSqrtHalf, kCos8, 1.f, kCos8, kSqrtHalf, kSin8,
-                             0.f, -kSin8, -kSqrtHalf, -kCos8, -1.f, -kCos8, -kSqrtHalf, -kSin8};
-    // forward: exp(-i*theta) = (c, -s); inverse: (c, s)
-    constexpr float wr = c[m];
-    constexpr float wi = INV ? s[m] : -s[m];
-    return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+  else if constexpr ((m & 3) == 2) {
+    // |cos| = |sin| = sqrt(1/2): w = sqrt(1/2) (sc + i si), sc, si = +-1
+    constexpr Konst h = konst(0.70710678118654752440);
+    constexpr float sc = kCos16[m] > 0 ? 1.f : -1.f;
+    constexpr float si = (kSin16[m] > 0) == INV ? 1.f : -1.f;  // forward: exp(-i theta)
+    return make_float2(kmul(sc * a.x - si * a.y, h), kmul(si * a.x + sc * a.y, h));
+  } else {
+    constexpr Konst wr = konst(kCos16[m]);
+    constexpr Konst wi = konst(INV ? kSin16[m] : -kSin16[m]);
+    constexpr Konst nwi = Konst{-wi.hi, -wi.lo};
+    return make_float2(kdot(a.x, wr, a.y, nwi), kdot(a.x, wi, a.y, wr));
   }
 }
 
@@ -217,11 +312,22 @@ __host__ __device__ constexpr int plan_ns(int p) {
   for (int q = 0; q < p; ++q) ns *= FftPlan<N, K>::R[q];
   return ns;
 }
+// Pass p's twiddles are multiples of 2 pi / 16 (NS R <= 16): the few
+// values cos, sin (pi m / 8) whose float roundings all shrink the modulus
+// (by up to 2^-24) and that carry half of the row or more, so their bias
+// adds up over the depth steps.  Such passes hold each twiddle as a float
+// pair, hi then lo (Konst), and multiply with the pair; other passes use
+// round_complex values.
+template <int N, int K = 0>
+__host__ __device__ constexpr bool plan_tw_pair(int p) {
+  return plan_ns<N, K>(p) > 1 && plan_ns<N, K>(p) * FftPlan<N, K>::R[p] <= 16;
+}
 // offset of pass p's twiddles in the concatenated table
 template <int N, int K = 0>
 __host__ __device__ constexpr int plan_tw_off(int p) {
   int off = 0;
-  for (int q = 0; q < p; ++q) off += (FftPlan<N, K>::R[q] - 1) * plan_ns<N, K>(q);
+  for (int q = 0; q < p; ++q)
+    off += (FftPlan<N, K>::R[q] - 1) * plan_ns<N, K>(q) * (plan_tw_pair<N, K>(q) ? 2 : 1);
   return off;
 }
 template <int N, int K = 0>
@@ -259,8 +365,10 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
       for (int r = 0; r < R; ++r) x[b + r * BPL] = base[68 * b + r * lidx(NB)];
     wave_lds_order();
   }
-  // twiddle base: entry (r-1)*NS + (j & (NS-1)), j = t + 64 b
-  const float2* twb = tw + plan_tw_off<N, K>(p) + (NS >= 64 ? t : (t & (NS - 1)));
+  // twiddle base: entry (r-1)*NS + (j & (NS-1)), j = t + 64 b (pairs: two
+  // float2 per entry)
+  constexpr bool PAIR = plan_tw_pair<N, K>(p);
+  const float2* twb = tw + plan_tw_off<N, K>(p) + (NS >= 64 ? t : (t & (NS - 1))) * (PAIR ? 2 : 1);
   // output base of butterfly j = t + 64 b
   int obase;
   if constexpr (NS == 1) obase = R * t + ((R * t) >> 4);
@@ -279,7 +387,10 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
     if constexpr (NS > 1) {
       constexpr int boff = NS >= 64 ? 64 : 0;  // (j & (NS-1)) advances by 64 per b when NS > 64
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[r] = cmul_tw<INV>(v[r], twb[(r - 1) * NS + (NS > 64 ? b * boff : 0)]);
+      for (int r = 1; r < R; ++r) {
+        if constexpr (PAIR) v[r] = cmul_tw_pair<INV>(v[r], twb[2 * (r - 1) * NS], twb[2 * (r - 1) * NS + 1]);
+        else v[r] = cmul_tw<INV>(v[r], twb[(r - 1) * NS + (NS > 64 ? b * boff : 0)]);
+      }
     }
     dft<R, INV>(v);
     if constexpr (TO_LDS) {
@@ -362,20 +473,27 @@ constexpr int kGenMaxRadix = 61;
 // radices transformed as a direct DFT (their own R-entry table follows the
 // pass twiddles); 2, 3, 4, 5, 8 and 16 are register butterflies
 __host__ __device__ constexpr bool gen_direct(int R) { return R > 5 && R != 8 && R != 16; }
-__host__ __device__ constexpr int gen_tw_cap(int N) { return N + 3 * kGenMaxRadix; }
+// register-butterfly passes of a runtime plan whose twiddles are few
+// distinct values (NS R <= 64) hold them as float pairs, as plan_tw_pair
+// passes do (the LDS-bound runtime passes take the extra multiplies for
+// free); their pair entries add at most 63 to the table
+__host__ __device__ constexpr bool gen_tw_pair(int R, int NS) { return NS > 1 && NS * R <= 64 && !gen_direct(R); }
+__host__ __device__ constexpr int gen_tw_cap(int N) { return N + 3 * kGenMaxRadix + 64; }
 
-constexpr float kSin3 = 0.86602540378443864676f;   // sin(2 pi / 3)
-constexpr float kCos5a = 0.30901699437494742410f;  // cos(2 pi / 5)
-constexpr float kCos5b = -0.80901699437494742410f; // cos(4 pi / 5)
-constexpr float kSin5a = 0.95105651629515357212f;  // sin(2 pi / 5)
-constexpr float kSin5b = 0.58778525229247312917f;  // sin(4 pi / 5)
+constexpr Konst kSin3 = konst(0.86602540378443864676);   // sin(2 pi / 3)
+constexpr Konst kCos5a = konst(0.30901699437494742410);  // cos(2 pi / 5)
+constexpr Konst kCos5b = konst(-0.80901699437494742410); // cos(4 pi / 5)
+constexpr Konst kSin5a = konst(0.95105651629515357212);  // sin(2 pi / 5)
+constexpr Konst kSin5b = konst(0.58778525229247312917);  // sin(4 pi / 5)
+constexpr Konst kSin5aN = konst(-0.95105651629515357212);
 
 template <bool INV>
 __device__ __forceinline__ void dft3(float2 (&v)[3]) {
   const float2 t1 = cadd(v[1], v[2]);
   const float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
   const float2 d = csub(v[1], v[2]);
-  const float2 t3 = mul_mi<INV>(cscale(d, kSin3));  // -i sin(2pi/3) (v1 - v2), conjugated for INV
+  // -i sin(2pi/3) (v1 - v2), conjugated for INV
+  const float2 t3 = mul_mi<INV>(make_float2(kmul(d.x, kSin3), kmul(d.y, kSin3)));
   v[0] = cadd(v[0], t1);
   v[1] = cadd(t2, t3);
   v[2] = csub(t2, t3);
@@ -386,11 +504,11 @@ __device__ __forceinline__ void dft5(float2 (&v)[5]) {
   const float2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
   const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
   const float2 a = v[0];
-  const float2 p1 = make_float2(a.x + kCos5a * b1.x + kCos5b * b2.x, a.y + kCos5a * b1.y + kCos5b * b2.y);
-  const float2 p2 = make_float2(a.x + kCos5b * b1.x + kCos5a * b2.x, a.y + kCos5b * b1.y + kCos5a * b2.y);
+  const float2 p1 = make_float2(a.x + kdot(b1.x, kCos5a, b2.x, kCos5b), a.y + kdot(b1.y, kCos5a, b2.y, kCos5b));
+  const float2 p2 = make_float2(a.x + kdot(b1.x, kCos5b, b2.x, kCos5a), a.y + kdot(b1.y, kCos5b, b2.y, kCos5a));
   // -i (s1 d1 + s2 d2) and -i (s2 d1 - s1 d2) (conjugated for INV)
-  const float2 q1 = mul_mi<INV>(make_float2(kSin5a * d1.x + kSin5b * d2.x, kSin5a * d1.y + kSin5b * d2.y));
-  const float2 q2 = mul_mi<INV>(make_float2(kSin5b * d1.x - kSin5a * d2.x, kSin5b * d1.y - kSin5a * d2.y));
+  const float2 q1 = mul_mi<INV>(make_float2(kdot(d1.x, kSin5a, d2.x, kSin5b), kdot(d1.y, kSin5a, d2.y, kSin5b)));
+  const float2 q2 = mul_mi<INV>(make_float2(kdot(d1.x, kSin5b, d2.x, kSin5aN), kdot(d1.y, kSin5b, d2.y, kSin5aN)));
   v[0] = make_float2(a.x + b1.x + b2.x, a.y + b1.y + b2.y);
   v[1] = cadd(p1, q1);
   v[4] = csub(p1, q1);
@@ -414,7 +532,13 @@ __device__ __forceinline__ void gen_bfly(const float2* __restrict__ A, float2* _
   float2 v[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = A[lidx(j + r * NB)];
-  if (NS > 1) {
+  if (gen_tw_pair(R, NS)) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      const int e = 2 * ((r - 1) * NS + m);
+      v[r] = cmul_tw_pair<INV>(v[r], tw[e], tw[e + 1]);
+    }
+  } else if (NS > 1) {
 #pragma unroll
     for (int r = 1; r < R; ++r) v[r] = cmul_tw<INV>(v[r], tw[(r - 1) * NS + m]);
   }
@@ -508,7 +632,7 @@ __device__ __forceinline__ void fft_row_gen(float2 (&x)[N / 64], float2* buf, co
       default: gen_pass_dft<INV>(A, B, twp, twp + (R - 1) * NS, n, R, NS, t); break;
     }
     wave_lds_order();
-    off += (R - 1) * NS + (gen_direct(R) ? R : 0);
+    off += (R - 1) * NS * (gen_tw_pair(R, NS) ? 2 : 1) + (gen_direct(R) ? R : 0);
     NS *= R;
     float2* T = A;
     A = B;
@@ -520,7 +644,7 @@ __device__ __forceinline__ void fft_row_gen(float2 (&x)[N / 64], float2* buf, co
 }
 
 // Fills the concatenated per-pass twiddle table of FftPlan<N, K> (double
-// precision, rounded once).
+// precision, rounded once by round_complex).
 template <int N, int K = 0>
 __global__ void twiddle_table_kernel(float2* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -529,13 +653,19 @@ __global__ void twiddle_table_kernel(float2* out) {
   while (p + 1 < FftPlan<N, K>::P && plan_tw_off<N, K>(p + 1) <= i) ++p;
   const int R = FftPlan<N, K>::R[p];
   const int NS = plan_ns<N, K>(p);
-  const int k = i - plan_tw_off<N, K>(p);
+  const bool pair = plan_tw_pair<N, K>(p);
+  const int k = (i - plan_tw_off<N, K>(p)) >> (pair ? 1 : 0);
   const int r = k / NS + 1;
   const int jm = k % NS;
   const double a = -2.0 * 3.14159265358979323846 * (double)(jm * r) / (double)(NS * R);
   double s, c;
   sincos(a, &s, &c);
-  out[i] = make_float2((float)c, (float)s);
+  if (!pair) {
+    out[i] = round_complex(c, s);
+  } else {
+    const float2 hi = make_float2((float)c, (float)s);
+    out[i] = ((i - plan_tw_off<N, K>(p)) & 1) ? make_float2((float)(c - (double)hi.x), (float)(s - (double)hi.y)) : hi;
+  }
 }
 
 }  // namespace wemfwi

@@ -277,7 +277,8 @@ bool fft_gen_plan(int n, int maxr, FftGen* f) {
 
 // Twiddle table of a runtime plan (layout of fft_row_gen), padded with
 // zeros to the gen_tw_cap(N) entries a launch stages in LDS; double
-// precision rounded once.  Cached per (n, N, device).
+// precision rounded once (round_complex), or split into float pairs in
+// gen_tw_pair passes.  Cached per (n, N, device).
 static std::map<long long, float2*>& gen_twiddle_cache() {
   static std::map<long long, float2*> c;
   return c;
@@ -293,27 +294,42 @@ int gen_twiddles(const FftGen& f, int N, const float2** out) {
     *out = it->second;
     return WEMFWI_OK;
   }
+  size_t total = 0;
+  for (int p = 0, NS = 1; p < f.np; NS *= gen_radix(f, p), ++p) {
+    const int R = gen_radix(f, p);
+    total += (size_t)(R - 1) * NS * (gen_tw_pair(R, NS) ? 2 : 1) + (gen_direct(R) ? R : 0);
+  }
+  if (total > (size_t)gen_tw_cap(N)) return fail(WEMFWI_ERR_UNSUPPORTED, "FFT plan: twiddle table too large");
   std::vector<float2> h((size_t)gen_tw_cap(N), make_float2(0.f, 0.f));
   size_t off = 0;
   int NS = 1;
   for (int p = 0; p < f.np; ++p) {
     const int R = gen_radix(f, p);
+    const bool pair = gen_tw_pair(R, NS);
     for (int r = 1; r < R; ++r)
       for (int m = 0; m < NS; ++m) {
         const double a = -2.0 * M_PI * (double)((long long)m * r) / (double)((long long)NS * R);
-        h[off + (size_t)(r - 1) * NS + m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        const double c = std::cos(a), s = std::sin(a);
+        const size_t e = (size_t)(r - 1) * NS + m;
+        if (pair) {
+          // hi then lo of each part (cmul_tw_pair)
+          const float2 hi = make_float2((float)c, (float)s);
+          h[off + 2 * e] = hi;
+          h[off + 2 * e + 1] = make_float2((float)(c - (double)hi.x), (float)(s - (double)hi.y));
+        } else {
+          h[off + e] = round_complex(c, s);
+        }
       }
-    off += (size_t)(R - 1) * NS;
+    off += (size_t)(R - 1) * NS * (pair ? 2 : 1);
     if (gen_direct(R)) {
       for (int q = 0; q < R; ++q) {
         const double a = -2.0 * M_PI * (double)q / (double)R;
-        h[off + q] = make_float2((float)std::cos(a), (float)std::sin(a));
+        h[off + q] = round_complex(std::cos(a), std::sin(a));
       }
       off += R;
     }
     NS *= R;
   }
-  if (off > h.size()) return fail(WEMFWI_ERR_UNSUPPORTED, "FFT plan: twiddle table too large");
   float2* d = nullptr;
   int rc = dev_alloc(&d, h.size(), "twiddle table");
   if (rc) return rc;

@@ -24,6 +24,8 @@ namespace wemfwi {
 // Only k = 0..n/2 is used: the grid satisfies k[n-i] = -k[i], so the
 // multiplier of index k is entry min(k, n-k); n is the FFT length, NH the
 // entries per (frequency, slot) (N/2 + 1 of the row's N lanes, >= n/2 + 1).
+// Rounded by round_complex: the table of a depth repeats wherever the
+// reference slowness does.
 __global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __restrict__ wlist,
                                    int nw, const double2* __restrict__ sref, int nslots, int N,
                                    int NH, double dk, double dz, double eps) {
@@ -47,7 +49,7 @@ __global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __re
     const double amp = exp(im * dz) / (double)N;
     double sn, cs;
     sincos(re * dz, &sn, &cs);
-    ptab[i] = make_float2((float)(amp * cs), (float)(-amp * sn));
+    ptab[i] = round_complex(amp * cs, -amp * sn);
   }
 }
 
