@@ -160,6 +160,25 @@ int wemfwi_model_set_freqs(wemfwi_model* model, int nfreq, const float* freq_hz,
 int wemfwi_oneway(wemfwi_model* model, int up, int adjoint, int nrows, const float* freq_hz,
                   const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
 
+/* wemfwi_oneway without device memory allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as wemfwi_oneway, with these conditions: the
+ * model's phase tables must already hold every freq_hz[i]
+ * (wemfwi_model_set_freqs, or an earlier wemfwi_oneway call), else
+ * WEMFWI_ERR_ARG; the sweep runs the FFT plan the autotuner already chose
+ * for this variant (plan 0 if it has not tuned it: no tuning launch); nrows
+ * must not exceed wemfwi_oneway_noalloc_max_rows(model) (the 64 MiB device
+ * scratch array holds N complex scratch values and one row descriptor per
+ * row), else WEMFWI_ERR_ARG.  Every call on a GPU shares that GPU's scratch
+ * array: calls that overlap on one GPU (other host threads, or streams not
+ * ordered with each other) race. */
+int wemfwi_oneway_noalloc(wemfwi_model* model, int up, int adjoint, int nrows, const float* freq_hz,
+                          const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
+/* Largest nrows wemfwi_oneway_noalloc accepts for this model (0 for a null
+ * model). */
+long long wemfwi_oneway_noalloc_max_rows(const wemfwi_model* model);
+
 /* Recursive Down / Up with separate source and output buffers:
  * out = (I - L)^-1 src, i.e. the in-place recursion of wemfwi_oneway applied
  * to a copy of src, without the copy.  No single reference call corresponds;

@@ -342,6 +342,16 @@ int gen_twiddles(const FftGen& f, int N, const float2** out) {
 
 int twiddles(int n, const float2** out) { return twiddles(n, 0, out); }
 
+int twiddles_cached(int n, int plan, const float2** out) {
+  auto& c = twiddle_cache();
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  auto it = c.find((n * 8 + plan) * 64 + dev);
+  if (it == c.end()) return WEMFWI_ERR_ARG;
+  *out = it->second;
+  return WEMFWI_OK;
+}
+
 int twiddles(int n, int plan, const float2** out) {
   auto& c = twiddle_cache();
   int dev = 0;
@@ -597,12 +607,16 @@ int Model::set_freqs(int n, const float* freq_hz, hipStream_t s) {
 
 // Maps rows to the current frequency set, rebuilding it when a row's
 // frequency is not in it.
+bool Model::has_freqs(int n, const float* freq_hz) const {
+  bool have = !freqs.empty();
+  for (int i = 0; i < n && have; ++i) have = std::binary_search(freqs.begin(), freqs.end(), freq_hz[i]);
+  return have;
+}
+
 int Model::phase_tables(int nrows, const float* freq_hz, std::vector<float>& wrow,
                         std::vector<const float2*>& prow, hipStream_t s) {
   if (int rc = require_fft()) return rc;
-  bool have = !freqs.empty();
-  for (int i = 0; i < nrows && have; ++i) have = std::binary_search(freqs.begin(), freqs.end(), freq_hz[i]);
-  if (!have) {
+  if (!has_freqs(nrows, freq_hz)) {
     int rc = set_freqs(nrows, freq_hz, s);
     if (rc) return rc;
   }
@@ -620,6 +634,19 @@ int Model::phase_tables(int nrows, const float* freq_hz, std::vector<float>& wro
 int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf* in, cf* out, int add,
                   int recursive, hipStream_t s) {
   if (nrows < 0 || !freq_hz || !in || !out) return fail(WEMFWI_ERR_ARG, "oneway: null argument");
+  if (nrows == 0 || (in == out && !add)) return oneway_on(up, adjoint, nrows, freq_hz, in, out, add, recursive,
+                                                          nullptr, nullptr, false, s);
+  int rc = require_fft();
+  if (rc) return rc;
+  if ((rc = scratch.reserve((size_t)nrows * N * sizeof(float2), "scratch rows"))) return rc;
+  if ((rc = rows.reserve((size_t)nrows * sizeof(SweepRow), "row descriptors"))) return rc;
+  return oneway_on(up, adjoint, nrows, freq_hz, in, out, add, recursive, (float2*)scratch.ptr,
+                   (SweepRow*)rows.ptr, false, s);
+}
+
+int Model::oneway_on(int up, int adjoint, int nrows, const float* freq_hz, const cf* in, cf* out, int add,
+                     int recursive, float2* scr, SweepRow* drows, bool fixed, hipStream_t s) {
+  if (nrows < 0 || !freq_hz || !in || !out) return fail(WEMFWI_ERR_ARG, "oneway: null argument");
   if (nrows == 0) return WEMFWI_OK;
   const size_t nzx = (size_t)d.nz * d.nx;
   hipError_t e;
@@ -630,6 +657,9 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
     e = hipStreamSynchronize(s);
     return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "oneway");
   }
+  if (int rc = require_fft()) return rc;
+  if (fixed && !has_freqs(nrows, freq_hz))
+    return fail(WEMFWI_ERR_ARG, "oneway: a frequency has no phase table (prepare them with set_freqs first)");
   std::vector<float> wrow;
   std::vector<const float2*> prow;
   int rc = phase_tables(nrows, freq_hz, wrow, prow, s);
@@ -652,17 +682,15 @@ int Model::oneway(int up, int adjoint, int nrows, const float* freq_hz, const cf
     h[i].omul = nullptr;
     h[i].src_skip = 0;
   }
-  if ((rc = scratch.reserve((size_t)nrows * N * sizeof(float2), "scratch rows"))) return rc;
-  for (int i = 0; i < nrows; ++i) h[i].scratch = (float2*)scratch.ptr + (size_t)i * N;
-  if ((rc = rows.reserve(h.size() * sizeof(SweepRow), "row descriptors"))) return rc;
-  e = hipMemcpyAsync(rows.ptr, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice, s);
+  for (int i = 0; i < nrows; ++i) h[i].scratch = scr + (size_t)i * N;
+  e = hipMemcpyAsync(drows, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "row upload");
   const bool down = !up;
   const int step = (down != (adjoint != 0)) ? 1 : -1;
   {
     KernelTimer kt("oneway_kernel");
     kt.begin(s);
-    e = launch_oneway(N, (const SweepRow*)rows.ptr, nrows, d_tables, geom(), d.nz, step, adjoint != 0, d_tw, s);
+    e = launch_oneway(N, drows, nrows, d_tables, geom(), d.nz, step, adjoint != 0, d_tw, s, !fixed);
     kt.end(s);
     if (e != hipSuccess) return hip_fail(e, "oneway kernel");
   }

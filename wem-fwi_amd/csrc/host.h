@@ -109,6 +109,15 @@ struct Model {
                    hipStream_t s);
   int oneway(int up, int adjoint, int nrows, const float* freq_hz, const std::complex<float>* in,
              std::complex<float>* out, int add, int recursive, hipStream_t s);
+  // oneway on caller-provided device memory: scratch rows [nrows][N] (scr)
+  // and nrows row descriptors (drows).  fixed: allocate nothing -- the phase
+  // tables of every frequency must exist (set_freqs) and the sweep runs the
+  // FFT plan already chosen (no autotuning launch)
+  int oneway_on(int up, int adjoint, int nrows, const float* freq_hz, const std::complex<float>* in,
+                std::complex<float>* out, int add, int recursive, float2* scr, SweepRow* drows, bool fixed,
+                hipStream_t s);
+  // frequencies of freq_hz[0..n) missing from the current phase table
+  bool has_freqs(int n, const float* freq_hz) const;
   int onestep(int adjoint, int nrows, const float* freq_hz, const int* depth, const std::complex<float>* in,
               std::complex<float>* out, int add, hipStream_t s);
 };

@@ -50,14 +50,20 @@ int twiddle_table_size(int N, int plan);
 hipError_t launch_twiddles(int N, int plan, float2* out, hipStream_t s);
 // twiddle table of plan `plan` of length N on the current device (cached)
 int twiddles(int n, int plan, const float2** out);
+// the table of plan `plan` of length n if it was built before (no allocation)
+int twiddles_cached(int n, int plan, const float2** out);
 // JSON list of the sweep autotuner's decisions
 std::string autotune_report();
 
 hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const double2* sref,
                               int nslots, int N, int NH, double dk, double dz, double eps,
                               hipStream_t stream);
+// tune = false: no autotuning launch and no twiddle-table allocation; the
+// plan is the forced or recorded one when its table already exists, else
+// plan 0 with tw
 hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
-                         StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s);
+                         StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s,
+                         bool tune = true);
 hipError_t launch_onestep(int N, const StepRow* rows, int nrows, const DepthTables* models,
                           StepGeom g, bool adj, const float2* tw, hipStream_t s);
 

@@ -338,7 +338,7 @@ static int env_int(const char* name, int dflt) {
 
 template <int N>
 static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz, int step,
-                      bool adj, hipStream_t s) {
+                      bool adj, hipStream_t s, bool tune) {
   const int np = FftPlans<N>::value;
   const int forced = env_int("WEMFWI_FFT_PLAN", -1);
   if (forced >= 0) return forced < np ? forced : 0;
@@ -348,6 +348,7 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
   std::lock_guard<std::mutex> lock(tune_mutex());
   for (const TuneEntry& e : tune_table())
     if (e.N == N && e.full == g.full && e.ssf == g.ssf && e.adj == (int)adj && e.dev == dev) return e.plan;
+  if (!tune) return 0;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
   // a sample of rows like the first one, on scratch buffers
@@ -431,13 +432,18 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
 
 template <int N>
 static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
-                                  int step, bool adj, const float2* tw, hipStream_t s) {
+                                  int step, bool adj, const float2* tw, hipStream_t s, bool tune) {
   if (g.fg.n != N) {
     if (g.ssf) return launch_oneway_v<Var<N, false, true, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
     return launch_oneway_v<Var<N, false, false, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
   }
-  const int k = tuned_plan<N>(rows, nrows, models, g, nz, step, adj, s);
-  if (k != 0 && twiddles(N, k, &tw) != WEMFWI_OK) return hipErrorInvalidValue;
+  int k = tuned_plan<N>(rows, nrows, models, g, nz, step, adj, s, tune);
+  if (k != 0) {
+    const float2* twk = nullptr;
+    if ((tune ? twiddles(N, k, &twk) : twiddles_cached(N, k, &twk)) == WEMFWI_OK) tw = twk;
+    else if (tune) return hipErrorInvalidValue;
+    else k = 0;  // plan 0's table is the model's own (tw)
+  }
   return launch_oneway_plan<N>(k, rows, nrows, models, g, nz, step, adj, tw, s);
 }
 
@@ -464,14 +470,14 @@ std::string autotune_report() {
 }
 
 hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
-                         StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s) {
+                         StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s, bool tune) {
   if (nrows <= 0 || nz < 2) return hipSuccess;
   switch (N) {
-    case 128: return launch_oneway_n<128>(rows, nrows, models, g, nz, step, adj, tw, s);
-    case 256: return launch_oneway_n<256>(rows, nrows, models, g, nz, step, adj, tw, s);
-    case 512: return launch_oneway_n<512>(rows, nrows, models, g, nz, step, adj, tw, s);
-    case 1024: return launch_oneway_n<1024>(rows, nrows, models, g, nz, step, adj, tw, s);
-    case 2048: return launch_oneway_n<2048>(rows, nrows, models, g, nz, step, adj, tw, s);
+    case 128: return launch_oneway_n<128>(rows, nrows, models, g, nz, step, adj, tw, s, tune);
+    case 256: return launch_oneway_n<256>(rows, nrows, models, g, nz, step, adj, tw, s, tune);
+    case 512: return launch_oneway_n<512>(rows, nrows, models, g, nz, step, adj, tw, s, tune);
+    case 1024: return launch_oneway_n<1024>(rows, nrows, models, g, nz, step, adj, tw, s, tune);
+    case 2048: return launch_oneway_n<2048>(rows, nrows, models, g, nz, step, adj, tw, s, tune);
     default: return hipErrorInvalidValue;
   }
 }

@@ -64,6 +64,8 @@ SIGNATURES = {
     "wemfwi_oneway": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _i, _vp]),
     "wemfwi_onestep": (_i, [_vp, _i, _i, _fp, _ip, _vp, _vp, _i, _vp]),
     "wemfwi_oneway_recursive": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _vp]),
+    "wemfwi_oneway_noalloc": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _i, _vp]),
+    "wemfwi_oneway_noalloc_max_rows": (C.c_longlong, [_vp]),
     "wemfwi_oneway_batch": (_i, [_i, _i, _i, _vp, _fp, _vp, _vp, _vp]),
     "wemfwi_batch_create": (_i, [_i, _vp, _fp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "wemfwi_batch_run": (_i, [_vp, _i, _i, _vp]),
