@@ -378,7 +378,7 @@ int twiddles(int n, int plan, const float2** out) {
 
 Model::~Model() {
   void* ptrs[] = {d_slot_begin, d_slot_ref, d_rid_lane, d_one_ref, d_phi_lane, d_taper_lane, d_slot_sref, d_tables,
-                  d_scoef, d_refl, d_drefl};
+                  d_scoef, d_refl, d_drefl, d_phi_max};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ptab.release();
@@ -540,6 +540,12 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
   t.nx = d.nx;
   t.tap = d.tap;
   t.taper = d.taper > 0 ? 1 : 0;
+  t.phi_real = 1;
+  for (size_t i = 0; i < phi.size() && t.phi_real; ++i) t.phi_real = phi[i].y == 0.f;
+  std::vector<float> phi_max(d.nz, 0.f);
+  for (size_t i = 0; i < nzx; ++i) phi_max[i / d.nx] = std::max(phi_max[i / d.nx], std::fabs(phi[i].x));
+  if ((rc = dev_upload(&d_phi_max, phi_max.data(), phi_max.size(), "phi bound"))) return rc;
+  t.phi_max = d_phi_max;
   if ((rc = dev_upload(&d_tables, &t, 1, "depth tables"))) return rc;
   if (fft_ok) {
     if (nk == N) rc = twiddles(N, &d_tw);

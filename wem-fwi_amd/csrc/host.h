@@ -77,6 +77,7 @@ struct Model {
   float2* d_phi_lane = nullptr;         // [nz][N] lane layout
   float* d_taper_lane = nullptr;        // [N] lane layout
   double2* d_slot_sref = nullptr;
+  float* d_phi_max = nullptr;  // [nz] largest |Re phi| of each depth
   float2* d_scoef = nullptr;   // [ntaylor+1][nz][nx] scatterer coefficients
   float2* d_refl = nullptr;    // [nz][nx] reflection coefficient (Reflect.h:19-26)
   float2* d_drefl = nullptr;   // [4][nz][nx] a1, b1, a2, b2 of dReflect.h:18-27,40-47

@@ -129,6 +129,8 @@ struct DepthTables {
   int nx;                       // propagation width of this model
   int tap;                      // FFT zero padding per side (N = nx + 2 tap)
   int taper;                    // 1: a taper is applied
+  int phi_real;                 // 1: every split-step exponent is real (no amplitude term)
+  const float* phi_max;         // [nz] largest |Re phi| of the depth's samples
 };
 
 struct StepGeom {
@@ -198,6 +200,21 @@ __device__ __forceinline__ void sincos_cw(float x, float* s, float* c) {
   *c = ((iq + 1) & 2) ? -cc : cc;
 }
 
+// sincos_cw for |x| <= 0.78: the reduction is the identity there (q = 0),
+// so the result is bitwise that of sincos_cw
+__device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
+  const float r2 = x * x;
+  float ps = fmaf(r2, 2.7557319224e-6f, -1.9841269841e-4f);
+  ps = fmaf(r2, ps, 8.3333333333e-3f);
+  ps = fmaf(r2, ps, -1.6666666667e-1f);
+  *s = fmaf(x * r2, ps, x);
+  float pc = fmaf(r2, -2.7557319224e-7f, 2.4801587302e-5f);
+  pc = fmaf(r2, pc, -1.3888888889e-3f);
+  pc = fmaf(r2, pc, 4.1666666667e-2f);
+  pc = fmaf(r2, pc, -0.5f);
+  *c = fmaf(r2, pc, 1.0f);
+}
+
 __device__ __forceinline__ bool in_row(int x, int nx) { return (unsigned)x < (unsigned)nx; }
 
 // Sample x = t + 64 e - tap of a row of nx values starting at p, 0 for
@@ -236,14 +253,38 @@ __device__ __forceinline__ void ss_taper_row(float2 (&f)[V::E], const DepthTable
   }
   if constexpr (V::SSF) {
     const brsrc r = make_rsrc(dt.phi_lane + (size_t)d * V::n, (unsigned)V::n * 8u);
+    // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr)).  Two
+    // shortcuts, both bitwise the general form and uniform over the wave:
+    // w max_x |pr| <= 0.78 at this depth (no argument reduction: smooth
+    // models at seismic frequencies), and a real exponent without taper
+    // (amplitude 1)
+    const bool small = w * __builtin_bit_cast(float, ldu((const int*)dt.phi_max, d)) <= 0.78f;
+    if (!g.taper && dt.phi_real) {
+      if (small) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float2 pv = bld2(r, (unsigned)(t * E + e) * 8u, 0u);
-      // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr))
-      float sn, cs;
-      sincos_cw(w * pv.x, &sn, &cs);
-      const float a = tv[e] * exp2f(w * pv.y * 1.4426950408889634f);
-      f[e] = make_float2(a * cs, -a * sn);
+        for (int e = 0; e < E; ++e) {
+          float sn, cs;
+          sincos_small(w * bld2(r, (unsigned)(t * E + e) * 8u, 0u).x, &sn, &cs);
+          f[e] = make_float2(cs, -sn);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          float sn, cs;
+          sincos_cw(w * bld2(r, (unsigned)(t * E + e) * 8u, 0u).x, &sn, &cs);
+          f[e] = make_float2(cs, -sn);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float2 pv = bld2(r, (unsigned)(t * E + e) * 8u, 0u);
+        float sn, cs;
+        if (small) sincos_small(w * pv.x, &sn, &cs);
+        else sincos_cw(w * pv.x, &sn, &cs);
+        const float a = tv[e] * exp2f(w * pv.y * 1.4426950408889634f);
+        f[e] = make_float2(a * cs, -a * sn);
+      }
     }
   } else {
 #pragma unroll
@@ -542,10 +583,16 @@ __device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, floa
     const int r = ldu(dt.slot_ref, s);
     const float2* P = ptab + (size_t)s * (N / 2 + 1);
     float2 Z[E];
+    // several references: each reads back only the parked samples it
+    // selects (each sample read once over the references)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const float2 v = multi ? bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e) : st[e];
-      Z[e] = (all || rid_at(rid, e) == r) ? v : make_float2(0.f, 0.f);
+      if (multi) {
+        Z[e] = make_float2(0.f, 0.f);
+        if (rid_at(rid, e) == r) Z[e] = bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e);
+      } else {
+        Z[e] = (all || rid_at(rid, e) == r) ? st[e] : make_float2(0.f, 0.f);
+      }
     }
     fft_v<V, false>(Z, buf, tw, t, g);
     const bool first = s == s0;
