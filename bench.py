@@ -16,13 +16,17 @@ decides there) and give the per-launch kernel times of the roofline.  The
 metric is wavefield updates per second (x.z.w, summed over the shots), the
 reference's unit of work.
 
-Also reported in the same line: forward and adjoint sweep rates on their own,
-and the FWI gradient (WEM forward + residual + exact Born adjoint, 512 x 300,
-128 frequencies, 32 shots) with all rows batched against the same library
-called once per (shot, frequency) row (a timed sample of rows, scaled), with
-the relative difference of the two gradients on the sampled rows; the batched
-gradient again with wavefield rows kept every 25 depth steps (recomputed
-segment by segment), which must equal the store-all gradient bitwise.
+Also reported in the same line: forward and adjoint sweep rates on their own;
+the same grid, rows and model with SSF and one reference (two FFTs and the
+split-step correction at every depth step) as a sweep line against the HBM
+roofline; and the FWI gradient (WEM forward + residual + exact Born adjoint,
+512 x 300, 128 frequencies, 32 shots) with all rows batched against the same
+library called once per frequency with all shots of it (a timed sample of
+frequencies, scaled), with the relative difference of the two gradients on
+the sampled frequencies; the batched gradient again with wavefield rows kept
+every 25 depth steps (recomputed segment by segment), which must equal the
+store-all gradient bitwise.  The CPU leg also checks 2 x 2 rows of that
+gradient and its misfit against the FP64 oracle.
 
 Also the uneven batch: 64 shots with their own apertures (256-2048) and
 bands (32-192 frequencies), nz = 400, Down forward of all rows in one batched
@@ -80,6 +84,10 @@ CONFIGS = {
     # PSPI downward continuation (the headline configuration)
     "pspi1024": dict(nx=1024, nz=500, nw=256, fmin=2.0, fmax=30.0, ns=16, nref=4, prop="pspi",
                      seed=2, model="layers"),
+    # the PSPI workload's grid, rows and model with one reference per depth
+    # and the split-step correction (SSF): two FFTs per depth step, every step
+    "ssf1024": dict(nx=1024, nz=500, nw=256, fmin=2.0, fmax=30.0, ns=16, nref=1, prop="ssf",
+                    seed=2, model="layers"),
     # split-step Fourier, single shot (the CPU-runnable configuration)
     "ssf256": dict(nx=256, nz=200, nw=64, fmin=2.0, fmax=20.0, ns=1, nref=1, prop="ssf",
                    seed=1, model="gradient"),
@@ -152,6 +160,59 @@ def cpu_baseline(cfg, slow, freqs, src_ix, nrows_sample):
             "single_thread_value": 2.0 * nx * (nz - 1) * n1 / dt1 / 1e9,
             "single_thread_sample": f"{n1} rows, {dt1:.2f} s",
             "host": host_cpu()}
+
+
+def fwi_oracle_check():
+    """The CPU leg's check of the FWI gradient workload against the FP64
+    oracle (the oracle is the checker here, never the thing timed): misfit
+    and exact Born-adjoint gradient of 2 shots x 2 frequencies (shots 0 and
+    31, 2 and 15 Hz) of the 512 x 300 workload, batched on the GPU, store-all
+    and checkpointed every 25 depth steps, against wem_forward /
+    born_adjoint(mode="exact") of the oracle on the same rows."""
+    import torch
+    import wemfwi
+    from oracle import wem_oracle as O
+    from wemfwi import ops
+    c = FWI
+    nx, nz, nw, ns = c["nx"], c["nz"], c["nw"], c["ns"]
+    freqs = (c["fmin"] + np.arange(nw) * (c["fmax"] - c["fmin"]) / (nw - 1)).astype(np.float32)
+    wav = ricker_spectrum(freqs)
+    src_ix = [int(round(8 + s * (nx - 16) / (ns - 1))) for s in range(ns)]
+    shots, fidx = [0, ns - 1], [0, nw - 1]
+    s_true, s_start = fwi_models()
+    den = np.ones((nz, nx), np.complex64)
+    sv = O.Survey(nx=nx, nz=nz, dx=10.0, dz=10.0, freq=freqs[fidx], index=np.arange(len(fidx)),
+                  wave_f=wav[fidx], src_iz=[0] * len(shots), src_ix=[src_ix[s] for s in shots],
+                  rec_iz=[0] * nx, rec_ix=list(range(nx)), nt_f=len(fidx), tap=0, pad=0, nref=1, eps=0.04,
+                  prop="ssf", ntaylor=1, ngs=0, ngr=0, ng=0)
+    g = ops.Survey(sv.src_iz, sv.src_ix, sv.rec_iz, sv.rec_ix, sv.freq, sv.index, sv.wave_f, sv.nt_f)
+    m_true = wemfwi.Model(s_true, 10.0, 10.0, den=den)
+    m0 = wemfwi.Model(s_start, 10.0, 10.0, den=den)
+    obs = torch.zeros(g.data_shape, dtype=torch.complex64, device="cuda")
+    ops.wem_forward(m_true, g, obs)
+    res = torch.zeros_like(obs)
+    ops.wem_forward(m0, g, res)
+    res.sub_(obs)
+    J = 0.5 * float((res.to(torch.complex128).abs() ** 2).sum())
+    t0 = time.perf_counter()
+    res_o = O.wem_forward(sv, s_start, den) - O.wem_forward(sv, s_true, den)
+    J_o = 0.5 * float(np.sum(np.abs(res_o) ** 2))
+    gS_o, _ = O.born_adjoint(sv, s_start, den, res_o, mode="exact")
+    t_oracle = time.perf_counter() - t0
+    out = {"rows": "shots 0, 31 x 2, 15 Hz of the 512 x 300 workload", "misfit_rel_err": abs(J - J_o) / J_o,
+           "oracle_s": round(t_oracle, 1)}
+    for k in (0, CKPT):
+        g.set_checkpoint(k)
+        gS = torch.zeros((nz, nx), dtype=torch.complex64, device="cuda")
+        gR = torch.zeros_like(gS)
+        ops.born_adjoint(m0, g, gS, gR, res, exact=True)
+        a = gS.cpu().numpy().astype(np.complex128)
+        out["gradient_rel_err" + ("" if k == 0 else f"_checkpoint_{k}")] = float(
+            np.linalg.norm(a - gS_o) / np.linalg.norm(gS_o))
+    g.close()
+    m_true.close()
+    m0.close()
+    return out
 
 
 def host_cpu() -> str:
@@ -239,10 +300,11 @@ def fd_check(sv, obs, s0, den, gS, ndir, seed=44, rel_step=1e-3):
                     "slownesses' dependence on S"}
 
 
-def fwi_gradient_bench(sample_rows: int, fd_dirs: int = 20):
+def fwi_gradient_bench(sample_freqs: int, fd_dirs: int = 20):
     """Wall time of one FWI gradient (WEM forward of the start model, residual,
     exact Born adjoint) with all (shot, frequency) rows batched, against the
-    same library called once per (shot, frequency) row on a sample of rows."""
+    same library called once per frequency (all shots of it in one call) on a
+    sample of frequencies."""
     import torch
     import wemfwi
     from wemfwi import ops
@@ -292,23 +354,25 @@ def fwi_gradient_bench(sample_rows: int, fd_dirs: int = 20):
     t_ckpt = time.perf_counter() - t0
     ckpt_equal = bool(torch.equal(gS, g_all[0]) and torch.equal(gR, g_all[1]))
     sv.set_checkpoint(0)
-    # plain loop over a sample of rows (one survey per row, built beforehand)
+    # the plain per-frequency loop: one gradient call per frequency with all
+    # shots of that frequency (one survey per frequency, built beforehand),
+    # timed on a sample of frequencies and scaled
     total = ns * nw
-    stride = max(1, total // sample_rows)
-    rows = [(s, i) for s in range(ns) for i in range(nw)][::stride][:sample_rows]
-    row_sv = [survey([s], [i]) for s, i in rows]
-    row_obs = [obs[s:s + 1, :, i:i + 1].contiguous() for s, i in rows]
-    d1 = torch.zeros((1, nx, 1), dtype=torch.complex64, device=dev)
+    stride = max(1, nw // max(1, sample_freqs))
+    fsample = list(range(nw))[::stride][:sample_freqs]
+    f_sv = [survey(list(range(ns)), [i]) for i in fsample]
+    f_obs = [obs[:, :, i:i + 1].contiguous() for i in fsample]
+    d1 = torch.zeros((ns, nx, 1), dtype=torch.complex64, device=dev)
     pS = torch.zeros_like(gS)
     pR = torch.zeros_like(gS)
 
     def plain():
         pS.zero_()
         pR.zero_()
-        for k in range(len(rows)):
-            ops.wem_forward(m0, row_sv[k], d1)
-            d1.sub_(row_obs[k])
-            ops.born_adjoint(m0, row_sv[k], pS, pR, d1, add=True, exact=True)
+        for k in range(len(fsample)):
+            ops.wem_forward(m0, f_sv[k], d1)
+            d1.sub_(f_obs[k])
+            ops.born_adjoint(m0, f_sv[k], pS, pR, d1, add=True, exact=True)
 
     plain()  # warmup
     torch.cuda.synchronize()
@@ -316,34 +380,40 @@ def fwi_gradient_bench(sample_rows: int, fd_dirs: int = 20):
     plain()
     torch.cuda.synchronize()
     t_plain_sample = time.perf_counter() - t0
-    # the batched gradient restricted to the same rows, for the accuracy check
+    # the batched gradient restricted to the same frequencies, for the check
     bS = torch.zeros_like(gS)
     bR = torch.zeros_like(gS)
-    for s in sorted(set(s for s, _ in rows)):
-        idx = [i for (ss, i) in rows if ss == s]
-        svs = survey([s], idx)
-        d = torch.zeros(svs.data_shape, dtype=torch.complex64, device=dev)
-        ops.wem_forward(m0, svs, d)
-        d.sub_(obs[s:s + 1, :, idx].contiguous())
-        ops.born_adjoint(m0, svs, bS, bR, d, add=True, exact=True)
+    svs = survey(list(range(ns)), fsample)
+    d = torch.zeros(svs.data_shape, dtype=torch.complex64, device=dev)
+    ops.wem_forward(m0, svs, d)
+    d.sub_(obs[:, :, fsample].contiguous())
+    ops.born_adjoint(m0, svs, bS, bR, d, exact=True)
     torch.cuda.synchronize()
     rel = float(torch.linalg.vector_norm((pS - bS).to(torch.complex128)) /
                 torch.linalg.vector_norm(bS.to(torch.complex128)))
-    t_plain = t_plain_sample * total / len(rows)
-    upd = 12.0 * nx * (nz - 1) * total
+    t_plain = t_plain_sample * nw / len(fsample)
+    # sweeps per (shot, frequency) row: executed here (WEM: Down, Up; Born
+    # adjoint: Down, Up^H, Down^H, LinDown^H, Up, LinUp^H) and in the
+    # reference's orchestration (wem_prop.cpp:52-74, born_prop.cpp:121-232)
+    sweeps_exec, sweeps_ref = 8, 12
+    upd_row = nx * (nz - 1) * total
     fd = fd_check(sv, obs, s_start, den, gS, fd_dirs) if fd_dirs > 0 else None
     return {"config": f"ssf nx={nx} nz={nz} nw={nw} {c['fmin']:g}-{c['fmax']:g} Hz ns={ns} nr={nx}",
             "batched_ms": round(t_batched * 1e3, 2),
-            "batched_gupdates_per_s": round(upd / t_batched / 1e9, 2),
+            "sweeps_executed": sweeps_exec,
+            "sweeps_reference": sweeps_ref,
+            "executed_gupdates_per_s": round(sweeps_exec * upd_row / t_batched / 1e9, 2),
+            "reference_equivalent_gupdates_per_s": round(sweeps_ref * upd_row / t_batched / 1e9, 2),
             "checkpoint_every": CKPT,
             "checkpointed_ms": round(t_ckpt * 1e3, 2),
             "checkpointed_bitwise_equal": ckpt_equal,
             "adjoint_row_bytes": {"store_all": 4 * nz * nx * 8,
                                   "checkpointed": (4 * (CKPT + 2) + 2 * (-(-(nz - 1) // CKPT)) + 2) * nx * 8},
-            "plain_loop_ms": round(t_plain * 1e3, 1),
-            "plain_loop_sample": f"{len(rows)} of {total} rows timed ({t_plain_sample:.2f} s), scaled to all rows",
-            "speedup": round(t_plain / t_batched, 2),
-            "rel_err_plain_vs_batched": rel,
+            "per_frequency_loop_ms": round(t_plain * 1e3, 1),
+            "per_frequency_loop_sample": f"{len(fsample)} of {nw} frequencies timed, one call per frequency "
+                                         f"with all {ns} shots ({t_plain_sample:.2f} s), scaled to all",
+            "speedup_vs_per_frequency_loop": round(t_plain / t_batched, 2),
+            "rel_err_per_frequency_vs_batched": rel,
             "fd_check": fd}
 
 
@@ -611,6 +681,55 @@ def extended_gradient_bench(shots, one, grouped, model_img, nh, nrows, nsample, 
             "misfit_rel_err_row_loop_vs_batched_on_sample": abs(Jl - Jb) / Jb}
 
 
+def sweep_line(config: str, reps: int):
+    """Down forward (sources -> wavefield) and Down adjoint sweeps of a
+    configuration's rows on their own: the sweep kernel's mean launch time
+    (HIP events inside the library) against the HBM roofline."""
+    import torch
+    import wemfwi
+    cfg = CONFIGS[config]
+    nx, nz, nw, ns = cfg["nx"], cfg["nz"], cfg["nw"], cfg["ns"]
+    freqs = (cfg["fmin"] + np.arange(nw) * (cfg["fmax"] - cfg["fmin"]) / (nw - 1)).astype(np.float32)
+    src_ix = [32 + 64 * s for s in range(ns)]
+    slow = (1.0 / velocity(cfg) ** 2).astype(np.complex64)
+    model = wemfwi.Model(slow, 10.0, 10.0, nref=cfg["nref"], prop=cfg["prop"], eps=0.04)
+    nrows = ns * nw
+    f_rows = np.tile(freqs, ns)
+    dev = torch.device("cuda:0")
+    S = torch.zeros((nrows, nz, nx), dtype=torch.complex64, device=dev)
+    spec = torch.from_numpy(np.tile(ricker_spectrum(freqs), ns)).to(dev)
+    ix = torch.tensor(np.repeat(src_ix, nw), device=dev)
+    S[torch.arange(nrows, device=dev), 0, ix] = spec
+    W = torch.empty_like(S)
+
+    def step():
+        model.oneway_recursive(False, False, f_rows, S, W)
+        model.oneway(False, True, f_rows, W)
+
+    step()  # warm-up (the autotuner decides here)
+    torch.cuda.synchronize()
+    wemfwi.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wemfwi.timing(False)
+    k_ms, k_n = wemfwi.kernel_time("oneway_kernel")
+    avg = k_ms / max(k_n, 1)
+    upd = nx * (nz - 1) * nrows
+    achieved = BYTES_PER_UPDATE * upd / (avg / 1e3) / 1e9
+    finite = bool(torch.isfinite(W.abs()).all())
+    model.close()
+    del S, W
+    return {"config": f"{cfg['prop']} nref={cfg['nref']} nx={nx} nz={nz} nw={nw} ns={ns} ({nrows} rows)",
+            "kernel": "oneway_kernel", "kernel_ms": round(avg, 3),
+            "gupdates_per_s": round(upd / (avg / 1e3) / 1e9, 2),
+            "wall_gupdates_per_s": round(2 * upd * reps / (t1 - t0) / 1e9, 2),
+            "achieved_gbs": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": BYTES_PER_UPDATE * upd, "finite": finite}
+
+
 def measured_traffic(config: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this configuration (mean over its launches, each
@@ -641,8 +760,8 @@ def main():
     ap.add_argument("--config", default="pspi1024", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="rows of the CPU baseline sample (0: skip)")
-    ap.add_argument("--fwi-sample", type=int, default=64,
-                    help="rows of the plain per-row loop in the FWI gradient comparison (0: skip)")
+    ap.add_argument("--fwi-sample", type=int, default=16,
+                    help="frequencies of the per-frequency loop in the FWI gradient comparison (0: skip)")
     ap.add_argument("--fd-directions", type=int, default=20,
                     help="central-difference directions of the FWI gradient check (0: skip)")
     ap.add_argument("--uneven-sample", type=int, default=32,
@@ -785,6 +904,8 @@ def main():
     line["autotune"] = [e for e in wemfwi.autotune_report() if e["N"] == nx]
     line["fwd_gupdates_per_s"] = round(upd_sweep / t_fwd / 1e9, 3)
     line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
+    if args.config == "pspi1024":
+        line["ssf_sweep"] = sweep_line("ssf1024", reps)
     if args.fwi_sample > 0:
         line["fwi_gradient"] = fwi_gradient_bench(args.fwi_sample, args.fd_directions)
     if args.uneven_sample > 0:
@@ -792,6 +913,8 @@ def main():
         line["uneven_offset_image"] = uneven_image_bench(args.uneven_sample)
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)
+        if args.fwi_sample > 0:
+            line["fwi_gradient"]["fp64_oracle_check"] = fwi_oracle_check()
     print(json.dumps(line), flush=True)
 
 
