@@ -25,7 +25,7 @@ NZ = 501
 TOL = 1e-5
 
 # nx (= FFT length), number of compiled plans (1: runtime plan or one plan)
-CASES = [(128, 1), (256, 1), (512, 1), (1024, 4), (2048, 3), (500, 1), (600, 1), (1000, 1)]
+CASES = [(128, 1), (256, 1), (512, 1), (1024, 3), (2048, 2), (500, 1), (600, 1), (1000, 1)]
 
 
 @pytest.fixture

@@ -132,13 +132,10 @@ __device__ __forceinline__ float kdot(float x, Konst a, float y, Konst b) {
   return fmaf(x, a.hi, fmaf(y, b.hi, fmaf(x, a.lo, y * b.lo)));
 }
 
-// padded LDS index (one pad slot every 16 complex)
+// padded LDS index (one pad slot every 16 complex): the layout of the
+// runtime-planned passes and of the power-of-two passes unless lay_m says
+// otherwise
 __host__ __device__ constexpr int lidx(int i) { return i + (i >> 4); }
-
-template <int N>
-struct LdsSize {
-  static constexpr int value = N + N / 16;
-};
 
 // ---------------------------------------------------------------- small DFTs
 
@@ -281,11 +278,6 @@ struct FftPlan<1024, 2> {
   static constexpr int R[P] = {4, 16, 16};
 };
 template <>
-struct FftPlan<1024, 3> {
-  static constexpr int P = 3;
-  static constexpr int R[P] = {8, 8, 16};
-};
-template <>
 struct FftPlan<2048, 0> {
   static constexpr int P = 3;
   static constexpr int R[P] = {16, 8, 16};
@@ -293,17 +285,14 @@ struct FftPlan<2048, 0> {
 template <>
 struct FftPlan<2048, 1> {
   static constexpr int P = 3;
-  static constexpr int R[P] = {8, 16, 16};
-};
-template <>
-struct FftPlan<2048, 2> {
-  static constexpr int P = 3;
   static constexpr int R[P] = {16, 16, 8};
 };
-// number of plans of a length
+// number of plans of a length (dropped after measurement: 1024 as
+// {8, 8, 16}, whose NS = 8 pass needs more LDS than a 1024 row has for a
+// conflict-free layout, and 2048 as {8, 16, 16}, which spills registers)
 template <int N>
 struct FftPlans {
-  static constexpr int value = N == 1024 ? 4 : (N == 2048 ? 3 : 1);
+  static constexpr int value = N == 1024 ? 3 : (N == 2048 ? 2 : 1);
 };
 
 template <int N, int K = 0>
@@ -335,6 +324,69 @@ __host__ __device__ constexpr int plan_tw_size() {
   return plan_tw_off<N, K>(FftPlan<N, K>::P);
 }
 
+// LDS layout between pass p and pass p + 1: element i at
+// i + (i >> lay_a) * lay_m.  LDS serves a 64-bit access 16 lanes at a time
+// over 32 banks, so 16 consecutive lanes must hit 16 distinct bank pairs.
+// The next pass reads 16 consecutive elements per 16 lanes (free of
+// conflicts under any such layout).  A pass with NS < 16 writes, per 16
+// lanes, 16/NS runs of NS consecutive elements NS R apart: the runs get
+// the offsets 0, NS, 2 NS, ... modulo 16 from m = NS with a = log2(NS R)
+// (a = 4 when NS R < 16).  Other passes (NS = 1: one run per lane R apart;
+// NS >= 16: a run per 16 lanes) are conflict-free with one pad per 16.
+template <int N, int K = 0>
+__host__ __device__ constexpr int lay_m(int p) {
+  const int ns = plan_ns<N, K>(p);
+  return ns > 1 && ns < 16 ? ns : 1;
+}
+template <int N, int K = 0>
+__host__ __device__ constexpr int lay_a(int p) {
+  const int ns = plan_ns<N, K>(p), w = ns * FftPlan<N, K>::R[p];
+  if (!(ns > 1 && ns < 16) || w % 16 != 0) return 4;
+  int a = 0;
+  while ((1 << a) < w) ++a;
+  return a;
+}
+template <int N, int K, int p>
+__host__ __device__ constexpr int lay(int i) {
+  return i + (i >> lay_a<N, K>(p)) * lay_m<N, K>(p);
+}
+// LDS complexes per row: the largest layout over the plans of a length
+template <int N, int K = 0>
+__host__ __device__ constexpr int plan_lds_size() {
+  int m = N + N / 16;
+  for (int p = 0; p + 1 < FftPlan<N, K>::P; ++p) {
+    const int v = N - 1 + ((N - 1) >> lay_a<N, K>(p)) * lay_m<N, K>(p) + 1;
+    m = v > m ? v : m;
+  }
+  return m;
+}
+template <int N, int K = 0>
+__host__ __device__ constexpr int lds_size_k() {
+  if constexpr (K + 1 < FftPlans<N>::value) {
+    const int a = plan_lds_size<N, K>(), b = lds_size_k<N, K + 1>();
+    return a > b ? a : b;
+  } else {
+    return plan_lds_size<N, K>();
+  }
+}
+template <int N>
+struct LdsSize {
+  static constexpr int value = lds_size_k<N>();
+};
+// reads of pass p (layout of boundary p - 1) split as lane base + constant:
+// lay(t + c) == lay(t) + lay(c) for every lane t < 64 and offset c of the pass
+template <int N, int K, int p>
+__host__ __device__ constexpr bool lay_reads_split() {
+  constexpr int R = FftPlan<N, K>::R[p], NB = N / R, BPL = N / 64 / R;
+  for (int b = 0; b < BPL; ++b)
+    for (int r = 0; r < R; ++r)
+      for (int t = 0; t < 64; ++t) {
+        const int c = 64 * b + r * NB;
+        if (lay<N, K, p - 1>(t + c) != lay<N, K, p - 1>(t) + lay<N, K, p - 1>(c)) return false;
+      }
+  return true;
+}
+
 // ---------------------------------------------------------------- passes
 
 __device__ __forceinline__ void wave_lds_order() {
@@ -357,12 +409,14 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
   static_assert(BPL * R == E, "radix must divide the per-lane count");
   static_assert(NS <= 64 || 64 * (BPL - 1) < NS, "twiddle index wraps");
   if constexpr (FROM_LDS) {
-    // element t + 64 b + r NB: lidx = (t + t/16) + 68 b + r (NB + NB/16)
-    const float2* base = buf + (t + (t >> 4));
+    // element t + 64 b + r NB in the layout of the previous boundary: a lane
+    // base plus compile-time offsets
+    static_assert(lay_reads_split<N, K, p>(), "LDS layout does not split into lane base + offset");
+    const float2* base = buf + lay<N, K, p - 1>(t);
 #pragma unroll
     for (int b = 0; b < BPL; ++b)
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[b + r * BPL] = base[68 * b + r * lidx(NB)];
+      for (int r = 0; r < R; ++r) x[b + r * BPL] = base[lay<N, K, p - 1>(64 * b + r * NB)];
     wave_lds_order();
   }
   // twiddle base: entry (r-1)*NS + (j & (NS-1)), j = t + 64 b (pairs: two
@@ -395,6 +449,8 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
     dft<R, INV>(v);
     if constexpr (TO_LDS) {
       // output index (j/NS)*NS*R + j%NS + r NS, j = t + 64 b
+      static_assert((NS > 1 && NS < 16) || (lay_a<N, K>(p) == 4 && lay_m<N, K>(p) == 1),
+                    "these write paths address the one-pad-per-16 layout");
       if constexpr (NS == 1) {
         // j R + r = R t + 64 R b + r  ->  lidx = obase + 68 R b + r
 #pragma unroll
@@ -416,13 +472,10 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
           buf[obase + c + (c >> 4)] = v[r];
         }
       } else {
-        // NS < 16: full padded index per output
+        // NS < 16: the boundary's own layout (lay_m, lay_a), per output
         const int u = (t / NS) * NS * R + (t & (NS - 1));
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int i = u + (64 * b / NS) * NS * R + r * NS;
-          buf[i + (i >> 4)] = v[r];
-        }
+        for (int r = 0; r < R; ++r) buf[lay<N, K, p>(u + (64 * b / NS) * NS * R + r * NS)] = v[r];
       }
     } else {
 #pragma unroll
