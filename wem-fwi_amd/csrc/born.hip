@@ -16,16 +16,13 @@ __device__ __forceinline__ void lrow(float2 (&v)[V::E], const float2* __restrict
   for (int e = 0; e < V::E; ++e) v[e] = row_sample<V>(p, g, t, e);
 }
 
+// p[x] += v for the model samples of the row (through the row's buffer
+// descriptor: padding samples read 0 and their stores are dropped)
 template <class V>
 __device__ __forceinline__ void acc_row(float2* __restrict__ p, const float2 (&v)[V::E], const StepGeom& g,
                                         int t) {
-  float2* b = p + (t - g.tap);
 #pragma unroll
-  for (int e = 0; e < V::E; ++e)
-    if (in_row(t + 64 * e - g.tap, g.nx)) {
-      const float2 o = ldg(b, (unsigned)(64 * e));
-      stg(b, (unsigned)(64 * e), cadd(o, v[e]));
-    }
+  for (int e = 0; e < V::E; ++e) row_store<V>(p, cadd(row_sample<V>(p, g, t, e), v[e]), g, t, e);
 }
 
 // LinDown / LinUp (LinOneWay.cpp:33-142) as independent (row, depth) items.
@@ -112,10 +109,8 @@ __global__ void __launch_bounds__(64 * kLinWaves)
   if (R.add) {
     acc_row<V>(R.out, st, g, t);
   } else {
-    float2* b = R.out + (t - g.tap);
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      if (in_row(t + 64 * e - g.tap, g.nx)) stg(b, (unsigned)(64 * e), st[e]);
+    for (int e = 0; e < E; ++e) row_store<V>(R.out, st[e], g, t, e);
   }
 }
 

@@ -128,14 +128,18 @@ __global__ void __launch_bounds__(kTX)
       const float2* d = D + S.roff + r0 * rstride + (long long)z * nx;
       const float2* q = R + S.roff + r0 * rstride + (long long)z * nx;
       for (int i = t; i < seg; i += kTX) {
-        const int xx = a0 - hh + i;
-        const bool in = xx >= 0 && xx < nx;
+        // column xx of depth z of row r0 + k, through that row's buffer
+        // descriptor (nx samples): columns outside [0, nx) read 0
+        const unsigned off = (unsigned)(a0 - hh + i) * 8u;
         float2 vd[kRows], vr[kRows];
 #pragma unroll
         for (int k = 0; k < kRows; ++k) {
-          const bool ok = in && k < kr;
-          vd[k] = ok ? ldg(d + k * rstride, (size_t)xx) : make_float2(0.f, 0.f);
-          vr[k] = ok ? ldg(q + k * rstride, (size_t)xx) : make_float2(0.f, 0.f);
+          // rows past the shot's last (k >= kr) are never addressed: their
+          // loads read row kr - 1 and are replaced by 0
+          const int kk = k < kr ? k : kr - 1;
+          vd[k] = bld2(make_rsrc(d + kk * rstride, (unsigned)nx * 8u), off, 0u);
+          vr[k] = bld2(make_rsrc(q + kk * rstride, (unsigned)nx * 8u), off, 0u);
+          if (k >= kr) vd[k] = vr[k] = make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int k = 0; k < kRows; ++k) {
@@ -216,9 +220,11 @@ __global__ void __launch_bounds__(kTX)
   const int X = order[blockIdx.x / nz] * kTX;
   const int t = threadIdx.x;
   for (int i = t; i < nh * wi; i += kTX) {
+    // image column X - hh + c of lag j through the image row's buffer
+    // descriptor: columns outside [0, nxi) read 0
     const int j = i / wi, c = i - j * wi;
-    const int xc = X - hh + c;
-    si[i] = (xc >= 0 && xc < nxi) ? img[j * hstride + (long long)z * nxi + xc] : make_float2(0.f, 0.f);
+    si[i] = bld2(make_rsrc(img + j * hstride + (long long)z * nxi, (unsigned)nxi * 8u), (unsigned)(X - hh + c) * 8u,
+                 0u);
   }
   for (int sh = 0; sh < nshots; ++sh) {
     const UShot S = shots[sh];
@@ -232,11 +238,16 @@ __global__ void __launch_bounds__(kTX)
       const float2* d = D + S.roff + r0 * rstride + (long long)z * nx;
       __syncthreads();
       for (int i = t; i < wd; i += kTX) {
-        const int xx = a0 - 2 * hh + i;
-        const bool in = xx >= 0 && xx < nx;
+        // column xx of depth z of row r0 + k through the row's buffer
+        // descriptor: columns outside [0, nx) read 0
+        const unsigned off = (unsigned)(a0 - 2 * hh + i) * 8u;
         float2 v[kRows];
 #pragma unroll
-        for (int k = 0; k < kRows; ++k) v[k] = (in && k < kr) ? ldg(d + k * rstride, (size_t)xx) : make_float2(0.f, 0.f);
+        for (int k = 0; k < kRows; ++k) {
+          const int kk = k < kr ? k : kr - 1;  // rows past the shot's last: row kr - 1, value 0
+          v[k] = bld2(make_rsrc(d + kk * rstride, (unsigned)nx * 8u), off, 0u);
+          if (k >= kr) v[k] = make_float2(0.f, 0.f);
+        }
 #pragma unroll
         for (int k = 0; k < kRows; ++k) sd[k * WD + i] = v[k];
       }
