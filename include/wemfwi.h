@@ -295,6 +295,26 @@ int wemfwi_born_adjoint(wemfwi_model* model, wemfwi_survey* survey, wemfwi_compl
                         wemfwi_complex* dden, const wemfwi_complex* data, int add, int mode,
                         void* stream);
 
+/* The L2 data-misfit gradient of one model as a replayable graph
+ * (scripts/fwi.py:70-131 with WEM.py:52-81: J = 1/2 ||WEM(m) - d||^2 and its
+ * gradient Born(m)^H (WEM(m) - d), which the reference computes with one
+ * WEM::forward and one BornFull::adjoint call).  create runs the whole
+ * sequence once (res = WEM(m) - obs, the FP64 misfit, gS / gR =
+ * wemfwi_born_adjoint(res) with the given mode, honouring the survey's
+ * checkpoint setting) and captures it as a hipGraph; launch replays it and
+ * returns J.  obs, res [ns][nr][nt] and gS, gR [nz][nx] are device buffers
+ * bound at creation; the plan holds every device buffer of the call until
+ * destroyed; a new model needs a new plan.  Results are bitwise those of the
+ * eager calls. */
+typedef struct wemfwi_gradient wemfwi_gradient;
+int wemfwi_gradient_create(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* obs,
+                           wemfwi_complex* res, wemfwi_complex* gS, wemfwi_complex* gR, int mode, void* stream,
+                           wemfwi_gradient** plan);
+int wemfwi_gradient_launch(wemfwi_gradient* plan, void* stream, double* misfit);
+/* J of the last run (creation or launch) */
+int wemfwi_gradient_misfit(wemfwi_gradient* plan, double* misfit);
+int wemfwi_gradient_destroy(wemfwi_gradient* plan);
+
 /* extWEM / extBornFull (ext_wem_prop.cpp:8-59, ext_born_prop.cpp:8-194): one
  * background model per frequency (models[nw]); perturbations
  * [nw][nz][nx].  extborn_adjoint mode 0 reproduces the reference's serial

@@ -64,6 +64,13 @@ constexpr int kRunRows = 64;
 hipError_t launch_reduce_rows(double2* out, const float2* in, long long stride, long long n, int nrows, int phase,
                               bool acc, hipStream_t s, const float2* in2 = nullptr);
 hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s);
+// zero rows x width 8-byte words at base + row * pitch (a kernel, capturable)
+hipError_t launch_zero2d(void* base, long long pitch_words, long long width_words, long long rows, hipStream_t s);
+// res -= obs over n samples and *half_sum = 0.5 sum |res|^2 in FP64, in a
+// fixed order (kMisfitParts partial sums)
+constexpr int kMisfitParts = 256;
+hipError_t launch_residual_sumsq(float2* res, const float2* obs, long long n, double* part, double* half_sum,
+                                 hipStream_t s);
 // Phshift rows (w, sref per row; device arrays)
 hipError_t launch_phshift(float2* out, const float2* in, int nk, double dk, double dz, double eps, const double* w,
                           const double2* sref, int nrows, bool adjoint, bool add, hipStream_t s);

@@ -2,6 +2,8 @@
 // injection/recording and the deterministic frequency/shot reduction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "born.h"
 #include "sweep.hpp"
 
@@ -248,6 +250,51 @@ __global__ void gather_mul_kernel(float2* __restrict__ out, const float2* __rest
     const long long x = (long long)r * nx + loc[j];
     out[x] = cmul(in[x], mult[i]);
   }
+}
+
+// rows x width 8-byte words at base + row * pitch set to zero (a kernel,
+// not a memset: the clears of an operator call may be captured in a graph)
+__global__ void zero2d_kernel(unsigned long long* __restrict__ base, long long pitch, long long width,
+                              long long rows) {
+  const long long n = width * rows;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long r = i / width;
+    base[r * pitch + (i - r * width)] = 0ull;
+  }
+}
+
+// res[i] -= obs[i] and the FP64 partial sums of |res[i]|^2: part[b] = the
+// sum over the elements i = b * blockDim + t + k * gridDim * blockDim of
+// block b, in a fixed order (a strided loop per thread, then a tree over
+// the block)
+constexpr int kSumThreads = 256;
+__global__ void __launch_bounds__(kSumThreads)
+    residual_sumsq_kernel(float2* __restrict__ res, const float2* __restrict__ obs, long long n,
+                          double* __restrict__ part) {
+  __shared__ double red[kSumThreads];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 r = csub(res[i], obs[i]);
+    res[i] = r;
+    acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kSumThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// out[0] = 0.5 * sum of the nparts partial sums, in order
+__global__ void half_sum_kernel(double* __restrict__ out, const double* __restrict__ part, int nparts) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < nparts; ++b) s += part[b];
+  out[0] = 0.5 * s;
 }
 
 // out += in
@@ -575,6 +622,22 @@ hipError_t launch_gather_mul(float2* out, const float2* in, const float2* mult, 
   const long long n = (long long)nrows * nloc;
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(gather_mul_kernel, grid1d(n), dim3(256), 0, s, out, in, mult, loc, nloc, nx, nrows);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero2d(void* base, long long pitch_words, long long width_words, long long rows, hipStream_t s) {
+  const long long n = width_words * rows;
+  if (n <= 0) return hipSuccess;
+  const int grid = (int)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero2d_kernel, dim3(grid), dim3(256), 0, s, (unsigned long long*)base, pitch_words, width_words,
+                     rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual_sumsq(float2* res, const float2* obs, long long n, double* part, double* half_sum,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(residual_sumsq_kernel, dim3(kMisfitParts), dim3(kSumThreads), 0, s, res, obs, n, part);
+  hipLaunchKernelGGL(half_sum_kernel, dim3(1), dim3(64), 0, s, half_sum, part, kMisfitParts);
   return hipGetLastError();
 }
 

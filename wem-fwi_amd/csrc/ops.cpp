@@ -159,15 +159,67 @@ static int build_plan(DevPlan& P, const std::vector<int>& iz, const std::vector<
 
 // ------------------------------------------------------------------ batches
 
+#define RCCHK_PUT(expr)    \
+  do {                     \
+    int _rc = (expr);      \
+    if (_rc) return _rc;   \
+  } while (0)
+
+// Recording of one operator call's device buffers and host decisions, so
+// that the same call can run again under stream capture: the recording pass
+// runs eagerly and keeps every buffer it allocates (descriptor tables each
+// in a buffer of their own, uploaded once); the replay pass takes the same
+// buffers in the same order, uploads nothing, waits for nothing and makes
+// the same decisions, so that only kernel launches and device copies reach
+// the stream.  Set for the calling thread by the gradient plan while it
+// builds (g_arena); null everywhere else.
+struct Arena {
+  bool replay = false;
+  std::vector<std::pair<void*, size_t>> bufs;
+  size_t next = 0;
+  std::vector<long long> decisions;
+  size_t next_decision = 0;
+  ~Arena() {
+    for (auto& b : bufs) cached_free(b.first);
+  }
+  int take(void** p, size_t bytes, const char* what) {
+    if (!replay) {
+      if (int rc = cached_malloc(p, bytes, what)) return rc;
+      bufs.push_back({*p, bytes});
+      return WEMFWI_OK;
+    }
+    if (next >= bufs.size() || bufs[next].second != bytes)
+      return fail(WEMFWI_ERR_ARG, "recorded call: replay does not match the recording");
+    *p = bufs[next++].first;
+    return WEMFWI_OK;
+  }
+  long long decide(long long v) {
+    if (!replay) {
+      decisions.push_back(v);
+      return v;
+    }
+    return next_decision < decisions.size() ? decisions[next_decision++] : v;
+  }
+  void rewind() {
+    replay = true;
+    next = next_decision = 0;
+  }
+};
+static thread_local Arena* g_arena = nullptr;
+
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
-  ~DevBuf() { cached_free(p); }
+  bool recorded = false;  // owned by the arena of a recorded call
+  ~DevBuf() {
+    if (!recorded) cached_free(p);
+  }
   int alloc(size_t bytes, const char* what) {
-    cached_free(p);
+    if (!recorded) cached_free(p);
     p = nullptr;
     n = bytes;
-    if (int rc = cached_malloc(&p, bytes, what)) {
+    recorded = g_arena != nullptr;
+    if (int rc = recorded ? g_arena->take(&p, bytes, what) : cached_malloc(&p, bytes, what)) {
       n = 0;
       return rc;
     }
@@ -181,6 +233,15 @@ struct DevBuf {
 
 template <typename T>
 static int put(DevBuf& b, const std::vector<T>& h, hipStream_t s, const char* what) {
+  if (g_arena) {
+    // a buffer of its own for every table of a recorded call (a replayed
+    // launch reads what it read when recorded), uploaded when recorded
+    if (h.empty()) return WEMFWI_OK;
+    RCCHK_PUT(b.alloc(h.size() * sizeof(T), what));
+    if (g_arena->replay) return WEMFWI_OK;
+    hipError_t e = hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e == hipSuccess ? WEMFWI_OK : hip_fail(e, what);
+  }
   if (b.n < h.size() * sizeof(T)) {
     int rc = b.alloc(h.size() * sizeof(T), what);
     if (rc) return rc;
@@ -222,6 +283,7 @@ struct Rows {
 // WEMFWI_CHUNK_ROWS rows when that variable is set (to bound memory, and to
 // exercise chunked calls on small problems).
 static size_t chunk_rows(size_t per_row_bytes, size_t nrows) {
+  if (g_arena && g_arena->replay) return (size_t)g_arena->decide((long long)nrows);
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr == 0) fr = (size_t)8 << 30;
   const size_t budget = fr / 2;
@@ -231,7 +293,10 @@ static size_t chunk_rows(size_t per_row_bytes, size_t nrows) {
     if (cap > 0) c = std::min(c, (size_t)cap);
   }
   if (c < 1) c = 1;
-  return std::min(c, nrows);
+  c = std::min(c, nrows);
+  // a replayed call cuts its rows as when it was recorded (free memory has
+  // changed since)
+  return g_arena ? (size_t)g_arena->decide((long long)c) : c;
 }
 
 // Context of one operator call: geometry shared by all models, device-side
@@ -584,17 +649,21 @@ struct Ctx {
   int zero_depths(float2* W, int n, int lo, int hi, size_t wstride = 0, int zbase = 0) {
     if (lo > hi || n <= 0) return WEMFWI_OK;
     const size_t pitch = wstride ? wstride : nzx;
-    HIPCHK(hipMemset2DAsync(W + (size_t)(lo - zbase) * nx, pitch * sizeof(float2), 0,
-                            (size_t)(hi - lo + 1) * nx * sizeof(float2), (size_t)n, s),
-           "memset");
+    return zero_rows(W + (size_t)(lo - zbase) * nx, pitch, (size_t)(hi - lo + 1) * nx, n);
+  }
+  // n rows of width samples, pitch samples apart (kernels, not memsets:
+  // a recorded call may be captured in a graph)
+  int zero_rows(float2* p, size_t pitch, size_t width, int n) {
+    HIPCHK(launch_zero2d(p, (long long)pitch, (long long)width, n, s), "zero rows");
     return WEMFWI_OK;
   }
-
+  // bytes: a multiple of 8
   int zero(void* p, size_t bytes) {
-    HIPCHK(hipMemsetAsync(p, 0, bytes, s), "memset");
+    HIPCHK(launch_zero2d(p, (long long)(bytes / 8), (long long)(bytes / 8), 1, s), "zero");
     return WEMFWI_OK;
   }
   int sync() {
+    if (g_arena) return WEMFWI_OK;  // a recorded call is one stream-ordered sequence
     HIPCHK(hipStreamSynchronize(s), "stream sync");
     return WEMFWI_OK;
   }
@@ -818,7 +887,7 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
     RCCHK(c.inject_receivers(r, Q, data, sv->d.ngr, sv->d.offset));   // Q = conj(r) R + R^H d
     RCCHK(c.sweep(r, 0, 1, Q, Q));
     // LinDown^H stores rows 0..nz-2 of cs (no fill); row nz-1 starts at 0
-    HIPCHK(hipMemset2DAsync(cs + last, c.nzx * sizeof(float2), 0, c.nx * sizeof(float2), n, c.s), "memset");
+    RCCHK(c.zero_rows(cs + last, c.nzx, c.nx, n));
     // LinDown^H; each D row d < nz-1 it reads becomes r * D (the Up source)
     RCCHK(c.lin(r, +1, 1, exact, bg, Q, 0, cs, true, true));
     {
@@ -834,8 +903,7 @@ static int born_adjoint_all(Ctx& c, const std::vector<Model*>& fm, Per per, doub
     if (ext_rowlast) {
       for (int i = 0; i < n; ++i)
         if (r.shot[i] != ns - 1)
-          HIPCHK(hipMemsetAsync(cs + (size_t)i * c.nzx + (size_t)(c.nz - 1) * c.nx, 0, c.nx * sizeof(float2), c.s),
-                 "memset");
+          RCCHK(c.zero_rows(cs + (size_t)i * c.nzx + (size_t)(c.nz - 1) * c.nx, c.nx, c.nx, 1));
     }
     RCCHK(reduce_chunk(c, r, sliced, accS, cs, c.nzx, 0, 0, c.nz));
     RCCHK(c.sync());
@@ -951,9 +1019,7 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
       if (j > 0) RCCHK(c.copy_rows(cq, nx, Q + oa, W, n));
       // LinDown^H stores the rows [a, b) of cs (no fill); the last segment's
       // row nz - 1 = b starts at 0
-      if (last)
-        HIPCHK(hipMemset2DAsync(cs + (size_t)(b - zb) * nx, W * sizeof(float2), 0, nx * sizeof(float2), n, c.s),
-               "memset");
+      if (last) RCCHK(c.zero_rows(cs + (size_t)(b - zb) * nx, W, nx, n));
       // LinDown^H over [a, b); the D rows it reads become r * D (Up sources)
       RCCHK(c.lin_seg(r, +1, exact, bg, Q, cs, W, zb, a, b - a, true, true));
       if (last) {  // the Up recursion starts from r * D at the last depth
@@ -971,8 +1037,7 @@ static int born_adjoint_ckpt(Ctx& c, const std::vector<Model*>& fm, Per per, dou
       if (ext_rowlast && last) {
         for (int i = 0; i < n; ++i)
           if (r.shot[i] != ns - 1)
-            HIPCHK(hipMemsetAsync(cs + (size_t)i * W + (size_t)(nz - 1 - zb) * nx, 0, nx * sizeof(float2), c.s),
-                   "memset");
+            RCCHK(c.zero_rows(cs + (size_t)i * W + (size_t)(nz - 1 - zb) * nx, nx, nx, 1));
       }
       RCCHK(reduce_chunk(c, r, sliced, accS, cs, W, zb, a, e));
     }
@@ -1121,6 +1186,73 @@ static int bornrefl_adjoint(Model* m, Survey* sv, float2* image, int nh, const f
   return c.sync();
 }
 
+// ------------------------------------------------------------------ gradient plan
+
+// The FWI gradient of one model, survey and observed data as a replayable
+// hipGraph: res = WEM(m) - obs, J = 0.5 ||res||^2 (FP64, fixed order) and
+// (gS, gR) = Born(m)^H res with every (shot, frequency) row batched
+// (scripts/fwi.py:70-131 with WEM.py:52-81: the objective and its gradient
+// for one model).  Built by running the sequence once eagerly with a
+// recording arena (every device buffer kept, every descriptor table in a
+// buffer of its own) and once more under stream capture replaying that
+// arena, so the graph holds kernel launches and device copies only.  The
+// plan keeps every buffer of the call (for store-all gradients 4 wavefields
+// per row of a chunk) until it is destroyed; the model, survey, observed data
+// and output buffers are bound at creation.
+struct GradientPlan {
+  Model* m = nullptr;
+  Survey* sv = nullptr;
+  const float2* obs = nullptr;
+  float2 *res = nullptr, *gS = nullptr, *gR = nullptr;
+  int exact = 1;
+  Arena arena;
+  void* d_acc = nullptr;  // [kMisfitParts] partial sums, then J
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~GradientPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    cached_free(d_acc);
+  }
+  double* d_J() const { return (double*)d_acc + kMisfitParts; }
+  int run(hipStream_t s) {
+    std::vector<Model*> fm(sv->d.nw, m);
+    RCCHK(wem_forward(fm, Per::Freq, sv, res, 0, s));
+    const long long n = (long long)sv->d.ns * sv->d.nr * sv->d.nt;
+    HIPCHK(launch_residual_sumsq(res, obs, n, (double*)d_acc, d_J(), s), "misfit");
+    return born_adjoint(fm, Per::Freq, sv, gS, gR, 0, res, 0, exact, false, s);
+  }
+  int build(hipStream_t s) {
+    RCCHK(cached_malloc(&d_acc, (kMisfitParts + 1) * sizeof(double), "misfit partial sums"));
+    g_arena = &arena;
+    int rc = run(s);  // recording pass (eager: its results are the gradient of this call)
+    hipError_t e = hipStreamSynchronize(s);
+    if (rc == WEMFWI_OK && e != hipSuccess) rc = hip_fail(e, "gradient plan");
+    if (rc == WEMFWI_OK) {
+      arena.rewind();
+      e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "begin capture");
+      } else {
+        rc = run(cap);
+        hipError_t e2 = hipStreamEndCapture(cap, &graph);
+        if (rc == WEMFWI_OK && e2 != hipSuccess) rc = hip_fail(e2, "end capture");
+        if (rc == WEMFWI_OK && arena.next != arena.bufs.size())
+          rc = fail(WEMFWI_ERR_ARG, "gradient plan: replay does not match the recording");
+      }
+      if (rc == WEMFWI_OK) {
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "graph instantiate");
+      }
+    }
+    g_arena = nullptr;
+    return rc;
+  }
+};
+
 }  // namespace wemfwi
 
 // ------------------------------------------------------------------ C ABI
@@ -1129,6 +1261,10 @@ using namespace wemfwi;
 
 struct wemfwi_survey {
   Survey s;
+};
+
+struct wemfwi_gradient {
+  GradientPlan p;
 };
 
 extern "C" {
@@ -1197,6 +1333,49 @@ int wemfwi_born_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi
   std::vector<Model*> fm(survey->s.d.nw, &model->m);
   return born_forward(fm, Per::Freq, &survey->s, (const float2*)dslow, (const float2*)dden, 0, (float2*)data, add,
                       (hipStream_t)stream);
+}
+
+int wemfwi_gradient_create(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* obs,
+                           wemfwi_complex* res, wemfwi_complex* gS, wemfwi_complex* gR, int mode, void* stream,
+                           wemfwi_gradient** out) {
+  if (!model || !survey || !obs || !res || !gS || !gR || !out) return fail(WEMFWI_ERR_ARG, "gradient_create: null argument");
+  *out = nullptr;
+  wemfwi_gradient* h = new (std::nothrow) wemfwi_gradient();
+  if (!h) return fail(WEMFWI_ERR_ALLOC, "gradient_create: out of host memory");
+  GradientPlan& p = h->p;
+  p.m = &model->m;
+  p.sv = &survey->s;
+  p.obs = (const float2*)obs;
+  p.res = (float2*)res;
+  p.gS = (float2*)gS;
+  p.gR = (float2*)gR;
+  p.exact = mode != 0;
+  if (int rc = p.build((hipStream_t)stream)) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return WEMFWI_OK;
+}
+
+int wemfwi_gradient_launch(wemfwi_gradient* plan, void* stream, double* misfit) {
+  if (!plan || !plan->p.exec) return fail(WEMFWI_ERR_ARG, "gradient_launch: null plan");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipGraphLaunch(plan->p.exec, s);
+  if (e == hipSuccess && misfit) e = hipMemcpyAsync(misfit, plan->p.d_J(), sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "gradient launch");
+}
+
+int wemfwi_gradient_misfit(wemfwi_gradient* plan, double* misfit) {
+  if (!plan || !misfit) return fail(WEMFWI_ERR_ARG, "gradient_misfit: null argument");
+  hipError_t e = hipMemcpy(misfit, plan->p.d_J(), sizeof(double), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "gradient misfit");
+}
+
+int wemfwi_gradient_destroy(wemfwi_gradient* plan) {
+  delete plan;
+  return WEMFWI_OK;
 }
 
 int wemfwi_born_adjoint(wemfwi_model* model, wemfwi_survey* survey, wemfwi_complex* dslow, wemfwi_complex* dden,

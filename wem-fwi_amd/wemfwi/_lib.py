@@ -65,6 +65,10 @@ SIGNATURES = {
     "wemfwi_onestep": (_i, [_vp, _i, _i, _fp, _ip, _vp, _vp, _i, _vp]),
     "wemfwi_oneway_recursive": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _vp]),
     "wemfwi_oneway_noalloc": (_i, [_vp, _i, _i, _i, _fp, _vp, _vp, _i, _vp]),
+    "wemfwi_gradient_create": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, C.POINTER(_vp)]),
+    "wemfwi_gradient_launch": (_i, [_vp, _vp, C.POINTER(C.c_double)]),
+    "wemfwi_gradient_misfit": (_i, [_vp, C.POINTER(C.c_double)]),
+    "wemfwi_gradient_destroy": (_i, [_vp]),
     "wemfwi_oneway_noalloc_max_rows": (C.c_longlong, [_vp]),
     "wemfwi_oneway_batch": (_i, [_i, _i, _i, _vp, _fp, _vp, _vp, _vp]),
     "wemfwi_batch_create": (_i, [_i, _vp, _fp, _vp, _vp, _vp, C.POINTER(_vp)]),

@@ -137,6 +137,49 @@ def born_adjoint(model: Model, survey: Survey, dslow: torch.Tensor, dden: torch.
     return dslow, dden
 
 
+class GradientPlan:
+    """The FWI gradient of one model as a replayable graph
+    (wemfwi_gradient_create): res = WEM(model) - obs, J = 0.5 ||res||^2 and
+    (gS, gR) = Born(model)^H res (exact adjoint by default), every
+    (shot, frequency) row batched.  Creating the plan computes the gradient
+    once; run() replays the captured graph and returns J; gS, gR and res
+    hold the results.  A new model needs a new plan."""
+
+    def __init__(self, model: Model, survey: Survey, obs: torch.Tensor, exact: bool = True, stream=None):
+        _dev_c64(obs, "obs")
+        self.obs = obs
+        self.res = torch.empty_like(obs)
+        self.gS = torch.empty((model.nz, model.nx), dtype=torch.complex64, device=obs.device)
+        self.gR = torch.empty_like(self.gS)
+        h = C.c_void_p()
+        check(lib.wemfwi_gradient_create(model._h, survey._h, _ptr(obs), _ptr(self.res), _ptr(self.gS),
+                                         _ptr(self.gR), int(bool(exact)), _stream_ptr(stream), C.byref(h)))
+        self._h = h
+        self._keep = (model, survey)
+
+    @property
+    def misfit(self) -> float:
+        j = C.c_double()
+        check(lib.wemfwi_gradient_misfit(self._h, C.byref(j)))
+        return j.value
+
+    def run(self, stream=None) -> float:
+        j = C.c_double()
+        check(lib.wemfwi_gradient_launch(self._h, _stream_ptr(stream), C.byref(j)))
+        return j.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.wemfwi_gradient_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def extborn_forward(models: Sequence[Model], survey: Survey, dslow, dden, data, add=False,
                     stream=None):
     for n, t in (("dslow", dslow), ("dden", dden), ("data", data)):
