@@ -447,6 +447,44 @@ int wemfwi_uneven_offset_gradient(const wemfwi_image_shot* shots, int ns, int ng
                                   const wemfwi_complex* model, wemfwi_complex* grad, double* misfit, int add,
                                   void* stream);
 
+/* BornFull::forward / adjoint (BornFull.cpp:64-117, born_prop.cpp:11-235) of
+ * shots with uneven apertures and bands: shot s propagates on its own
+ * background `model` (nz x nx_s, its aperture; any FFT length the models
+ * support) placed at column x0 of the perturbation grid [nz][nx_model]
+ * (columns x0 + a outside [0, nx_model) are zero, as the reference's
+ * zero-padded perturbation, born_prop.cpp:52-58); its rows are its nw
+ * frequencies; the source (spread ngs) and the receivers (spread ngr) sit on
+ * the aperture grid; data is the shot's device array [nr][nw] (written by the
+ * forward, read by the adjoint).  All rows of a call are swept in batched
+ * launches (one per kernel variant: FFT length x full row x split step) and
+ * LinDown / LinUp likewise; injection, recording and the FP64 reduction run
+ * per shot in the order of a one-shot call, so every shot's data and every
+ * shot's gradient slice equal those of wemfwi_born_forward /
+ * wemfwi_born_adjoint on that shot alone (a one-shot survey with bins
+ * 0..nw-1, nt = nw, offset 0) bitwise.  The adjoint sums the shots' slices
+ * in shot order in double precision into dslow / dden [nz][nx_model]
+ * (device; add = 0 overwrites them); mode as born_adjoint.  Whole
+ * wavefields are kept (the survey checkpoint setting does not apply);
+ * consecutive shots are grouped so that a group's wavefields fit half of
+ * the free device memory.  Frequencies, wavelets and device positions are
+ * host arrays. */
+typedef struct wemfwi_born_shot {
+  wemfwi_model* model;
+  int x0;
+  int nw;
+  const float* freq_hz;
+  const wemfwi_complex* wavelet;
+  int src_iz, src_ix;
+  int nr;
+  const int* rec_iz;
+  const int* rec_ix;
+  wemfwi_complex* data;
+} wemfwi_born_shot;
+int wemfwi_uneven_born_forward(const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
+                               const wemfwi_complex* dslow, const wemfwi_complex* dden, int add, void* stream);
+int wemfwi_uneven_born_adjoint(const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
+                               wemfwi_complex* dslow, wemfwi_complex* dden, int add, int mode, void* stream);
+
 /* ---------------------------------------------------------------- extended FWI
  * Operators of the frequency-extended objective of scripts/efwi.py:160-190
  * (operator/src/python/Operator.py:228-292,619-775), on device models

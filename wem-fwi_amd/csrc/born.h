@@ -80,6 +80,10 @@ hipError_t launch_gather_mul(float2* out, const float2* in, const float2* mult, 
 hipError_t launch_gather_sum(float2* out, long long n, const long long* ptr, const long long* src, const float2* in,
                              bool add, hipStream_t s);
 hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hipStream_t s);
+// out[z][a] = in[z][x0 + a] (0 outside [0, nxg)), z < nz, a < nxw; and its
+// adjoint acc[z][x0 + a] += in[z][a] (FP64 accumulator [nz][nxg])
+hipError_t launch_window(float2* out, const float2* in, int nz, int nxw, int nxg, int x0, hipStream_t s);
+hipError_t launch_window_add(double2* acc, const float2* in, int nz, int nxw, int nxg, int x0, hipStream_t s);
 // W holds rows zbase.. of each [nz][nx] field; sources outside [zlo, zhi]
 // (cells outside [clo, chi), with cbase = zbase*nx) are skipped
 hipError_t launch_inject_point(float2* W, long long wstride, int nx, const int* iz, const int* ix,

@@ -51,6 +51,11 @@ __global__ void __launch_bounds__(64 * kLinWaves)
   const int d = dlo + (item - row * nd);
   const LinRow R = ldu_struct(rows, row);
   const DepthTables dt = ldu_struct(models, R.model);
+  // the row's width, padding and taper come from its model (a launch may
+  // hold rows of different apertures; it fixes the FFT length)
+  g.nx = dt.nx;
+  g.tap = dt.tap;
+  g.taper = dt.taper;
   float2* buf = bufs[wv];
   const size_t nx = g.nx;
   const int r0 = d - R.zbase;  // buffer row of depth d
@@ -336,6 +341,36 @@ __global__ void d2f_kernel(float2* __restrict__ out, const double2* __restrict__
       y += out[i].y;
     }
     out[i] = make_float2((float)x, (float)y);
+  }
+}
+
+// Aperture window of a perturbation grid: out[z][a] = in[z][x0 + a] for
+// 0 <= x0 + a < nxg, 0 outside (the zero padding of born_prop.cpp:52-58).
+__global__ void window_kernel(float2* __restrict__ out, const float2* __restrict__ in, int nz, int nxw, int nxg,
+                              int x0) {
+  const long long n = (long long)nz * nxw;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long z = i / nxw;
+    const int x = x0 + (int)(i - z * nxw);
+    out[i] = (x >= 0 && x < nxg) ? in[z * nxg + x] : make_float2(0.f, 0.f);
+  }
+}
+
+// Its adjoint into an FP64 accumulator: acc[z][x0 + a] += in[z][a] for the
+// columns inside [0, nxg).
+__global__ void window_add_kernel(double2* __restrict__ acc, const float2* __restrict__ in, int nz, int nxw,
+                                  int nxg, int x0) {
+  const long long n = (long long)nz * nxw;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long z = i / nxw;
+    const int x = x0 + (int)(i - z * nxw);
+    if (x < 0 || x >= nxg) continue;
+    const float2 v = in[i];
+    double2& o = acc[z * nxg + x];
+    o.x += (double)v.x;
+    o.y += (double)v.y;
   }
 }
 
@@ -650,6 +685,20 @@ hipError_t launch_add(float2* out, const float2* in, long long n, hipStream_t s)
 hipError_t launch_d2f(float2* out, const double2* in, long long n, bool acc, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(d2f_kernel, grid1d(n), dim3(256), 0, s, out, in, n, acc ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_window(float2* out, const float2* in, int nz, int nxw, int nxg, int x0, hipStream_t s) {
+  const long long n = (long long)nz * nxw;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(window_kernel, grid1d(n), dim3(256), 0, s, out, in, nz, nxw, nxg, x0);
+  return hipGetLastError();
+}
+
+hipError_t launch_window_add(double2* acc, const float2* in, int nz, int nxw, int nxg, int x0, hipStream_t s) {
+  const long long n = (long long)nz * nxw;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(window_add_kernel, grid1d(n), dim3(256), 0, s, acc, in, nz, nxw, nxg, x0);
   return hipGetLastError();
 }
 

@@ -732,7 +732,7 @@ static int batch_streams(hipStream_t (&out)[4]) {
 // tables and scratch rows stay on the device between runs.
 
 int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf* const* in, cf* const* out,
-                     hipStream_t s, const int* src_lo, const int* src_hi) {
+                     hipStream_t s, const int* src_lo, const int* src_hi, const RowProducts* prod) {
   if (nrows < 1 || !ms || !freq_hz || !in || !out) return fail(WEMFWI_ERR_ARG, "batch: invalid argument");
   std::vector<int> midx(nrows);
   for (int i = 0; i < nrows; ++i) {
@@ -788,9 +788,9 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     h[k].nsteps = -1;
     h[k].init = nullptr;
     h[k].zbase = 0;
-    h[k].smul = nullptr;
-    h[k].out2 = nullptr;
-    h[k].omul = nullptr;
+    h[k].smul = (prod && prod->smul) ? prod->smul[i] : nullptr;
+    h[k].out2 = (prod && prod->out2) ? prod->out2[i] : nullptr;
+    h[k].omul = (prod && prod->omul) ? prod->omul[i] : nullptr;
     h[k].src_skip = (src_lo && src_hi) ? 1 : 0;
     h[k].src_lo = (src_lo && src_hi) ? src_lo[i] : 0;
     h[k].src_hi = (src_lo && src_hi) ? src_hi[i] : -1;

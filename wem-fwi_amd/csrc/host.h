@@ -141,11 +141,18 @@ struct BatchPlan {
   const DepthTables* d_tabs = nullptr;
   std::vector<hipEvent_t> events;      // fork / join events of enqueue (one more than groups)
   ~BatchPlan();
+  // Optional per-row products of the sweep kernel (SweepRow smul, out2,
+  // omul): arrays of nrows pointers, NULL for none
+  struct RowProducts {
+    const float2* const* smul = nullptr;
+    float2* const* out2 = nullptr;
+    const float2* const* omul = nullptr;
+  };
   // src_lo / src_hi (optional, per row): only the source rows of those
   // depths are read (the sweep kernel's src_skip)
   int build(int nrows, Model* const* ms, const float* freq_hz, const std::complex<float>* const* in,
             std::complex<float>* const* out, hipStream_t s, const int* src_lo = nullptr,
-            const int* src_hi = nullptr);
+            const int* src_hi = nullptr, const RowProducts* prod = nullptr);
   // WEMFWI_ERR_ARG if a model's phase table was rebuilt since planning
   int check() const;
   // launches the groups behind the work already on s and makes s wait for

@@ -1253,6 +1253,473 @@ struct GradientPlan {
   }
 };
 
+// ------------------------------------------------------------------ uneven Born
+
+// BornFull::full_fwd / full_adj (born_prop.cpp:75-113, 178-232) of shots that
+// each carry their own background on their own aperture (nz x nx_s, placed
+// at column x0 of the perturbation grid) and their own frequencies.  The
+// rows of a group of consecutive shots (shot order, then frequency order)
+// share one set of wavefield buffers; every sweep is one batch plan over all
+// of them (one launch per kernel variant) and every LinDown / LinUp one
+// launch per FFT length; injection, recording, row products and the FP64
+// reductions run per shot with the arguments and in the order of
+// born_forward / born_adjoint_all on a one-shot survey, so every shot's data
+// and gradient slice are bitwise those of such a call.
+struct UnevenBorn {
+  const wemfwi_born_shot* shots = nullptr;
+  int s0 = 0, s1 = 0, nz = 0, ngs = 0, ngr = 0;
+  hipStream_t s = nullptr;
+  std::vector<Model*> rm;      // model of each row
+  std::vector<float> rf;       // frequency of each row
+  std::vector<size_t> roff;    // first sample of each row in a group buffer
+  std::vector<int> first;      // first row of each shot of the group (one more entry)
+  size_t total = 0;            // samples of a group buffer
+  std::vector<size_t> woff;    // first sample of each shot's [nz][nx_s] slot
+  size_t wtotal = 0;
+  std::vector<DevPlan> rplan;  // receiver plans, one per shot
+  std::vector<Model*> models;  // distinct models of the group
+  std::vector<int> midx;       // row -> distinct model
+  std::vector<float> wrow;
+  std::vector<const float2*> prow;
+  DevBuf d_tabs, d_lin, d_mul, d_iz, d_ix, d_val, d_gw, d_vb;
+
+  ~UnevenBorn() {
+    for (auto& p : rplan) p.release();
+  }
+  const wemfwi_born_shot& S(int k) const { return shots[s0 + k]; }
+  int nshots() const { return s1 - s0; }
+  int nx(int k) const { return S(k).model->m.d.nx; }
+  size_t nzx(int k) const { return (size_t)nz * nx(k); }
+  int nw(int k) const { return first[k + 1] - first[k]; }
+  // first sample of shot k's rows in a group buffer
+  size_t base(int k) const { return nw(k) ? roff[first[k]] : 0; }
+
+  int init(const wemfwi_born_shot* sh, int a, int b, int nz_, int ngs_, int ngr_, hipStream_t st) {
+    shots = sh;
+    s0 = a;
+    s1 = b;
+    nz = nz_;
+    ngs = ngs_;
+    ngr = ngr_;
+    s = st;
+    first.assign(1, 0);
+    for (int k = 0; k < nshots(); ++k) {
+      Model* m = &S(k).model->m;
+      woff.push_back(wtotal);
+      wtotal += nzx(k);
+      for (int i = 0; i < S(k).nw; ++i) {
+        rm.push_back(m);
+        rf.push_back(S(k).freq_hz[i]);
+        roff.push_back(total);
+        total += nzx(k);
+      }
+      first.push_back((int)rm.size());
+    }
+    const int n = (int)rm.size();
+    midx.resize(n);
+    for (int i = 0; i < n; ++i) {
+      auto it = std::find(models.begin(), models.end(), rm[i]);
+      midx[i] = (int)(it - models.begin());
+      if (it == models.end()) models.push_back(rm[i]);
+    }
+    // phase tables: every model gets the frequencies of its rows
+    wrow.resize(n);
+    prow.resize(n);
+    for (size_t u = 0; u < models.size(); ++u) {
+      std::vector<int> idx;
+      std::vector<float> f;
+      for (int i = 0; i < n; ++i)
+        if (midx[i] == (int)u) {
+          idx.push_back(i);
+          f.push_back(rf[i]);
+        }
+      std::vector<float> w;
+      std::vector<const float2*> p;
+      RCCHK(models[u]->phase_tables((int)f.size(), f.data(), w, p, s));
+      for (size_t k = 0; k < idx.size(); ++k) {
+        wrow[idx[k]] = w[k];
+        prow[idx[k]] = p[k];
+      }
+    }
+    std::vector<DepthTables> ht;
+    for (Model* m : models) ht.push_back(m->h_tables);
+    RCCHK(put(d_tabs, ht, s, "model tables"));
+    // per-row source positions and values, data columns
+    std::vector<int> hiz(std::max(n, 1)), hix(std::max(n, 1));
+    std::vector<float2> hval(std::max(n, 1));
+    std::vector<long long> hvb(std::max(n, 1));
+    for (int k = 0; k < nshots(); ++k)
+      for (int i = 0; i < nw(k); ++i) {
+        const int r = first[k] + i;
+        hiz[r] = S(k).src_iz;
+        hix[r] = S(k).src_ix;
+        hval[r] = make_float2(S(k).wavelet[i].re, S(k).wavelet[i].im);
+        hvb[r] = i;  // data [nr][nw]: receiver stride nw, column i
+      }
+    RCCHK(put(d_iz, hiz, s, "source iz"));
+    RCCHK(put(d_ix, hix, s, "source ix"));
+    RCCHK(put(d_val, hval, s, "source values"));
+    RCCHK(put(d_vb, hvb, s, "data columns"));
+    RCCHK(put(d_gw, cos2_window(ngs), s, "source spread"));
+    rplan.resize(nshots());
+    for (int k = 0; k < nshots(); ++k) {
+      const std::vector<int> iz(S(k).rec_iz, S(k).rec_iz + S(k).nr), ix(S(k).rec_ix, S(k).rec_ix + S(k).nr);
+      RCCHK(build_plan(rplan[k], iz, ix, ngr, 0, nx(k), nz));
+    }
+    return WEMFWI_OK;
+  }
+
+  // one sweep of every row of the group (batch plan: one launch per kernel
+  // variant); lo / hi: per row, the only source depths read (none: all);
+  // smul_refl: source rows multiplied by the row model's reflection
+  // coefficients as they are read; q2: second output conj(r) * result
+  int sweep(int up, int adjoint, const float2* in, float2* out, const std::vector<int>* lo = nullptr,
+            const std::vector<int>* hi = nullptr, bool smul_refl = false, float2* q2 = nullptr) {
+    const int n = (int)rm.size();
+    if (n == 0) return WEMFWI_OK;
+    std::vector<const cf*> pin(n);
+    std::vector<cf*> pout(n);
+    std::vector<const float2*> sm, om;
+    std::vector<float2*> o2;
+    BatchPlan::RowProducts pr;
+    for (int i = 0; i < n; ++i) {
+      pin[i] = (const cf*)(in + roff[i]);
+      pout[i] = (cf*)(out + roff[i]);
+    }
+    if (smul_refl) {
+      sm.resize(n);
+      for (int i = 0; i < n; ++i) sm[i] = rm[i]->d_refl;
+      pr.smul = sm.data();
+    }
+    if (q2) {
+      o2.resize(n);
+      om.resize(n);
+      for (int i = 0; i < n; ++i) {
+        o2[i] = q2 + roff[i];
+        om[i] = rm[i]->d_refl;
+      }
+      pr.out2 = o2.data();
+      pr.omul = om.data();
+    }
+    BatchPlan p;
+    RCCHK(p.build(n, rm.data(), rf.data(), pin.data(), pout.data(), s, lo ? lo->data() : nullptr,
+                  hi ? hi->data() : nullptr, &pr));
+    return p.run(up, adjoint, s);
+  }
+
+  // LinDown (dir = +1) / LinUp (dir = -1) of every row, one launch per FFT
+  // length and Taylor order; m: per row, the perturbation (forward) or the
+  // wavefield row (adjoint)
+  int lin(int dir, int adjoint, int exact, float2* bg, const std::vector<const float2*>& m, float2* out,
+          bool post_refl = false, bool assign = false) {
+    const int n = (int)rm.size();
+    if (n == 0) return WEMFWI_OK;
+    std::vector<long long> key(n);
+    for (int i = 0; i < n; ++i) key[i] = ((long long)rm[i]->N * 4096 + rm[i]->nk) * 64 + rm[i]->d.ntaylor;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    std::vector<LinRow> h(n);
+    for (int k = 0; k < n; ++k) {
+      const int i = order[k];
+      h[k] = LinRow{bg + roff[i], m[i], out + roff[i], prow[i], wrow[i], midx[i], 0,
+                    post_refl ? rm[i]->d_refl : nullptr, assign ? 1 : 0};
+    }
+    RCCHK(put(d_lin, h, s, "lin rows"));
+    for (int k0 = 0; k0 < n;) {
+      int k1 = k0 + 1;
+      while (k1 < n && key[order[k1]] == key[order[k0]]) ++k1;
+      const Model* mg = rm[order[k0]];
+      HIPCHK(launch_lin(mg->N, d_lin.as<LinRow>() + k0, k1 - k0, d_tabs.as<DepthTables>(), mg->geom(), nz, dir,
+                        adjoint != 0, exact != 0, mg->d_tw, s),
+             "lin kernel");
+      k0 = k1;
+    }
+    HIPCHK(hipStreamSynchronize(s), "lin");
+    return WEMFWI_OK;
+  }
+
+  // per-row pointers: shot k's slot of a [nz][nx_s]-per-shot buffer, or the
+  // row itself in a group buffer
+  std::vector<const float2*> per_shot(const float2* w) const {
+    std::vector<const float2*> v(rm.size());
+    for (int k = 0; k < nshots(); ++k)
+      for (int r = first[k]; r < first[k + 1]; ++r) v[r] = w + woff[k];
+    return v;
+  }
+  std::vector<const float2*> per_row(const float2* w) const {
+    std::vector<const float2*> v(rm.size());
+    for (size_t r = 0; r < rm.size(); ++r) v[r] = w + roff[r];
+    return v;
+  }
+  // depths [lo, hi] of shot k's rows
+  int zero_depths(float2* W, int k, int lo, int hi) {
+    if (lo > hi || nw(k) == 0) return WEMFWI_OK;
+    HIPCHK(launch_zero2d(W + base(k) + (size_t)lo * nx(k), (long long)nzx(k), (long long)(hi - lo + 1) * nx(k),
+                         nw(k), s),
+           "zero rows");
+    return WEMFWI_OK;
+  }
+  int inject_sources(float2* W) {
+    for (int k = 0; k < nshots(); ++k) {
+      if (nw(k) == 0) continue;
+      const int r = first[k];
+      HIPCHK(launch_inject_point(W + base(k), (long long)nzx(k), nx(k), d_iz.as<int>() + r, d_ix.as<int>() + r,
+                                 d_val.as<float2>() + r, d_gw.as<float>(), ngs, 0, nw(k), s),
+             "inject sources");
+    }
+    return WEMFWI_OK;
+  }
+  int inject_receivers(float2* W) {
+    for (int k = 0; k < nshots(); ++k) {
+      if (nw(k) == 0) continue;
+      HIPCHK(launch_inject_csr(W + base(k), (long long)nzx(k), rplan[k].inj, (const float2*)S(k).data,
+                               d_vb.as<long long>() + first[k], nw(k), nw(k), s, 0, 0, (long long)nzx(k)),
+             "inject receivers");
+    }
+    return WEMFWI_OK;
+  }
+  int record(const float2* W) {
+    for (int k = 0; k < nshots(); ++k) {
+      if (nw(k) == 0) continue;
+      HIPCHK(launch_record((float2*)S(k).data, d_vb.as<long long>() + first[k], nw(k), W + base(k),
+                           (long long)nzx(k), rplan[k].rec, nw(k), s),
+             "record receivers");
+    }
+    HIPCHK(hipStreamSynchronize(s), "record");
+    return WEMFWI_OK;
+  }
+  int mul_rows(std::vector<MulRow>& rows, size_t len, bool acc) {
+    if (rows.empty()) return WEMFWI_OK;
+    RCCHK(put(d_mul, rows, s, "product rows"));
+    HIPCHK(launch_mul_rows(d_mul.as<MulRow>(), (int)rows.size(), (long long)len, false, acc, s), "mul rows");
+    return WEMFWI_OK;
+  }
+  // receiver depth range of shot k ([lo, hi], lo > hi without receivers)
+  void rec_depths(int k, int& lo, int& hi) const {
+    lo = 1 << 30;
+    hi = -1;
+    for (int r = 0; r < S(k).nr; ++r) {
+      lo = std::min(lo, S(k).rec_iz[r]);
+      hi = std::max(hi, S(k).rec_iz[r]);
+    }
+  }
+  std::vector<int> row_vec(const std::vector<int>& per_shot_v) const {
+    std::vector<int> v(rm.size());
+    for (int k = 0; k < nshots(); ++k)
+      for (int r = first[k]; r < first[k + 1]; ++r) v[r] = per_shot_v[k];
+    return v;
+  }
+};
+
+// born_forward's sequence on the rows of one group of shots
+static int uneven_born_forward_group(UnevenBorn& u, const float2* dS, const float2* dR, int nxg) {
+  const int K = u.nshots(), nz = u.nz;
+  hipStream_t s = u.s;
+  DevBuf WS, WR, DRb, B, S1, T;
+  RCCHK(WS.alloc(u.wtotal * sizeof(float2), "perturbation windows"));
+  RCCHK(WR.alloc(u.wtotal * sizeof(float2), "perturbation windows"));
+  RCCHK(DRb.alloc(u.wtotal * sizeof(float2), "dr"));
+  float2 *ws = WS.as<float2>(), *wr = WR.as<float2>(), *dr = DRb.as<float2>();
+  for (int k = 0; k < K; ++k) {
+    const Model& m = u.S(k).model->m;
+    HIPCHK(launch_window(ws + u.woff[k], dS, nz, u.nx(k), nxg, u.S(k).x0, s), "window");
+    HIPCHK(launch_window(wr + u.woff[k], dR, nz, u.nx(k), nxg, u.S(k).x0, s), "window");
+    HIPCHK(launch_dreflect_fwd(dr + u.woff[k], m.d_drefl, ws + u.woff[k], wr + u.woff[k], nz, u.nx(k), false, s),
+           "dreflect");
+  }
+  if (u.total == 0) return WEMFWI_OK;
+  RCCHK(B.alloc(u.total * sizeof(float2), "background"));
+  RCCHK(S1.alloc(u.total * sizeof(float2), "scattered"));
+  RCCHK(T.alloc(u.total * sizeof(float2), "scattered"));
+  float2 *bg = B.as<float2>(), *s1 = S1.as<float2>(), *tt = T.as<float2>();
+  std::vector<int> src(K);
+  for (int k = 0; k < K; ++k) src[k] = u.S(k).src_iz;
+  const std::vector<int> slo = u.row_vec(src);
+  // D = Down(inj_src): only the source depth is cleared and read as a source
+  for (int k = 0; k < K; ++k) RCCHK(u.zero_depths(bg, k, src[k], src[k]));
+  RCCHK(u.inject_sources(bg));
+  RCCHK(u.sweep(0, 0, bg, bg, &slo, &slo));
+  const std::vector<const float2*> mS = u.per_shot(ws);
+  // s1 = Down(LinDown(dS; D)), recorded
+  HIPCHK(launch_zero2d(s1, (long long)u.total, (long long)u.total, 1, s), "zero");
+  RCCHK(u.lin(+1, 0, 0, bg, mS, s1));
+  RCCHK(u.sweep(0, 0, s1, s1));
+  RCCHK(u.record(s1));
+  // s2 = Up(D * dr + r * s1), recorded
+  for (int k = 0; k < K; ++k) {
+    if (u.nw(k) == 0) continue;
+    std::vector<MulRow> ic, rs1;
+    for (int r = u.first[k]; r < u.first[k + 1]; ++r) {
+      ic.push_back(MulRow{tt + u.roff[r], bg + u.roff[r], dr + u.woff[k]});
+      rs1.push_back(MulRow{tt + u.roff[r], u.S(k).model->m.d_refl, s1 + u.roff[r]});
+    }
+    RCCHK(u.mul_rows(ic, u.nzx(k), false));
+    RCCHK(u.mul_rows(rs1, u.nzx(k), true));
+  }
+  RCCHK(u.sweep(1, 0, tt, tt));
+  RCCHK(u.record(tt));
+  // U = Up(r * D); s3 = Up(LinUp(dS; U)), recorded
+  RCCHK(u.sweep(1, 0, bg, bg, nullptr, nullptr, true));
+  HIPCHK(launch_zero2d(s1, (long long)u.total, (long long)u.total, 1, s), "zero");
+  RCCHK(u.lin(-1, 0, 0, bg, mS, s1));
+  RCCHK(u.sweep(1, 0, s1, s1));
+  RCCHK(u.record(s1));
+  return WEMFWI_OK;
+}
+
+// born_adjoint_all and born_adjoint_out on the rows of one group of shots;
+// each shot's float gradient slice is added to the FP64 accumulators gS, gR
+// [nz][nxg] at its columns, shot after shot
+static int uneven_born_adjoint_group(UnevenBorn& u, double2* gS, double2* gR, int nxg, int exact) {
+  const int K = u.nshots(), nz = u.nz;
+  hipStream_t s = u.s;
+  if (u.total == 0) return WEMFWI_OK;
+  DevBuf B, Rb, Qb, CS, AS, AD, FS, FR, FD;
+  RCCHK(B.alloc(u.total * sizeof(float2), "background"));
+  RCCHK(Rb.alloc(u.total * sizeof(float2), "receiver wavefield"));
+  RCCHK(Qb.alloc(u.total * sizeof(float2), "receiver wavefield"));
+  RCCHK(CS.alloc(u.total * sizeof(float2), "dS contributions"));
+  RCCHK(AS.alloc(u.wtotal * sizeof(double2), "dS accumulators"));
+  RCCHK(AD.alloc(u.wtotal * sizeof(double2), "dr accumulators"));
+  float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>();
+  double2 *as = AS.as<double2>(), *ad = AD.as<double2>();
+  HIPCHK(launch_zero2d(as, (long long)(2 * u.wtotal), (long long)(2 * u.wtotal), 1, s), "zero");
+  HIPCHK(launch_zero2d(ad, (long long)(2 * u.wtotal), (long long)(2 * u.wtotal), 1, s), "zero");
+  std::vector<int> src(K), rlo(K), rhi(K);
+  for (int k = 0; k < K; ++k) {
+    src[k] = u.S(k).src_iz;
+    u.rec_depths(k, rlo[k], rhi[k]);
+  }
+  const std::vector<int> slo = u.row_vec(src), rl = u.row_vec(rlo), rh = u.row_vec(rhi);
+  // D = Down(inj_src); R = Up^H(R^H d) with Q = conj(r) R on every row
+  for (int k = 0; k < K; ++k) RCCHK(u.zero_depths(bg, k, src[k], src[k]));
+  RCCHK(u.inject_sources(bg));
+  RCCHK(u.sweep(0, 0, bg, bg, &slo, &slo));
+  for (int k = 0; k < K; ++k) RCCHK(u.zero_depths(R, k, rlo[k], rhi[k]));
+  RCCHK(u.inject_receivers(R));
+  RCCHK(u.sweep(1, 1, R, R, &rl, &rh, false, Q));
+  // dr += conj(D) * R, per shot over its rows (runs from its first row)
+  for (int k = 0; k < K; ++k) {
+    if (u.nw(k) == 0) continue;
+    HIPCHK(launch_reduce_rows(ad + u.woff[k], bg + u.base(k), (long long)u.nzx(k), (long long)u.nzx(k), u.nw(k), 0,
+                              true, s, R + u.base(k)),
+           "reduce");
+  }
+  // Q = Down^H(conj(r) R + R^H d)
+  RCCHK(u.inject_receivers(Q));
+  RCCHK(u.sweep(0, 1, Q, Q));
+  // LinDown^H stores rows 0..nz-2 of cs; row nz-1 starts at 0; each D row it
+  // reads becomes r * D
+  const size_t last = (size_t)(nz - 1);
+  for (int k = 0; k < K; ++k)
+    if (u.nw(k))
+      HIPCHK(launch_zero2d(cs + u.base(k) + last * u.nx(k), (long long)u.nzx(k), u.nx(k), u.nw(k), s), "zero rows");
+  RCCHK(u.lin(+1, 1, exact, bg, u.per_row(Q), cs, true, true));
+  for (int k = 0; k < K; ++k) {
+    if (u.nw(k) == 0) continue;
+    std::vector<MulRow> rbg;
+    const size_t lo = last * u.nx(k);
+    for (int r = u.first[k]; r < u.first[k + 1]; ++r)
+      rbg.push_back(MulRow{bg + u.roff[r] + lo, u.S(k).model->m.d_refl + lo, bg + u.roff[r] + lo});
+    RCCHK(u.mul_rows(rbg, (size_t)u.nx(k), false));
+  }
+  // U = Up(r D); dS += LinUp^H(R; U)
+  RCCHK(u.sweep(1, 0, bg, bg));
+  RCCHK(u.lin(-1, 1, exact, bg, u.per_row(R), cs));
+  size_t wmax = 0;
+  for (int k = 0; k < K; ++k) wmax = std::max(wmax, u.nzx(k));
+  RCCHK(FS.alloc(wmax * sizeof(float2), "tmp"));
+  RCCHK(FR.alloc(wmax * sizeof(float2), "tmp"));
+  RCCHK(FD.alloc(wmax * sizeof(float2), "tmp"));
+  for (int k = 0; k < K; ++k) {
+    if (u.nw(k) == 0) continue;
+    const Model& m = u.S(k).model->m;
+    const long long n = (long long)u.nzx(k);
+    HIPCHK(launch_reduce_rows(as + u.woff[k], cs + u.base(k), n, n, u.nw(k), 0, true, s), "reduce");
+    // dReflect^H of the shot's dr, then its slice into the sums over shots
+    HIPCHK(launch_d2f(FS.as<float2>(), as + u.woff[k], n, false, s), "d2f");
+    HIPCHK(launch_d2f(FD.as<float2>(), ad + u.woff[k], n, false, s), "d2f");
+    HIPCHK(launch_zero2d(FR.p, n, n, 1, s), "zero");
+    HIPCHK(launch_dreflect_adj(FS.as<float2>(), FR.as<float2>(), m.d_drefl, FD.as<float2>(), nz, u.nx(k), s),
+           "dreflect adjoint");
+    HIPCHK(launch_window_add(gS, FS.as<float2>(), nz, u.nx(k), nxg, u.S(k).x0, s), "window add");
+    HIPCHK(launch_window_add(gR, FR.as<float2>(), nz, u.nx(k), nxg, u.S(k).x0, s), "window add");
+  }
+  HIPCHK(hipStreamSynchronize(s), "uneven born adjoint");
+  return WEMFWI_OK;
+}
+
+// Validation and grouping of both entry points (gS / gR null: the forward)
+static int uneven_born(const char* name, const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nxg,
+                       const float2* dS, const float2* dR, float2* gS, float2* gR, int add, int exact,
+                       hipStream_t s) {
+  const std::string who = name;
+  const bool fwd = gS == nullptr;
+  if ((!shots && ns > 0) || ns < 0 || ngs < 0 || ngr < 0 || nz < 2 || nxg < 1)
+    return fail(WEMFWI_ERR_ARG, who + ": invalid argument");
+  if (fwd ? (!dS || !dR) : (!gS || !gR)) return fail(WEMFWI_ERR_ARG, who + ": null perturbation");
+  // validate every shot before anything is queued
+  for (int sh = 0; sh < ns; ++sh) {
+    const wemfwi_born_shot& S = shots[sh];
+    const std::string at = who + ": shot " + std::to_string(sh);
+    if (!S.model || S.nw < 0 || (S.nw > 0 && (!S.freq_hz || !S.wavelet || !S.data)) || S.nr < 0 ||
+        (S.nr > 0 && (!S.rec_iz || !S.rec_ix)))
+      return fail(WEMFWI_ERR_ARG, at + ": null argument");
+    const Model& m = S.model->m;
+    if (m.d.nz != nz) return fail(WEMFWI_ERR_ARG, at + ": model nz differs");
+    if (S.src_iz < 0 || S.src_iz >= nz) return fail(WEMFWI_ERR_ARG, at + ": source depth outside the model");
+    for (int r = 0; r < S.nr; ++r)
+      if (S.rec_iz[r] < 0 || S.rec_iz[r] >= nz) return fail(WEMFWI_ERR_ARG, at + ": receiver depth outside the model");
+    if (S.nw > 0) RCCHK(m.require_fft());
+  }
+  size_t budget = 0;
+  {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr == 0) fr = (size_t)8 << 30;
+    budget = fr / 2;
+  }
+  DevBuf GS, GR;
+  const size_t npix = (size_t)nz * nxg;
+  if (fwd) {
+    for (int sh = 0; sh < ns; ++sh)
+      if (shots[sh].nw > 0 && shots[sh].nr > 0 && !add)
+        HIPCHK(launch_zero2d(shots[sh].data, (long long)shots[sh].nr * shots[sh].nw,
+                             (long long)shots[sh].nr * shots[sh].nw, 1, s),
+               "zero data");
+  } else {
+    RCCHK(GS.alloc(npix * sizeof(double2), "gradient accumulator"));
+    RCCHK(GR.alloc(npix * sizeof(double2), "gradient accumulator"));
+    HIPCHK(launch_zero2d(GS.p, (long long)(2 * npix), (long long)(2 * npix), 1, s), "zero");
+    HIPCHK(launch_zero2d(GR.p, (long long)(2 * npix), (long long)(2 * npix), 1, s), "zero");
+  }
+  // groups of consecutive shots whose wavefields fit half the free memory
+  const size_t nbuf = fwd ? 3 : 4;
+  int s0 = 0;
+  while (s0 < ns) {
+    int s1 = s0;
+    size_t bytes = 0;
+    while (s1 < ns) {
+      const Model& m = shots[s1].model->m;
+      const size_t b = (nbuf * (size_t)shots[s1].nw * sizeof(float2) + 4 * sizeof(double2)) * m.d.nz * m.d.nx;
+      if (s1 > s0 && bytes + b > budget) break;
+      bytes += b;
+      ++s1;
+    }
+    UnevenBorn u;
+    RCCHK(u.init(shots, s0, s1, nz, ngs, ngr, s));
+    if (fwd) RCCHK(uneven_born_forward_group(u, dS, dR, nxg));
+    else RCCHK(uneven_born_adjoint_group(u, GS.as<double2>(), GR.as<double2>(), nxg, exact));
+    s0 = s1;
+  }
+  if (!fwd) {
+    HIPCHK(launch_d2f(gS, GS.as<double2>(), (long long)npix, add != 0, s), "gradient");
+    HIPCHK(launch_d2f(gR, GR.as<double2>(), (long long)npix, add != 0, s), "gradient");
+  }
+  HIPCHK(hipStreamSynchronize(s), name);
+  return WEMFWI_OK;
+}
+
 }  // namespace wemfwi
 
 // ------------------------------------------------------------------ C ABI
@@ -1480,6 +1947,19 @@ int wemfwi_gather_sum(long long nout, const long long* ptr, const long long* src
   HIPCHK(launch_gather_sum((float2*)out, nout, ptr, src, (const float2*)in, add != 0, s), "gather_sum");
   HIPCHK(hipStreamSynchronize(s), "gather_sum");
   return WEMFWI_OK;
+}
+
+int wemfwi_uneven_born_forward(const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
+                               const wemfwi_complex* dslow, const wemfwi_complex* dden, int add, void* stream) {
+  return uneven_born("uneven_born_forward", shots, ns, ngs, ngr, nz, nx_model, (const float2*)dslow,
+                     (const float2*)dden, nullptr, nullptr, add, 0, (hipStream_t)stream);
+}
+
+int wemfwi_uneven_born_adjoint(const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
+                               wemfwi_complex* dslow, wemfwi_complex* dden, int add, int mode, void* stream) {
+  if (mode != 0 && mode != 1) return fail(WEMFWI_ERR_ARG, "uneven_born_adjoint: mode must be 0 or 1");
+  return uneven_born("uneven_born_adjoint", shots, ns, ngs, ngr, nz, nx_model, nullptr, nullptr, (float2*)dslow,
+                     (float2*)dden, add, mode, (hipStream_t)stream);
 }
 
 int wemfwi_bornrefl_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* image,

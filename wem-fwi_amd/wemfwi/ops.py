@@ -306,6 +306,52 @@ def uneven_offset_gradient(shots: Sequence[ImageShot], nh: int, model: torch.Ten
     return float(j.value)
 
 
+class BornShot(ImageShot):
+    """One shot of the uneven Born operators: its background `model` on its
+    aperture (nz x nx_s), placed at column x0 of the perturbation grid; its
+    frequencies and source spectrum; its source point and receivers on the
+    aperture grid; its data, a CUDA complex64 tensor [nr][nw] (written by
+    uneven_born_forward, read by uneven_born_adjoint)."""
+
+
+def _born_shots(shots: Sequence[BornShot]):
+    return (ImageShotDesc * max(1, len(shots)))(*[sh.desc() for sh in shots])
+
+
+def uneven_born_forward(shots: Sequence[BornShot], dslow: torch.Tensor, dden: torch.Tensor, ngs: int = 0,
+                        ngr: int = 0, add: bool = False, stream=None):
+    """BornFull::forward of shots with their own apertures and bands
+    (BornFull.cpp:64-117 per shot): every shot's data [nr][nw] receives the
+    scattered field of the perturbations dslow, dden [nz][nx_model] (CUDA
+    complex64) windowed to its aperture; all rows swept in batched
+    launches."""
+    _dev_c64(dslow, "dslow")
+    _dev_c64(dden, "dden")
+    if dslow.dim() != 2 or tuple(dden.shape) != tuple(dslow.shape):
+        raise ValueError("dslow and dden must both be [nz][nx_model]")
+    arr = _born_shots(shots)
+    check(lib.wemfwi_uneven_born_forward(C.cast(arr, C.c_void_p), len(shots), int(ngs), int(ngr),
+                                         int(dslow.shape[0]), int(dslow.shape[1]), _ptr(dslow), _ptr(dden),
+                                         int(add), _stream_ptr(stream)))
+    return [sh.data for sh in shots]
+
+
+def uneven_born_adjoint(shots: Sequence[BornShot], dslow: torch.Tensor, dden: torch.Tensor, ngs: int = 0,
+                        ngr: int = 0, add: bool = False, exact: bool = False, stream=None):
+    """BornFull::adjoint of the same shots: the gradient slices of every
+    shot's data, summed over shots in shot order in FP64 into dslow, dden
+    [nz][nx_model]; exact selects the exact Scatter adjoint (as born_adjoint)."""
+    _dev_c64(dslow, "dslow")
+    _dev_c64(dden, "dden")
+    if dslow.dim() != 2 or tuple(dden.shape) != tuple(dslow.shape):
+        raise ValueError("dslow and dden must both be [nz][nx_model]")
+    arr = _born_shots(shots)
+    check(lib.wemfwi_uneven_born_adjoint(C.cast(arr, C.c_void_p), len(shots), int(ngs), int(ngr),
+                                         int(dslow.shape[0]), int(dslow.shape[1]), _ptr(dslow), _ptr(dden),
+                                         int(add), int(exact), _stream_ptr(stream)))
+    return dslow, dden
+
+
 def shotwem_forward(models: Sequence[Model], survey: Survey, data: torch.Tensor, add: bool = False,
                     stream=None):
     """WEM with one background model per shot (conformal-map grids)."""
