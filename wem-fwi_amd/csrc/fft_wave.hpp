@@ -389,6 +389,24 @@ __host__ __device__ constexpr bool lay_reads_split() {
 
 // ---------------------------------------------------------------- passes
 
+// Copies the N samples of a whole row from global memory into buf[0..N) in
+// natural order by LDS DMA (buffer_load_dwordx4 ... lds: 64 lanes x 16 bytes
+// per instruction; one lane offset t * 16 and a scalar offset per
+// instruction, no VGPRs held for the data).  Marked non-temporal: each row
+// is read once per sweep.  The wave must not access buf[0..N) until the
+// copy has arrived (the compiler waits for it, vmcnt, before the next LDS
+// read of buf).
+template <int N>
+__device__ __forceinline__ void row_to_lds(const float2* src, float2* buf, int t) {
+  static_assert(N % 128 == 0, "a DMA instruction moves 128 samples");
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(src), (short)0, N * 8, 0x00020000);
+#pragma unroll
+  for (int i = 0; i < N / 128; ++i)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(buf + 128 * i), 16,
+                                             t * 16, i * 1024, 0, 2);
+}
+
 __device__ __forceinline__ void wave_lds_order() {
   // LDS instructions of one wavefront execute in order; this keeps the
   // compiler from moving LDS accesses across a pass boundary.
@@ -398,9 +416,13 @@ __device__ __forceinline__ void wave_lds_order() {
 
 // Pass p of the plan.  FROM_LDS: inputs come from LDS (else from x in the
 // strided natural layout); TO_LDS: outputs go to LDS (else to x).
+// pre (last pass only, or NULL): a whole row whose copy into buf[0..N) by
+// LDS DMA is started as soon as this pass's reads of buf have completed, so
+// that it overlaps the pass's arithmetic and whatever follows the transform.
 template <int N, int K, int p, bool INV, bool FROM_LDS, bool TO_LDS>
 __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __restrict__ buf,
-                                              const float2* __restrict__ tw, int t) {
+                                              const float2* __restrict__ tw, int t,
+                                              const float2* pre = nullptr) {
   constexpr int E = N / 64;
   constexpr int R = FftPlan<N, K>::R[p];
   constexpr int NS = plan_ns<N, K>(p);
@@ -418,6 +440,13 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
 #pragma unroll
       for (int r = 0; r < R; ++r) x[b + r * BPL] = base[lay<N, K, p - 1>(64 * b + r * NB)];
     wave_lds_order();
+    if constexpr (!TO_LDS) {
+      if (pre) {
+        // the reads above must have returned before the DMA overwrites buf
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        row_to_lds<N>(pre, buf, t);
+      }
+    }
   }
   // twiddle base: entry (r-1)*NS + (j & (NS-1)), j = t + 64 b (pairs: two
   // float2 per entry)
@@ -486,19 +515,24 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
 }
 
 template <int N, int K, bool INV, int p>
-__device__ __forceinline__ void fft_passes(float2 (&x)[N / 64], float2* buf, const float2* tw, int t) {
+__device__ __forceinline__ void fft_passes(float2 (&x)[N / 64], float2* buf, const float2* tw, int t,
+                                           const float2* pre) {
   constexpr int P = FftPlan<N, K>::P;
+  static_assert(P >= 2, "the last pass reads buf");
   if constexpr (p < P) {
-    stockham_pass<N, K, p, INV, (p > 0), (p < P - 1)>(x, buf, tw, t);
-    fft_passes<N, K, INV, p + 1>(x, buf, tw, t);
+    stockham_pass<N, K, p, INV, (p > 0), (p < P - 1)>(x, buf, tw, t, p == P - 1 ? pre : nullptr);
+    fft_passes<N, K, INV, p + 1>(x, buf, tw, t, pre);
   }
 }
 
 // Unnormalised DFT (forward: exp(-2 pi i jk/N); inverse: exp(+2 pi i jk/N))
-// of the row held in x (strided natural layout in and out).
+// of the row held in x (strided natural layout in and out).  pre: a whole
+// row of N samples copied into buf[0..N) by LDS DMA during the last pass
+// (NULL: none).
 template <int N, bool INV, int K = 0>
-__device__ __forceinline__ void fft_row(float2 (&x)[N / 64], float2* buf, const float2* tw, int t) {
-  fft_passes<N, K, INV, 0>(x, buf, tw, t);
+__device__ __forceinline__ void fft_row(float2 (&x)[N / 64], float2* buf, const float2* tw, int t,
+                                        const float2* pre = nullptr) {
+  fft_passes<N, K, INV, 0>(x, buf, tw, t, pre);
 }
 
 // ---------------------------------------------------------------- any length

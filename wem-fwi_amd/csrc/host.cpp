@@ -562,6 +562,7 @@ StepGeom Model::geom() const {
   g.ssf = d.pspi ? 0 : 1;
   g.full = (d.tap == 0 && d.nx == N) ? 1 : 0;
   g.fg = fg;
+  g.pf = 0;
   g.ntaylor = d.ntaylor;
   g.taper = d.taper > 0 ? 1 : 0;
   return g;
@@ -696,7 +697,9 @@ int Model::oneway_on(int up, int adjoint, int nrows, const float* freq_hz, const
   {
     KernelTimer kt("oneway_kernel");
     kt.begin(s);
-    e = launch_oneway(N, drows, nrows, d_tables, geom(), d.nz, step, adjoint != 0, d_tw, s, !fixed);
+    StepGeom g = geom();
+    g.pf = 1;  // every row reads its source row at every depth
+    e = launch_oneway(N, drows, nrows, d_tables, g, d.nz, step, adjoint != 0, d_tw, s, !fixed);
     kt.end(s);
     if (e != hipSuccess) return hip_fail(e, "oneway kernel");
   }
@@ -800,7 +803,11 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     int k1 = k0 + 1;
     while (k1 < nrows && key[order[k1]] == key[order[k0]]) ++k1;
     const Model* mg = ms[order[k0]];
-    groups.push_back(Group{mg->N, k0, k1 - k0, mg->geom(), mg->d.nz, mg->d_tw});
+    StepGeom g = mg->geom();
+    // every row reads a plain source row at every depth: the full-row
+    // kernels may prefetch it
+    g.pf = (!(src_lo && src_hi) && !(prod && prod->smul)) ? 1 : 0;
+    groups.push_back(Group{mg->N, k0, k1 - k0, g, mg->d.nz, mg->d_tw});
     k0 = k1;
   }
   std::vector<DepthTables> ht;

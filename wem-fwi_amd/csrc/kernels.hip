@@ -95,7 +95,7 @@ struct Occupancy {
   static constexpr int value = N <= 256 ? 8 : (N == 512 ? 6 : (N == 1024 ? 4 : 2));
 };
 
-template <class V, bool ADJ>
+template <class V, bool ADJ, bool PF>
 __global__ void __launch_bounds__(64 * WavesV<V>::value)
     __attribute__((amdgpu_waves_per_eu(V::GEN ? 2 : Occupancy<V::n>::value, 8)))
     oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
@@ -144,16 +144,39 @@ __global__ void __launch_bounds__(64 * WavesV<V>::value)
       row_store<V>(R.out2 + (size_t)(first - zb) * nx, cmulc(st[e], row_sample<V>(R.omul + (size_t)first * nx, g, t, e)),
                    g, t, e);
   }
+  // PF (full rows whose every depth has a plain source row; the launch
+  // guarantees it): the source row is read by LDS DMA into the wave's
+  // transform buffer, started once the step's last inverse transform has
+  // read that buffer for the last time (in its last pass, or right after the
+  // split step's table loads), so that the row's HBM latency overlaps the
+  // rest of the step's arithmetic instead of stalling the source add.
+  constexpr bool PFX = PF && V::FULL;
   for (int k = 0; k < nsteps; ++k) {
     const int cur = first + k * step;
     const int nxt = cur + step;
     if (!recursive && k > 0) load_row<V>(st, R.in + (size_t)(cur - zb) * nx, g, t);
-    if constexpr (ADJ) step_adjoint<V, true>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
-    else step_forward<V, true>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch);
-    if (!R.src_skip || (nxt >= R.src_lo && nxt <= R.src_hi)) {
+    const bool src_here = PFX || !R.src_skip || (nxt >= R.src_lo && nxt <= R.src_hi);
+    const float2* pre = nullptr;
+    if constexpr (PFX) {
+      pre = R.in + (size_t)(nxt - zb) * nx;
+      __builtin_assume(pre != nullptr);
+    }
+    bool in_lds;
+    if constexpr (ADJ) in_lds = step_adjoint<V, true>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
+    else in_lds = step_forward<V, true>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
+    if (src_here) {
       float2 src[E];
-      load_row<V>(src, R.in + (size_t)(nxt - zb) * nx, g, t);
-      if (R.smul) {
+      if (PFX && in_lds) {
+        // the DMA writes LDS behind the memory counter: wait for it
+        // explicitly (the compiler does not track the copy through the
+        // generic buffer pointer)
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int e = 0; e < E; ++e) src[e] = buf[t + 64 * e];
+      } else {
+        load_row<V>(src, R.in + (size_t)(nxt - zb) * nx, g, t);
+      }
+      if (!PFX && R.smul) {
 #pragma unroll
         for (int e = 0; e < E; ++e) src[e] = cmul(src[e], row_sample<V>(R.smul + (size_t)nxt * nx, g, t, e));
       }
@@ -277,10 +300,21 @@ static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTa
                                   int step, bool adj, const float2* tw, hipStream_t s) {
   constexpr int kWavesPerBlock = WavesV<V>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+  // the source-row prefetch only on full rows, when the caller states that
+  // every row reads a plain source row at every depth (g.pf)
+  if constexpr (V::FULL) {
+    if (g.pf) {
+      if (adj)
+        hipLaunchKernelGGL((oneway_kernel<V, true, true>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+      else
+        hipLaunchKernelGGL((oneway_kernel<V, false, true>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+      return hipGetLastError();
+    }
+  }
   if (adj)
-    hipLaunchKernelGGL((oneway_kernel<V, true>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, true, false>), grid, block, 0, s, rows, nrows, models, g, step, tw);
   else
-    hipLaunchKernelGGL((oneway_kernel<V, false>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, false, false>), grid, block, 0, s, rows, nrows, models, g, step, tw);
   return hipGetLastError();
 }
 

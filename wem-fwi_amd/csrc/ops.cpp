@@ -399,8 +399,10 @@ struct Ctx {
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
     RCCHK(put(d_rows, hsw, s, "sweep rows"));
     const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
-    HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, step,
-                         adjoint != 0, m0->d_tw, s),
+    StepGeom g = m0->geom();
+    g.pf = (!src_rows && !smul) ? 1 : 0;  // plain source rows at every depth
+    HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), g, nz, step, adjoint != 0,
+                         m0->d_tw, s),
            "oneway kernel");
     return sync();
   }
@@ -462,8 +464,10 @@ struct Ctx {
     for (int i = 0; i < n; ++i) hsw[i].scratch = scr.as<float2>() + (size_t)i * N;
     RCCHK(put(d_rows, hsw, s, "sweep rows"));
     const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
-    HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), m0->geom(), nz, step,
-                         adjoint != 0, m0->d_tw, s),
+    StepGeom g = m0->geom();
+    g.pf = src_rows ? 0 : 1;  // plain source rows at every depth
+    HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), g, nz, step, adjoint != 0,
+                         m0->d_tw, s),
            "oneway kernel");
     return WEMFWI_OK;
   }

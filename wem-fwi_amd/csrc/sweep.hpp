@@ -141,6 +141,9 @@ struct StepGeom {
   int ntaylor;  // Taylor order of the scatterer
   int taper;    // 1: a taper is applied (width > 0)
   FftGen fg;    // FFT length and pass plan when it is not the launch's N
+  int pf;       // sweeps: 1 when every row of the launch reads a plain source
+                // row at every depth (no src_skip, no smul): full rows then
+                // prefetch it by LDS DMA
 };
 
 // Compile-time variants of the step: FULL (no padding samples), SSF
@@ -168,11 +171,13 @@ struct TwV {
   static constexpr int value = V::GEN ? gen_tw_cap(V::n) : plan_tw_size<V::n, V::K>();
 };
 
-// the row transform of a variant
+// the row transform of a variant; pre (register-planned lengths only): a
+// whole row copied into buf[0..N) by LDS DMA during the last pass
 template <class V, bool INV>
-__device__ __forceinline__ void fft_v(float2 (&x)[V::E], float2* buf, const float2* tw, int t, const StepGeom& g) {
+__device__ __forceinline__ void fft_v(float2 (&x)[V::E], float2* buf, const float2* tw, int t, const StepGeom& g,
+                                      const float2* pre = nullptr) {
   if constexpr (V::GEN) fft_row_gen<V::n, INV>(x, buf, tw, t, g.fg);
-  else fft_row<V::n, INV, V::K>(x, buf, tw, t);
+  else fft_row<V::n, INV, V::K>(x, buf, tw, t, pre);
 }
 
 // sin and cos of x for |x| < 1e5: Cody-Waite reduction by a three-part pi/2
@@ -238,10 +243,13 @@ __device__ __forceinline__ void row_store(float2* p, float2 v, const StepGeom& g
 }
 
 // taper * exp(-i w phi) of the E samples of this lane (0 on padding samples
-// when a taper or the split step is on; 1 otherwise)
+// when a taper or the split step is on; 1 otherwise).  pre (or NULL): a whole
+// row copied into buf[0..N) by LDS DMA, started right after this depth's
+// table loads have been issued (the memory counter is in order: a DMA issued
+// first would hold up the table loads behind its HBM round trip).
 template <class V>
 __device__ __forceinline__ void ss_taper_row(float2 (&f)[V::E], const DepthTables& dt, const StepGeom& g, int d,
-                                             float w, int t) {
+                                             float w, int t, const float2* pre = nullptr, float2* buf = nullptr) {
   constexpr int E = V::E;
   float tv[E];
 #pragma unroll
@@ -251,6 +259,15 @@ __device__ __forceinline__ void ss_taper_row(float2 (&f)[V::E], const DepthTable
 #pragma unroll
     for (int e = 0; e < E; ++e) tv[e] = __builtin_bit_cast(float, bld1(r, (unsigned)(t * E + e) * 4u, 0u));
   }
+  auto start_dma = [&]() {
+    if constexpr (!V::GEN) {
+      if (pre) {
+        __builtin_amdgcn_sched_barrier(0);
+        row_to_lds<V::n>(pre, buf, t);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  };
   if constexpr (V::SSF) {
     const brsrc r = make_rsrc(dt.phi_lane + (size_t)d * V::n, (unsigned)V::n * 8u);
     // exp(-i w (pr + i pi)) = exp(w pi) (cos(w pr) - i sin(w pr)).  Two
@@ -260,33 +277,42 @@ __device__ __forceinline__ void ss_taper_row(float2 (&f)[V::E], const DepthTable
     // (amplitude 1)
     const bool small = w * __builtin_bit_cast(float, ldu((const int*)dt.phi_max, d)) <= 0.78f;
     if (!g.taper && dt.phi_real) {
+      // the real parts only
+      float px[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) px[e] = __builtin_bit_cast(float, bld1(r, (unsigned)(t * E + e) * 8u, 0u));
+      start_dma();
       if (small) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           float sn, cs;
-          sincos_small(w * bld2(r, (unsigned)(t * E + e) * 8u, 0u).x, &sn, &cs);
+          sincos_small(w * px[e], &sn, &cs);
           f[e] = make_float2(cs, -sn);
         }
       } else {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           float sn, cs;
-          sincos_cw(w * bld2(r, (unsigned)(t * E + e) * 8u, 0u).x, &sn, &cs);
+          sincos_cw(w * px[e], &sn, &cs);
           f[e] = make_float2(cs, -sn);
         }
       }
     } else {
+      float2 pv[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) pv[e] = bld2(r, (unsigned)(t * E + e) * 8u, 0u);
+      start_dma();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const float2 pv = bld2(r, (unsigned)(t * E + e) * 8u, 0u);
         float sn, cs;
-        if (small) sincos_small(w * pv.x, &sn, &cs);
-        else sincos_cw(w * pv.x, &sn, &cs);
-        const float a = tv[e] * exp2f(w * pv.y * 1.4426950408889634f);
+        if (small) sincos_small(w * pv[e].x, &sn, &cs);
+        else sincos_cw(w * pv[e].x, &sn, &cs);
+        const float a = tv[e] * exp2f(w * pv[e].y * 1.4426950408889634f);
         f[e] = make_float2(a * cs, -a * sn);
       }
     }
   } else {
+    start_dma();
 #pragma unroll
     for (int e = 0; e < E; ++e) f[e] = make_float2(tv[e], 0.f);
   }
@@ -352,22 +378,25 @@ __device__ __forceinline__ void load_samples(float2 (&v)[V::E], const float2* p,
 // them across an FFT costs more registers (and waves per SIMD) than their
 // latency costs.
 template <class V>
-__device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+__device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                                 const float2* tw, int t, float2* scr);
+                                                 const float2* tw, int t, float2* scr, const float2* pre);
 
 //
 // SCR: one code path for every depth (step_forward_scr): at depths with
 // several active references the spectrum is parked in the wave's scratch row
 // (global memory, written and re-read by the same lane) instead of registers,
 // which keeps the step within 128 VGPRs.
+// pre (SCR only; FULL rows of register-planned lengths): a whole row copied
+// into buf[0..N) by LDS DMA during the step's last inverse transform;
+// returns whether that copy was started (buf then holds it once it arrives)
 template <class V, bool SCR = false>
-__device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+__device__ __forceinline__ bool step_forward(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                              const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                             const float2* tw, int t, float2* scr = nullptr) {
+                                             const float2* tw, int t, float2* scr = nullptr,
+                                             const float2* pre = nullptr) {
   if constexpr (SCR) {
-    step_forward_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr);
-    return;
+    return step_forward_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr, pre);
   }
   constexpr int N = V::n;
   constexpr int E = V::E;
@@ -415,22 +444,23 @@ __device__ __forceinline__ void step_forward(float2 (&st)[V::E], int d, float w,
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmul(st[e], f[e]);
   }
+  return false;
 }
 
 // out = P_d^H(in); SCR as in step_forward (the split-step-corrected input row
 // is parked in the scratch row at depths with several references)
 template <class V>
-__device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+__device__ __forceinline__ bool step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                                 const float2* tw, int t, float2* scr);
+                                                 const float2* tw, int t, float2* scr, const float2* pre);
 
 template <class V, bool SCR = false>
-__device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+__device__ __forceinline__ bool step_adjoint(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                              const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                             const float2* tw, int t, float2* scr = nullptr) {
+                                             const float2* tw, int t, float2* scr = nullptr,
+                                             const float2* pre = nullptr) {
   if constexpr (SCR) {
-    step_adjoint_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr);
-    return;
+    return step_adjoint_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr, pre);
   }
   constexpr int N = V::n;
   constexpr int E = V::E;
@@ -481,6 +511,7 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w,
     for (int e = 0; e < E; ++e)
       if (!in_row(t + 64 * e - g.tap, g.nx)) st[e] = make_float2(0.f, 0.f);
   }
+  return false;
 }
 
 // One loop over the active references of the depth for both the single-
@@ -488,18 +519,19 @@ __device__ __forceinline__ void step_adjoint(float2 (&st)[V::E], int d, float w,
 // register allocator more than either path alone).
 //   forward: X = F in; for each reference r: Y = F^H (P_r X), out[x in r] = Y[x]
 template <class V>
-__device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+__device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                                 const float2* tw, int t, float2* scr) {
+                                                 const float2* tw, int t, float2* scr, const float2* pre) {
   constexpr int N = V::n;
   constexpr int E = V::E;
+  if constexpr (V::GEN) pre = nullptr;  // the runtime-planned transform has no DMA hook
   const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
   if (s0 == s1) {
     // no reference selects any x of this depth: Selector leaves the row zero
     // (RefSampler's stale buffer can leave a depth without a live reference)
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
-    return;
+    return false;
   }
   fft_v<V, false>(st, buf, tw, t, g);
   const bool multi = s1 - s0 > 1;
@@ -531,7 +563,9 @@ __device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, floa
 #pragma unroll
       for (int e = E / 2; e < E; ++e) Y[e] = cmul(st[e], ptab_v<V>(P, t, e, g));
     }
-    fft_v<V, true>(Y, buf, tw, t, g);
+    // the last inverse transform of the depth starts the DMA of pre, unless
+    // the split step / taper follows (it starts the DMA after its table loads)
+    fft_v<V, true>(Y, buf, tw, t, g, (s == s1 - 1 && !(V::SSF || g.taper)) ? pre : nullptr);
     if (all) {
 #pragma unroll
       for (int e = 0; e < E; ++e) st[e] = Y[e];
@@ -544,24 +578,26 @@ __device__ __forceinline__ void step_forward_scr(float2 (&st)[V::E], int d, floa
   }
   if (V::SSF || g.taper) {
     float2 f[E];
-    ss_taper_row<V>(f, dt, g, d, w, t);
+    ss_taper_row<V>(f, dt, g, d, w, t, pre, buf);
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = cmul(st[e], f[e]);
   }
+  return pre != nullptr;
 }
 
 //   adjoint: u = conj(f) in; acc = sum_r conj(P_r) F (u restricted to r); out = F^H acc
 template <class V>
-__device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
+__device__ __forceinline__ bool step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
-                                                 const float2* tw, int t, float2* scr) {
+                                                 const float2* tw, int t, float2* scr, const float2* pre) {
   constexpr int N = V::n;
   constexpr int E = V::E;
+  if constexpr (V::GEN) pre = nullptr;  // the runtime-planned transform has no DMA hook
   const int s0 = ldu(dt.slot_begin, d), s1 = ldu(dt.slot_begin, d + 1);
   if (s0 == s1) {  // the adjoint of a zero row
 #pragma unroll
     for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
-    return;
+    return false;
   }
   if (V::SSF || g.taper) {
     float2 f[E];
@@ -608,12 +644,13 @@ __device__ __forceinline__ void step_adjoint_scr(float2 (&st)[V::E], int d, floa
       st[e] = first ? z : cadd(st[e], z);
     }
   }
-  fft_v<V, true>(st, buf, tw, t, g);
+  fft_v<V, true>(st, buf, tw, t, g, pre);
   if constexpr (!V::FULL) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
       if (!in_row(t + 64 * e - g.tap, g.nx)) st[e] = make_float2(0.f, 0.f);
   }
+  return pre != nullptr;
 }
 
 // scatterer coefficient C_it of this lane's sample e (0 on padding samples);
