@@ -396,15 +396,18 @@ __host__ __device__ constexpr bool lay_reads_split() {
 // is read once per sweep.  The wave must not access buf[0..N) until the
 // copy has arrived (the compiler waits for it, vmcnt, before the next LDS
 // read of buf).
-template <int N>
+// N samples (a row) or, with M entries, a table of M complex values (the
+// range check of the buffer descriptor leaves the rest of the last 1 KB
+// block zero).  AUX: cache policy (2: non-temporal).
+template <int N, int M = N, int AUX = 2>
 __device__ __forceinline__ void row_to_lds(const float2* src, float2* buf, int t) {
-  static_assert(N % 128 == 0, "a DMA instruction moves 128 samples");
+  constexpr int BLOCKS = (M + 127) / 128;
   const __amdgpu_buffer_rsrc_t r =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(src), (short)0, N * 8, 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(src), (short)0, M * 8, 0x00020000);
 #pragma unroll
-  for (int i = 0; i < N / 128; ++i)
+  for (int i = 0; i < BLOCKS; ++i)
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(buf + 128 * i), 16,
-                                             t * 16, i * 1024, 0, 2);
+                                             t * 16, i * 1024, 0, AUX);
 }
 
 __device__ __forceinline__ void wave_lds_order() {
@@ -418,11 +421,12 @@ __device__ __forceinline__ void wave_lds_order() {
 // strided natural layout); TO_LDS: outputs go to LDS (else to x).
 // pre (last pass only, or NULL): a whole row whose copy into buf[0..N) by
 // LDS DMA is started as soon as this pass's reads of buf have completed, so
-// that it overlaps the pass's arithmetic and whatever follows the transform.
+// that it overlaps the pass's arithmetic and whatever follows the transform;
+// tab likewise, a phase table of N/2 + 1 entries (read from L2, cached).
 template <int N, int K, int p, bool INV, bool FROM_LDS, bool TO_LDS>
 __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __restrict__ buf,
                                               const float2* __restrict__ tw, int t,
-                                              const float2* pre = nullptr) {
+                                              const float2* pre = nullptr, const float2* tab = nullptr) {
   constexpr int E = N / 64;
   constexpr int R = FftPlan<N, K>::R[p];
   constexpr int NS = plan_ns<N, K>(p);
@@ -441,10 +445,11 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
       for (int r = 0; r < R; ++r) x[b + r * BPL] = base[lay<N, K, p - 1>(64 * b + r * NB)];
     wave_lds_order();
     if constexpr (!TO_LDS) {
-      if (pre) {
+      if (pre || tab) {
         // the reads above must have returned before the DMA overwrites buf
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        row_to_lds<N>(pre, buf, t);
+        if (pre) row_to_lds<N>(pre, buf, t);
+        else row_to_lds<N, N / 2 + 1, 0>(tab, buf, t);
       }
     }
   }
@@ -516,23 +521,24 @@ __device__ __forceinline__ void stockham_pass(float2 (&x)[N / 64], float2* __res
 
 template <int N, int K, bool INV, int p>
 __device__ __forceinline__ void fft_passes(float2 (&x)[N / 64], float2* buf, const float2* tw, int t,
-                                           const float2* pre) {
+                                           const float2* pre, const float2* tab) {
   constexpr int P = FftPlan<N, K>::P;
   static_assert(P >= 2, "the last pass reads buf");
   if constexpr (p < P) {
-    stockham_pass<N, K, p, INV, (p > 0), (p < P - 1)>(x, buf, tw, t, p == P - 1 ? pre : nullptr);
-    fft_passes<N, K, INV, p + 1>(x, buf, tw, t, pre);
+    stockham_pass<N, K, p, INV, (p > 0), (p < P - 1)>(x, buf, tw, t, p == P - 1 ? pre : nullptr,
+                                                        p == P - 1 ? tab : nullptr);
+    fft_passes<N, K, INV, p + 1>(x, buf, tw, t, pre, tab);
   }
 }
 
 // Unnormalised DFT (forward: exp(-2 pi i jk/N); inverse: exp(+2 pi i jk/N))
 // of the row held in x (strided natural layout in and out).  pre: a whole
-// row of N samples copied into buf[0..N) by LDS DMA during the last pass
-// (NULL: none).
+// row of N samples copied into buf[0..N) by LDS DMA during the last pass;
+// tab: a phase table of N/2 + 1 entries copied there instead (NULL: none).
 template <int N, bool INV, int K = 0>
 __device__ __forceinline__ void fft_row(float2 (&x)[N / 64], float2* buf, const float2* tw, int t,
-                                        const float2* pre = nullptr) {
-  fft_passes<N, K, INV, 0>(x, buf, tw, t, pre);
+                                        const float2* pre = nullptr, const float2* tab = nullptr) {
+  fft_passes<N, K, INV, 0>(x, buf, tw, t, pre, tab);
 }
 
 // ---------------------------------------------------------------- any length

@@ -563,6 +563,7 @@ StepGeom Model::geom() const {
   g.full = (d.tap == 0 && d.nx == N) ? 1 : 0;
   g.fg = fg;
   g.pf = 0;
+  g.tdma = 0;
   g.ntaylor = d.ntaylor;
   g.taper = d.taper > 0 ? 1 : 0;
   return g;
@@ -699,6 +700,11 @@ int Model::oneway_on(int up, int adjoint, int nrows, const float* freq_hz, const
     kt.begin(s);
     StepGeom g = geom();
     g.pf = 1;  // every row reads its source row at every depth
+    {
+      std::vector<float> f(freq_hz, freq_hz + nrows);
+      std::sort(f.begin(), f.end());
+      g.tdma = table_dma((size_t)nrows, (size_t)(std::unique(f.begin(), f.end()) - f.begin()));
+    }
     e = launch_oneway(N, drows, nrows, d_tables, g, d.nz, step, adjoint != 0, d_tw, s, !fixed);
     kt.end(s);
     if (e != hipSuccess) return hip_fail(e, "oneway kernel");
@@ -807,6 +813,12 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     // every row reads a plain source row at every depth: the full-row
     // kernels may prefetch it
     g.pf = (!(src_lo && src_hi) && !(prod && prod->smul)) ? 1 : 0;
+    {
+      std::vector<std::pair<int, float>> tabs;
+      for (int k = k0; k < k1; ++k) tabs.push_back({midx[order[k]], freq_hz[order[k]]});
+      std::sort(tabs.begin(), tabs.end());
+      g.tdma = table_dma((size_t)(k1 - k0), (size_t)(std::unique(tabs.begin(), tabs.end()) - tabs.begin()));
+    }
     groups.push_back(Group{mg->N, k0, k1 - k0, g, mg->d.nz, mg->d_tw});
     k0 = k1;
   }

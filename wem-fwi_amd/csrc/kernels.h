@@ -66,5 +66,10 @@ hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTabl
                          bool tune = true);
 hipError_t launch_onestep(int N, const StepRow* rows, int nrows, const DepthTables* models,
                           StepGeom g, bool adj, const float2* tw, hipStream_t s);
+// StepGeom.tdma of a sweep launch of `rows` rows that use `tables` distinct
+// phase tables: 1 when a table serves fewer than two rows on average (it is
+// then read from HBM by one row, and a DMA copy beats the row's serial
+// table loads); WEMFWI_TABLE_DMA=0 / 1 forces it off / on
+int table_dma(size_t rows, size_t tables);
 
 }  // namespace wemfwi

@@ -95,7 +95,7 @@ struct Occupancy {
   static constexpr int value = N <= 256 ? 8 : (N == 512 ? 6 : (N == 1024 ? 4 : 2));
 };
 
-template <class V, bool ADJ, bool PF>
+template <class V, bool ADJ, bool PF, bool TD>
 __global__ void __launch_bounds__(64 * WavesV<V>::value)
     __attribute__((amdgpu_waves_per_eu(V::GEN ? 2 : Occupancy<V::n>::value, 8)))
     oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
@@ -162,8 +162,8 @@ __global__ void __launch_bounds__(64 * WavesV<V>::value)
       __builtin_assume(pre != nullptr);
     }
     bool in_lds;
-    if constexpr (ADJ) in_lds = step_adjoint<V, true>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
-    else in_lds = step_forward<V, true>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
+    if constexpr (ADJ) in_lds = step_adjoint<V, true, TD>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
+    else in_lds = step_forward<V, true, TD>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
     if (src_here) {
       float2 src[E];
       if (PFX && in_lds) {
@@ -295,9 +295,10 @@ hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const d
   return hipGetLastError();
 }
 
-template <class V>
-static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
-                                  int step, bool adj, const float2* tw, hipStream_t s) {
+// TD: the phase-table DMA variant (g.tdma)
+template <class V, bool TD>
+static hipError_t launch_oneway_w(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int step,
+                                  bool adj, const float2* tw, hipStream_t s) {
   constexpr int kWavesPerBlock = WavesV<V>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   // the source-row prefetch only on full rows, when the caller states that
@@ -305,17 +306,24 @@ static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTa
   if constexpr (V::FULL) {
     if (g.pf) {
       if (adj)
-        hipLaunchKernelGGL((oneway_kernel<V, true, true>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+        hipLaunchKernelGGL((oneway_kernel<V, true, true, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
       else
-        hipLaunchKernelGGL((oneway_kernel<V, false, true>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+        hipLaunchKernelGGL((oneway_kernel<V, false, true, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
       return hipGetLastError();
     }
   }
   if (adj)
-    hipLaunchKernelGGL((oneway_kernel<V, true, false>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, true, false, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
   else
-    hipLaunchKernelGGL((oneway_kernel<V, false, false>), grid, block, 0, s, rows, nrows, models, g, step, tw);
+    hipLaunchKernelGGL((oneway_kernel<V, false, false, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
   return hipGetLastError();
+}
+
+template <class V>
+static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                                  int step, bool adj, const float2* tw, hipStream_t s) {
+  if (g.tdma && !V::GEN) return launch_oneway_w<V, true>(rows, nrows, models, g, step, adj, tw, s);
+  return launch_oneway_w<V, false>(rows, nrows, models, g, step, adj, tw, s);
 }
 
 // the four register-FFT variants of radix plan K
@@ -337,6 +345,12 @@ static hipError_t launch_oneway_plan(int k, const SweepRow* rows, int nrows, con
   }
   if (k != K) return hipErrorInvalidValue;
   return launch_oneway_k<N, K>(rows, nrows, models, g, nz, step, adj, tw, s);
+}
+
+int table_dma(size_t rows, size_t tables) {
+  const char* e = std::getenv("WEMFWI_TABLE_DMA");
+  if (e && *e) return std::atoi(e) ? 1 : 0;
+  return tables * 2 > rows ? 1 : 0;
 }
 
 // ------------------------------------------------------------------ autotuner

@@ -355,6 +355,13 @@ struct Ctx {
     }
     return WEMFWI_OK;
   }
+  // distinct phase tables ((model, frequency) pairs) of a chunk's rows
+  static size_t distinct_tables(const Rows& r) {
+    std::vector<std::pair<Model*, int>> v;
+    for (size_t i = 0; i < r.shot.size(); ++i) v.push_back({r.model[i], r.fi[i]});
+    std::sort(v.begin(), v.end());
+    return (size_t)(std::unique(v.begin(), v.end()) - v.begin());
+  }
   int model_index(Model* m) const {
     return (int)(std::find(models.begin(), models.end(), m) - models.begin());
   }
@@ -401,6 +408,7 @@ struct Ctx {
     const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
     StepGeom g = m0->geom();
     g.pf = (!src_rows && !smul) ? 1 : 0;  // plain source rows at every depth
+    g.tdma = table_dma((size_t)n, distinct_tables(r));
     HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), g, nz, step, adjoint != 0,
                          m0->d_tw, s),
            "oneway kernel");
@@ -466,6 +474,7 @@ struct Ctx {
     const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
     StepGeom g = m0->geom();
     g.pf = src_rows ? 0 : 1;  // plain source rows at every depth
+    g.tdma = table_dma((size_t)n, distinct_tables(r));
     HIPCHK(launch_oneway(N, d_rows.as<SweepRow>(), n, d_models.as<DepthTables>(), g, nz, step, adjoint != 0,
                          m0->d_tw, s),
            "oneway kernel");

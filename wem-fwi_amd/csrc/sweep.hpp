@@ -144,6 +144,10 @@ struct StepGeom {
   int pf;       // sweeps: 1 when every row of the launch reads a plain source
                 // row at every depth (no src_skip, no smul): full rows then
                 // prefetch it by LDS DMA
+  int tdma;     // sweeps: 1 when the rows of the launch rarely share a phase
+                // table (tables then come from HBM, not L2): the launch runs
+                // the kernels whose single-reference depths copy their table
+                // row into LDS by DMA
 };
 
 // Compile-time variants of the step: FULL (no padding samples), SSF
@@ -175,9 +179,17 @@ struct TwV {
 // whole row copied into buf[0..N) by LDS DMA during the last pass
 template <class V, bool INV>
 __device__ __forceinline__ void fft_v(float2 (&x)[V::E], float2* buf, const float2* tw, int t, const StepGeom& g,
-                                      const float2* pre = nullptr) {
+                                      const float2* pre = nullptr, const float2* tab = nullptr) {
   if constexpr (V::GEN) fft_row_gen<V::n, INV>(x, buf, tw, t, g.fg);
-  else fft_row<V::n, INV, V::K>(x, buf, tw, t, pre);
+  else fft_row<V::n, INV, V::K>(x, buf, tw, t, pre, tab);
+}
+
+// phase-table entry of lane sample k = t + 64 e from a copy of the table
+// (N/2 + 1 entries) in LDS: entry k for k <= N/2, N - k above
+template <int N>
+__device__ __forceinline__ float2 ptab_lds(const float2* T, int t, int e) {
+  if (e < N / 128) return T[t + 64 * e];
+  return T[(64 - t) + 64 * (N / 64 - 1 - e)];
 }
 
 // sin and cos of x for |x| < 1e5: Cody-Waite reduction by a three-part pi/2
@@ -377,7 +389,7 @@ __device__ __forceinline__ void load_samples(float2 (&v)[V::E], const float2* p,
 // [0,nx)).  Loads are issued where their values are first needed: holding
 // them across an FFT costs more registers (and waves per SIMD) than their
 // latency costs.
-template <class V>
+template <class V, bool TD>
 __device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                                  const float2* tw, int t, float2* scr, const float2* pre);
@@ -389,14 +401,17 @@ __device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, floa
 // which keeps the step within 128 VGPRs.
 // pre (SCR only; FULL rows of register-planned lengths): a whole row copied
 // into buf[0..N) by LDS DMA during the step's last inverse transform;
-// returns whether that copy was started (buf then holds it once it arrives)
-template <class V, bool SCR = false>
+// returns whether that copy was started (buf then holds it once it arrives).
+// TD (SCR only): single-reference depths copy their phase-table row into
+// buf by LDS DMA during the forward transform (launches whose rows rarely
+// share a table).
+template <class V, bool SCR = false, bool TD = false>
 __device__ __forceinline__ bool step_forward(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                              const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                              const float2* tw, int t, float2* scr = nullptr,
                                              const float2* pre = nullptr) {
   if constexpr (SCR) {
-    return step_forward_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr, pre);
+    return step_forward_scr<V, TD>(st, d, w, dt, g, ptab, buf, tw, t, scr, pre);
   }
   constexpr int N = V::n;
   constexpr int E = V::E;
@@ -449,18 +464,18 @@ __device__ __forceinline__ bool step_forward(float2 (&st)[V::E], int d, float w,
 
 // out = P_d^H(in); SCR as in step_forward (the split-step-corrected input row
 // is parked in the scratch row at depths with several references)
-template <class V>
+template <class V, bool TD>
 __device__ __forceinline__ bool step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                                  const float2* tw, int t, float2* scr, const float2* pre);
 
-template <class V, bool SCR = false>
+template <class V, bool SCR = false, bool TD = false>
 __device__ __forceinline__ bool step_adjoint(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                              const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                              const float2* tw, int t, float2* scr = nullptr,
                                              const float2* pre = nullptr) {
   if constexpr (SCR) {
-    return step_adjoint_scr<V>(st, d, w, dt, g, ptab, buf, tw, t, scr, pre);
+    return step_adjoint_scr<V, TD>(st, d, w, dt, g, ptab, buf, tw, t, scr, pre);
   }
   constexpr int N = V::n;
   constexpr int E = V::E;
@@ -518,7 +533,7 @@ __device__ __forceinline__ bool step_adjoint(float2 (&st)[V::E], int d, float w,
 // and the multi-reference case (two separate paths in one sweep loop cost the
 // register allocator more than either path alone).
 //   forward: X = F in; for each reference r: Y = F^H (P_r X), out[x in r] = Y[x]
-template <class V>
+template <class V, bool TD>
 __device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                                  const float2* tw, int t, float2* scr, const float2* pre) {
@@ -533,8 +548,13 @@ __device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, floa
     for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
     return false;
   }
-  fft_v<V, false>(st, buf, tw, t, g);
   const bool multi = s1 - s0 > 1;
+  // one reference: its phase-table row is copied into the transform buffer
+  // by LDS DMA during the forward transform's last pass (read from there,
+  // the N/2 + 1 loads do not queue one after another behind the register
+  // pressure of the step)
+  const float2* tab = (TD && !V::GEN && !multi) ? ptab + (size_t)s0 * (N / 2 + 1) : nullptr;
+  fft_v<V, false>(st, buf, tw, t, g, nullptr, tab);
   const bool all = V::FULL && ldu(dt.one_ref, d);
   const brsrc xs = make_rsrc(scr, (unsigned)N * 8u);  // the wave's scratch row, lane index t + 64 e
   if (multi) {
@@ -556,6 +576,10 @@ __device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, floa
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int e = E / 2; e < E; ++e) Y[e] = cmul(bld2(xs, (unsigned)t * 8u, 512u * (unsigned)e), ptab_v<V>(P, t, e, g));
+    } else if (TD && tab) {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the table copy has arrived
+#pragma unroll
+      for (int e = 0; e < E; ++e) Y[e] = cmul(st[e], ptab_lds<N>(buf, t, e));
     } else {
 #pragma unroll
       for (int e = 0; e < E / 2; ++e) Y[e] = cmul(st[e], ptab_v<V>(P, t, e, g));
@@ -586,7 +610,7 @@ __device__ __forceinline__ bool step_forward_scr(float2 (&st)[V::E], int d, floa
 }
 
 //   adjoint: u = conj(f) in; acc = sum_r conj(P_r) F (u restricted to r); out = F^H acc
-template <class V>
+template <class V, bool TD>
 __device__ __forceinline__ bool step_adjoint_scr(float2 (&st)[V::E], int d, float w, const DepthTables& dt,
                                                  const StepGeom& g, const float2* __restrict__ ptab, float2* buf,
                                                  const float2* tw, int t, float2* scr, const float2* pre) {
@@ -630,18 +654,30 @@ __device__ __forceinline__ bool step_adjoint_scr(float2 (&st)[V::E], int d, floa
         Z[e] = (all || rid_at(rid, e) == r) ? st[e] : make_float2(0.f, 0.f);
       }
     }
-    fft_v<V, false>(Z, buf, tw, t, g);
+    // one reference: its phase-table row arrives in the transform buffer by
+    // LDS DMA during the transform's last pass (as in the forward step)
+    const float2* tab = (TD && !V::GEN && !multi) ? P : nullptr;
+    fft_v<V, false>(Z, buf, tw, t, g, nullptr, tab);
     const bool first = s == s0;
+    if (TD && tab) {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the table copy has arrived
 #pragma unroll
-    for (int e = 0; e < E / 2; ++e) {
-      const float2 z = cmulc(Z[e], ptab_v<V>(P, t, e, g));
-      st[e] = first ? z : cadd(st[e], z);
-    }
-    __builtin_amdgcn_sched_barrier(0);
+      for (int e = 0; e < E; ++e) {
+        const float2 z = cmulc(Z[e], ptab_lds<N>(buf, t, e));
+        st[e] = first ? z : cadd(st[e], z);
+      }
+    } else {
 #pragma unroll
-    for (int e = E / 2; e < E; ++e) {
-      const float2 z = cmulc(Z[e], ptab_v<V>(P, t, e, g));
-      st[e] = first ? z : cadd(st[e], z);
+      for (int e = 0; e < E / 2; ++e) {
+        const float2 z = cmulc(Z[e], ptab_v<V>(P, t, e, g));
+        st[e] = first ? z : cadd(st[e], z);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = E / 2; e < E; ++e) {
+        const float2 z = cmulc(Z[e], ptab_v<V>(P, t, e, g));
+        st[e] = first ? z : cadd(st[e], z);
+      }
     }
   }
   fft_v<V, true>(st, buf, tw, t, g, pre);
