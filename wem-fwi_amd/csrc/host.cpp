@@ -869,8 +869,11 @@ int BatchPlan::enqueue(int up, int adjoint, hipStream_t s, bool tables) {
     e = hipStreamWaitEvent(sg, events[0], 0);
     if (e == hipSuccess) e = launch_oneway(G.N, d_rows + G.off, G.n, d_tabs, G.g, G.nz, step, adjoint != 0, G.tw, sg);
     if (e == hipSuccess) e = hipEventRecord(events[k + 1], sg);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, events[k + 1], 0);
   }
+  // the joins go in after every launch: with more streams than hardware
+  // queues a side stream can share the caller's queue, and a launch queued
+  // behind a join would wait for the groups before it
+  for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) e = hipStreamWaitEvent(s, events[k + 1], 0);
   return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch run");
 }
 
