@@ -917,7 +917,7 @@ def main():
                                 "flops_per_launch": flops,
                                 "flop_per_byte": round(flops / (BYTES_PER_UPDATE * upd_sweep), 2)}
     # the sweep autotuner's radix-plan choice for this workload's launches
-    line["autotune"] = [e for e in wemfwi.autotune_report() if e["N"] == nx]
+    line["autotune"] = [e for e in wemfwi.autotune_report() if e["kernel"] == "oneway_kernel" and e["N"] == nx]
     line["fwd_gupdates_per_s"] = round(upd_sweep / t_fwd / 1e9, 3)
     line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
     if args.config == "pspi1024":
@@ -926,6 +926,8 @@ def main():
         line["fwi_gradient"] = fwi_gradient_bench(args.fwi_sample, args.fd_directions)
     if args.uneven_sample > 0:
         line["uneven_batch"] = uneven_batch_bench(args.uneven_sample)
+        # the batch plan's launch-order decision (groups, order, best time of each candidate)
+        line["uneven_batch"]["launch_order"] = [e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"]
         line["uneven_offset_image"] = uneven_image_bench(args.uneven_sample)
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)

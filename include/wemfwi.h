@@ -603,7 +603,9 @@ int wemfwi_timing_get(const char* kernel, double* total_ms, long* count);
 /* Decisions of the sweep autotuner so far (no counterpart in the reference,
  * whose FFTW plans are chosen by FFTW_ESTIMATE, FFT1_c.cpp:34-38): a JSON
  * list with, per (FFT length, full row, split step, adjoint, device), the
- * plan chosen and the measured mean launch time of every radix plan.
+ * plan chosen and the measured mean launch time of every radix plan, and
+ * per batch group shape ("kernel": "batch_order") the launch order of the
+ * groups chosen and the best time of every candidate order.
  * Writes at most cap bytes (NUL-terminated) and returns the full length. */
 int wemfwi_autotune_report(char* buf, int cap);
 

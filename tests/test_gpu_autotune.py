@@ -87,8 +87,8 @@ def test_tuner_records_one_decision_per_variant(forced_plan):
     freqs = np.linspace(3, 20, 64).astype(np.float32)
     m.oneway(False, True, freqs, W)
     m.oneway(False, True, freqs, W)
-    rep = [e for e in wemfwi.autotune_report() if e["N"] == 2048 and e["full"] == 0 and e["ssf"] == 0
-           and e["adjoint"] == 1]
+    rep = [e for e in wemfwi.autotune_report() if e["kernel"] == "oneway_kernel" and e["N"] == 2048
+           and e["full"] == 0 and e["ssf"] == 0 and e["adjoint"] == 1]
     assert len(rep) == 1
     e = rep[0]
     assert len(e["ms"]) == 2 and all(v > 0 for v in e["ms"])

@@ -101,3 +101,51 @@ def test_batch_plan_runs_repeat_and_detect_rebuilt_tables():
     plan.close()
     for m in models:
         m.close()
+
+
+def test_batch_launch_order_is_timed_once_per_shape_and_reused():
+    """A plan whose runs can be repeated times its candidate launch orders on
+    its first run and keeps the fastest; a later plan of the same group shape
+    (here the one-call form, and an in-place one that could not be timed)
+    reuses that decision; every run stays bitwise equal to the per-model
+    sweeps."""
+    import torch
+    import wemfwi
+    from wemfwi import Model
+    from wemfwi.core import BatchPlan, oneway_batch
+    slows = [_slow(nz, nx, 30 + i) for i, (nz, nx, _, _, _) in enumerate(MODELS)]
+    models = [Model(s, 10.0, 10.0, tap=tap, pad=6, nref=nref, prop=prop, eps=0.04)
+              for s, (nz, nx, tap, prop, nref) in zip(slows, MODELS)]
+    rows = _rows()[:11]     # a group shape no other test builds
+    rng = np.random.default_rng(9)
+    src = [torch.from_numpy((rng.standard_normal(MODELS[mi][:2]) + 0j).astype(np.complex64)).cuda()
+           for mi, _ in rows]
+    ms, fs = [models[mi] for mi, _ in rows], [f for _, f in rows]
+    want = []
+    for (mi, f), a in zip(rows, src):
+        ref = torch.zeros_like(a)
+        models[mi].oneway_recursive(False, False, [f], a[None], ref[None])
+        want.append(ref)
+    before = [e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"]
+    out = [torch.zeros_like(a) for a in src]
+    plan = BatchPlan(ms, fs, src, out)
+    plan.run(False, False)
+    after = [e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"]
+    assert len(after) == len(before) + 1
+    e = after[-1]
+    ngroups = len(e["groups"])
+    assert ngroups >= 2 and sorted(e["order"]) == list(range(ngroups))
+    assert len(e["ms"]) == min(ngroups, 4) and all(v > 0 for v in e["ms"])
+    assert e["order"][0] == int(np.argmin(e["ms"]))   # candidate c starts with group c
+    for a, b in zip(out, want):
+        assert torch.equal(a, b)
+    plan.close()
+    out2 = [torch.zeros_like(a) for a in src]
+    oneway_batch(False, False, ms, fs, src, out2)
+    inpl = [a.clone() for a in src]
+    oneway_batch(False, False, ms, fs, inpl, inpl)
+    assert len([e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"]) == len(after)
+    for a, b, c in zip(out2, inpl, want):
+        assert torch.equal(a, c) and torch.equal(b, c)
+    for m in models:
+        m.close()

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -735,10 +736,11 @@ static int batch_streams(hipStream_t (&out)[4]) {
 // Plan of an uneven batch: Down/Up recursions out_i = (I - L_i)^-1 in_i of
 // rows that each have their own model (aperture, depth count, FFT length) and
 // frequency.  Rows are grouped by kernel variant (FFT length, full row, split
-// step); each group is one launch, the groups run on separate streams, the
-// group with the largest LDS footprint first so that its workgroups claim
-// compute units before the smaller ones fill the rest.  Descriptors, depth
-// tables and scratch rows stay on the device between runs.
+// step); each group is one launch, the groups run on separate streams, at
+// first the group with the largest LDS footprint first so that its
+// workgroups claim compute units before the smaller ones fill the rest, then
+// in the measured order (tune_order).  Descriptors, depth tables and scratch
+// rows stay on the device between runs.
 
 int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf* const* in, cf* const* out,
                      hipStream_t s, const int* src_lo, const int* src_hi, const RowProducts* prod) {
@@ -822,6 +824,27 @@ int BatchPlan::build(int nrows, Model* const* ms, const float* freq_hz, const cf
     groups.push_back(Group{mg->N, k0, k1 - k0, g, mg->d.nz, mg->d_tw});
     k0 = k1;
   }
+  launch_order.resize(groups.size());
+  std::iota(launch_order.begin(), launch_order.end(), 0);
+  // repeatable runs: no input row overlaps an output row (or an out2 row)
+  {
+    std::vector<std::pair<uintptr_t, uintptr_t>> outs;
+    for (int i = 0; i < nrows; ++i) {
+      const size_t bytes = (size_t)ms[i]->d.nz * ms[i]->d.nx * sizeof(float2);
+      outs.push_back({(uintptr_t)out[i], (uintptr_t)out[i] + bytes});
+      if (prod && prod->out2 && prod->out2[i]) outs.push_back({(uintptr_t)prod->out2[i], (uintptr_t)prod->out2[i] + bytes});
+    }
+    std::sort(outs.begin(), outs.end());
+    std::vector<uintptr_t> reach(outs.size());  // largest end among outs[0..k]
+    for (size_t k = 0; k < outs.size(); ++k) reach[k] = std::max(outs[k].second, k ? reach[k - 1] : 0);
+    rerunnable = true;
+    for (int i = 0; i < nrows && rerunnable; ++i) {
+      const uintptr_t a0 = (uintptr_t)in[i], a1 = a0 + (size_t)ms[i]->d.nz * ms[i]->d.nx * sizeof(float2);
+      // outs starting before a1: the one reaching furthest overlaps [a0, a1) iff it ends after a0
+      const size_t k = (size_t)(std::lower_bound(outs.begin(), outs.end(), std::make_pair(a1, (uintptr_t)0)) - outs.begin());
+      if (k > 0 && reach[k - 1] > a0) rerunnable = false;
+    }
+  }
   std::vector<DepthTables> ht;
   for (Model* m : models) ht.push_back(m->h_tables);
   const size_t rows_bytes = (nrows * sizeof(SweepRow) + 255) / 256 * 256;
@@ -853,6 +876,139 @@ int BatchPlan::enqueue(int up, int adjoint, hipStream_t s, bool tables) {
   if (tables)
     for (Model* m : models)
       if (int rc = m->enqueue_tables(s)) return rc;
+  if (!order_done)
+    if (int rc = tune_order(up, adjoint, s)) return rc;
+  return launch(up, adjoint, s, launch_order);
+}
+
+// Launch-order decisions per device and group shape (each group's FFT
+// length, variant flags, depth count and row count), shared by every plan
+// of that shape: plans built per call (the one-call batch, the uneven
+// imaging and gradient drivers) time the orders once per process, and
+// plans whose runs cannot be repeated reuse a decision of the same shape.
+struct OrderEntry {
+  std::vector<long long> key;
+  std::vector<int> order;
+  std::vector<float> ms;
+};
+static std::mutex& order_mutex() {
+  static std::mutex m;
+  return m;
+}
+static std::vector<OrderEntry>& order_table() {
+  static std::vector<OrderEntry> t;
+  return t;
+}
+
+static std::vector<long long> order_key(const std::vector<BatchPlan::Group>& groups) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::vector<long long> k{dev};
+  for (const BatchPlan::Group& G : groups)
+    for (long long v : {(long long)G.N, (long long)G.g.fg.n, (long long)G.g.full, (long long)G.g.ssf,
+                        (long long)G.g.pf, (long long)G.g.tdma, (long long)G.nz, (long long)G.n})
+      k.push_back(v);
+  return k;
+}
+
+std::string batch_order_report() {
+  std::lock_guard<std::mutex> lock(order_mutex());
+  std::string out;
+  for (const OrderEntry& e : order_table()) {
+    out += ", {\"kernel\": \"batch_order\", \"groups\": [";
+    for (size_t g = 0; (g + 1) * 8 < e.key.size(); ++g) {
+      char b[96];
+      std::snprintf(b, sizeof(b), "%s{\"N\": %lld, \"rows\": %lld}", g ? ", " : "", e.key[1 + 8 * g],
+                    e.key[8 + 8 * g]);
+      out += b;
+    }
+    out += "], \"order\": [";
+    for (size_t i = 0; i < e.order.size(); ++i) out += (i ? ", " : "") + std::to_string(e.order[i]);
+    out += "], \"ms\": [";
+    for (size_t i = 0; i < e.ms.size(); ++i) {
+      char b[32];
+      std::snprintf(b, sizeof(b), "%s%.3f", i ? ", " : "", e.ms[i]);
+      out += b;
+    }
+    out += "]}";
+  }
+  return out;
+}
+
+// Candidate launch orders, each timed twice (HIP events on s around the
+// fork and the joins, the work before them finished first): the current
+// order and each other group moved to the front.  Groups whose workgroups
+// fill a compute unit's LDS by themselves (rows of 2048 lanes) take whole
+// units, so which group starts first decides how the rest share the chip:
+// in the 64-shot survey of apertures 256-2048 the 1024-lane group ahead of
+// the 2048-lane one runs the batch in 13.4-13.7 ms against 14.6.  A run
+// inside stream capture keeps the current order and times nothing.
+// WEMFWI_BATCH_ORDER=0 keeps the largest-length-first order.
+int BatchPlan::tune_order(int up, int adjoint, hipStream_t s) {
+  const char* env = std::getenv("WEMFWI_BATCH_ORDER");
+  if (groups.size() < 2 || (env && *env && std::atoi(env) == 0)) {
+    order_done = true;
+    return WEMFWI_OK;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return WEMFWI_OK;
+  const std::vector<long long> key = order_key(groups);
+  {
+    std::lock_guard<std::mutex> lock(order_mutex());
+    for (const OrderEntry& e : order_table())
+      if (e.key == key) {
+        launch_order = e.order;
+        order_ms = e.ms;
+        order_done = true;
+        return WEMFWI_OK;
+      }
+  }
+  if (!rerunnable) {
+    order_done = true;
+    return WEMFWI_OK;
+  }
+  std::vector<std::vector<int>> cand;
+  for (size_t f = 0; f < groups.size() && cand.size() < 4; ++f) {
+    std::vector<int> o{(int)f};
+    for (size_t k = 0; k < groups.size(); ++k)
+      if (k != f) o.push_back((int)k);
+    cand.push_back(o);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  std::vector<float> ms(cand.size(), 0.f);
+  for (size_t c = 0; c < cand.size() && e == hipSuccess; ++c)
+    for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+      e = hipEventRecord(e0, s);
+      if (e != hipSuccess) break;
+      if (int rc = launch(up, adjoint, s, cand[c])) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+      }
+      e = hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e == hipSuccess && (rep == 0 || t < ms[c])) ms[c] = t;
+    }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(e, "batch order timing");
+  size_t best = 0;
+  for (size_t c = 1; c < cand.size(); ++c)
+    if (ms[c] < ms[best]) best = c;
+  launch_order = cand[best];
+  order_ms = ms;
+  order_done = true;
+  std::lock_guard<std::mutex> lock(order_mutex());
+  order_table().push_back(OrderEntry{key, launch_order, ms});
+  return WEMFWI_OK;
+}
+
+int BatchPlan::launch(int up, int adjoint, hipStream_t s, const std::vector<int>& ord) {
+  if (ord.size() != groups.size()) return fail(WEMFWI_ERR_ARG, "batch: launch order does not cover the groups");
   hipStream_t st[4];
   if (int rc = batch_streams(st)) return rc;
   while (events.size() < groups.size() + 1) {
@@ -863,8 +1019,8 @@ int BatchPlan::enqueue(int up, int adjoint, hipStream_t s, bool tables) {
   }
   hipError_t e = hipEventRecord(events[0], s);
   const int step = ((!up) != (adjoint != 0)) ? 1 : -1;
-  for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
-    const Group& G = groups[k];
+  for (size_t k = 0; k < ord.size() && e == hipSuccess; ++k) {
+    const Group& G = groups[ord[k]];
     hipStream_t sg = st[k % 4];
     e = hipStreamWaitEvent(sg, events[0], 0);
     if (e == hipSuccess) e = launch_oneway(G.N, d_rows + G.off, G.n, d_tabs, G.g, G.nz, step, adjoint != 0, G.tw, sg);
@@ -1136,7 +1292,9 @@ int wemfwi_timing(int enable) {
 }
 
 int wemfwi_autotune_report(char* buf, int cap) {
-  const std::string r = autotune_report();
+  std::string r = autotune_report();
+  const std::string b = batch_order_report();
+  if (!b.empty()) r.insert(r.size() - 1, r.size() > 2 ? b : b.substr(2));
   if (buf && cap > 0) {
     const size_t n = std::min((size_t)cap - 1, r.size());
     std::memcpy(buf, r.data(), n);

@@ -124,8 +124,8 @@ struct Model {
 };
 
 // Sweeps of rows with their own models (widths, bands) planned once: rows
-// grouped by kernel variant (largest FFT length first), descriptors and
-// scratch rows uploaded; run() launches the groups on side streams.
+// grouped by kernel variant, descriptors and scratch rows uploaded; run()
+// launches the groups on side streams in a measured order.
 struct BatchPlan {
   struct Group {
     int N, off, n;
@@ -140,6 +140,15 @@ struct BatchPlan {
   const SweepRow* d_rows = nullptr;
   const DepthTables* d_tabs = nullptr;
   std::vector<hipEvent_t> events;      // fork / join events of enqueue (one more than groups)
+  // launch order of the groups: largest FFT length first until the first
+  // enqueue outside stream capture has timed the candidate orders (each
+  // other group moved to the front) and kept the fastest, or found the
+  // decision of an earlier plan of the same group shape.  Only plans whose
+  // runs may be repeated are timed: no row reads memory that a row writes.
+  std::vector<int> launch_order;
+  std::vector<float> order_ms;         // best time of each candidate (empty: not timed)
+  bool order_done = false;
+  bool rerunnable = false;
   ~BatchPlan();
   // Optional per-row products of the sweep kernel (SweepRow smul, out2,
   // omul): arrays of nrows pointers, NULL for none
@@ -156,11 +165,19 @@ struct BatchPlan {
   // WEMFWI_ERR_ARG if a model's phase table was rebuilt since planning
   int check() const;
   // launches the groups behind the work already on s and makes s wait for
-  // them (no host synchronisation, so it can be captured into a graph);
-  // tables = true first recomputes the models' phase tables in place
+  // them (no host synchronisation, so it can be captured into a graph;
+  // the first call outside capture times the launch orders and waits for
+  // them); tables = true first recomputes the models' phase tables in place
   int enqueue(int up, int adjoint, hipStream_t s, bool tables);
   int run(int up, int adjoint, hipStream_t s);
+
+ private:
+  int launch(int up, int adjoint, hipStream_t s, const std::vector<int>& ord);
+  int tune_order(int up, int adjoint, hipStream_t s);
 };
+
+// JSON entries (each starting with ", ") of the launch-order decisions so far
+std::string batch_order_report();
 
 // cos^2 source/receiver spread weights of half-width ng (Injection.h:53-65)
 std::vector<float> cos2_window(int ng);

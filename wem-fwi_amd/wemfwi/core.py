@@ -259,7 +259,9 @@ def timing(enable: bool) -> None:
 def autotune_report() -> list:
     """The sweep autotuner's decisions so far: per (FFT length, full row,
     split step, adjoint, device) the radix plan chosen and the measured mean
-    launch time of every plan (ms)."""
+    launch time of every plan (ms), entries with "kernel": "oneway_kernel";
+    per batch group shape the launch order chosen and the best time of every
+    candidate order, entries with "kernel": "batch_order"."""
     import json
     n = lib.wemfwi_autotune_report(None, 0)
     buf = C.create_string_buffer(n + 1)
