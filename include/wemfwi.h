@@ -485,6 +485,40 @@ int wemfwi_uneven_born_forward(const wemfwi_born_shot* shots, int ns, int ngs, i
 int wemfwi_uneven_born_adjoint(const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
                                wemfwi_complex* dslow, wemfwi_complex* dden, int add, int mode, void* stream);
 
+/* extBornFull::forward / adjoint (extBornFull.cpp:11-121,
+ * ext_born_prop.cpp:8-194) of shots with uneven apertures and bands: as
+ * wemfwi_uneven_born_*, but frequency i of shot s propagates on its own
+ * background models[i] (nz x nx_s, the shot's aperture) and meets the
+ * perturbation slice w0 + i of dslow / dden [nslice][nz][nx_model] (the
+ * reference's one model slice per frequency).  Every shot's data, and every
+ * shot's contribution to each slice before the sum over shots (mode 1),
+ * equal wemfwi_extborn_forward / wemfwi_extborn_adjoint on that shot alone
+ * (models[0..nw), slices w0..w0+nw) bitwise.  The adjoint sums in shot
+ * order in double precision.  mode 0 reproduces the reference's serial shot
+ * loop (LinOneWay.cpp:64-65): the last depth row of a slice keeps, at each
+ * column, the LinUp term of the last shot whose aperture and band cover it
+ * (shots without such a successor keep theirs); mode 1 is the exact
+ * adjoint. */
+typedef struct wemfwi_extborn_shot {
+  wemfwi_model* const* models;
+  int x0;
+  int nw;
+  int w0;
+  const float* freq_hz;
+  const wemfwi_complex* wavelet;
+  int src_iz, src_ix;
+  int nr;
+  const int* rec_iz;
+  const int* rec_ix;
+  wemfwi_complex* data;
+} wemfwi_extborn_shot;
+int wemfwi_uneven_extborn_forward(const wemfwi_extborn_shot* shots, int ns, int ngs, int ngr, int nz,
+                                  int nx_model, int nslice, const wemfwi_complex* dslow,
+                                  const wemfwi_complex* dden, int add, void* stream);
+int wemfwi_uneven_extborn_adjoint(const wemfwi_extborn_shot* shots, int ns, int ngs, int ngr, int nz,
+                                  int nx_model, int nslice, wemfwi_complex* dslow, wemfwi_complex* dden,
+                                  int add, int mode, void* stream);
+
 /* ---------------------------------------------------------------- extended FWI
  * Operators of the frequency-extended objective of scripts/efwi.py:160-190
  * (operator/src/python/Operator.py:228-292,619-775), on device models

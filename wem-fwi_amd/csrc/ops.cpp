@@ -1278,13 +1278,32 @@ struct GradientPlan {
 // reductions run per shot with the arguments and in the order of
 // born_forward / born_adjoint_all on a one-shot survey, so every shot's data
 // and gradient slice are bitwise those of such a call.
+// ext (extBornFull, ext_born_prop.cpp:8-194): frequency i of a shot has its
+// own background fm[i] and perturbation slice w0 + i; the windows, the
+// dReflect products and the FP64 accumulators are then per row, not per
+// shot, and each row's contribution goes to its own slice.
+struct UShot {
+  std::vector<Model*> fm;  // background of each frequency (one entry: all)
+  int x0 = 0, nw = 0, w0 = 0;
+  const float* freq_hz = nullptr;
+  const wemfwi_complex* wavelet = nullptr;
+  int src_iz = 0, src_ix = 0, nr = 0;
+  const int* rec_iz = nullptr;
+  const int* rec_ix = nullptr;
+  wemfwi_complex* data = nullptr;
+  Model* model(int i) const { return fm.size() > 1 ? fm[i] : fm[0]; }
+};
+
 struct UnevenBorn {
-  const wemfwi_born_shot* shots = nullptr;
+  const UShot* shots = nullptr;
+  bool ext = false;
   int s0 = 0, s1 = 0, nz = 0, ngs = 0, ngr = 0;
   hipStream_t s = nullptr;
   std::vector<Model*> rm;      // model of each row
   std::vector<float> rf;       // frequency of each row
   std::vector<size_t> roff;    // first sample of each row in a group buffer
+  std::vector<int> rshot;      // shot (of the group) of each row
+  std::vector<int> rslice;     // perturbation slice of each row
   std::vector<int> first;      // first row of each shot of the group (one more entry)
   size_t total = 0;            // samples of a group buffer
   std::vector<size_t> woff;    // first sample of each shot's [nz][nx_s] slot
@@ -1299,16 +1318,21 @@ struct UnevenBorn {
   ~UnevenBorn() {
     for (auto& p : rplan) p.release();
   }
-  const wemfwi_born_shot& S(int k) const { return shots[s0 + k]; }
+  const UShot& S(int k) const { return shots[s0 + k]; }
   int nshots() const { return s1 - s0; }
-  int nx(int k) const { return S(k).model->m.d.nx; }
+  int nx(int k) const { return S(k).fm[0]->d.nx; }
   size_t nzx(int k) const { return (size_t)nz * nx(k); }
   int nw(int k) const { return first[k + 1] - first[k]; }
   // first sample of shot k's rows in a group buffer
   size_t base(int k) const { return nw(k) ? roff[first[k]] : 0; }
+  // row r's perturbation window / dReflect slot: its shot's slot, or (ext)
+  // the row's own
+  size_t wo(int r) const { return ext ? roff[r] : woff[rshot[r]]; }
+  size_t wsize() const { return ext ? total : wtotal; }
 
-  int init(const wemfwi_born_shot* sh, int a, int b, int nz_, int ngs_, int ngr_, hipStream_t st) {
+  int init(const UShot* sh, bool ext_, int a, int b, int nz_, int ngs_, int ngr_, hipStream_t st) {
     shots = sh;
+    ext = ext_;
     s0 = a;
     s1 = b;
     nz = nz_;
@@ -1317,13 +1341,14 @@ struct UnevenBorn {
     s = st;
     first.assign(1, 0);
     for (int k = 0; k < nshots(); ++k) {
-      Model* m = &S(k).model->m;
       woff.push_back(wtotal);
       wtotal += nzx(k);
       for (int i = 0; i < S(k).nw; ++i) {
-        rm.push_back(m);
+        rm.push_back(S(k).model(i));
         rf.push_back(S(k).freq_hz[i]);
         roff.push_back(total);
+        rshot.push_back(k);
+        rslice.push_back(ext ? S(k).w0 + i : 0);
         total += nzx(k);
       }
       first.push_back((int)rm.size());
@@ -1452,12 +1477,11 @@ struct UnevenBorn {
     return WEMFWI_OK;
   }
 
-  // per-row pointers: shot k's slot of a [nz][nx_s]-per-shot buffer, or the
-  // row itself in a group buffer
-  std::vector<const float2*> per_shot(const float2* w) const {
+  // per-row pointers into a window buffer (each row's window slot), or to
+  // the row itself in a group buffer
+  std::vector<const float2*> per_window(const float2* w) const {
     std::vector<const float2*> v(rm.size());
-    for (int k = 0; k < nshots(); ++k)
-      for (int r = first[k]; r < first[k + 1]; ++r) v[r] = w + woff[k];
+    for (size_t r = 0; r < rm.size(); ++r) v[r] = w + wo((int)r);
     return v;
   }
   std::vector<const float2*> per_row(const float2* w) const {
@@ -1525,21 +1549,28 @@ struct UnevenBorn {
   }
 };
 
-// born_forward's sequence on the rows of one group of shots
+// born_forward's sequence on the rows of one group of shots; dS, dR: the
+// perturbation slices [nslice][nz][nxg]
 static int uneven_born_forward_group(UnevenBorn& u, const float2* dS, const float2* dR, int nxg) {
   const int K = u.nshots(), nz = u.nz;
+  const size_t npix = (size_t)nz * nxg;
   hipStream_t s = u.s;
   DevBuf WS, WR, DRb, B, S1, T;
-  RCCHK(WS.alloc(u.wtotal * sizeof(float2), "perturbation windows"));
-  RCCHK(WR.alloc(u.wtotal * sizeof(float2), "perturbation windows"));
-  RCCHK(DRb.alloc(u.wtotal * sizeof(float2), "dr"));
+  RCCHK(WS.alloc(u.wsize() * sizeof(float2), "perturbation windows"));
+  RCCHK(WR.alloc(u.wsize() * sizeof(float2), "perturbation windows"));
+  RCCHK(DRb.alloc(u.wsize() * sizeof(float2), "dr"));
   float2 *ws = WS.as<float2>(), *wr = WR.as<float2>(), *dr = DRb.as<float2>();
   for (int k = 0; k < K; ++k) {
-    const Model& m = u.S(k).model->m;
-    HIPCHK(launch_window(ws + u.woff[k], dS, nz, u.nx(k), nxg, u.S(k).x0, s), "window");
-    HIPCHK(launch_window(wr + u.woff[k], dR, nz, u.nx(k), nxg, u.S(k).x0, s), "window");
-    HIPCHK(launch_dreflect_fwd(dr + u.woff[k], m.d_drefl, ws + u.woff[k], wr + u.woff[k], nz, u.nx(k), false, s),
-           "dreflect");
+    // one window per shot, or (ext) one per row from its slice
+    const int r0 = u.ext ? u.first[k] : 0, r1 = u.ext ? u.first[k + 1] : 1;
+    for (int r = r0; r < r1; ++r) {
+      const size_t o = u.ext ? u.roff[r] : u.woff[k];
+      const Model& m = u.ext ? *u.rm[r] : *u.S(k).fm[0];
+      const size_t q = u.ext ? (size_t)u.rslice[r] * npix : 0;
+      HIPCHK(launch_window(ws + o, dS + q, nz, u.nx(k), nxg, u.S(k).x0, s), "window");
+      HIPCHK(launch_window(wr + o, dR + q, nz, u.nx(k), nxg, u.S(k).x0, s), "window");
+      HIPCHK(launch_dreflect_fwd(dr + o, m.d_drefl, ws + o, wr + o, nz, u.nx(k), false, s), "dreflect");
+    }
   }
   if (u.total == 0) return WEMFWI_OK;
   RCCHK(B.alloc(u.total * sizeof(float2), "background"));
@@ -1553,7 +1584,7 @@ static int uneven_born_forward_group(UnevenBorn& u, const float2* dS, const floa
   for (int k = 0; k < K; ++k) RCCHK(u.zero_depths(bg, k, src[k], src[k]));
   RCCHK(u.inject_sources(bg));
   RCCHK(u.sweep(0, 0, bg, bg, &slo, &slo));
-  const std::vector<const float2*> mS = u.per_shot(ws);
+  const std::vector<const float2*> mS = u.per_window(ws);
   // s1 = Down(LinDown(dS; D)), recorded
   HIPCHK(launch_zero2d(s1, (long long)u.total, (long long)u.total, 1, s), "zero");
   RCCHK(u.lin(+1, 0, 0, bg, mS, s1));
@@ -1564,8 +1595,8 @@ static int uneven_born_forward_group(UnevenBorn& u, const float2* dS, const floa
     if (u.nw(k) == 0) continue;
     std::vector<MulRow> ic, rs1;
     for (int r = u.first[k]; r < u.first[k + 1]; ++r) {
-      ic.push_back(MulRow{tt + u.roff[r], bg + u.roff[r], dr + u.woff[k]});
-      rs1.push_back(MulRow{tt + u.roff[r], u.S(k).model->m.d_refl, s1 + u.roff[r]});
+      ic.push_back(MulRow{tt + u.roff[r], bg + u.roff[r], dr + u.wo(r)});
+      rs1.push_back(MulRow{tt + u.roff[r], u.rm[r]->d_refl, s1 + u.roff[r]});
     }
     RCCHK(u.mul_rows(ic, u.nzx(k), false));
     RCCHK(u.mul_rows(rs1, u.nzx(k), true));
@@ -1582,10 +1613,18 @@ static int uneven_born_forward_group(UnevenBorn& u, const float2* dS, const floa
 }
 
 // born_adjoint_all and born_adjoint_out on the rows of one group of shots;
-// each shot's float gradient slice is added to the FP64 accumulators gS, gR
-// [nz][nxg] at its columns, shot after shot
-static int uneven_born_adjoint_group(UnevenBorn& u, double2* gS, double2* gR, int nxg, int exact) {
+// each shot's float gradient slice (ext: each row's, into its own slice) is
+// added to the FP64 accumulators gS, gR [nslice][nz][nxg] at its columns,
+// shot after shot.  lastcut (ext, mode 0): per row, the columns [a, b) of
+// its shot's aperture whose last depth row a later shot overwrites in the
+// reference's serial loop
+// column ranges [a, b) per shot and frequency
+using ColCuts = std::vector<std::vector<std::vector<std::pair<int, int>>>>;
+
+static int uneven_born_adjoint_group(UnevenBorn& u, double2* gS, double2* gR, int nxg, int exact,
+                                     const ColCuts* lastcut) {
   const int K = u.nshots(), nz = u.nz;
+  const size_t npix = (size_t)nz * nxg;
   hipStream_t s = u.s;
   if (u.total == 0) return WEMFWI_OK;
   DevBuf B, Rb, Qb, CS, AS, AD, FS, FR, FD;
@@ -1593,12 +1632,12 @@ static int uneven_born_adjoint_group(UnevenBorn& u, double2* gS, double2* gR, in
   RCCHK(Rb.alloc(u.total * sizeof(float2), "receiver wavefield"));
   RCCHK(Qb.alloc(u.total * sizeof(float2), "receiver wavefield"));
   RCCHK(CS.alloc(u.total * sizeof(float2), "dS contributions"));
-  RCCHK(AS.alloc(u.wtotal * sizeof(double2), "dS accumulators"));
-  RCCHK(AD.alloc(u.wtotal * sizeof(double2), "dr accumulators"));
+  RCCHK(AS.alloc(u.wsize() * sizeof(double2), "dS accumulators"));
+  RCCHK(AD.alloc(u.wsize() * sizeof(double2), "dr accumulators"));
   float2 *bg = B.as<float2>(), *R = Rb.as<float2>(), *Q = Qb.as<float2>(), *cs = CS.as<float2>();
   double2 *as = AS.as<double2>(), *ad = AD.as<double2>();
-  HIPCHK(launch_zero2d(as, (long long)(2 * u.wtotal), (long long)(2 * u.wtotal), 1, s), "zero");
-  HIPCHK(launch_zero2d(ad, (long long)(2 * u.wtotal), (long long)(2 * u.wtotal), 1, s), "zero");
+  HIPCHK(launch_zero2d(as, (long long)(2 * u.wsize()), (long long)(2 * u.wsize()), 1, s), "zero");
+  HIPCHK(launch_zero2d(ad, (long long)(2 * u.wsize()), (long long)(2 * u.wsize()), 1, s), "zero");
   std::vector<int> src(K), rlo(K), rhi(K);
   for (int k = 0; k < K; ++k) {
     src[k] = u.S(k).src_iz;
@@ -1612,12 +1651,17 @@ static int uneven_born_adjoint_group(UnevenBorn& u, double2* gS, double2* gR, in
   for (int k = 0; k < K; ++k) RCCHK(u.zero_depths(R, k, rlo[k], rhi[k]));
   RCCHK(u.inject_receivers(R));
   RCCHK(u.sweep(1, 1, R, R, &rl, &rh, false, Q));
-  // dr += conj(D) * R, per shot over its rows (runs from its first row)
-  for (int k = 0; k < K; ++k) {
-    if (u.nw(k) == 0) continue;
-    HIPCHK(launch_reduce_rows(ad + u.woff[k], bg + u.base(k), (long long)u.nzx(k), (long long)u.nzx(k), u.nw(k), 0,
-                              true, s, R + u.base(k)),
-           "reduce");
+  // dr += conj(D) * R: per shot over its rows (runs from its first row), or
+  // (ext) every row into its own accumulator (one row per run: one launch)
+  if (u.ext) {
+    HIPCHK(launch_reduce_rows(ad, bg, (long long)u.total, (long long)u.total, 1, 0, true, s, R), "reduce");
+  } else {
+    for (int k = 0; k < K; ++k) {
+      if (u.nw(k) == 0) continue;
+      HIPCHK(launch_reduce_rows(ad + u.woff[k], bg + u.base(k), (long long)u.nzx(k), (long long)u.nzx(k), u.nw(k),
+                                0, true, s, R + u.base(k)),
+             "reduce");
+    }
   }
   // Q = Down^H(conj(r) R + R^H d)
   RCCHK(u.inject_receivers(Q));
@@ -1634,12 +1678,31 @@ static int uneven_born_adjoint_group(UnevenBorn& u, double2* gS, double2* gR, in
     std::vector<MulRow> rbg;
     const size_t lo = last * u.nx(k);
     for (int r = u.first[k]; r < u.first[k + 1]; ++r)
-      rbg.push_back(MulRow{bg + u.roff[r] + lo, u.S(k).model->m.d_refl + lo, bg + u.roff[r] + lo});
+      rbg.push_back(MulRow{bg + u.roff[r] + lo, u.rm[r]->d_refl + lo, bg + u.roff[r] + lo});
     RCCHK(u.mul_rows(rbg, (size_t)u.nx(k), false));
   }
   // U = Up(r D); dS += LinUp^H(R; U)
   RCCHK(u.sweep(1, 0, bg, bg));
   RCCHK(u.lin(-1, 1, exact, bg, u.per_row(R), cs));
+  if (lastcut) {
+    // the reference's serial loop: a later shot's LinDown::adjoint clears
+    // the last row of the slice (LinOneWay.cpp:64-65) where it covers it
+    for (int k = 0; k < K; ++k)
+      for (int r = u.first[k]; r < u.first[k + 1]; ++r)
+        for (const auto& c : (*lastcut)[u.s0 + k][r - u.first[k]])
+          HIPCHK(launch_zero2d(cs + u.roff[r] + last * u.nx(k) + c.first, c.second - c.first, c.second - c.first, 1,
+                               s),
+                 "zero row");
+  }
+  if (u.ext) {
+    HIPCHK(launch_reduce_rows(as, cs, (long long)u.total, (long long)u.total, 1, 0, true, s), "reduce");
+  } else {
+    for (int k = 0; k < K; ++k) {
+      if (u.nw(k) == 0) continue;
+      const long long n = (long long)u.nzx(k);
+      HIPCHK(launch_reduce_rows(as + u.woff[k], cs + u.base(k), n, n, u.nw(k), 0, true, s), "reduce");
+    }
+  }
   size_t wmax = 0;
   for (int k = 0; k < K; ++k) wmax = std::max(wmax, u.nzx(k));
   RCCHK(FS.alloc(wmax * sizeof(float2), "tmp"));
@@ -1647,44 +1710,81 @@ static int uneven_born_adjoint_group(UnevenBorn& u, double2* gS, double2* gR, in
   RCCHK(FD.alloc(wmax * sizeof(float2), "tmp"));
   for (int k = 0; k < K; ++k) {
     if (u.nw(k) == 0) continue;
-    const Model& m = u.S(k).model->m;
     const long long n = (long long)u.nzx(k);
-    HIPCHK(launch_reduce_rows(as + u.woff[k], cs + u.base(k), n, n, u.nw(k), 0, true, s), "reduce");
-    // dReflect^H of the shot's dr, then its slice into the sums over shots
-    HIPCHK(launch_d2f(FS.as<float2>(), as + u.woff[k], n, false, s), "d2f");
-    HIPCHK(launch_d2f(FD.as<float2>(), ad + u.woff[k], n, false, s), "d2f");
-    HIPCHK(launch_zero2d(FR.p, n, n, 1, s), "zero");
-    HIPCHK(launch_dreflect_adj(FS.as<float2>(), FR.as<float2>(), m.d_drefl, FD.as<float2>(), nz, u.nx(k), s),
-           "dreflect adjoint");
-    HIPCHK(launch_window_add(gS, FS.as<float2>(), nz, u.nx(k), nxg, u.S(k).x0, s), "window add");
-    HIPCHK(launch_window_add(gR, FR.as<float2>(), nz, u.nx(k), nxg, u.S(k).x0, s), "window add");
+    // dReflect^H of each window's dr, then its slice into the sums over shots
+    const int r0 = u.ext ? u.first[k] : 0, r1 = u.ext ? u.first[k + 1] : 1;
+    for (int r = r0; r < r1; ++r) {
+      const size_t o = u.ext ? u.roff[r] : u.woff[k];
+      const Model& m = u.ext ? *u.rm[r] : *u.S(k).fm[0];
+      const size_t q = u.ext ? (size_t)u.rslice[r] * npix : 0;
+      HIPCHK(launch_d2f(FS.as<float2>(), as + o, n, false, s), "d2f");
+      HIPCHK(launch_d2f(FD.as<float2>(), ad + o, n, false, s), "d2f");
+      HIPCHK(launch_zero2d(FR.p, n, n, 1, s), "zero");
+      HIPCHK(launch_dreflect_adj(FS.as<float2>(), FR.as<float2>(), m.d_drefl, FD.as<float2>(), nz, u.nx(k), s),
+             "dreflect adjoint");
+      HIPCHK(launch_window_add(gS + q, FS.as<float2>(), nz, u.nx(k), nxg, u.S(k).x0, s), "window add");
+      HIPCHK(launch_window_add(gR + q, FR.as<float2>(), nz, u.nx(k), nxg, u.S(k).x0, s), "window add");
+    }
   }
   HIPCHK(hipStreamSynchronize(s), "uneven born adjoint");
   return WEMFWI_OK;
 }
 
-// Validation and grouping of both entry points (gS / gR null: the forward)
-static int uneven_born(const char* name, const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nxg,
-                       const float2* dS, const float2* dR, float2* gS, float2* gR, int add, int exact,
-                       hipStream_t s) {
+// Validation and grouping of the entry points (gS / gR null: the forward);
+// nslice perturbation slices [nslice][nz][nxg] (1 unless ext)
+static int uneven_born(const char* name, const std::vector<UShot>& shots, bool ext, int nslice, int ngs, int ngr,
+                       int nz, int nxg, const float2* dS, const float2* dR, float2* gS, float2* gR, int add,
+                       int mode, hipStream_t s) {
   const std::string who = name;
   const bool fwd = gS == nullptr;
-  if ((!shots && ns > 0) || ns < 0 || ngs < 0 || ngr < 0 || nz < 2 || nxg < 1)
-    return fail(WEMFWI_ERR_ARG, who + ": invalid argument");
+  const int ns = (int)shots.size();
+  if (ngs < 0 || ngr < 0 || nz < 2 || nxg < 1 || nslice < 1) return fail(WEMFWI_ERR_ARG, who + ": invalid argument");
   if (fwd ? (!dS || !dR) : (!gS || !gR)) return fail(WEMFWI_ERR_ARG, who + ": null perturbation");
   // validate every shot before anything is queued
   for (int sh = 0; sh < ns; ++sh) {
-    const wemfwi_born_shot& S = shots[sh];
+    const UShot& S = shots[sh];
     const std::string at = who + ": shot " + std::to_string(sh);
-    if (!S.model || S.nw < 0 || (S.nw > 0 && (!S.freq_hz || !S.wavelet || !S.data)) || S.nr < 0 ||
+    if (S.fm.empty() || S.nw < 0 || (S.nw > 0 && (!S.freq_hz || !S.wavelet || !S.data)) || S.nr < 0 ||
         (S.nr > 0 && (!S.rec_iz || !S.rec_ix)))
       return fail(WEMFWI_ERR_ARG, at + ": null argument");
-    const Model& m = S.model->m;
-    if (m.d.nz != nz) return fail(WEMFWI_ERR_ARG, at + ": model nz differs");
+    for (Model* mp : S.fm) {
+      if (!mp) return fail(WEMFWI_ERR_ARG, at + ": null model");
+      if (mp->d.nz != nz) return fail(WEMFWI_ERR_ARG, at + ": model nz differs");
+      if (mp->d.nx != S.fm[0]->d.nx) return fail(WEMFWI_ERR_ARG, at + ": the frequencies' models differ in width");
+      if (S.nw > 0) RCCHK(mp->require_fft());
+    }
+    if (ext && (S.w0 < 0 || S.w0 + S.nw > nslice))
+      return fail(WEMFWI_ERR_ARG, at + ": band outside the perturbation slices");
     if (S.src_iz < 0 || S.src_iz >= nz) return fail(WEMFWI_ERR_ARG, at + ": source depth outside the model");
     for (int r = 0; r < S.nr; ++r)
       if (S.rec_iz[r] < 0 || S.rec_iz[r] >= nz) return fail(WEMFWI_ERR_ARG, at + ": receiver depth outside the model");
-    if (S.nw > 0) RCCHK(m.require_fft());
+  }
+  // ext, mode 0: per shot and frequency, the aperture columns [a, b) of the
+  // last depth row that a later shot with the same slice also covers
+  ColCuts cut;
+  const bool rowlast = ext && !fwd && mode == 0;
+  if (rowlast) {
+    cut.resize(ns);
+    for (int k = 0; k < ns; ++k) {
+      const int ax = shots[k].x0, bx = ax + shots[k].fm[0]->d.nx;
+      cut[k].resize(shots[k].nw);
+      for (int i = 0; i < shots[k].nw; ++i) {
+        const int q = shots[k].w0 + i;
+        std::vector<std::pair<int, int>> iv;
+        for (int k2 = k + 1; k2 < ns; ++k2)
+          if (q >= shots[k2].w0 && q < shots[k2].w0 + shots[k2].nw) {
+            const int a = std::max(ax, shots[k2].x0), b = std::min(bx, shots[k2].x0 + shots[k2].fm[0]->d.nx);
+            if (a < b) iv.push_back({a - ax, b - ax});
+          }
+        std::sort(iv.begin(), iv.end());
+        for (const auto& c : iv) {
+          if (!cut[k][i].empty() && c.first <= cut[k][i].back().second)
+            cut[k][i].back().second = std::max(cut[k][i].back().second, c.second);
+          else
+            cut[k][i].push_back(c);
+        }
+      }
+    }
   }
   size_t budget = 0;
   {
@@ -1693,7 +1793,7 @@ static int uneven_born(const char* name, const wemfwi_born_shot* shots, int ns, 
     budget = fr / 2;
   }
   DevBuf GS, GR;
-  const size_t npix = (size_t)nz * nxg;
+  const size_t npix = (size_t)nz * nxg, nacc = npix * (size_t)nslice;
   if (fwd) {
     for (int sh = 0; sh < ns; ++sh)
       if (shots[sh].nw > 0 && shots[sh].nr > 0 && !add)
@@ -1701,10 +1801,10 @@ static int uneven_born(const char* name, const wemfwi_born_shot* shots, int ns, 
                              (long long)shots[sh].nr * shots[sh].nw, 1, s),
                "zero data");
   } else {
-    RCCHK(GS.alloc(npix * sizeof(double2), "gradient accumulator"));
-    RCCHK(GR.alloc(npix * sizeof(double2), "gradient accumulator"));
-    HIPCHK(launch_zero2d(GS.p, (long long)(2 * npix), (long long)(2 * npix), 1, s), "zero");
-    HIPCHK(launch_zero2d(GR.p, (long long)(2 * npix), (long long)(2 * npix), 1, s), "zero");
+    RCCHK(GS.alloc(nacc * sizeof(double2), "gradient accumulator"));
+    RCCHK(GR.alloc(nacc * sizeof(double2), "gradient accumulator"));
+    HIPCHK(launch_zero2d(GS.p, (long long)(2 * nacc), (long long)(2 * nacc), 1, s), "zero");
+    HIPCHK(launch_zero2d(GR.p, (long long)(2 * nacc), (long long)(2 * nacc), 1, s), "zero");
   }
   // groups of consecutive shots whose wavefields fit half the free memory
   const size_t nbuf = fwd ? 3 : 4;
@@ -1713,21 +1813,23 @@ static int uneven_born(const char* name, const wemfwi_born_shot* shots, int ns, 
     int s1 = s0;
     size_t bytes = 0;
     while (s1 < ns) {
-      const Model& m = shots[s1].model->m;
-      const size_t b = (nbuf * (size_t)shots[s1].nw * sizeof(float2) + 4 * sizeof(double2)) * m.d.nz * m.d.nx;
+      const Model& m = *shots[s1].fm[0];
+      const size_t per = ext ? (nbuf + (fwd ? 3 : 4)) * (size_t)shots[s1].nw * sizeof(float2)
+                             : nbuf * (size_t)shots[s1].nw * sizeof(float2) + 4 * sizeof(double2);
+      const size_t b = per * m.d.nz * m.d.nx;
       if (s1 > s0 && bytes + b > budget) break;
       bytes += b;
       ++s1;
     }
     UnevenBorn u;
-    RCCHK(u.init(shots, s0, s1, nz, ngs, ngr, s));
+    RCCHK(u.init(shots.data(), ext, s0, s1, nz, ngs, ngr, s));
     if (fwd) RCCHK(uneven_born_forward_group(u, dS, dR, nxg));
-    else RCCHK(uneven_born_adjoint_group(u, GS.as<double2>(), GR.as<double2>(), nxg, exact));
+    else RCCHK(uneven_born_adjoint_group(u, GS.as<double2>(), GR.as<double2>(), nxg, mode, rowlast ? &cut : nullptr));
     s0 = s1;
   }
   if (!fwd) {
-    HIPCHK(launch_d2f(gS, GS.as<double2>(), (long long)npix, add != 0, s), "gradient");
-    HIPCHK(launch_d2f(gR, GR.as<double2>(), (long long)npix, add != 0, s), "gradient");
+    HIPCHK(launch_d2f(gS, GS.as<double2>(), (long long)nacc, add != 0, s), "gradient");
+    HIPCHK(launch_d2f(gR, GR.as<double2>(), (long long)nacc, add != 0, s), "gradient");
   }
   HIPCHK(hipStreamSynchronize(s), name);
   return WEMFWI_OK;
@@ -1962,17 +2064,87 @@ int wemfwi_gather_sum(long long nout, const long long* ptr, const long long* src
   return WEMFWI_OK;
 }
 
+static int ushots(const wemfwi_born_shot* shots, int ns, std::vector<UShot>& out) {
+  if ((!shots && ns > 0) || ns < 0) return fail(WEMFWI_ERR_ARG, "uneven_born: invalid argument");
+  out.resize(ns);
+  for (int k = 0; k < ns; ++k) {
+    const wemfwi_born_shot& S = shots[k];
+    UShot& u = out[k];
+    u.fm.assign(1, S.model ? &S.model->m : nullptr);
+    u.x0 = S.x0;
+    u.nw = S.nw;
+    u.freq_hz = S.freq_hz;
+    u.wavelet = S.wavelet;
+    u.src_iz = S.src_iz;
+    u.src_ix = S.src_ix;
+    u.nr = S.nr;
+    u.rec_iz = S.rec_iz;
+    u.rec_ix = S.rec_ix;
+    u.data = S.data;
+  }
+  return WEMFWI_OK;
+}
+
+static int ushots(const wemfwi_extborn_shot* shots, int ns, std::vector<UShot>& out) {
+  if ((!shots && ns > 0) || ns < 0) return fail(WEMFWI_ERR_ARG, "uneven_extborn: invalid argument");
+  out.resize(ns);
+  for (int k = 0; k < ns; ++k) {
+    const wemfwi_extborn_shot& S = shots[k];
+    UShot& u = out[k];
+    if (!S.models || S.nw < 1) {
+      u.fm.assign(1, nullptr);  // rejected by the validation
+    } else {
+      for (int i = 0; i < S.nw; ++i) u.fm.push_back(S.models[i] ? &S.models[i]->m : nullptr);
+    }
+    u.x0 = S.x0;
+    u.nw = S.nw;
+    u.w0 = S.w0;
+    u.freq_hz = S.freq_hz;
+    u.wavelet = S.wavelet;
+    u.src_iz = S.src_iz;
+    u.src_ix = S.src_ix;
+    u.nr = S.nr;
+    u.rec_iz = S.rec_iz;
+    u.rec_ix = S.rec_ix;
+    u.data = S.data;
+  }
+  return WEMFWI_OK;
+}
+
 int wemfwi_uneven_born_forward(const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
                                const wemfwi_complex* dslow, const wemfwi_complex* dden, int add, void* stream) {
-  return uneven_born("uneven_born_forward", shots, ns, ngs, ngr, nz, nx_model, (const float2*)dslow,
+  std::vector<UShot> u;
+  RCCHK(ushots(shots, ns, u));
+  return uneven_born("uneven_born_forward", u, false, 1, ngs, ngr, nz, nx_model, (const float2*)dslow,
                      (const float2*)dden, nullptr, nullptr, add, 0, (hipStream_t)stream);
 }
 
 int wemfwi_uneven_born_adjoint(const wemfwi_born_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
                                wemfwi_complex* dslow, wemfwi_complex* dden, int add, int mode, void* stream) {
   if (mode != 0 && mode != 1) return fail(WEMFWI_ERR_ARG, "uneven_born_adjoint: mode must be 0 or 1");
-  return uneven_born("uneven_born_adjoint", shots, ns, ngs, ngr, nz, nx_model, nullptr, nullptr, (float2*)dslow,
+  std::vector<UShot> u;
+  RCCHK(ushots(shots, ns, u));
+  return uneven_born("uneven_born_adjoint", u, false, 1, ngs, ngr, nz, nx_model, nullptr, nullptr, (float2*)dslow,
                      (float2*)dden, add, mode, (hipStream_t)stream);
+}
+
+int wemfwi_uneven_extborn_forward(const wemfwi_extborn_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
+                                  int nslice, const wemfwi_complex* dslow, const wemfwi_complex* dden, int add,
+                                  void* stream) {
+  std::vector<UShot> u;
+  RCCHK(ushots(shots, ns, u));
+  return uneven_born("uneven_extborn_forward", u, true, nslice, ngs, ngr, nz, nx_model, (const float2*)dslow,
+                     (const float2*)dden, nullptr, nullptr, add, 0, (hipStream_t)stream);
+}
+
+int wemfwi_uneven_extborn_adjoint(const wemfwi_extborn_shot* shots, int ns, int ngs, int ngr, int nz, int nx_model,
+                                  int nslice, wemfwi_complex* dslow, wemfwi_complex* dden, int add, int mode,
+                                  void* stream) {
+  if (mode != 0 && mode != 1) return fail(WEMFWI_ERR_ARG, "uneven_extborn_adjoint: mode must be 0 or 1");
+  std::vector<UShot> u;
+  RCCHK(ushots(shots, ns, u));
+  return uneven_born("uneven_extborn_adjoint", u, true, nslice, ngs, ngr, nz, nx_model, nullptr, nullptr,
+                     (float2*)dslow, (float2*)dden, add, mode, (hipStream_t)stream);
 }
 
 int wemfwi_bornrefl_forward(wemfwi_model* model, wemfwi_survey* survey, const wemfwi_complex* image,

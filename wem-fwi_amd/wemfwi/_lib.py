@@ -88,6 +88,8 @@ SIGNATURES = {
     "wemfwi_uneven_offset_gradient": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, C.POINTER(C.c_double), _i, _vp]),
     "wemfwi_uneven_born_forward": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _i, _vp]),
     "wemfwi_uneven_born_adjoint": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp]),
+    "wemfwi_uneven_extborn_forward": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _i, _vp]),
+    "wemfwi_uneven_extborn_adjoint": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp]),
     "wemfwi_wem_forward": (_i, [_vp, _vp, _vp, _i, _vp]),
     "wemfwi_born_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "wemfwi_born_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),

@@ -352,6 +352,83 @@ def uneven_born_adjoint(shots: Sequence[BornShot], dslow: torch.Tensor, dden: to
     return dslow, dden
 
 
+class ExtBornShotDesc(C.Structure):
+    _fields_ = [
+        ("models", C.c_void_p), ("x0", C.c_int), ("nw", C.c_int), ("w0", C.c_int), ("freq_hz", C.POINTER(C.c_float)),
+        ("wavelet", C.c_void_p), ("src_iz", C.c_int), ("src_ix", C.c_int), ("nr", C.c_int),
+        ("rec_iz", C.POINTER(C.c_int)), ("rec_ix", C.POINTER(C.c_int)), ("data", C.c_void_p),
+    ]
+
+
+class ExtBornShot:
+    """One shot of the uneven extBorn operators: one background model per
+    frequency (models[i], nz x nx_s, the shot's aperture) placed at column x0
+    of the perturbation grid; frequency i meets perturbation slice w0 + i;
+    its frequencies, source spectrum, source point, receivers (aperture grid)
+    and data, a CUDA complex64 tensor [nr][nw]."""
+
+    def __init__(self, models: Sequence[Model], x0: int, w0: int, freq_hz, wavelet, src_iz: int, src_ix: int,
+                 rec_iz, rec_ix, data: torch.Tensor):
+        self.models = list(models)
+        self.x0, self.w0 = int(x0), int(w0)
+        self.freq = np.ascontiguousarray(np.asarray(freq_hz, dtype=np.float32))
+        self.wavelet = np.ascontiguousarray(np.asarray(wavelet, dtype=np.complex64))
+        self.src = (int(src_iz), int(src_ix))
+        self.rec_iz, _ = _ip(rec_iz)
+        self.rec_ix, _ = _ip(rec_ix)
+        if len(self.models) != len(self.freq) or len(self.wavelet) != len(self.freq):
+            raise ValueError("ExtBornShot: one model and one wavelet sample per frequency")
+        if len(self.rec_iz) != len(self.rec_ix):
+            raise ValueError("ExtBornShot: receiver list sizes differ")
+        _dev_c64(data, "data")
+        if tuple(data.shape) != (len(self.rec_iz), len(self.freq)):
+            raise ValueError("ExtBornShot: data must be [nr][nw]")
+        self.data = data
+        self._mh = _models(self.models)
+
+    def desc(self) -> ExtBornShotDesc:
+        return ExtBornShotDesc(models=C.cast(self._mh, C.c_void_p), x0=self.x0, nw=len(self.freq), w0=self.w0,
+                               freq_hz=self.freq.ctypes.data_as(C.POINTER(C.c_float)),
+                               wavelet=self.wavelet.ctypes.data, src_iz=self.src[0], src_ix=self.src[1],
+                               nr=len(self.rec_iz), rec_iz=self.rec_iz.ctypes.data_as(C.POINTER(C.c_int)),
+                               rec_ix=self.rec_ix.ctypes.data_as(C.POINTER(C.c_int)), data=_ptr(self.data))
+
+
+def _ext_slices(dslow: torch.Tensor, dden: torch.Tensor):
+    _dev_c64(dslow, "dslow")
+    _dev_c64(dden, "dden")
+    if dslow.dim() != 3 or tuple(dden.shape) != tuple(dslow.shape):
+        raise ValueError("dslow and dden must both be [nslice][nz][nx_model]")
+
+
+def uneven_extborn_forward(shots: Sequence[ExtBornShot], dslow: torch.Tensor, dden: torch.Tensor, ngs: int = 0,
+                           ngr: int = 0, add: bool = False, stream=None):
+    """extBornFull::forward of shots with their own apertures and bands
+    (ext_born_prop.cpp:8-97 per shot): every shot's data [nr][nw] receives
+    the scattered field of its frequencies' perturbation slices w0 + i of
+    dslow, dden [nslice][nz][nx_model] (CUDA complex64) windowed to its
+    aperture; all rows swept in batched launches."""
+    _ext_slices(dslow, dden)
+    arr = (ExtBornShotDesc * max(1, len(shots)))(*[sh.desc() for sh in shots])
+    check(lib.wemfwi_uneven_extborn_forward(C.cast(arr, C.c_void_p), len(shots), int(ngs), int(ngr),
+                                            int(dslow.shape[1]), int(dslow.shape[2]), int(dslow.shape[0]),
+                                            _ptr(dslow), _ptr(dden), int(add), _stream_ptr(stream)))
+    return [sh.data for sh in shots]
+
+
+def uneven_extborn_adjoint(shots: Sequence[ExtBornShot], dslow: torch.Tensor, dden: torch.Tensor, ngs: int = 0,
+                           ngr: int = 0, add: bool = False, exact: bool = False, stream=None):
+    """extBornFull::adjoint of the same shots into the slices dslow, dden
+    [nslice][nz][nx_model], summed over shots in shot order in FP64; exact =
+    False keeps the reference's last-row rule of its serial shot loop."""
+    _ext_slices(dslow, dden)
+    arr = (ExtBornShotDesc * max(1, len(shots)))(*[sh.desc() for sh in shots])
+    check(lib.wemfwi_uneven_extborn_adjoint(C.cast(arr, C.c_void_p), len(shots), int(ngs), int(ngr),
+                                            int(dslow.shape[1]), int(dslow.shape[2]), int(dslow.shape[0]),
+                                            _ptr(dslow), _ptr(dden), int(add), int(exact), _stream_ptr(stream)))
+    return dslow, dden
+
+
 def shotwem_forward(models: Sequence[Model], survey: Survey, data: torch.Tensor, add: bool = False,
                     stream=None):
     """WEM with one background model per shot (conformal-map grids)."""
