@@ -148,7 +148,8 @@ __global__ void __launch_bounds__(kTX)
         }
       }
       __syncthreads();
-      if (x >= 0 && x < nx) {
+      if (NHM > 21 && x >= 0 && x < nx) {
+        // 41 lags: one at a time (their FP64 sums already fill the registers)
         for (int k = 0; k < kr; ++k) {
 #pragma unroll
           for (int j = 0; j < NHM; ++j) {
@@ -158,6 +159,34 @@ __global__ void __launch_bounds__(kTX)
               ax[j] += v.x;
               ay[j] += v.y;
             }
+          }
+        }
+      } else if (x >= 0 && x < nx) {
+        for (int k = 0; k < kr; ++k) {
+          // every lag without a branch: the staged segments are 0 outside
+          // [0, nx), so an absent term is a zero product, which leaves the
+          // (never negative-zero) accumulator as it is; lags past nh read a
+          // valid pair and are never written out
+          // (chunks of up to 11 lags in flight at a time)
+          constexpr int CH = NHM < 11 ? NHM : 11;
+#pragma unroll
+          for (int j0 = 0; j0 < NHM; j0 += CH) {
+            float2 pr[CH], pd[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              const int h = min(j0 + c - hh, hh);
+              pr[c] = sr[k * kSeg + t + hh + h];
+              pd[c] = sd[k * kSeg + t + hh - h];
+            }
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              if (j0 + c < NHM) {
+                const float2 v = cmulc(pr[c], pd[c]);
+                ax[j0 + c] += v.x;
+                ay[j0 + c] += v.y;
+              }
+            }
+            __builtin_amdgcn_sched_barrier(0);
           }
         }
       }
@@ -254,16 +283,31 @@ __global__ void __launch_bounds__(kTX)
       __syncthreads();
       if (y >= 0 && y < nx) {
         for (int k = 0; k < kr; ++k) {
+          // every lag without a branch: y and y - 2h inside [0, nx) put y - h
+          // inside too, and the staged D segment is 0 outside [0, nx), so an
+          // absent term is a zero product (the accumulator, never negative
+          // zero, keeps its value); lags past nh add exact zeros
+          // (chunks of up to 11 lags in flight at a time)
+          constexpr int CH = NHM < 11 ? NHM : 11;
           float ax = 0.f, ay = 0.f;
 #pragma unroll
-          for (int j = 0; j < NHM; ++j) {
-            const int h = j - hh;
-            if (j < nh && y - h >= 0 && y - h < nx && y - 2 * h >= 0 && y - 2 * h < nx) {
-              const float2 p = si[j * wi + t - h + hh];
-              const float2 b = sd[k * WD + t - 2 * h + 2 * hh];
-              ax = fmaf(p.x, b.x, fmaf(-p.y, b.y, ax));
-              ay = fmaf(p.x, b.y, fmaf(p.y, b.x, ay));
+          for (int j0 = 0; j0 < NHM; j0 += CH) {
+            float2 pi[CH], pb[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              const int j = j0 + c;
+              const int h = min(j - hh, hh);
+              pi[c] = (j < nh && j < NHM) ? si[j * wi + t - h + hh] : make_float2(0.f, 0.f);
+              pb[c] = sd[k * WD + t - 2 * h + 2 * hh];
             }
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              if (j0 + c < NHM) {
+                ax = fmaf(pi[c].x, pb[c].x, fmaf(-pi[c].y, pb[c].y, ax));
+                ay = fmaf(pi[c].x, pb[c].y, fmaf(pi[c].y, pb[c].x, ay));
+              }
+            }
+            __builtin_amdgcn_sched_barrier(0);
           }
           U[S.roff + (r0 + k) * rstride + (long long)z * nx + y] = make_float2(ax, ay);
         }
