@@ -337,6 +337,16 @@ def test_uneven_offset_kernels_tiles(nhm):
             assert (j * wi + t - h + hh).min() >= j * wi and (j * wi + t - h + hh).max() < (j + 1) * wi
             b = t - 2 * h + 2 * hh
             assert b.min() >= 0 and b.max() < wd
+        # the branch-free lag loops run all NHM lags of the template with
+        # h = min(j - hh, hh) (lags past nh read a valid pair, never used;
+        # the scatter reads no image value for them)
+        assert "const int h = min(j0 + c - hh, hh);" in src or "const int h = min(j - hh, hh);" in src
+        for j in range(nhm):
+            h = min(j - hh, hh)
+            for idx in (t + hh + h, t + hh - h):
+                assert idx.min() >= 0 and idx.max() < seg
+            b = t - 2 * h + 2 * hh
+            assert b.min() >= 0 and b.max() < wd
 
 
 def test_uneven_image_columns_inside_the_image():
