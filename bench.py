@@ -594,7 +594,11 @@ def uneven_image_bench(sample_rows: int):
     rows = [(s, i) for s in range(len(survey)) for i in range(len(survey[s][2]))]
     upd = 2.0 * sum(survey[s][1] * (nz - 1) for s, _ in rows)
     img = torch.zeros((nh, nz, W0), dtype=torch.complex64, device=dev)
-    uneven_offset_image_adjoint(shots, nh, img)  # warmup (plans, phase tables)
+    # warm-up: plans, phase tables, and the batch plans' launch orders (the
+    # in-place sweeps of the call time one candidate order per call until
+    # each has run twice)
+    for _ in range(9):
+        uneven_offset_image_adjoint(shots, nh, img)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     uneven_offset_image_adjoint(shots, nh, img)
@@ -664,7 +668,8 @@ def extended_gradient_bench(shots, one, grouped, model_img, nh, nrows, nsample, 
     scale = float(torch.linalg.vector_norm(I)) or 1.0
     I.mul_(1.0 / scale)
     g = torch.zeros_like(I)
-    uneven_offset_gradient(shots, nh, I, g)   # warmup
+    for _ in range(9):                        # warm-up, launch orders as above
+        uneven_offset_gradient(shots, nh, I, g)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J = uneven_offset_gradient(shots, nh, I, g)

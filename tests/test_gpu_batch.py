@@ -149,3 +149,42 @@ def test_batch_launch_order_is_timed_once_per_shape_and_reused():
         assert torch.equal(a, c) and torch.equal(b, c)
     for m in models:
         m.close()
+
+
+def test_in_place_batch_times_one_order_per_call():
+    """In-place plans cannot repeat a run: each call of a new group shape
+    launches the candidate order tried least so far and times it; after two
+    calls per candidate the shape has its decision.  Every call stays
+    bitwise equal to the per-model sweeps."""
+    import torch
+    import wemfwi
+    from wemfwi import Model
+    from wemfwi.core import oneway_batch
+    slows = [_slow(nz, nx, 40 + i) for i, (nz, nx, _, _, _) in enumerate(MODELS)]
+    models = [Model(s, 10.0, 10.0, tap=tap, pad=6, nref=nref, prop=prop, eps=0.04)
+              for s, (nz, nx, tap, prop, nref) in zip(slows, MODELS)]
+    rows = _rows()[:9]     # a group shape no other test builds
+    rng = np.random.default_rng(21)
+    src = [torch.from_numpy((rng.standard_normal(MODELS[mi][:2]) + 0j).astype(np.complex64)).cuda()
+           for mi, _ in rows]
+    ms, fs = [models[mi] for mi, _ in rows], [f for _, f in rows]
+    want = []
+    for (mi, f), a in zip(rows, src):
+        ref = torch.zeros_like(a)
+        models[mi].oneway_recursive(True, True, [f], a[None], ref[None])
+        want.append(ref)
+    ngroups = len({(MODELS[mi][1], MODELS[mi][2], MODELS[mi][3]) for mi, _ in rows})
+    before = len([e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"])
+    calls = 2 * min(ngroups, 4)
+    for k in range(calls):
+        inpl = [a.clone() for a in src]
+        oneway_batch(True, True, ms, fs, inpl, inpl)
+        for a, b in zip(inpl, want):
+            assert torch.equal(a, b), f"call {k}"
+        n = len([e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"])
+        assert n == before + (1 if k == calls - 1 else 0), f"call {k}: decision count {n}"
+    e = [e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"][-1]
+    assert len(e["ms"]) == min(ngroups, 4) and all(v > 0 for v in e["ms"])
+    assert e["order"][0] == int(np.argmin(e["ms"]))
+    for m in models:
+        m.close()

@@ -884,8 +884,10 @@ int BatchPlan::enqueue(int up, int adjoint, hipStream_t s, bool tables) {
 // Launch-order decisions per device and group shape (each group's FFT
 // length, variant flags, depth count and row count), shared by every plan
 // of that shape: plans built per call (the one-call batch, the uneven
-// imaging and gradient drivers) time the orders once per process, and
-// plans whose runs cannot be repeated reuse a decision of the same shape.
+// imaging and gradient drivers) time the orders once per process; plans
+// whose runs cannot be repeated (in place) reuse a decision of the same
+// shape, or time one candidate per run() until every candidate has run
+// twice (run_trial).
 struct OrderEntry {
   std::vector<long long> key;
   std::vector<int> order;
@@ -897,6 +899,17 @@ static std::mutex& order_mutex() {
 }
 static std::vector<OrderEntry>& order_table() {
   static std::vector<OrderEntry> t;
+  return t;
+}
+// shapes being timed one run at a time (plans whose runs cannot be repeated)
+struct TrialEntry {
+  std::vector<long long> key;
+  std::vector<std::vector<int>> cand;
+  std::vector<float> ms;
+  std::vector<int> tries;
+};
+static std::vector<TrialEntry>& trial_table() {
+  static std::vector<TrialEntry> t;
   return t;
 }
 
@@ -953,6 +966,7 @@ int BatchPlan::tune_order(int up, int adjoint, hipStream_t s) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return WEMFWI_OK;
   const std::vector<long long> key = order_key(groups);
+  shape = key;
   {
     std::lock_guard<std::mutex> lock(order_mutex());
     for (const OrderEntry& e : order_table())
@@ -964,16 +978,12 @@ int BatchPlan::tune_order(int up, int adjoint, hipStream_t s) {
       }
   }
   if (!rerunnable) {
+    // timed across runs instead (run_trial); enqueue keeps the current order
     order_done = true;
+    online = true;
     return WEMFWI_OK;
   }
-  std::vector<std::vector<int>> cand;
-  for (size_t f = 0; f < groups.size() && cand.size() < 4; ++f) {
-    std::vector<int> o{(int)f};
-    for (size_t k = 0; k < groups.size(); ++k)
-      if (k != f) o.push_back((int)k);
-    cand.push_back(o);
-  }
+  const std::vector<std::vector<int>> cand = candidates();
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -1007,6 +1017,73 @@ int BatchPlan::tune_order(int up, int adjoint, hipStream_t s) {
   return WEMFWI_OK;
 }
 
+// the current order and each other group moved to the front (at most 4)
+std::vector<std::vector<int>> BatchPlan::candidates() const {
+  std::vector<std::vector<int>> cand;
+  for (size_t f = 0; f < groups.size() && cand.size() < 4; ++f) {
+    std::vector<int> o{(int)f};
+    for (size_t k = 0; k < groups.size(); ++k)
+      if (k != f) o.push_back((int)k);
+    cand.push_back(o);
+  }
+  return cand;
+}
+
+// One run of an online plan: the candidate order of its shape tried least so
+// far (results do not depend on the order), timed with HIP events on s.
+int BatchPlan::run_trial(int up, int adjoint, hipStream_t s) {
+  std::vector<int> ord;
+  size_t ci = 0;
+  {
+    std::lock_guard<std::mutex> lock(order_mutex());
+    for (const OrderEntry& e : order_table())
+      if (e.key == shape) {
+        ord = e.order;  // decided meanwhile (another plan of the shape)
+        break;
+      }
+    if (ord.empty()) {
+      auto& tt = trial_table();
+      auto it = std::find_if(tt.begin(), tt.end(), [&](const TrialEntry& e) { return e.key == shape; });
+      if (it == tt.end()) {
+        const auto cand = candidates();
+        tt.push_back(TrialEntry{shape, cand, std::vector<float>(cand.size(), 0.f), std::vector<int>(cand.size(), 0)});
+        it = tt.end() - 1;
+      }
+      for (size_t c = 1; c < it->cand.size(); ++c)
+        if (it->tries[c] < it->tries[ci]) ci = c;
+      ord = it->cand[ci];
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  int rc = WEMFWI_OK;
+  if (e == hipSuccess) rc = launch(up, adjoint, s, ord);
+  if (e == hipSuccess && rc == WEMFWI_OK) e = hipEventRecord(e1, s);
+  if (e == hipSuccess && rc == WEMFWI_OK) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess && rc == WEMFWI_OK) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "batch run");
+  std::lock_guard<std::mutex> lock(order_mutex());
+  auto& tt = trial_table();
+  auto it = std::find_if(tt.begin(), tt.end(), [&](const TrialEntry& x) { return x.key == shape; });
+  if (it == tt.end() || it->cand[ci] != ord) return WEMFWI_OK;
+  it->ms[ci] = it->tries[ci] ? std::min(it->ms[ci], t) : t;
+  it->tries[ci] += 1;
+  if (*std::min_element(it->tries.begin(), it->tries.end()) >= 2) {
+    size_t best = 0;
+    for (size_t c = 1; c < it->cand.size(); ++c)
+      if (it->ms[c] < it->ms[best]) best = c;
+    order_table().push_back(OrderEntry{shape, it->cand[best], it->ms});
+    tt.erase(it);
+  }
+  return WEMFWI_OK;
+}
+
 int BatchPlan::launch(int up, int adjoint, hipStream_t s, const std::vector<int>& ord) {
   if (ord.size() != groups.size()) return fail(WEMFWI_ERR_ARG, "batch: launch order does not cover the groups");
   hipStream_t st[4];
@@ -1034,6 +1111,17 @@ int BatchPlan::launch(int up, int adjoint, hipStream_t s, const std::vector<int>
 }
 
 int BatchPlan::run(int up, int adjoint, hipStream_t s) {
+  if (int rc = check()) return rc;
+  if (!order_done)
+    if (int rc = tune_order(up, adjoint, s)) return rc;
+  if (online) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+      if (int rc = run_trial(up, adjoint, s)) return rc;
+      hipError_t e = hipStreamSynchronize(s);
+      return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch run");
+    }
+  }
   if (int rc = enqueue(up, adjoint, s, false)) return rc;
   hipError_t e = hipStreamSynchronize(s);
   return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "batch run");

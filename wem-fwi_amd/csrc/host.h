@@ -149,6 +149,12 @@ struct BatchPlan {
   std::vector<float> order_ms;         // best time of each candidate (empty: not timed)
   bool order_done = false;
   bool rerunnable = false;
+  // a plan whose runs cannot be repeated and whose shape has no decision
+  // yet: each run() (which waits for its work anyway) launches the candidate
+  // order tried least so far for that shape and records its time; after two
+  // runs of every candidate the shape has its decision
+  bool online = false;
+  std::vector<long long> shape;
   ~BatchPlan();
   // Optional per-row products of the sweep kernel (SweepRow smul, out2,
   // omul): arrays of nrows pointers, NULL for none
@@ -174,6 +180,8 @@ struct BatchPlan {
  private:
   int launch(int up, int adjoint, hipStream_t s, const std::vector<int>& ord);
   int tune_order(int up, int adjoint, hipStream_t s);
+  int run_trial(int up, int adjoint, hipStream_t s);
+  std::vector<std::vector<int>> candidates() const;
 };
 
 // JSON entries (each starting with ", ") of the launch-order decisions so far
