@@ -921,8 +921,10 @@ def main():
                                 "frac_of_scalar_fp32": round(tfl / VALU_SCALAR_TFLOPS, 4),
                                 "flops_per_launch": flops,
                                 "flop_per_byte": round(flops / (BYTES_PER_UPDATE * upd_sweep), 2)}
-    # the sweep autotuner's radix-plan choice for this workload's launches
-    line["autotune"] = [e for e in wemfwi.autotune_report() if e["kernel"] == "oneway_kernel" and e["N"] == nx]
+    # the sweep autotuner's radix-plan and workgroup-tile choices for this workload's launches
+    line["autotune"] = [e for e in wemfwi.autotune_report()
+                        if (e["kernel"] == "oneway_kernel" and e["N"] == nx)
+                        or (e["kernel"] == "oneway_tile" and e["N"] == nx and e["rows"] == nrows)]
     line["fwd_gupdates_per_s"] = round(upd_sweep / t_fwd / 1e9, 3)
     line["adj_gupdates_per_s"] = round(upd_sweep / t_adj / 1e9, 3)
     if args.config == "pspi1024":
@@ -932,7 +934,12 @@ def main():
     if args.uneven_sample > 0:
         line["uneven_batch"] = uneven_batch_bench(args.uneven_sample)
         # the batch plan's launch-order decision (groups, order, best time of each candidate)
-        line["uneven_batch"]["launch_order"] = [e for e in wemfwi.autotune_report() if e["kernel"] == "batch_order"]
+        rep = wemfwi.autotune_report()
+        line["uneven_batch"]["launch_order"] = [e for e in rep if e["kernel"] == "batch_order"]
+        # the workgroup tile each FFT-length group of the batch runs with
+        groups = {(g["N"], g["rows"]) for e in line["uneven_batch"]["launch_order"] for g in e["groups"]}
+        line["uneven_batch"]["tiles"] = [{"N": e["N"], "rows": e["rows"], "tile": e["tile"], "ms": e["ms"]}
+                                         for e in rep if e["kernel"] == "oneway_tile" and (e["N"], e["rows"]) in groups]
         line["uneven_offset_image"] = uneven_image_bench(args.uneven_sample)
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)

@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libwemfwi.so")
-SOURCES = ["kernels.hip", "born.hip", "host.cpp", "ops.cpp", "params.cpp", "traces.hip", "cmap.cpp", "efwi.hip", "interp.hip", "uneven.hip", "noalloc.hip"]
+SOURCES = ["kernels.hip", "sweep_tile.hip", "born.hip", "host.cpp", "ops.cpp", "params.cpp", "traces.hip", "cmap.cpp", "efwi.hip", "interp.hip", "uneven.hip", "noalloc.hip"]
 # Host-only sources compiled as plain C++: the conformal-map geometry must see
 # the C library's complex functions (std::abs of std::complex<float> is glibc's
 # hypotf there; HIP compile mode substitutes an inline formula that rounds

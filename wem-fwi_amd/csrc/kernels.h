@@ -64,6 +64,12 @@ hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const d
 hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
                          StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s,
                          bool tune = true);
+// sweep launch of radix plan k with the half workgroup tile (half the waves
+// per workgroup of launch_oneway's full tile); only for the FFT lengths of
+// oneway_half_tile
+hipError_t launch_oneway_half(int N, int k, const SweepRow* rows, int nrows, const DepthTables* models,
+                              StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s);
+constexpr bool oneway_half_tile(int N) { return N == 1024 || N == 2048; }
 hipError_t launch_onestep(int N, const StepRow* rows, int nrows, const DepthTables* models,
                           StepGeom g, bool adj, const float2* tw, hipStream_t s);
 // StepGeom.tdma of a sweep launch of `rows` rows that use `tables` distinct

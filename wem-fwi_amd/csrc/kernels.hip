@@ -11,6 +11,7 @@
 #include "../../include/wemfwi.h"
 #include "kernels.h"
 #include "sweep.hpp"
+#include "sweep_kernel.hpp"
 
 namespace wemfwi {
 
@@ -50,146 +51,6 @@ __global__ void phase_table_kernel(float2* __restrict__ ptab, const double* __re
     double sn, cs;
     sincos(re * dz, &sn, &cs);
     ptab[i] = round_complex(amp * cs, -amp * sn);
-  }
-}
-
-// ------------------------------------------------------------------ rows
-
-template <class V>
-__device__ __forceinline__ void load_row(float2 (&v)[V::E], const float2* __restrict__ p, const StepGeom& g,
-                                         int t) {
-  load_samples<V>(v, p, g, t);
-}
-
-template <class V>
-__device__ __forceinline__ void store_row(float2* __restrict__ p, const float2 (&v)[V::E], const StepGeom& g,
-                                          int t) {
-#pragma unroll
-  for (int e = 0; e < V::E; ++e) row_store<V>(p, v[e], g, t, e);
-}
-
-// Waves (rows) per workgroup: the twiddle table in LDS is shared by the
-// waves of a workgroup.  For N = 1024 two workgroups of 8 waves (2 x 77.6 KB
-// of LDS) give 16 waves per CU, i.e. 4 per SIMD; for N = 2048 one workgroup
-// of 8 waves (155 KB) gives 2 per SIMD.
-template <int N>
-struct WavesPerBlock {
-  static constexpr int value = N >= 1024 ? 8 : 4;
-};
-// GEN rows use two LDS row buffers per wave: 4 waves of 2 x 34.8 KB at
-// N = 2048
-template <class V>
-struct WavesV {
-  static constexpr int value = (V::GEN && V::n == 2048) ? 4 : WavesPerBlock<V::n>::value;
-};
-
-// In-place or out-of-place Down/Up recursion (OneWay.cpp:32-109) of one
-// (shot, frequency) row per wavefront.  step = +1 runs iz = 0..nz-2 (Down
-// forward, Up adjoint), step = -1 runs iz = nz-1..1 (Down adjoint, Up forward).
-// Forward steps use depth cur, adjoint steps depth next.  The row's width,
-// padding, taper and depth count come from its model, so one launch may hold
-// rows of different apertures; the launch fixes the FFT length and variant.
-// Waves per SIMD the register allocator targets for an FFT length.
-template <int N>
-struct Occupancy {
-  static constexpr int value = N <= 256 ? 8 : (N == 512 ? 6 : (N == 1024 ? 4 : 2));
-};
-
-template <class V, bool ADJ, bool PF, bool TD>
-__global__ void __launch_bounds__(64 * WavesV<V>::value)
-    __attribute__((amdgpu_waves_per_eu(V::GEN ? 2 : Occupancy<V::n>::value, 8)))
-    oneway_kernel(const SweepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
-                  StepGeom g, int step, const float2* __restrict__ twg) {
-  constexpr int N = V::n;
-  constexpr int E = V::E;
-  constexpr int kWavesPerBlock = WavesV<V>::value;
-  __shared__ float2 tw[TwV<V>::value];
-  __shared__ float2 bufs[kWavesPerBlock][LdsV<V>::value];
-  const int t = threadIdx.x & 63;
-  const int wv = wave_id();
-  for (int i = threadIdx.x; i < TwV<V>::value; i += blockDim.x) tw[i] = ldg(twg, (size_t)i);
-  __syncthreads();
-  const int row = blockIdx.x * kWavesPerBlock + wv;
-  if (row >= nrows) return;
-  const SweepRow R = ldu_struct(rows, row);
-  const DepthTables dt = ldu_struct(models, R.model);
-  g.nx = dt.nx;
-  g.tap = dt.tap;
-  g.taper = dt.taper;
-  float2* buf = bufs[wv];
-  const int nx = g.nx;
-  const bool recursive = R.recursive != 0;
-  // a row covers depths first .. first + step * nsteps (a whole sweep, or a
-  // segment started from a checkpointed state row)
-  const int first = R.first >= 0 ? R.first : (step > 0 ? 0 : dt.nz - 1);
-  const int nsteps = R.nsteps >= 0 ? R.nsteps : dt.nz - 1;
-  const int zb = R.zbase;
-  float2 st[E];
-  if (R.init || !R.src_skip || (first >= R.src_lo && first <= R.src_hi)) {
-    load_row<V>(st, R.init ? R.init : R.in + (size_t)(first - zb) * nx, g, t);
-  } else {
-#pragma unroll
-    for (int e = 0; e < E; ++e) st[e] = make_float2(0.f, 0.f);
-  }
-  if (R.smul && !R.init) {
-#pragma unroll
-    for (int e = 0; e < E; ++e) st[e] = cmul(st[e], row_sample<V>(R.smul + (size_t)first * nx, g, t, e));
-  }
-  // the first row is stored unless it already holds the state (in place,
-  // read as it is); with src_skip it may not have been initialised
-  if (R.in != R.out || R.init || R.smul || R.src_skip) store_row<V>(R.out + (size_t)(first - zb) * nx, st, g, t);
-  if (R.out2) {
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-      row_store<V>(R.out2 + (size_t)(first - zb) * nx, cmulc(st[e], row_sample<V>(R.omul + (size_t)first * nx, g, t, e)),
-                   g, t, e);
-  }
-  // PF (full rows whose every depth has a plain source row; the launch
-  // guarantees it): the source row is read by LDS DMA into the wave's
-  // transform buffer, started once the step's last inverse transform has
-  // read that buffer for the last time (in its last pass, or right after the
-  // split step's table loads), so that the row's HBM latency overlaps the
-  // rest of the step's arithmetic instead of stalling the source add.
-  constexpr bool PFX = PF && V::FULL;
-  for (int k = 0; k < nsteps; ++k) {
-    const int cur = first + k * step;
-    const int nxt = cur + step;
-    if (!recursive && k > 0) load_row<V>(st, R.in + (size_t)(cur - zb) * nx, g, t);
-    const bool src_here = PFX || !R.src_skip || (nxt >= R.src_lo && nxt <= R.src_hi);
-    const float2* pre = nullptr;
-    if constexpr (PFX) {
-      pre = R.in + (size_t)(nxt - zb) * nx;
-      __builtin_assume(pre != nullptr);
-    }
-    bool in_lds;
-    if constexpr (ADJ) in_lds = step_adjoint<V, true, TD>(st, nxt, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
-    else in_lds = step_forward<V, true, TD>(st, cur, R.w, dt, g, R.ptab, buf, tw, t, R.scratch, pre);
-    if (src_here) {
-      float2 src[E];
-      if (PFX && in_lds) {
-        // the DMA writes LDS behind the memory counter: wait for it
-        // explicitly (the compiler does not track the copy through the
-        // generic buffer pointer)
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-#pragma unroll
-        for (int e = 0; e < E; ++e) src[e] = buf[t + 64 * e];
-      } else {
-        load_row<V>(src, R.in + (size_t)(nxt - zb) * nx, g, t);
-      }
-      if (!PFX && R.smul) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) src[e] = cmul(src[e], row_sample<V>(R.smul + (size_t)nxt * nx, g, t, e));
-      }
-#pragma unroll
-      for (int e = 0; e < E; ++e) st[e] = cadd(st[e], src[e]);
-    }
-    store_row<V>(R.out + (size_t)(nxt - zb) * nx, st, g, t);
-    if (R.out2) {
-#pragma unroll
-      for (int e = 0; e < E; ++e)
-        row_store<V>(R.out2 + (size_t)(nxt - zb) * nx, cmulc(st[e], row_sample<V>(R.omul + (size_t)nxt * nx, g, t, e)),
-                     g, t, e);
-    }
   }
 }
 
@@ -295,58 +156,6 @@ hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const d
   return hipGetLastError();
 }
 
-// TD: the phase-table DMA variant (g.tdma)
-template <class V, bool TD>
-static hipError_t launch_oneway_w(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int step,
-                                  bool adj, const float2* tw, hipStream_t s) {
-  constexpr int kWavesPerBlock = WavesV<V>::value;
-  dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
-  // the source-row prefetch only on full rows, when the caller states that
-  // every row reads a plain source row at every depth (g.pf)
-  if constexpr (V::FULL) {
-    if (g.pf) {
-      if (adj)
-        hipLaunchKernelGGL((oneway_kernel<V, true, true, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
-      else
-        hipLaunchKernelGGL((oneway_kernel<V, false, true, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
-      return hipGetLastError();
-    }
-  }
-  if (adj)
-    hipLaunchKernelGGL((oneway_kernel<V, true, false, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
-  else
-    hipLaunchKernelGGL((oneway_kernel<V, false, false, TD>), grid, block, 0, s, rows, nrows, models, g, step, tw);
-  return hipGetLastError();
-}
-
-template <class V>
-static hipError_t launch_oneway_v(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
-                                  int step, bool adj, const float2* tw, hipStream_t s) {
-  if (g.tdma && !V::GEN) return launch_oneway_w<V, true>(rows, nrows, models, g, step, adj, tw, s);
-  return launch_oneway_w<V, false>(rows, nrows, models, g, step, adj, tw, s);
-}
-
-// the four register-FFT variants of radix plan K
-template <int N, int K>
-static hipError_t launch_oneway_k(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
-                                  int step, bool adj, const float2* tw, hipStream_t s) {
-  if (g.full && g.ssf)
-    return launch_oneway_v<Var<N, true, true, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
-  if (g.full) return launch_oneway_v<Var<N, true, false, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
-  if (g.ssf) return launch_oneway_v<Var<N, false, true, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
-  return launch_oneway_v<Var<N, false, false, false, K>>(rows, nrows, models, g, nz, step, adj, tw, s);
-}
-
-template <int N, int K = 0>
-static hipError_t launch_oneway_plan(int k, const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g,
-                                     int nz, int step, bool adj, const float2* tw, hipStream_t s) {
-  if constexpr (K + 1 < FftPlans<N>::value) {
-    if (k != K) return launch_oneway_plan<N, K + 1>(k, rows, nrows, models, g, nz, step, adj, tw, s);
-  }
-  if (k != K) return hipErrorInvalidValue;
-  return launch_oneway_k<N, K>(rows, nrows, models, g, nz, step, adj, tw, s);
-}
-
 int table_dma(size_t rows, size_t tables) {
   const char* e = std::getenv("WEMFWI_TABLE_DMA");
   if (e && *e) return std::atoi(e) ? 1 : 0;
@@ -363,11 +172,29 @@ int table_dma(size_t rows, size_t tables) {
 // the fastest of three timed with HIP events) and keeps the fastest plan for
 // every later launch.  WEMFWI_FFT_PLAN=k forces plan k, WEMFWI_AUTOTUNE=0 keeps plan 0;
 // a launch inside stream capture uses the recorded choice or plan 0.
+//
+// The workgroup tile of 1024- and 2048-lane sweeps is picked the same way
+// per row count: the first launch of each (N, row kind, variant, plan, row
+// count, device) times the full tile against the half tile on exactly as
+// many sample rows (at most kTileMaxRows) over at most kTileSteps depth
+// steps, and keeps the half tile when it is at least 2 % faster.  The tile
+// does not change the arithmetic of a row (one wavefront per row either
+// way), only how the rows of a launch spread over the CUs.
+// WEMFWI_SWEEP_TILE=full|half forces one; launches of more rows, inside
+// stream capture before their row count was timed, or with autotuning off
+// take the full tile.
 constexpr int kTuneSteps = 32;
+constexpr int kTileSteps = 16;
+constexpr int kTileMaxRows = 16384;
 
 struct TuneEntry {
   int N, full, ssf, adj, dev, plan, rows, steps;
   std::vector<float> ms;  // mean launch time of each plan
+};
+
+struct TileEntry {
+  int N, full, ssf, adj, pf, tdma, dev, plan, rows, steps, half;
+  float ms[2];  // full tile, half tile
 };
 
 static std::mutex& tune_mutex() {
@@ -378,51 +205,42 @@ static std::vector<TuneEntry>& tune_table() {
   static std::vector<TuneEntry> t;
   return t;
 }
+static std::vector<TileEntry>& tile_table() {
+  static std::vector<TileEntry> t;
+  return t;
+}
 
 static int env_int(const char* name, int dflt) {
   const char* e = std::getenv(name);
   return (e && *e) ? std::atoi(e) : dflt;
 }
 
-template <int N>
-static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz, int step,
-                      bool adj, hipStream_t s, bool tune) {
-  const int np = FftPlans<N>::value;
-  const int forced = env_int("WEMFWI_FFT_PLAN", -1);
-  if (forced >= 0) return forced < np ? forced : 0;
-  if (np < 2 || env_int("WEMFWI_AUTOTUNE", 1) == 0) return 0;
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lock(tune_mutex());
-  for (const TuneEntry& e : tune_table())
-    if (e.N == N && e.full == g.full && e.ssf == g.ssf && e.adj == (int)adj && e.dev == dev) return e.plan;
-  if (!tune) return 0;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
-  // a sample of rows like the first one, on scratch buffers
-  if (hipStreamSynchronize(s) != hipSuccess) return 0;
-  SweepRow r0;
-  DepthTables dt;
-  if (hipMemcpy(&r0, rows, sizeof(r0), hipMemcpyDeviceToHost) != hipSuccess) return 0;
-  if (hipMemcpy(&dt, models + r0.model, sizeof(dt), hipMemcpyDeviceToHost) != hipSuccess) return 0;
-  const int T = std::min(dt.nz - 1, kTuneSteps);
-  const size_t cap_rows = ((size_t)1 << 30) / ((size_t)(T + 1) * std::max(dt.nx, 1) * sizeof(float2) + 1);
-  const int R = (int)std::min<size_t>((size_t)std::min(nrows, 4096), std::max<size_t>(cap_rows, 1));
-  TuneEntry ent{N, g.full, g.ssf, (int)adj, dev, 0, R, T, {}};
-  if (T < 1 || R < 1) {
-    tune_table().push_back(ent);
-    return 0;
-  }
-  const size_t rowlen = (size_t)(T + 1) * dt.nx;
-  float2 *buf = nullptr, *scr = nullptr;
+// R scratch rows of T + 1 zeroed depths, each a copy of a launch's first
+// row descriptor (its model, frequency and phase table), for timing launches
+// that leave the caller's rows alone.
+struct TuneSample {
+  float2* buf = nullptr;
+  float2* scr = nullptr;
   SweepRow* drows = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  bool ok = hipMalloc((void**)&buf, (size_t)R * rowlen * sizeof(float2)) == hipSuccess &&
-            hipMalloc((void**)&scr, (size_t)R * N * sizeof(float2)) == hipSuccess &&
-            hipMalloc((void**)&drows, (size_t)R * sizeof(SweepRow)) == hipSuccess &&
-            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
-  if (ok) ok = hipMemsetAsync(buf, 0, (size_t)R * rowlen * sizeof(float2), s) == hipSuccess;
-  if (ok) {
+  TuneSample() = default;
+  TuneSample(const TuneSample&) = delete;
+  TuneSample& operator=(const TuneSample&) = delete;
+  ~TuneSample() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(drows);
+    (void)hipFree(scr);
+    (void)hipFree(buf);
+  }
+  bool make(const SweepRow& r0, int nx, int N, int R, int T, int step, hipStream_t s) {
+    const size_t rowlen = (size_t)(T + 1) * nx;
+    bool ok = hipMalloc((void**)&buf, (size_t)R * rowlen * sizeof(float2)) == hipSuccess &&
+              hipMalloc((void**)&scr, (size_t)R * N * sizeof(float2)) == hipSuccess &&
+              hipMalloc((void**)&drows, (size_t)R * sizeof(SweepRow)) == hipSuccess &&
+              hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    if (ok) ok = hipMemsetAsync(buf, 0, (size_t)R * rowlen * sizeof(float2), s) == hipSuccess;
+    if (!ok) return false;
     std::vector<SweepRow> h(R, r0);
     for (int i = 0; i < R; ++i) {
       h[i].in = buf + (size_t)i * rowlen;
@@ -438,30 +256,79 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
       h[i].omul = nullptr;
       h[i].src_skip = 0;
     }
-    ok = hipMemcpy(drows, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice) == hipSuccess;
+    return hipMemcpy(drows, h.data(), h.size() * sizeof(SweepRow), hipMemcpyHostToDevice) == hipSuccess;
   }
-  float best = 0.f;
-  for (int k = 0; ok && k < np; ++k) {
-    const float2* tw = nullptr;
-    if (twiddles(N, k, &tw) != WEMFWI_OK) {
-      ok = false;
-      break;
-    }
-    ok = launch_oneway_plan<N>(k, drows, R, models, g, nz, step, adj, tw, s) == hipSuccess;
-    float ms = 0.f;
-    for (int it = 0; ok && it < 3; ++it) {
-      ok = hipEventRecord(e0, s) == hipSuccess &&
-           launch_oneway_plan<N>(k, drows, R, models, g, nz, step, adj, tw, s) == hipSuccess &&
-           hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+  // one warm-up launch, then the fastest of three timed ones
+  template <class F>
+  bool time(F&& launch, hipStream_t s, float* best) {
+    if (launch() != hipSuccess) return false;
+    for (int it = 0; it < 3; ++it) {
       float m1 = 0.f;
-      ok = ok && hipEventElapsedTime(&m1, e0, e1) == hipSuccess;
-      if (ok && (it == 0 || m1 < ms)) ms = m1;
+      if (hipEventRecord(e0, s) != hipSuccess || launch() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+          hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&m1, e0, e1) != hipSuccess)
+        return false;
+      if (it == 0 || m1 < *best) *best = m1;
     }
-    if (!ok) break;
-    ent.ms.push_back(ms);
-    if (k == 0 || ms < best) {
-      best = ms;
-      ent.plan = k;
+    return true;
+  }
+};
+
+// the first row's descriptor and model of a launch (after the stream drains)
+static bool first_row(const SweepRow* rows, const DepthTables* models, hipStream_t s, SweepRow* r0,
+                      DepthTables* dt) {
+  return hipStreamSynchronize(s) == hipSuccess &&
+         hipMemcpy(r0, rows, sizeof(*r0), hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(dt, models + r0->model, sizeof(*dt), hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+template <int N>
+static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz, int step,
+                      bool adj, hipStream_t s, bool tune) {
+  const int np = FftPlans<N>::value;
+  const int forced = env_int("WEMFWI_FFT_PLAN", -1);
+  if (forced >= 0) return forced < np ? forced : 0;
+  if (np < 2 || env_int("WEMFWI_AUTOTUNE", 1) == 0) return 0;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  for (const TuneEntry& e : tune_table())
+    if (e.N == N && e.full == g.full && e.ssf == g.ssf && e.adj == (int)adj && e.dev == dev) return e.plan;
+  if (!tune || capturing(s)) return 0;
+  SweepRow r0;
+  DepthTables dt;
+  if (!first_row(rows, models, s, &r0, &dt)) return 0;
+  const int T = std::min(dt.nz - 1, kTuneSteps);
+  const size_t cap_rows = ((size_t)1 << 30) / ((size_t)(T + 1) * std::max(dt.nx, 1) * sizeof(float2) + 1);
+  const int R = (int)std::min<size_t>((size_t)std::min(nrows, 4096), std::max<size_t>(cap_rows, 1));
+  TuneEntry ent{N, g.full, g.ssf, (int)adj, dev, 0, R, T, {}};
+  if (T < 1 || R < 1) {
+    tune_table().push_back(ent);
+    return 0;
+  }
+  bool ok;
+  {
+    TuneSample smp;
+    ok = smp.make(r0, dt.nx, N, R, T, step, s);
+    float best = 0.f;
+    for (int k = 0; ok && k < np; ++k) {
+      const float2* tw = nullptr;
+      if (twiddles(N, k, &tw) != WEMFWI_OK) {
+        ok = false;
+        break;
+      }
+      float ms = 0.f;
+      ok = smp.time([&] { return launch_oneway_plan<N>(k, smp.drows, R, models, g, nz, step, adj, tw, s); }, s, &ms);
+      if (!ok) break;
+      ent.ms.push_back(ms);
+      if (k == 0 || ms < best) {
+        best = ms;
+        ent.plan = k;
+      }
     }
   }
   if (!ok) {
@@ -469,13 +336,44 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
     ent.plan = 0;
     ent.ms.clear();
   }
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(drows);
-  (void)hipFree(scr);
-  (void)hipFree(buf);
   tune_table().push_back(ent);
   return ent.plan;
+}
+
+// true: launch plan k of the sweep with the half tile
+template <int N>
+static bool tuned_half(int k, const float2* tw, const SweepRow* rows, int nrows, const DepthTables* models,
+                       StepGeom g, int nz, int step, bool adj, hipStream_t s, bool tune) {
+  if (!oneway_half_tile(N)) return false;
+  if (const char* e = std::getenv("WEMFWI_SWEEP_TILE"); e && *e) return std::string(e) == "half";
+  if (env_int("WEMFWI_AUTOTUNE", 1) == 0 || nrows > kTileMaxRows) return false;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  for (const TileEntry& e : tile_table())
+    if (e.N == N && e.full == g.full && e.ssf == g.ssf && e.adj == (int)adj && e.pf == g.pf && e.tdma == g.tdma &&
+        e.dev == dev && e.plan == k && e.rows == nrows)
+      return e.half != 0;
+  if (!tune || capturing(s)) return false;
+  SweepRow r0;
+  DepthTables dt;
+  if (!first_row(rows, models, s, &r0, &dt)) return false;
+  const size_t per = (size_t)nrows * std::max(dt.nx, 1) * sizeof(float2);  // one depth of every row
+  const size_t cap = ((size_t)1 << 30) / per;                                // depths within 1 GiB
+  const int T = std::min(std::min(dt.nz - 1, kTileSteps), (int)std::min<size_t>(cap > 0 ? cap - 1 : 0, kTileSteps));
+  TileEntry ent{N, g.full, g.ssf, (int)adj, g.pf, g.tdma, dev, k, nrows, T, 0, {0.f, 0.f}};
+  if (T >= 2) {
+    TuneSample smp;
+    bool ok = smp.make(r0, dt.nx, N, nrows, T, step, s);
+    ok = ok && smp.time([&] { return launch_oneway_plan<N>(k, smp.drows, nrows, models, g, nz, step, adj, tw, s); },
+                        s, &ent.ms[0]);
+    ok = ok && smp.time([&] { return launch_oneway_half(N, k, smp.drows, nrows, models, g, nz, step, adj, tw, s); },
+                        s, &ent.ms[1]);
+    if (ok) ent.half = ent.ms[1] < 0.98f * ent.ms[0] ? 1 : 0;
+    else (void)hipGetLastError();
+  }
+  tile_table().push_back(ent);
+  return ent.half != 0;
 }
 
 template <int N>
@@ -492,6 +390,8 @@ static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTa
     else if (tune) return hipErrorInvalidValue;
     else k = 0;  // plan 0's table is the model's own (tw)
   }
+  if (tuned_half<N>(k, tw, rows, nrows, models, g, nz, step, adj, s, tune))
+    return launch_oneway_half(N, k, rows, nrows, models, g, nz, step, adj, tw, s);
   return launch_oneway_plan<N>(k, rows, nrows, models, g, nz, step, adj, tw, s);
 }
 
@@ -513,6 +413,17 @@ std::string autotune_report() {
       out += v;
     }
     out += "]}";
+  }
+  for (size_t i = 0; i < tile_table().size(); ++i) {
+    const TileEntry& e = tile_table()[i];
+    char line[400];
+    std::snprintf(line, sizeof(line),
+                  "%s{\"kernel\": \"oneway_tile\", \"N\": %d, \"full\": %d, \"ssf\": %d, \"adjoint\": %d, "
+                  "\"pf\": %d, \"tdma\": %d, \"device\": %d, \"rows\": %d, \"steps\": %d, \"plan\": %d, "
+                  "\"tile\": \"%s\", \"ms\": [%.4f, %.4f]}",
+                  (i || !tune_table().empty()) ? ", " : "", e.N, e.full, e.ssf, e.adj, e.pf, e.tdma, e.dev, e.rows,
+                  e.steps, e.plan, e.half ? "half" : "full", e.ms[0], e.ms[1]);
+    out += line;
   }
   return out + "]";
 }
