@@ -273,10 +273,11 @@ struct TuneSample {
   }
 };
 
-// the first row's descriptor and model of a launch (after the stream drains)
-static bool first_row(const SweepRow* rows, const DepthTables* models, hipStream_t s, SweepRow* r0,
-                      DepthTables* dt) {
-  return hipStreamSynchronize(s) == hipSuccess &&
+// the first row's descriptor and model of a launch, after the whole device
+// drains: a tuning launch timed beside other work (a batch plan's groups run
+// on several streams) would measure the sharing, not the launch
+static bool first_row(const SweepRow* rows, const DepthTables* models, SweepRow* r0, DepthTables* dt) {
+  return hipDeviceSynchronize() == hipSuccess &&
          hipMemcpy(r0, rows, sizeof(*r0), hipMemcpyDeviceToHost) == hipSuccess &&
          hipMemcpy(dt, models + r0->model, sizeof(*dt), hipMemcpyDeviceToHost) == hipSuccess;
 }
@@ -301,7 +302,7 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
   if (!tune || capturing(s)) return 0;
   SweepRow r0;
   DepthTables dt;
-  if (!first_row(rows, models, s, &r0, &dt)) return 0;
+  if (!first_row(rows, models, &r0, &dt)) return 0;
   const int T = std::min(dt.nz - 1, kTuneSteps);
   const size_t cap_rows = ((size_t)1 << 30) / ((size_t)(T + 1) * std::max(dt.nx, 1) * sizeof(float2) + 1);
   const int R = (int)std::min<size_t>((size_t)std::min(nrows, 4096), std::max<size_t>(cap_rows, 1));
@@ -357,7 +358,7 @@ static bool tuned_half(int k, const float2* tw, const SweepRow* rows, int nrows,
   if (!tune || capturing(s)) return false;
   SweepRow r0;
   DepthTables dt;
-  if (!first_row(rows, models, s, &r0, &dt)) return false;
+  if (!first_row(rows, models, &r0, &dt)) return false;
   const size_t per = (size_t)nrows * std::max(dt.nx, 1) * sizeof(float2);  // one depth of every row
   const size_t cap = ((size_t)1 << 30) / per;                                // depths within 1 GiB
   const int T = std::min(std::min(dt.nz - 1, kTileSteps), (int)std::min<size_t>(cap > 0 ? cap - 1 : 0, kTileSteps));
