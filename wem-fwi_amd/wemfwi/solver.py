@@ -290,8 +290,10 @@ def shot_range(ns: int, rank: int, world: int) -> Tuple[int, int]:
 
 def allreduce_value_and_grad(f: float, g: Vec) -> Tuple[float, Vec]:
     """Sum of every rank's objective and gradient (the one exchange of the
-    shot-sharded problem).  gloo reduces host copies; nccl (RCCL) reduces the
-    device tensors.  Without an initialised process group it is the identity."""
+    shot-sharded problem), in double precision (tensors are summed as
+    complex128 / float64 and rounded back once).  gloo reduces host copies; nccl (RCCL)
+    reduces the device tensors.  Without an initialised process group it is
+    the identity."""
     import torch.distributed as dist
     if not (dist.is_available() and dist.is_initialized()) or dist.get_world_size() == 1:
         return f, g
@@ -300,11 +302,13 @@ def allreduce_value_and_grad(f: float, g: Vec) -> Tuple[float, Vec]:
     for t in g:
         w = t.detach().to("cpu", copy=True) if host else t.detach().clone()
         if w.is_complex():
-            w = torch.view_as_real(w.to(torch.complex128) if host else w).contiguous()
+            w = torch.view_as_real(w.to(torch.complex128)).contiguous()
             dist.all_reduce(w)
             w = torch.view_as_complex(w).to(t.dtype)
         else:
+            w = w.to(torch.float64)
             dist.all_reduce(w)
+            w = w.to(t.dtype)
         out.append(w.to(t.device) if host else w)
     fv = torch.tensor([float(f)], dtype=torch.float64, device="cpu" if host else g[0].device)
     dist.all_reduce(fv)
