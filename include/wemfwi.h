@@ -563,7 +563,8 @@ int wemfwi_interp_filter(int kind, int nd, float dd, int nm, float dm, float a, 
 /* out[(a n_out + i) inner + b] (+)= sum_j F[i][j] in[(a n_in + j) inner + b]
  * for a device float matrix F[n_out][n_in] (transpose = 1: F stored
  * [n_in][n_out], applied transposed): one axis of the separable
- * interpolation (Interpolation3D.h:21-95).  in != out. */
+ * interpolation (Interpolation3D.h:21-95).  in != out.  Each output row
+ * sums only over its band of nonzero taps (found on the device per call). */
 int wemfwi_axis_apply(const float* F, int n_out, int n_in, long long outer, long long inner, int transpose,
                       const wemfwi_complex* in, wemfwi_complex* out, int add, void* stream);
 

@@ -16,12 +16,33 @@
 namespace wemfwi {
 namespace {
 
+// Nonzero band of each output row of the filter: band[i] = [first, last + 1)
+// of the j with F[i][j] != 0 (F[j][i] for the transpose; [0, 0) for an
+// all-zero row).  One thread per output row, run once per apply: the rows of
+// the interpolation filters hold a few taps near the diagonal and, from the
+// even extension, a few more near the ends.
+__global__ void axis_band_kernel(int2* __restrict__ band, const float* __restrict__ F, int n_out, int n_in,
+                                 int transpose) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  int lo = n_in, hi = 0;
+  for (int j = 0; j < n_in; ++j) {
+    const float f = transpose ? F[(long long)j * n_out + i] : F[(long long)i * n_in + j];
+    if (f != 0.f) {
+      lo = j < lo ? j : lo;
+      hi = j + 1;
+    }
+  }
+  band[i] = lo < hi ? make_int2(lo, hi) : make_int2(0, 0);
+}
+
 // out[(a n_out + i) inner + b] (+)= sum_j F[i][j] in[(a n_in + j) inner + b]
-// (transpose: F[j][i] with F stored [n_in][n_out]); one thread per output
-// sample, double accumulation.
+// (transpose: F[j][i] with F stored [n_in][n_out]) over j in row i's band;
+// one thread per output sample, double accumulation in increasing j (zero
+// entries inside the band add nothing, as outside it).
 __global__ void axis_matrix_kernel(float2* __restrict__ out, const float2* __restrict__ in,
-                                   const float* __restrict__ F, int n_out, int n_in, long long outer,
-                                   long long inner, int transpose, int add) {
+                                   const float* __restrict__ F, const int2* __restrict__ band, int n_out, int n_in,
+                                   long long outer, long long inner, int transpose, int add) {
   const long long total = outer * n_out * inner;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (long long)gridDim.x * blockDim.x) {
@@ -30,8 +51,9 @@ __global__ void axis_matrix_kernel(float2* __restrict__ out, const float2* __res
     const int i = (int)(r % n_out);
     const long long a = r / n_out;
     const float2* x = in + a * n_in * inner + b;
+    const int2 bd = band[i];
     double sr = 0.0, si = 0.0;
-    for (int j = 0; j < n_in; ++j) {
+    for (int j = bd.x; j < bd.y; ++j) {
       const float f = transpose ? F[(long long)j * n_out + i] : F[(long long)i * n_in + j];
       if (f != 0.f) {
         const float2 v = x[(long long)j * inner];
@@ -136,11 +158,22 @@ int wemfwi_axis_apply(const float* F, int n_out, int n_in, long long outer, long
   const long long total = outer * n_out * inner;
   if (total == 0) return WEMFWI_OK;
   hipStream_t s = (hipStream_t)stream;
+  // the bands live in stream-ordered memory of this call (concurrent calls
+  // on other streams keep their own)
+  int2* band = nullptr;
+  hipError_t e = hipMallocAsync((void**)&band, (size_t)n_out * sizeof(int2), s);
+  if (e != hipSuccess) return hip_fail(e, "axis_apply band");
+  hipLaunchKernelGGL(axis_band_kernel, dim3((n_out + 255) / 256), dim3(256), 0, s, band, F, n_out, n_in, transpose);
+  e = hipGetLastError();
   long long blocks = (total + 255) / 256;
   if (blocks > 1048576) blocks = 1048576;
-  hipLaunchKernelGGL(axis_matrix_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (float2*)out, (const float2*)in,
-                     F, n_out, n_in, outer, inner, transpose, add);
-  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(axis_matrix_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (float2*)out,
+                       (const float2*)in, F, band, n_out, n_in, outer, inner, transpose, add);
+    e = hipGetLastError();
+  }
+  const hipError_t ef = hipFreeAsync(band, s);
+  if (e == hipSuccess) e = ef;
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? WEMFWI_OK : hip_fail(e, "axis_apply");
 }
