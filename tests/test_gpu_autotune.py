@@ -115,6 +115,8 @@ TILE_CASES = [
     (1000, 12, 2, "pspi", "0", False),   # padded 1024 row
     (2048, 0, 2, "pspi", "1", True),
     (2040, 4, 1, "ssf", "0", False),     # padded 2048 row
+    (600, 0, 1, "ssf", "0", False),      # nk = 600: runtime-planned passes in a 1024 row
+    (1000, 0, 2, "pspi", "1", True),     # nk = 1000, runtime-planned
 ]
 
 

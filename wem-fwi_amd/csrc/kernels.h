@@ -65,11 +65,11 @@ hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTabl
                          StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s,
                          bool tune = true);
 // sweep launch of radix plan k with the half workgroup tile (half the waves
-// per workgroup of launch_oneway's full tile); only for the FFT lengths of
-// oneway_half_tile
+// per workgroup of launch_oneway's full tile); only where oneway_half_tile
+// holds (gen: rows whose FFT length g.fg.n is not N, k unused)
 hipError_t launch_oneway_half(int N, int k, const SweepRow* rows, int nrows, const DepthTables* models,
                               StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s);
-constexpr bool oneway_half_tile(int N) { return N == 1024 || N == 2048; }
+constexpr bool oneway_half_tile(int N, bool gen) { return gen ? N == 1024 : (N == 1024 || N == 2048); }
 hipError_t launch_onestep(int N, const StepRow* rows, int nrows, const DepthTables* models,
                           StepGeom g, bool adj, const float2* tw, hipStream_t s);
 // StepGeom.tdma of a sweep launch of `rows` rows that use `tables` distinct

@@ -173,9 +173,10 @@ int table_dma(size_t rows, size_t tables) {
 // every later launch.  WEMFWI_FFT_PLAN=k forces plan k, WEMFWI_AUTOTUNE=0 keeps plan 0;
 // a launch inside stream capture uses the recorded choice or plan 0.
 //
-// The workgroup tile of 1024- and 2048-lane sweeps is picked the same way
-// per row count: the first launch of each (N, row kind, variant, plan, row
-// count, device) times the full tile against the half tile on exactly as
+// The workgroup tile of 1024- and 2048-lane sweeps (of 1024-lane ones for
+// runtime-planned FFT lengths) is picked the same way per row count: the
+// first launch of each (N, FFT length, row kind, variant, plan, row count,
+// device) times the full tile against the half tile on exactly as
 // many sample rows (at most kTileMaxRows) over at most kTileSteps depth
 // steps, and keeps the half tile when it is at least 2 % faster.  The tile
 // does not change the arithmetic of a row (one wavefront per row either
@@ -193,7 +194,7 @@ struct TuneEntry {
 };
 
 struct TileEntry {
-  int N, full, ssf, adj, pf, tdma, dev, plan, rows, steps, half;
+  int N, nk, full, ssf, adj, pf, tdma, dev, plan, rows, steps, half;
   float ms[2];  // full tile, half tile
 };
 
@@ -341,19 +342,21 @@ static int tuned_plan(const SweepRow* rows, int nrows, const DepthTables* models
   return ent.plan;
 }
 
-// true: launch plan k of the sweep with the half tile
+// true: launch plan k of the sweep (the runtime-planned passes when the FFT
+// length g.fg.n is not N) with the half tile
 template <int N>
 static bool tuned_half(int k, const float2* tw, const SweepRow* rows, int nrows, const DepthTables* models,
                        StepGeom g, int nz, int step, bool adj, hipStream_t s, bool tune) {
-  if (!oneway_half_tile(N)) return false;
+  const bool gen = g.fg.n != N;
+  if (!oneway_half_tile(N, gen)) return false;
   if (const char* e = std::getenv("WEMFWI_SWEEP_TILE"); e && *e) return std::string(e) == "half";
   if (env_int("WEMFWI_AUTOTUNE", 1) == 0 || nrows > kTileMaxRows) return false;
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lock(tune_mutex());
   for (const TileEntry& e : tile_table())
-    if (e.N == N && e.full == g.full && e.ssf == g.ssf && e.adj == (int)adj && e.pf == g.pf && e.tdma == g.tdma &&
-        e.dev == dev && e.plan == k && e.rows == nrows)
+    if (e.N == N && e.nk == g.fg.n && e.full == g.full && e.ssf == g.ssf && e.adj == (int)adj && e.pf == g.pf &&
+        e.tdma == g.tdma && e.dev == dev && e.plan == k && e.rows == nrows)
       return e.half != 0;
   if (!tune || capturing(s)) return false;
   SweepRow r0;
@@ -362,12 +365,16 @@ static bool tuned_half(int k, const float2* tw, const SweepRow* rows, int nrows,
   const size_t per = (size_t)nrows * std::max(dt.nx, 1) * sizeof(float2);  // one depth of every row
   const size_t cap = ((size_t)1 << 30) / per;                                // depths within 1 GiB
   const int T = std::min(std::min(dt.nz - 1, kTileSteps), (int)std::min<size_t>(cap > 0 ? cap - 1 : 0, kTileSteps));
-  TileEntry ent{N, g.full, g.ssf, (int)adj, g.pf, g.tdma, dev, k, nrows, T, 0, {0.f, 0.f}};
+  TileEntry ent{N, g.fg.n, g.full, g.ssf, (int)adj, g.pf, g.tdma, dev, k, nrows, T, 0, {0.f, 0.f}};
   if (T >= 2) {
     TuneSample smp;
     bool ok = smp.make(r0, dt.nx, N, nrows, T, step, s);
-    ok = ok && smp.time([&] { return launch_oneway_plan<N>(k, smp.drows, nrows, models, g, nz, step, adj, tw, s); },
-                        s, &ent.ms[0]);
+    ok = ok && smp.time(
+                   [&] {
+                     return gen ? launch_oneway_gen<N>(smp.drows, nrows, models, g, nz, step, adj, tw, s)
+                                : launch_oneway_plan<N>(k, smp.drows, nrows, models, g, nz, step, adj, tw, s);
+                   },
+                   s, &ent.ms[0]);
     ok = ok && smp.time([&] { return launch_oneway_half(N, k, smp.drows, nrows, models, g, nz, step, adj, tw, s); },
                         s, &ent.ms[1]);
     if (ok) ent.half = ent.ms[1] < 0.98f * ent.ms[0] ? 1 : 0;
@@ -381,8 +388,9 @@ template <int N>
 static hipError_t launch_oneway_n(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
                                   int step, bool adj, const float2* tw, hipStream_t s, bool tune) {
   if (g.fg.n != N) {
-    if (g.ssf) return launch_oneway_v<Var<N, false, true, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
-    return launch_oneway_v<Var<N, false, false, true>>(rows, nrows, models, g, nz, step, adj, tw, s);
+    if (tuned_half<N>(0, tw, rows, nrows, models, g, nz, step, adj, s, tune))
+      return launch_oneway_half(N, 0, rows, nrows, models, g, nz, step, adj, tw, s);
+    return launch_oneway_gen<N>(rows, nrows, models, g, nz, step, adj, tw, s);
   }
   int k = tuned_plan<N>(rows, nrows, models, g, nz, step, adj, s, tune);
   if (k != 0) {
@@ -419,10 +427,10 @@ std::string autotune_report() {
     const TileEntry& e = tile_table()[i];
     char line[400];
     std::snprintf(line, sizeof(line),
-                  "%s{\"kernel\": \"oneway_tile\", \"N\": %d, \"full\": %d, \"ssf\": %d, \"adjoint\": %d, "
+                  "%s{\"kernel\": \"oneway_tile\", \"N\": %d, \"nk\": %d, \"full\": %d, \"ssf\": %d, \"adjoint\": %d, "
                   "\"pf\": %d, \"tdma\": %d, \"device\": %d, \"rows\": %d, \"steps\": %d, \"plan\": %d, "
                   "\"tile\": \"%s\", \"ms\": [%.4f, %.4f]}",
-                  (i || !tune_table().empty()) ? ", " : "", e.N, e.full, e.ssf, e.adj, e.pf, e.tdma, e.dev, e.rows,
+                  (i || !tune_table().empty()) ? ", " : "", e.N, e.nk, e.full, e.ssf, e.adj, e.pf, e.tdma, e.dev, e.rows,
                   e.steps, e.plan, e.half ? "half" : "full", e.ms[0], e.ms[1]);
     out += line;
   }

@@ -197,6 +197,14 @@ static hipError_t launch_oneway_k(const SweepRow* rows, int nrows, const DepthTa
   return launch_oneway_v<Var<N, false, false, false, K>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
 }
 
+// rows whose FFT length is not N (runtime-planned passes, GEN variants)
+template <int N, bool H = false>
+static hipError_t launch_oneway_gen(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
+                                    int step, bool adj, const float2* tw, hipStream_t s) {
+  if (g.ssf) return launch_oneway_v<Var<N, false, true, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+  return launch_oneway_v<Var<N, false, false, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+}
+
 template <int N, bool H = false, int K = 0>
 static hipError_t launch_oneway_plan(int k, const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g,
                                      int nz, int step, bool adj, const float2* tw, hipStream_t s) {
