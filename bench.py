@@ -155,7 +155,7 @@ def cpu_baseline(cfg, slow, freqs, src_ix, nrows_sample):
     dt1 = time.perf_counter() - t0
     return {"value": upd / dt / 1e9, "unit": "Gupdates/s", "cores": cores, "kind": "port",
             "sample": f"{len(rows)} of {nw * len(src_ix)} (shot, frequency) rows, Down forward + "
-                      f"adjoint, FP32 C restatement (oracle/cpu_wem.c, OpenMP, radix-2 FFT), "
+                      f"adjoint, FP32 C restatement (oracle/cpu_wem.c, OpenMP, radix-4 Stockham FFT), "
                       f"{dt:.2f} s",
             "single_thread_value": 2.0 * nx * (nz - 1) * n1 / dt1 / 1e9,
             "single_thread_sample": f"{n1} rows, {dt1:.2f} s",

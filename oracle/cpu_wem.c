@@ -13,8 +13,8 @@
  *   row parallelism  one (shot, frequency) row per task, static schedule, as
  *                    tbb::blocked_range2d + static_partitioner in
  *                    propagator/src/wem_prop.cpp:11-12,76
- * The FFT is an iterative radix-2 Cooley-Tukey transform (power-of-two
- * lengths), FFTW being absent on the hosts this runs on.
+ * The FFT is a radix-4 Stockham transform (power-of-two lengths), FFTW
+ * being absent on the hosts this runs on.
  */
 #include <complex.h>
 #include <math.h>
@@ -32,30 +32,80 @@ static int ilog2(int n) {
   return (1 << l) == n ? l : -1;
 }
 
-/* in-place unnormalised DFT, sign -1 forward, +1 backward (FFTW convention) */
-static void fft_r2(cf* x, int n, int sign, const cf* tw /* n/2 forward twiddles */) {
-  int j = 0;
-  for (int i = 1; i < n; ++i) {
-    int bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) {
-      cf t = x[i];
-      x[i] = x[j];
-      x[j] = t;
+/*
+ * Unnormalised DFT of x (sign -1 forward, +1 backward, FFTW's convention),
+ * a Stockham autosort transform: a radix-2 pass first when log2 n is odd,
+ * then radix-4 passes, between x and the scratch row tmp (n entries); the
+ * inner loops run over contiguous butterflies.  Pass p of sub-length NS
+ * multiplies input r of butterfly j (m = j mod NS) by tw[off + (r-1) NS + m]
+ * = exp(-2 pi i m r / (4 NS)) (conjugated backward), then writes output s at
+ * (j / NS) 4 NS + m + s NS.
+ */
+static void fft_s4(cf* x, cf* tmp, int n, int sign, const cf* tw) {
+  cf *a = x, *b = tmp;
+  int ns = 1;
+  if (ilog2(n) & 1) {
+    const int h = n / 2;
+    for (int j = 0; j < h; ++j) {
+      b[2 * j] = a[j] + a[j + h];
+      b[2 * j + 1] = a[j] - a[j + h];
     }
+    cf* t = a;
+    a = b;
+    b = t;
+    ns = 2;
   }
-  for (int len = 2; len <= n; len <<= 1) {
-    const int half = len >> 1, step = n / len;
-    for (int i = 0; i < n; i += len)
-      for (int k = 0; k < half; ++k) {
-        cf w = tw[k * step];
-        if (sign > 0) w = conjf(w);
-        const cf u = x[i + k], v = x[i + k + half] * w;
-        x[i + k] = u + v;
-        x[i + k + half] = u - v;
+  size_t off = 0;
+  const int q4 = n / 4;
+  while (ns < n) {
+    const cf* t3 = tw + off;
+    for (int q = 0; q < n / (4 * ns); ++q) {
+      const cf* in = a + (size_t)q * ns;
+      cf* out = b + (size_t)q * 4 * ns;
+      for (int m = 0; m < ns; ++m) {
+        cf a0 = in[m], a1 = in[m + q4], a2 = in[m + 2 * q4], a3 = in[m + 3 * q4];
+        if (ns > 1) {
+          cf w1 = t3[m], w2 = t3[ns + m], w3 = t3[2 * ns + m];
+          if (sign > 0) {
+            w1 = conjf(w1);
+            w2 = conjf(w2);
+            w3 = conjf(w3);
+          }
+          a1 *= w1;
+          a2 *= w2;
+          a3 *= w3;
+        }
+        const cf s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3;
+        const cf d13 = (a1 - a3) * (sign < 0 ? -I : I);
+        out[m] = s02 + s13;
+        out[m + ns] = d02 + d13;
+        out[m + 2 * ns] = s02 - s13;
+        out[m + 3 * ns] = d02 - d13;
       }
+    }
+    off += (size_t)3 * ns;
+    ns *= 4;
+    cf* t = a;
+    a = b;
+    b = t;
   }
+  if (a != x) memcpy(x, a, sizeof(cf) * n);
+}
+
+/* fft_s4's twiddle table for length n (fewer than n entries), each
+ * evaluated in double and rounded once */
+static cf* s4_twiddles(int n) {
+  cf* tw = (cf*)malloc(sizeof(cf) * (size_t)(n > 4 ? n : 4));
+  size_t off = 0;
+  for (int ns = (ilog2(n) & 1) ? 2 : 1; ns < n; ns *= 4) {
+    for (int r = 1; r < 4; ++r)
+      for (int m = 0; m < ns; ++m) {
+        const double a = -2.0 * M_PI * (double)m * r / (4.0 * ns);
+        tw[off + (size_t)(r - 1) * ns + m] = (float)cos(a) + I * (float)sin(a);
+      }
+    off += (size_t)3 * ns;
+  }
+  return tw;
 }
 
 typedef struct {
@@ -66,7 +116,7 @@ typedef struct {
   const int* refid; /* [nz][nx] */
   float* k;         /* [nk] */
   float* filt;      /* [nx] */
-  cf* tw;           /* [nk/2] */
+  cf* tw;           /* fft_s4 twiddles of length nk */
 } Grid;
 
 /* ps_fwd / ps_adj (phase_shift.ispc:2-32) in the reference's float formula */
@@ -94,11 +144,11 @@ static void ss_factor(cf* out, const Grid* g, int iz, float w, cf s_ref, const i
   }
 }
 
-static void step_fwd(const Grid* g, int iz, float w, const cf* in, cf* out, cf* X, cf* Y, cf* row) {
+static void step_fwd(const Grid* g, int iz, float w, const cf* in, cf* out, cf* X, cf* Y, cf* row, cf* F) {
   const float sc = 1.0f / sqrtf((float)g->nk);
   memset(X, 0, sizeof(cf) * g->nk);
   for (int x = 0; x < g->nx; ++x) X[x + g->tap] = in[x];
-  fft_r2(X, g->nk, -1, g->tw);
+  fft_s4(X, F, g->nk, -1, g->tw);
   for (int i = 0; i < g->nk; ++i) X[i] *= sc;
   memset(out, 0, sizeof(cf) * g->nx);
   const int* rid = g->refid + (size_t)iz * g->nx;
@@ -108,7 +158,7 @@ static void step_fwd(const Grid* g, int iz, float w, const cf* in, cf* out, cf* 
     if (!any) continue;
     const cf s = g->sref[(size_t)iz * g->nref + r];
     phase(Y, X, g, w, s, 0, 0);
-    fft_r2(Y, g->nk, +1, g->tw);
+    fft_s4(Y, F, g->nk, +1, g->tw);
     for (int x = 0; x < g->nx; ++x) row[x] = Y[x + g->tap] * sc * g->filt[x];
     if (!g->pspi) ss_factor(row, g, iz, w, s, rid, r, 0);
     for (int x = 0; x < g->nx; ++x)
@@ -116,7 +166,7 @@ static void step_fwd(const Grid* g, int iz, float w, const cf* in, cf* out, cf* 
   }
 }
 
-static void step_adj(const Grid* g, int iz, float w, const cf* in, cf* out, cf* X, cf* Y, cf* row) {
+static void step_adj(const Grid* g, int iz, float w, const cf* in, cf* out, cf* X, cf* Y, cf* row, cf* F) {
   const float sc = 1.0f / sqrtf((float)g->nk);
   memset(X, 0, sizeof(cf) * g->nk);
   const int* rid = g->refid + (size_t)iz * g->nx;
@@ -129,11 +179,11 @@ static void step_adj(const Grid* g, int iz, float w, const cf* in, cf* out, cf* 
     if (!g->pspi) ss_factor(row, g, iz, w, s, rid, r, 1);
     memset(Y, 0, sizeof(cf) * g->nk);
     for (int x = 0; x < g->nx; ++x) Y[x + g->tap] = row[x] * g->filt[x];
-    fft_r2(Y, g->nk, -1, g->tw);
+    fft_s4(Y, F, g->nk, -1, g->tw);
     for (int i = 0; i < g->nk; ++i) Y[i] *= sc;
     phase(X, Y, g, w, s, 1, 1);
   }
-  fft_r2(X, g->nk, +1, g->tw);
+  fft_s4(X, F, g->nk, +1, g->tw);
   for (int x = 0; x < g->nx; ++x) out[x] = X[x + g->tap] * sc;
 }
 
@@ -162,7 +212,7 @@ int cpu_oneway(int nrows, int nz, int nx, int tap, int taper, int nref, int pspi
   if (ilog2(g.nk) < 0) return -1;
   g.k = (float*)calloc(g.nk, sizeof(float));
   g.filt = (float*)malloc(sizeof(float) * nx);
-  g.tw = (cf*)malloc(sizeof(cf) * (g.nk / 2 + 1));
+  g.tw = s4_twiddles(g.nk);
   const float dk = (float)(2 * M_PI / (dx * (g.nk - 1)));
   for (int ik = 0; ik < g.nk / 2; ++ik) g.k[ik] = ik * dk;
   for (int ik = 1; ik < g.nk / 2; ++ik) g.k[g.nk - ik] = -g.k[ik];
@@ -171,7 +221,6 @@ int cpu_oneway(int nrows, int nz, int nx, int tap, int taper, int nref, int pspi
     g.filt[i] = 0.5f * (float)(1 - cos(M_PI * i / (taper - 1)));
     g.filt[nx - 1 - i] = g.filt[i];
   }
-  for (int i = 0; i < g.nk / 2; ++i) g.tw[i] = cexpf(-2.0f * (float)M_PI * I * (float)i / (float)g.nk);
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
@@ -183,6 +232,7 @@ int cpu_oneway(int nrows, int nz, int nx, int tap, int taper, int nref, int pspi
     cf* Y = (cf*)malloc(sizeof(cf) * g.nk);
     cf* row = (cf*)malloc(sizeof(cf) * nx);
     cf* tmp = (cf*)malloc(sizeof(cf) * nx);
+    cf* F = (cf*)malloc(sizeof(cf) * g.nk);
 #pragma omp for schedule(static)
     for (int i = 0; i < nrows; ++i) {
       cf* w3 = (cf*)W + (size_t)i * nz * nx;
@@ -190,8 +240,8 @@ int cpu_oneway(int nrows, int nz, int nx, int tap, int taper, int nref, int pspi
       for (int s = 0; s < nz - 1; ++s) {
         const int cur = fwd_dir ? s : nz - 1 - s;
         const int nxt = fwd_dir ? cur + 1 : cur - 1;
-        if (adjoint) step_adj(&g, nxt, w, w3 + (size_t)cur * nx, tmp, X, Y, row);
-        else step_fwd(&g, cur, w, w3 + (size_t)cur * nx, tmp, X, Y, row);
+        if (adjoint) step_adj(&g, nxt, w, w3 + (size_t)cur * nx, tmp, X, Y, row, F);
+        else step_fwd(&g, cur, w, w3 + (size_t)cur * nx, tmp, X, Y, row, F);
         for (int x = 0; x < nx; ++x) w3[(size_t)nxt * nx + x] += tmp[x];
       }
     }
@@ -199,6 +249,7 @@ int cpu_oneway(int nrows, int nz, int nx, int tap, int taper, int nref, int pspi
     free(Y);
     free(row);
     free(tmp);
+    free(F);
   }
   free(g.k);
   free(g.filt);
