@@ -47,10 +47,10 @@ CASES = [
     (300, 0, 0, 3, "pspi"),                # 300 = 4.3.5.5; 512
     (375, 5, 0, 1, "ssf"),                 # 385 = 5.7.11; 512
     (500, 0, 10, 2, "ssf"),                # 500 = 4.5.5.5 (a reference run length); 512
-    (600, 0, 0, 4, "pspi"),                # 600 = 4.2.3.5.5 (a reference run length); 1024
-    (1000, 0, 0, 1, "ssf"),                # 1000 = 4.2.5.5.5; 1024
-    (1922, 15, 0, 2, "ssf"),               # 1952 = 4.4.2.61; 2048
-    (1536, 0, 0, 1, "pspi"),               # 1536 = 4.4.4.4.2.3; 2048
+    (600, 0, 0, 4, "pspi"),                # 600 = 8.3.5.5 (a reference run length); 1024
+    (1000, 0, 0, 1, "ssf"),                # 1000 = 8.5.5.5; 1024
+    (1922, 15, 0, 2, "ssf"),               # 1952 = 16.2.61; 2048
+    (1536, 0, 0, 1, "pspi"),               # 1536 = 16.16.2.3; 2048
 ]
 
 
@@ -215,3 +215,51 @@ def test_lengths_without_a_plan_are_rejected():
         with pytest.raises(WemfwiError) as e:
             m.oneway(False, False, np.array([5.0], np.float32), W)
         assert e.value.code == 3
+
+
+@pytest.fixture
+def gen_env():
+    import os
+    keys = ("WEMFWI_GEN_FIXED", "WEMFWI_SWEEP_TILE")
+    old = {k: os.environ.get(k) for k in keys}
+    yield os.environ
+    for k, v in old.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+
+
+@pytest.mark.parametrize("nx,tap,nref,prop,tile", [
+    (500, 0, 1, "ssf", "full"),      # nk = 500: compile-time chain 4.5.5.5 in 512-lane rows
+    (490, 5, 3, "pspi", "full"),     # nk = 500 with FFT padding
+    (600, 0, 2, "pspi", "full"),     # nk = 600: compile-time chain 8.3.5.5 in 1024-lane rows
+    (600, 0, 1, "ssf", "half"),      # the same in the half workgroup tile
+])
+def test_fixed_chain_is_bitwise_the_runtime_chain(nx, tap, nref, prop, tile, gen_env):
+    """The compile-time pass chains of the reference's run lengths do the
+    runtime chain's butterflies in the same order: bit-identical sweeps in
+    every direction, and oracle parity."""
+    nz = 12
+    m, s, p, ref = _setup(nx, tap, 0, nref, prop, nz, seed=4)
+    rng = np.random.default_rng(5)
+    nrows = 9
+    W = np.zeros((nrows, nz, nx), dtype=np.complex64)
+    W[:, 0, nx // 3: nx // 3 + 6] = rng.standard_normal((nrows, 6)) + 1j * rng.standard_normal((nrows, 6))
+    W[:, nz - 1, nx // 2: nx // 2 + 6] = rng.standard_normal((nrows, 6)) + 1j * rng.standard_normal((nrows, 6))
+    freqs = np.linspace(3.0, 27.0, nrows).astype(np.float32)
+    gen_env["WEMFWI_SWEEP_TILE"] = tile
+    for up, adjoint in ((False, False), (False, True), (True, False), (True, True)):
+        got = {}
+        for fixed in ("1", "0"):
+            gen_env["WEMFWI_GEN_FIXED"] = fixed
+            print(f"nk={nx + 2 * tap} {prop} tile={tile} fixed={fixed} up={up} adjoint={adjoint}", flush=True)
+            t = _c(W)
+            m.oneway(up, adjoint, freqs, t)
+            got[fixed] = t.cpu().numpy()
+        assert np.array_equal(got["1"].view(np.uint32), got["0"].view(np.uint32)), (up, adjoint)
+        fn = {(False, False): O.down_fwd, (False, True): O.down_adj,
+              (True, False): O.up_fwd, (True, True): O.up_adj}[(up, adjoint)]
+        for i in (0, nrows - 1):
+            want = fn(W[i].astype(complex), 2 * np.pi * float(freqs[i]), s, ref, p)
+            assert rel(got["1"][i], want) < TOL, (up, adjoint, i, rel(got["1"][i], want))

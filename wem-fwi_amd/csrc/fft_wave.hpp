@@ -736,6 +736,62 @@ __device__ __forceinline__ void fft_row_gen(float2 (&x)[N / 64], float2* buf, co
   wave_lds_order();
 }
 
+// Pass chains fixed at compile time for the FFT lengths of the reference's
+// own runs (nk = 500 in 512-lane rows, nk = 600 in 1024-lane rows): the
+// passes fft_gen_plan builds for them (500 = 4 5 5 5, 600 = 8 3 5 5), over
+// the same twiddle table, with every radix, length, sub-length and table
+// offset a constant, so that the index arithmetic folds.  Same butterflies
+// in the same order as fft_row_gen: bit-identical results.  ID 0: none.
+template <int ID>
+struct GenFixed;
+template <>
+struct GenFixed<1> {
+  static constexpr int n = 500, P = 4;
+  static constexpr int R[P] = {4, 5, 5, 5};
+};
+template <>
+struct GenFixed<2> {
+  static constexpr int n = 600, P = 4;
+  static constexpr int R[P] = {8, 3, 5, 5};
+};
+__host__ __device__ constexpr int gen_fixed_id(int N, int n) {
+  return (N == 512 && n == 500) ? 1 : ((N == 1024 && n == 600) ? 2 : 0);
+}
+
+template <int N, bool INV, int ID, int p, int NS, int OFF>
+__device__ __forceinline__ void gen_fixed_passes(float2*& A, float2*& B, const float2* tw, int t) {
+  if constexpr (p < GenFixed<ID>::P) {
+    constexpr int R = GenFixed<ID>::R[p];
+    constexpr int n = GenFixed<ID>::n;
+    constexpr bool kBig = gen_max_radix(N) == 16;
+    static_assert(!gen_direct(R) && (R <= 5 || kBig), "fixed chains use register butterflies");
+    gen_pass_r<R, INV, kBig>(A, B, tw + OFF, n, NS, t);
+    wave_lds_order();
+    float2* T = A;
+    A = B;
+    B = T;
+    gen_fixed_passes<N, INV, ID, p + 1, NS * R, OFF + (R - 1) * NS * (gen_tw_pair(R, NS) ? 2 : 1)>(A, B, tw, t);
+  }
+}
+
+// fft_row_gen for a fixed chain (f.n == GenFixed<ID>::n)
+template <int N, bool INV, int ID>
+__device__ __forceinline__ void fft_row_gen_fixed(float2 (&x)[N / 64], float2* buf, const float2* tw, int t) {
+  constexpr int E = N / 64;
+  constexpr int n = GenFixed<ID>::n;
+  static_assert(n <= N, "the chain's length exceeds the row");
+  float2* A = buf;
+  float2* B = buf + LdsSize<N>::value;
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (t + 64 * e < n) A[lidx(t + 64 * e)] = x[e];
+  wave_lds_order();
+  gen_fixed_passes<N, INV, ID, 0, 1, 0>(A, B, tw, t);
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = t + 64 * e < n ? A[lidx(t + 64 * e)] : make_float2(0.f, 0.f);
+  wave_lds_order();
+}
+
 // Fills the concatenated per-pass twiddle table of FftPlan<N, K> (double
 // precision, rounded once by round_complex).
 template <int N, int K = 0>

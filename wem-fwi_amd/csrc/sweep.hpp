@@ -171,8 +171,13 @@ struct LdsV {
   static constexpr int value = V::GEN ? 2 * LdsSize<V::n>::value : LdsSize<V::n>::value;
 };
 template <class V>
+__host__ __device__ constexpr int tw_entries() {
+  if constexpr (V::GEN) return gen_tw_cap(V::n);
+  else return plan_tw_size<V::n, V::K>();
+}
+template <class V>
 struct TwV {
-  static constexpr int value = V::GEN ? gen_tw_cap(V::n) : plan_tw_size<V::n, V::K>();
+  static constexpr int value = tw_entries<V>();
 };
 
 // the row transform of a variant; pre (register-planned lengths only): a
@@ -180,7 +185,8 @@ struct TwV {
 template <class V, bool INV>
 __device__ __forceinline__ void fft_v(float2 (&x)[V::E], float2* buf, const float2* tw, int t, const StepGeom& g,
                                       const float2* pre = nullptr, const float2* tab = nullptr) {
-  if constexpr (V::GEN) fft_row_gen<V::n, INV>(x, buf, tw, t, g.fg);
+  if constexpr (V::GEN && V::K > 0) fft_row_gen_fixed<V::n, INV, V::K>(x, buf, tw, t);
+  else if constexpr (V::GEN) fft_row_gen<V::n, INV>(x, buf, tw, t, g.fg);
   else fft_row<V::n, INV, V::K>(x, buf, tw, t, pre, tab);
 }
 

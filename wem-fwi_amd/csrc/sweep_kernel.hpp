@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "../../include/wemfwi.h"
 #include "kernels.h"
 #include "sweep.hpp"
@@ -197,10 +199,20 @@ static hipError_t launch_oneway_k(const SweepRow* rows, int nrows, const DepthTa
   return launch_oneway_v<Var<N, false, false, false, K>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
 }
 
-// rows whose FFT length is not N (runtime-planned passes, GEN variants)
+// rows whose FFT length is not N (GEN variants): the compile-time pass
+// chain of their length where there is one (GenFixed; WEMFWI_GEN_FIXED=0
+// keeps the runtime chain), else the runtime-planned passes
 template <int N, bool H = false>
 static hipError_t launch_oneway_gen(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
                                     int step, bool adj, const float2* tw, hipStream_t s) {
+  constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
+  if constexpr (ID > 0) {
+    const char* e = std::getenv("WEMFWI_GEN_FIXED");
+    if (gen_fixed_id(N, g.fg.n) == ID && !(e && *e && std::atoi(e) == 0)) {
+      if (g.ssf) return launch_oneway_v<Var<N, false, true, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+      return launch_oneway_v<Var<N, false, false, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+    }
+  }
   if (g.ssf) return launch_oneway_v<Var<N, false, true, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
   return launch_oneway_v<Var<N, false, false, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
 }
