@@ -263,3 +263,35 @@ def test_fixed_chain_is_bitwise_the_runtime_chain(nx, tap, nref, prop, tile, gen
         for i in (0, nrows - 1):
             want = fn(W[i].astype(complex), 2 * np.pi * float(freqs[i]), s, ref, p)
             assert rel(got["1"][i], want) < TOL, (up, adjoint, i, rel(got["1"][i], want))
+
+
+def test_born_fixed_chain_is_bitwise_the_runtime_chain(gen_env):
+    """nk = 600 (a compile-time chain): WEM, Born forward and exact adjoint
+    bit-identical with the runtime chain (sweeps and LinDown/LinUp)."""
+    import torch
+    from wemfwi import Model
+    from wemfwi.ops import Survey, born_adjoint, born_forward
+    sv, slow, den = _born_case(500, 25, 25)
+    sp, dp = O.edge_pad(slow, sv.pad), O.edge_pad(den, sv.pad)
+    m = Model(sp.astype(np.complex64), sv.dx, sv.dz, tap=sv.tap, pad=sv.pad, nref=sv.nref, prop=sv.prop,
+              eps=sv.eps, ntaylor=sv.ntaylor, den=dp.astype(np.complex64))
+    s = Survey(sv.src_iz, sv.src_ix, sv.rec_iz, sv.rec_ix, sv.freq, sv.index, sv.wave_f[sv.index], sv.nt_f,
+               ngs=sv.ngs, ngr=sv.ngr, offset=sv.pad)
+    rng = np.random.default_rng(8)
+    nxp = sv.nx + 2 * sv.pad
+    dS = _c(rng.standard_normal((sv.nz, nxp)) * 1e-8)
+    dR = _c(rng.standard_normal((sv.nz, nxp)) * 0.1)
+    data = np.zeros((sv.ns, sv.nr, sv.nt_f), complex)
+    data[:, :, sv.index] = rng.standard_normal((sv.ns, sv.nr, len(sv.index)))
+    got = {}
+    for fixed in ("1", "0"):
+        gen_env["WEMFWI_GEN_FIXED"] = fixed
+        print(f"born nk=600 fixed={fixed}", flush=True)
+        d = torch.zeros(s.data_shape, dtype=torch.complex64, device="cuda")
+        born_forward(m, s, dS, dR, d)
+        gS = torch.zeros((sv.nz, nxp), dtype=torch.complex64, device="cuda")
+        gR = torch.zeros_like(gS)
+        born_adjoint(m, s, gS, gR, _c(data), exact=True)
+        got[fixed] = [a.cpu().numpy() for a in (d, gS, gR)]
+    for a, b in zip(got["1"], got["0"]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -32,12 +32,13 @@ __device__ __forceinline__ void acc_row(float2* __restrict__ p, const float2 (&v
 // d = 1..nz-1, neighbour d-1.
 //   forward: out[d+dir] += P_d Sc_d (bg[d] * m[d])
 //   adjoint: out[d]     += conj(bg[d]) * Sc^H_d P^H_d (in[d+dir])
-template <int N, bool ADJ, bool EXACT, bool GEN>
+// FX: the compile-time pass chain of GEN rows (GenFixed id, 0: runtime)
+template <int N, bool ADJ, bool EXACT, bool GEN, int FX = 0>
 __global__ void __launch_bounds__(64 * kLinWaves)
     __attribute__((amdgpu_waves_per_eu(N <= 256 ? 4 : 2, 8)))
     lin_kernel(const LinRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models, StepGeom g,
                int dlo, int nd, int dir, const float2* __restrict__ twg) {
-  using V = Var<N, false, true, GEN>;
+  using V = Var<N, false, true, GEN, FX>;
   constexpr int E = N / 64;
   __shared__ float2 tw[TwV<V>::value];
   __shared__ float2 bufs[kLinWaves][LdsV<V>::value];
@@ -528,27 +529,33 @@ static dim3 grid1d(long long n, int block = 256) {
   return dim3((unsigned)b);
 }
 
-template <int N, bool GEN>
+template <int N, bool GEN, int FX = 0>
 static hipError_t launch_lin_v(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int dlo,
                                int nd, int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
   const long long items = (long long)nrows * nd;
   dim3 grid((unsigned)((items + kLinWaves - 1) / kLinWaves)), block(64 * kLinWaves);
   if (!adj)
-    hipLaunchKernelGGL((lin_kernel<N, false, false, GEN>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir,
-                       tw);
+    hipLaunchKernelGGL((lin_kernel<N, false, false, GEN, FX>), grid, block, 0, s, rows, nrows, models, g, dlo, nd,
+                       dir, tw);
   else if (exact)
-    hipLaunchKernelGGL((lin_kernel<N, true, true, GEN>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir,
-                       tw);
+    hipLaunchKernelGGL((lin_kernel<N, true, true, GEN, FX>), grid, block, 0, s, rows, nrows, models, g, dlo, nd,
+                       dir, tw);
   else
-    hipLaunchKernelGGL((lin_kernel<N, true, false, GEN>), grid, block, 0, s, rows, nrows, models, g, dlo, nd, dir,
-                       tw);
+    hipLaunchKernelGGL((lin_kernel<N, true, false, GEN, FX>), grid, block, 0, s, rows, nrows, models, g, dlo, nd,
+                       dir, tw);
   return hipGetLastError();
 }
 
 template <int N>
 static hipError_t launch_lin_n(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int dlo,
                                int nd, int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
-  if (g.fg.n != N) return launch_lin_v<N, true>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
+  if (g.fg.n != N) {
+    constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
+    if constexpr (ID > 0)
+      if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled())
+        return launch_lin_v<N, true, ID>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
+    return launch_lin_v<N, true>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
+  }
   return launch_lin_v<N, false>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
 }
 

@@ -1,6 +1,8 @@
 // Device-side descriptors shared by the kernels and the host layer.
 #pragma once
 
+#include <cstdlib>
+
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -64,6 +66,13 @@ hipError_t launch_phase_table(float2* ptab, const double* wlist, int nw, const d
 hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTables* models,
                          StepGeom g, int nz, int step, bool adj, const float2* tw, hipStream_t s,
                          bool tune = true);
+// the compile-time pass chains of GenFixed lengths are used unless
+// WEMFWI_GEN_FIXED=0 (the runtime chain: same results, slower)
+inline bool gen_fixed_enabled() {
+  const char* e = std::getenv("WEMFWI_GEN_FIXED");
+  return !(e && *e && std::atoi(e) == 0);
+}
+
 // sweep launch of radix plan k with the half workgroup tile (half the waves
 // per workgroup of launch_oneway's full tile); only where oneway_half_tile
 // holds (gen: rows whose FFT length g.fg.n is not N, k unused)

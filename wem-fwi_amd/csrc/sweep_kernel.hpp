@@ -4,8 +4,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <cstdlib>
-
 #include "../../include/wemfwi.h"
 #include "kernels.h"
 #include "sweep.hpp"
@@ -207,8 +205,7 @@ static hipError_t launch_oneway_gen(const SweepRow* rows, int nrows, const Depth
                                     int step, bool adj, const float2* tw, hipStream_t s) {
   constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
   if constexpr (ID > 0) {
-    const char* e = std::getenv("WEMFWI_GEN_FIXED");
-    if (gen_fixed_id(N, g.fg.n) == ID && !(e && *e && std::atoi(e) == 0)) {
+    if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled()) {
       if (g.ssf) return launch_oneway_v<Var<N, false, true, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
       return launch_oneway_v<Var<N, false, false, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
     }
