@@ -34,11 +34,16 @@ call against one call per shot and one call per row (sampled); and the
 subsurface-offset migration (nh = 11) of the same survey in one call against
 one call per (shot, frequency) row (sampled).
 
+Also Down sweeps of 1024 rows at the reference's own FFT lengths nk = 500
+and 600 (compile-time pass chains, and the runtime chain for comparison)
+beside the powers of two 512 and 1024.
+
 The gradient line also carries a central-difference check on 20 random
 directions (--fd-directions 0 skips it).
 
 Prints one JSON line.  --cpu-sample 0 skips the CPU baseline, --fwi-sample 0
-the gradient comparison, --uneven-sample 0 the uneven batch.
+the gradient comparison, --uneven-sample 0 the uneven batch,
+--any-length-rows 0 the nk = 500 / 600 sweeps.
 """
 
 from __future__ import annotations
@@ -751,6 +756,44 @@ def sweep_line(config: str, reps: int):
             "algorithmic_bytes": BYTES_PER_UPDATE * upd, "finite": finite}
 
 
+def any_length_line(nrows: int):
+    """Down sweeps (SSF nref = 1, nz = 300, full rows) at the reference's own
+    FFT lengths nk = 500 and 600 (rows of 512 and 1024 lanes, compile-time
+    pass chains; also timed with the runtime chain, WEMFWI_GEN_FIXED=0) and at
+    the neighbouring powers of two: the sweep kernel's mean launch time (HIP
+    events inside the library), nrows rows each."""
+    import torch
+    import wemfwi
+    nz = 300
+    out = {"config": f"ssf nref=1 nz={nz}, {nrows} rows of 2-30 Hz, Down forward in place"}
+    for nk, chain in ((500, "fixed"), (500, "runtime"), (512, "pow2"), (600, "fixed"), (600, "runtime"),
+                      (1024, "pow2")):
+        os.environ["WEMFWI_GEN_FIXED"] = "0" if chain == "runtime" else "1"
+        rng = np.random.default_rng(3)
+        v = 1500.0 + 2000.0 * np.linspace(0, 1, nz)[:, None] + 50.0 * rng.standard_normal((nz, nk))
+        model = wemfwi.Model((1.0 / v ** 2).astype(np.complex64), 10.0, 10.0, nref=1, prop="ssf", eps=0.04)
+        f_rows = (2.0 + 28.0 * np.arange(nrows) / max(nrows - 1, 1)).astype(np.float32)
+        W = torch.zeros((nrows, nz, nk), dtype=torch.complex64, device="cuda:0")
+        W[:, 0, nk // 3] = 1.0
+        model.oneway(False, False, f_rows, W)  # warm-up (the autotuner decides here)
+        torch.cuda.synchronize()
+        wemfwi.timing(True)
+        for _ in range(3):
+            model.oneway(False, False, f_rows, W)
+        torch.cuda.synchronize()
+        k_ms, k_n = wemfwi.kernel_time("oneway_kernel")
+        wemfwi.timing(False)
+        avg = k_ms / max(k_n, 1)
+        out[f"nk{nk}_{chain}"] = {"kernel_ms": round(avg, 3),
+                                  "gupdates_per_s": round(nk * (nz - 1) * nrows / (avg / 1e3) / 1e9, 1),
+                                  "finite": bool(torch.isfinite(torch.view_as_real(W)).all())}
+        model.close()
+        del W
+        torch.cuda.empty_cache()
+    os.environ.pop("WEMFWI_GEN_FIXED", None)
+    return out
+
+
 def measured_traffic(config: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this configuration (mean over its launches, each
@@ -787,6 +830,8 @@ def main():
                     help="central-difference directions of the FWI gradient check (0: skip)")
     ap.add_argument("--uneven-sample", type=int, default=32,
                     help="rows of the per-row loop in the uneven-batch comparison (0: skip)")
+    ap.add_argument("--any-length-rows", type=int, default=1024,
+                    help="rows of the nk = 500 / 600 sweep comparison (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -941,6 +986,8 @@ def main():
         line["uneven_batch"]["tiles"] = [{"N": e["N"], "rows": e["rows"], "tile": e["tile"], "ms": e["ms"]}
                                          for e in rep if e["kernel"] == "oneway_tile" and (e["N"], e["rows"]) in groups]
         line["uneven_offset_image"] = uneven_image_bench(args.uneven_sample)
+    if args.any_length_rows > 0:
+        line["any_length_sweeps"] = any_length_line(args.any_length_rows)
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(cfg, slow, freqs, src_ix, args.cpu_sample)
         if args.fwi_sample > 0:
