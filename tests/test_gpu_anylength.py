@@ -295,3 +295,40 @@ def test_born_fixed_chain_is_bitwise_the_runtime_chain(gen_env):
         got[fixed] = [a.cpu().numpy() for a in (d, gS, gR)]
     for a, b in zip(got["1"], got["0"]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("nx,tap,nref,prop", [(500, 0, 2, "pspi"), (580, 10, 1, "ssf")])
+def test_unit_ops_fixed_chain_is_bitwise_the_runtime_chain(nx, tap, nref, prop, gen_env):
+    """Single depth steps and the scatterer at nk = 500 / 600: compile-time
+    and runtime chains bit-identical (their oracle parity is checked above)."""
+    import torch
+    from wemfwi.api import Hypercube, Vector
+    from wemfwi.unit import Scatter
+    nz = 10
+    m, s, p, ref = _setup(nx, tap, 0, nref, prop, nz)
+    rng = np.random.default_rng(2)
+    x = (rng.standard_normal((4, nx)) + 1j * rng.standard_normal((4, nx))).astype(np.complex64)
+    freqs = np.array([2.0, 7.5, 17.25, 30.0], dtype=np.float32)
+    depth = np.array([0, 3, 6, 9], dtype=np.int32)
+    par = {"tap": tap, "pad": 0, "nref": nref, "prop": prop, "eps": 0.04, "ntaylor": 2}
+    vs = Vector(Hypercube([nx, nz], [0.0, 0.0], [10.0, 10.0]), arr=_c(s))
+    got = {}
+    for fixed in ("1", "0"):
+        gen_env["WEMFWI_GEN_FIXED"] = fixed
+        print(f"unit ops nk={nx + 2 * tap} fixed={fixed}", flush=True)
+        res = []
+        for adjoint in (False, True):
+            out = torch.zeros((4, nx), dtype=torch.complex64, device="cuda")
+            m.onestep(adjoint, freqs, depth, _c(x), out)
+            res.append(out.cpu().numpy())
+        op = Scatter(vs, 2, par, adjoint_mode="exact")
+        op.setFreq(9.0)
+        op.setDepth(4)
+        y = torch.zeros(nx, dtype=torch.complex64, device="cuda")
+        op.forward(_c(x[0]), y)
+        g = torch.zeros_like(y)
+        op.adjoint(g, _c(x[1]))
+        res += [y.cpu().numpy(), g.cpu().numpy()]
+        got[fixed] = res
+    for a, b in zip(got["1"], got["0"]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

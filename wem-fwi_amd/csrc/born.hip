@@ -95,11 +95,11 @@ __global__ void __launch_bounds__(64 * kLinWaves)
 }
 
 // Scatter as a unit operator on independent rows: out (+)= Sc_d(in).
-template <int N, bool ADJ, bool EXACT, bool GEN>
+template <int N, bool ADJ, bool EXACT, bool GEN, int FX = 0>
 __global__ void __launch_bounds__(64 * kLinWaves)
     scatter_kernel(const StepRow* __restrict__ rows, int nrows, const DepthTables* __restrict__ models,
                    StepGeom g, const float2* __restrict__ twg) {
-  using V = Var<N, false, true, GEN>;
+  using V = Var<N, false, true, GEN, FX>;
   constexpr int E = N / 64;
   __shared__ float2 tw[TwV<V>::value];
   __shared__ float2 bufs[kLinWaves][LdsV<V>::value];
@@ -577,23 +577,29 @@ hipError_t launch_lin(int N, const LinRow* rows, int nrows, const DepthTables* m
   }
 }
 
-template <int N, bool GEN>
+template <int N, bool GEN, int FX = 0>
 static hipError_t launch_scatter_v(const StepRow* rows, int nrows, const DepthTables* models, StepGeom g,
                                    bool adj, bool exact, const float2* tw, hipStream_t s) {
   dim3 grid((nrows + kLinWaves - 1) / kLinWaves), block(64 * kLinWaves);
   if (!adj)
-    hipLaunchKernelGGL((scatter_kernel<N, false, false, GEN>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((scatter_kernel<N, false, false, GEN, FX>), grid, block, 0, s, rows, nrows, models, g, tw);
   else if (exact)
-    hipLaunchKernelGGL((scatter_kernel<N, true, true, GEN>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((scatter_kernel<N, true, true, GEN, FX>), grid, block, 0, s, rows, nrows, models, g, tw);
   else
-    hipLaunchKernelGGL((scatter_kernel<N, true, false, GEN>), grid, block, 0, s, rows, nrows, models, g, tw);
+    hipLaunchKernelGGL((scatter_kernel<N, true, false, GEN, FX>), grid, block, 0, s, rows, nrows, models, g, tw);
   return hipGetLastError();
 }
 
 template <int N>
 static hipError_t launch_scatter_n(const StepRow* rows, int nrows, const DepthTables* models, StepGeom g,
                                    bool adj, bool exact, const float2* tw, hipStream_t s) {
-  if (g.fg.n != N) return launch_scatter_v<N, true>(rows, nrows, models, g, adj, exact, tw, s);
+  if (g.fg.n != N) {
+    constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
+    if constexpr (ID > 0)
+      if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled())
+        return launch_scatter_v<N, true, ID>(rows, nrows, models, g, adj, exact, tw, s);
+    return launch_scatter_v<N, true>(rows, nrows, models, g, adj, exact, tw, s);
+  }
   return launch_scatter_v<N, false>(rows, nrows, models, g, adj, exact, tw, s);
 }
 

@@ -450,11 +450,12 @@ hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTabl
   }
 }
 
-// single steps use the general variant (padding and split step allowed)
-template <int N, bool GEN>
+// single steps use the general variant (padding and split step allowed);
+// FX: the compile-time pass chain of GEN rows (GenFixed id, 0: runtime)
+template <int N, bool GEN, int FX = 0>
 static hipError_t launch_onestep_v(const StepRow* rows, int nrows, const DepthTables* models,
                                    StepGeom g, bool adj, const float2* tw, hipStream_t s) {
-  using V = Var<N, false, true, GEN>;
+  using V = Var<N, false, true, GEN, FX>;
   constexpr int kWavesPerBlock = WavesV<V>::value;
   dim3 grid((nrows + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   if (adj)
@@ -467,7 +468,13 @@ static hipError_t launch_onestep_v(const StepRow* rows, int nrows, const DepthTa
 template <int N>
 static hipError_t launch_onestep_n(const StepRow* rows, int nrows, const DepthTables* models,
                                    StepGeom g, bool adj, const float2* tw, hipStream_t s) {
-  if (g.fg.n != N) return launch_onestep_v<N, true>(rows, nrows, models, g, adj, tw, s);
+  if (g.fg.n != N) {
+    constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
+    if constexpr (ID > 0)
+      if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled())
+        return launch_onestep_v<N, true, ID>(rows, nrows, models, g, adj, tw, s);
+    return launch_onestep_v<N, true>(rows, nrows, models, g, adj, tw, s);
+  }
   return launch_onestep_v<N, false>(rows, nrows, models, g, adj, tw, s);
 }
 
