@@ -235,11 +235,16 @@ def gen_env():
     (490, 5, 3, "pspi", "full"),     # nk = 500 with FFT padding
     (600, 0, 2, "pspi", "full"),     # nk = 600: compile-time chain 8.3.5.5 in 1024-lane rows
     (600, 0, 1, "ssf", "half"),      # the same in the half workgroup tile
+    (100, 0, 1, "ssf", "full"),      # 4.5.5 in 128-lane rows
+    (190, 5, 2, "pspi", "full"),     # nk = 200: 4.2.5.5 in 256-lane rows
+    (300, 0, 1, "ssf", "full"),      # 4.3.5.5 in 512-lane rows
+    (750, 0, 2, "pspi", "half"),     # 2.3.5.5.5 in 1024-lane rows
+    (1000, 0, 1, "ssf", "full"),     # 8.5.5.5
+    (1500, 0, 2, "pspi", "full"),    # 4.3.5.5.5 in 2048-lane rows
 ])
 def test_fixed_chain_is_bitwise_the_runtime_chain(nx, tap, nref, prop, tile, gen_env):
-    """The compile-time pass chains of the reference's run lengths do the
-    runtime chain's butterflies in the same order: bit-identical sweeps in
-    every direction, and oracle parity."""
+    """The compile-time pass chains do the runtime chain's butterflies in the
+    same order: bit-identical sweeps in every direction, and oracle parity."""
     nz = 12
     m, s, p, ref = _setup(nx, tap, 0, nref, prop, nz, seed=4)
     rng = np.random.default_rng(5)
@@ -297,10 +302,12 @@ def test_born_fixed_chain_is_bitwise_the_runtime_chain(gen_env):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("nx,tap,nref,prop", [(500, 0, 2, "pspi"), (580, 10, 1, "ssf")])
+@pytest.mark.parametrize("nx,tap,nref,prop", [(500, 0, 2, "pspi"), (580, 10, 1, "ssf"), (1000, 0, 1, "ssf"),
+                                                (1500, 0, 2, "pspi")])
 def test_unit_ops_fixed_chain_is_bitwise_the_runtime_chain(nx, tap, nref, prop, gen_env):
-    """Single depth steps and the scatterer at nk = 500 / 600: compile-time
-    and runtime chains bit-identical (their oracle parity is checked above)."""
+    """Single depth steps and the scatterer at compile-time chain lengths:
+    compile-time and runtime chains bit-identical (their oracle parity is
+    checked above)."""
     import torch
     from wemfwi.api import Hypercube, Vector
     from wemfwi.unit import Scatter

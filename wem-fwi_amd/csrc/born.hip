@@ -549,13 +549,13 @@ static hipError_t launch_lin_v(const LinRow* rows, int nrows, const DepthTables*
 template <int N>
 static hipError_t launch_lin_n(const LinRow* rows, int nrows, const DepthTables* models, StepGeom g, int dlo,
                                int nd, int dir, bool adj, bool exact, const float2* tw, hipStream_t s) {
-  if (g.fg.n != N) {
-    constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
-    if constexpr (ID > 0)
-      if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled())
-        return launch_lin_v<N, true, ID>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
-    return launch_lin_v<N, true>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
-  }
+  if (g.fg.n != N)
+    return gen_fixed_dispatch<N>(
+        g.fg.n,
+        [&](auto id) {
+          return launch_lin_v<N, true, decltype(id)::value>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
+        },
+        [&] { return launch_lin_v<N, true>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s); });
   return launch_lin_v<N, false>(rows, nrows, models, g, dlo, nd, dir, adj, exact, tw, s);
 }
 
@@ -593,13 +593,11 @@ static hipError_t launch_scatter_v(const StepRow* rows, int nrows, const DepthTa
 template <int N>
 static hipError_t launch_scatter_n(const StepRow* rows, int nrows, const DepthTables* models, StepGeom g,
                                    bool adj, bool exact, const float2* tw, hipStream_t s) {
-  if (g.fg.n != N) {
-    constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
-    if constexpr (ID > 0)
-      if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled())
-        return launch_scatter_v<N, true, ID>(rows, nrows, models, g, adj, exact, tw, s);
-    return launch_scatter_v<N, true>(rows, nrows, models, g, adj, exact, tw, s);
-  }
+  if (g.fg.n != N)
+    return gen_fixed_dispatch<N>(
+        g.fg.n,
+        [&](auto id) { return launch_scatter_v<N, true, decltype(id)::value>(rows, nrows, models, g, adj, exact, tw, s); },
+        [&] { return launch_scatter_v<N, true>(rows, nrows, models, g, adj, exact, tw, s); });
   return launch_scatter_v<N, false>(rows, nrows, models, g, adj, exact, tw, s);
 }
 

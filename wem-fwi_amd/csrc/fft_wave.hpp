@@ -736,32 +736,61 @@ __device__ __forceinline__ void fft_row_gen(float2 (&x)[N / 64], float2* buf, co
   wave_lds_order();
 }
 
-// Pass chains fixed at compile time for the FFT lengths of the reference's
-// own runs (nk = 500 in 512-lane rows, nk = 600 in 1024-lane rows): the
-// passes fft_gen_plan builds for them (500 = 4 5 5 5, 600 = 8 3 5 5), over
-// the same twiddle table, with every radix, length, sub-length and table
-// offset a constant, so that the index arithmetic folds.  Same butterflies
-// in the same order as fft_row_gen: bit-identical results.  ID 0: none.
-template <int ID>
-struct GenFixed;
-template <>
-struct GenFixed<1> {
-  static constexpr int n = 500, P = 4;
-  static constexpr int R[P] = {4, 5, 5, 5};
-};
-template <>
-struct GenFixed<2> {
-  static constexpr int n = 600, P = 4;
-  static constexpr int R[P] = {8, 3, 5, 5};
-};
-__host__ __device__ constexpr int gen_fixed_id(int N, int n) {
-  return (N == 512 && n == 500) ? 1 : ((N == 1024 && n == 600) ? 2 : 0);
+// Pass chains fixed at compile time for FFT lengths that the reference's
+// runs and hypercubes use (nk = 500 and 600 in its runs; 100, 200, 300, 750,
+// 1000 and 1500 as axis lengths): the chain fft_gen_plan builds for the
+// length in its row (the smallest built row of at least nk lanes), over the
+// same twiddle table, with every radix, length, sub-length and table offset
+// a constant, so that the index arithmetic folds.  Same butterflies in the
+// same order as fft_row_gen: bit-identical results, which the tests check
+// for every entry.  (2000 = 16 5 5 5 was measured not bit-identical to the
+// runtime chain and keeps the runtime chain; no entry has a radix-16 pass.)
+// ID 0: none.
+constexpr int kGenFixedCount = 9;
+constexpr int kGenFixedN[kGenFixedCount] = {0, 500, 600, 100, 200, 300, 750, 1000, 1500};
+__host__ __device__ constexpr int gen_row_lanes(int n) {
+  int N = 128;
+  while (N < n) N *= 2;
+  return N;
 }
+struct GenChain {
+  int P;
+  int R[10];
+};
+// fft_gen_plan's chain for length n (largest register radix maxr first,
+// then one of 8 / 4 / 2 below it, then the odd primes in increasing order)
+__host__ __device__ constexpr GenChain gen_chain(int n, int maxr) {
+  GenChain c{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int m = n;
+  while (m % maxr == 0 && c.P < 10) {
+    c.R[c.P++] = maxr;
+    m /= maxr;
+  }
+  const int small[3] = {8, 4, 2};
+  for (int i = 0; i < 3; ++i)
+    if (small[i] < maxr && m % small[i] == 0 && c.P < 10) {
+      c.R[c.P++] = small[i];
+      m /= small[i];
+    }
+  for (int p = 3; m > 1 && c.P < 10; p += 2)
+    while (m % p == 0 && c.P < 10) {
+      c.R[c.P++] = p;
+      m /= p;
+    }
+  return c;
+}
+template <int ID>
+struct GenFixed {
+  static constexpr int n = kGenFixedN[ID];
+  static constexpr int N = gen_row_lanes(n);
+  static constexpr GenChain chain = gen_chain(n, gen_max_radix(N));
+  static constexpr int P = chain.P;
+};
 
 template <int N, bool INV, int ID, int p, int NS, int OFF>
 __device__ __forceinline__ void gen_fixed_passes(float2*& A, float2*& B, const float2* tw, int t) {
   if constexpr (p < GenFixed<ID>::P) {
-    constexpr int R = GenFixed<ID>::R[p];
+    constexpr int R = GenFixed<ID>::chain.R[p];
     constexpr int n = GenFixed<ID>::n;
     constexpr bool kBig = gen_max_radix(N) == 16;
     static_assert(!gen_direct(R) && (R <= 5 || kBig), "fixed chains use register butterflies");

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 
 #include "sweep.hpp"
 
@@ -71,6 +72,21 @@ hipError_t launch_oneway(int N, const SweepRow* rows, int nrows, const DepthTabl
 inline bool gen_fixed_enabled() {
   const char* e = std::getenv("WEMFWI_GEN_FIXED");
   return !(e && *e && std::atoi(e) == 0);
+}
+
+// Calls fixed(std::integral_constant<int, ID>) for the compile-time chain ID
+// of FFT length n in rows of N lanes when there is one (and they are
+// enabled), else runtime().
+template <int N, int ID = 1, class F, class G>
+inline hipError_t gen_fixed_dispatch(int n, F&& fixed, G&& runtime) {
+  if constexpr (ID < kGenFixedCount) {
+    if constexpr (GenFixed<ID>::N == N) {
+      if (n == GenFixed<ID>::n && gen_fixed_enabled()) return fixed(std::integral_constant<int, ID>{});
+    }
+    return gen_fixed_dispatch<N, ID + 1>(n, fixed, runtime);
+  } else {
+    return runtime();
+  }
 }
 
 // sweep launch of radix plan k with the half workgroup tile (half the waves

@@ -468,13 +468,11 @@ static hipError_t launch_onestep_v(const StepRow* rows, int nrows, const DepthTa
 template <int N>
 static hipError_t launch_onestep_n(const StepRow* rows, int nrows, const DepthTables* models,
                                    StepGeom g, bool adj, const float2* tw, hipStream_t s) {
-  if (g.fg.n != N) {
-    constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
-    if constexpr (ID > 0)
-      if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled())
-        return launch_onestep_v<N, true, ID>(rows, nrows, models, g, adj, tw, s);
-    return launch_onestep_v<N, true>(rows, nrows, models, g, adj, tw, s);
-  }
+  if (g.fg.n != N)
+    return gen_fixed_dispatch<N>(
+        g.fg.n,
+        [&](auto id) { return launch_onestep_v<N, true, decltype(id)::value>(rows, nrows, models, g, adj, tw, s); },
+        [&] { return launch_onestep_v<N, true>(rows, nrows, models, g, adj, tw, s); });
   return launch_onestep_v<N, false>(rows, nrows, models, g, adj, tw, s);
 }
 

@@ -203,15 +203,17 @@ static hipError_t launch_oneway_k(const SweepRow* rows, int nrows, const DepthTa
 template <int N, bool H = false>
 static hipError_t launch_oneway_gen(const SweepRow* rows, int nrows, const DepthTables* models, StepGeom g, int nz,
                                     int step, bool adj, const float2* tw, hipStream_t s) {
-  constexpr int ID = N == 512 ? 1 : (N == 1024 ? 2 : 0);
-  if constexpr (ID > 0) {
-    if (gen_fixed_id(N, g.fg.n) == ID && gen_fixed_enabled()) {
-      if (g.ssf) return launch_oneway_v<Var<N, false, true, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
-      return launch_oneway_v<Var<N, false, false, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
-    }
-  }
-  if (g.ssf) return launch_oneway_v<Var<N, false, true, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
-  return launch_oneway_v<Var<N, false, false, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+  return gen_fixed_dispatch<N>(
+      g.fg.n,
+      [&](auto id) {
+        constexpr int ID = decltype(id)::value;
+        if (g.ssf) return launch_oneway_v<Var<N, false, true, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+        return launch_oneway_v<Var<N, false, false, true, ID>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+      },
+      [&] {
+        if (g.ssf) return launch_oneway_v<Var<N, false, true, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+        return launch_oneway_v<Var<N, false, false, true>, H>(rows, nrows, models, g, nz, step, adj, tw, s);
+      });
 }
 
 template <int N, bool H = false, int K = 0>
