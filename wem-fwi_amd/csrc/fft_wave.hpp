@@ -786,6 +786,22 @@ struct GenFixed {
   static constexpr GenChain chain = gen_chain(n, gen_max_radix(N));
   static constexpr int P = chain.P;
 };
+// the chains as documented (500 = 4 5 5 5, 600 = 8 3 5 5, ...)
+constexpr bool gen_chain_is(const GenChain& c, const int* r, int p) {
+  if (c.P != p) return false;
+  for (int i = 0; i < p; ++i)
+    if (c.R[i] != r[i]) return false;
+  return true;
+}
+namespace gen_chain_check {
+constexpr int c500[] = {4, 5, 5, 5}, c600[] = {8, 3, 5, 5}, c100[] = {4, 5, 5}, c200[] = {4, 2, 5, 5},
+              c300[] = {4, 3, 5, 5}, c750[] = {2, 3, 5, 5, 5}, c1000[] = {8, 5, 5, 5}, c1500[] = {4, 3, 5, 5, 5};
+static_assert(gen_chain_is(GenFixed<1>::chain, c500, 4) && gen_chain_is(GenFixed<2>::chain, c600, 4) &&
+                  gen_chain_is(GenFixed<3>::chain, c100, 3) && gen_chain_is(GenFixed<4>::chain, c200, 4) &&
+                  gen_chain_is(GenFixed<5>::chain, c300, 4) && gen_chain_is(GenFixed<6>::chain, c750, 5) &&
+                  gen_chain_is(GenFixed<7>::chain, c1000, 4) && gen_chain_is(GenFixed<8>::chain, c1500, 5),
+              "compile-time chains");
+}  // namespace gen_chain_check
 
 template <int N, bool INV, int ID, int p, int NS, int OFF>
 __device__ __forceinline__ void gen_fixed_passes(float2*& A, float2*& B, const float2* tw, int t) {

@@ -406,6 +406,12 @@ int Model::build(const wemfwi_model_desc& desc, const cf* slow_h, const cf* den_
     fft_ok = fft_size_supported(N);
   } else {
     fft_ok = nk <= N && fft_gen_plan(nk, gen_max_radix(N), &fg);
+    // the compile-time chain of this length (if any) runs over the twiddle
+    // table of this runtime plan: they must be the same chain
+    const GenChain c = gen_chain(nk, gen_max_radix(N));
+    bool same = fft_ok && c.P == fg.np;
+    for (int p = 0; same && p < fg.np; ++p) same = gen_radix(fg, p) == c.R[p];
+    if (fft_ok && !same) return fail(WEMFWI_ERR_UNSUPPORTED, "model: FFT pass plans disagree");
   }
   const size_t nzx = (size_t)d.nz * d.nx;
   slow.assign(slow_h, slow_h + nzx);
